@@ -1,0 +1,195 @@
+"""Python mirror of the reference's Core/Mode call surface over libfxg.so.
+
+`FxCore` stands where mpifxcorr's Core stands for one configuration
+(mpifxcorr/src/core.h:52-169): `processdata` takes one slot -- the subint offsets
+{scan, sec, ns}, each datastream's data buffer and its control buffer
+{scan, sec, ns, validflags...} -- and returns the slot's results array in the layout of
+Configuration::populateResultLengths (configuration.cpp:2348-2630).  The Mode getters of
+mode.h:138-203 (getFreqs, getDataWeight) and Model::calculateDelayInterpolator
+(model.cpp:219-289) are exposed for inspection.  Every call goes through the C ABI of
+include/fxg.h; there is no host fallback: if libfxg.so is missing or no device is present
+the constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import fxg_abi as abi
+from .config import Config
+
+_LIB = None
+LIB_PATH = Path(__file__).resolve().parent / "libfxg.so"
+
+
+class FxgError(RuntimeError):
+    pass
+
+
+def load_library() -> C.CDLL:
+    """Load the in-tree libfxg.so and declare its entry points.  Raises if absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        raise FxgError(f"{LIB_PATH} is not built: run `python -m difx_amd.build` first")
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    lib.fxg_create.argtypes = [P(abi.fxg_config), P(C.c_void_p)]
+    lib.fxg_destroy.argtypes = [C.c_void_p]
+    lib.fxg_strerror.argtypes = [C.c_int]
+    lib.fxg_strerror.restype = C.c_char_p
+    lib.fxg_last_error.argtypes = [C.c_void_p]
+    lib.fxg_last_error.restype = C.c_char_p
+    lib.fxg_get_layout.argtypes = [C.c_void_p, P(abi.fxg_layout)]
+    lib.fxg_data_bytes.argtypes = [C.c_void_p, C.c_int32, P(C.c_int64)]
+    lib.fxg_result_offsets.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32)]
+    lib.fxg_process_subint.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_void_p), P(C.c_int32),
+                                       P(C.c_void_p), C.c_void_p]
+    lib.fxg_process_subint_device.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_void_p), P(C.c_int32),
+                                              P(C.c_void_p), C.c_void_p, C.c_void_p]
+    lib.fxg_ring_length.argtypes = [C.c_void_p]
+    lib.fxg_submit_subint.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_void_p), P(C.c_int32),
+                                      P(C.c_void_p), P(C.c_int32)]
+    lib.fxg_wait.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+    lib.fxg_get_spectrum.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    lib.fxg_get_data_weights.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+    lib.fxg_unpack_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int64,
+                                      C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.fxg_delay_interpolator.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]
+    _LIB = lib
+    return lib
+
+
+def _ptr_array(ptrs: Sequence[int]):
+    arr = (C.c_void_p * len(ptrs))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p
+    return arr
+
+
+class FxCore:
+    """One configuration's device-resident FX pipeline (Core + its Modes)."""
+
+    def __init__(self, cfg: Config):
+        self.lib = load_library()
+        self.cfg = cfg
+        self._abi, self._keep = cfg.to_abi()
+        h = C.c_void_p()
+        rc = self.lib.fxg_create(C.byref(self._abi), C.byref(h))
+        if rc != abi.FXG_OK:
+            raise FxgError(f"fxg_create failed: {self.lib.fxg_strerror(rc).decode()}")
+        self.h = h
+        lay = abi.fxg_layout()
+        self._check(self.lib.fxg_get_layout(self.h, C.byref(lay)))
+        self.coreresultlength = int(lay.coreresultlength)
+        self.blockspersend = int(lay.blockspersend)
+        self.ndatastreams = len(cfg.datastreams)
+
+    def _check(self, rc: int):
+        if rc != abi.FXG_OK:
+            msg = self.lib.fxg_last_error(self.h).decode() if getattr(self, "h", None) else ""
+            raise FxgError(f"{self.lib.fxg_strerror(rc).decode()}: {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fxg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- layout ----
+    def data_bytes(self, ds: int) -> int:
+        v = C.c_int64()
+        self._check(self.lib.fxg_data_bytes(self.h, ds, C.byref(v)))
+        return int(v.value)
+
+    def result_offsets(self):
+        nf, nb, nd = len(self.cfg.freqs), len(self.cfg.baselines), self.ndatastreams
+        b = np.zeros(max(nf * nb, 1), np.int32)
+        w = np.zeros(max(nf * nb, 1), np.int32)
+        a = np.zeros(nd, np.int32)
+        aw = np.zeros(nd, np.int32)
+        P = C.POINTER(C.c_int32)
+        self._check(self.lib.fxg_result_offsets(self.h, b.ctypes.data_as(P), w.ctypes.data_as(P),
+                                                a.ctypes.data_as(P), aw.ctypes.data_as(P)))
+        return b[: nf * nb].reshape(nf, nb), w[: nf * nb].reshape(nf, nb), a, aw
+
+    # ---- Core::processdata ----
+    def processdata(self, offsets: Sequence[int], data: Sequence[np.ndarray],
+                    control: Sequence[np.ndarray]) -> np.ndarray:
+        """Host buffers in, results (complex64[coreresultlength]) out."""
+        offs = (C.c_int32 * 3)(*[int(x) for x in offsets])
+        data = [np.ascontiguousarray(d, dtype=np.uint8) for d in data]
+        ctrl = [np.ascontiguousarray(c, dtype=np.int32) for c in control]
+        nbytes = (C.c_int32 * len(data))(*[len(d) for d in data])
+        out = np.zeros(self.coreresultlength, np.complex64)
+        self._check(self.lib.fxg_process_subint(
+            self.h, offs, _ptr_array([d.ctypes.data for d in data]), nbytes,
+            _ptr_array([c.ctypes.data for c in ctrl]), out.ctypes.data))
+        return out
+
+    def processdata_device(self, offsets, d_data: Sequence[int], nbytes: Sequence[int],
+                           control: Sequence[np.ndarray], d_results: int, stream: int = 0):
+        """Device-resident variant: d_data are device pointers (e.g. torch tensor
+        data_ptr()), d_results a device pointer of coreresultlength cf32; enqueued on
+        `stream` (a hipStream_t handle, 0 = the context's own stream)."""
+        offs = (C.c_int32 * 3)(*[int(x) for x in offsets])
+        ctrl = [np.ascontiguousarray(c, dtype=np.int32) for c in control]
+        self._ctrl_keep = ctrl
+        nb = (C.c_int32 * len(nbytes))(*[int(x) for x in nbytes])
+        self._check(self.lib.fxg_process_subint_device(
+            self.h, offs, _ptr_array(list(d_data)), nb, _ptr_array([c.ctypes.data for c in ctrl]),
+            C.c_void_p(d_results), C.c_void_p(stream) if stream else None))
+
+    def submit(self, offsets, data, control) -> int:
+        offs = (C.c_int32 * 3)(*[int(x) for x in offsets])
+        data = [np.ascontiguousarray(d, dtype=np.uint8) for d in data]
+        ctrl = [np.ascontiguousarray(c, dtype=np.int32) for c in control]
+        nbytes = (C.c_int32 * len(data))(*[len(d) for d in data])
+        hnd = C.c_int32()
+        self._check(self.lib.fxg_submit_subint(
+            self.h, offs, _ptr_array([d.ctypes.data for d in data]), nbytes,
+            _ptr_array([c.ctypes.data for c in ctrl]), C.byref(hnd)))
+        return int(hnd.value)
+
+    def wait(self, handle: int) -> np.ndarray:
+        out = np.zeros(self.coreresultlength, np.complex64)
+        self._check(self.lib.fxg_wait(self.h, handle, out.ctypes.data))
+        return out
+
+    # ---- Mode getters ----
+    def getFreqs(self, ds: int, band: int, fft: int) -> np.ndarray:
+        C_ = self.cfg.freqs[self.cfg.datastreams[ds].recfreqs[0]].nchan
+        out = np.zeros(C_, np.complex64)
+        self._check(self.lib.fxg_get_spectrum(self.h, ds, band, fft, out.ctypes.data))
+        return out
+
+    def getDataWeights(self, ds: int) -> np.ndarray:
+        out = np.zeros(self.blockspersend, np.float32)
+        self._check(self.lib.fxg_get_data_weights(self.h, ds, out.ctypes.data))
+        return out
+
+    def calculateDelayInterpolator(self, ds: int, sec: int, ns: int) -> np.ndarray:
+        out = (C.c_double * 3)()
+        self._check(self.lib.fxg_delay_interpolator(self.h, ds, sec, ns, out))
+        return np.array(out[:])
+
+    def unpack_device(self, ds: int, d_data: int, nbytes: int, startsample: int, nsamples: int,
+                      d_out: int, d_good: int, stream: int = 0):
+        self._check(self.lib.fxg_unpack_device(self.h, ds, C.c_void_p(d_data), nbytes, startsample,
+                                               nsamples, C.c_void_p(d_out), C.c_void_p(d_good),
+                                               C.c_void_p(stream) if stream else None))

@@ -1,0 +1,1195 @@
+// Host layer of libfxg: the C ABI of include/fxg.h.
+//
+// fxg_create plays the part of Configuration (validation, derived quantities, the result
+// layout of populateResultLengths, configuration.cpp:2348-2630) and of the Core constructor
+// (device-resident buffers, core.cpp:23-154).  fxg_process_subint_device plays
+// Core::processdata (core.cpp:657-1129): per subint it runs the setup kernel (delay
+// interpolators and frame validity), then per chunk of FFTs the fused F-engine and the
+// X-engine, then the accumulation into the slot's results (core.cpp:1166-1376, 1605-1938).
+#include "fxg_internal.h"
+#include "../../include/fxg.h"
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr double TWO_PI = 6.283185307179586476925286766559;  // architecture.h.in:430
+constexpr double TINY = 0.000000001;                          // mode.cpp:27
+constexpr int FLAGS_PER_INT = 30;                             // mpifxcorr.h:29
+constexpr int INVALID_SUBINT = -99999999;                     // mode.h:222
+constexpr int RING = 4;                                       // RECEIVE_RING_LENGTH, core.cpp:156
+
+// configuration.cpp:38-51
+int calcstridelength(int n) {
+  if (n <= 16) return n;
+  for (int g = (int)std::sqrt(n + 0.1); g > 0; --g)
+    if (n % g == 0) return n / g;
+  return n;
+}
+
+bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+struct Slot {
+  hipStream_t copy = nullptr;
+  hipEvent_t copied = nullptr, done = nullptr;
+  std::vector<uint8_t*> d_data;
+  std::vector<long long> cap;
+  float2* d_results = nullptr;
+  float* h_results = nullptr;     // pinned
+  bool busy = false;
+};
+
+struct Staging {
+  DevDsRun* h_run = nullptr;      // pinned
+  int32_t* h_flags = nullptr;     // pinned
+  hipEvent_t used = nullptr;
+  bool pending = false;
+};
+
+}  // namespace
+
+struct fxg_ctx {
+  std::string err;
+  int device = 0;
+  // configuration copy
+  std::vector<fxg_freq> freqs;
+  std::vector<fxg_datastream> dss;
+  std::vector<fxg_baseline> bls;
+  std::vector<std::vector<double>> models;
+  long long subint_ns = 0;
+  int guard_ns = 0, rotorder = 1, nbuffered = 10, crossautos = 1, xengine = 0;
+  double model_inc = 120.0, scan_start = 0.0;
+  // derived
+  int nds = 0, nfreqs = 0, nbl = 0;
+  int B = 0, L = 0, iscomplex = 0, flaglen = 0;
+  std::vector<long long> databytes;
+  int bpa = 1;
+  long long coreresultlength = 0;
+  std::vector<int> boff, bwoff, acoff, acwoff;
+  std::vector<int> used, equiv;
+  // device tables
+  std::vector<DevDs> h_ds;
+  DevDs* d_ds = nullptr;
+  DevDsRun* d_run = nullptr;
+  double* d_model = nullptr;
+  float* d_chanfreq = nullptr;
+  uint8_t* d_fvalid = nullptr;
+  long long fvalid_total = 0;
+  int maxframes = 0;
+  float2* d_twiddle = nullptr;
+  float* d_weights = nullptr;
+  int32_t* d_flags = nullptr;
+  // chunk buffers
+  int chunkcap = 0;
+  float2* d_spec32 = nullptr;
+  uint32_t* d_spec16 = nullptr;
+  long long spec_elems = 0;
+  // plain X
+  std::vector<DevProd> prods;
+  DevProd* d_prods = nullptr;
+  float2* d_xacc = nullptr;
+  long long xacc_len = 0;
+  int maxchan = 0;
+  // MFMA X
+  std::vector<DevXGroup> groups;
+  std::vector<std::vector<std::pair<int, int>>> group_rows;   // (ds, band) per row
+  std::map<std::pair<int, int>, std::pair<int, int>> rowof;   // (ds, recband) -> (group, row)
+  DevXGroup* d_groups = nullptr;
+  float* d_vacc = nullptr;
+  long long vacc_len = 0;
+  float* d_xscale = nullptr;
+  long long xscale_len = 0;
+  // accumulation entries
+  std::vector<FinPlain> finp;
+  std::vector<FinMfma> finm;
+  std::vector<FinWeight> finw;
+  FinPlain* d_finp = nullptr;
+  FinMfma* d_finm = nullptr;
+  FinWeight* d_finw = nullptr;
+  int maxoutp = 0, maxoutm = 0;
+  // streams, staging, ring
+  hipStream_t stream = nullptr;
+  Staging stage[2];
+  int stage_next = 0;
+  Slot slots[RING];
+  int last_chunk0 = 0, last_nffts = 0;
+  std::mutex mu;
+};
+
+namespace {
+
+int fail(fxg_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(call)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (call);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return fail(ctx, FXG_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e));       \
+  } while (0)
+
+template <typename T>
+int dev_upload(fxg_ctx* ctx, T** dptr, const std::vector<T>& h) {
+  size_t bytes = std::max<size_t>(h.size() * sizeof(T), 16);
+  HIPCHK(hipMalloc((void**)dptr, bytes));
+  if (!h.empty()) HIPCHK(hipMemcpy(*dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return FXG_OK;
+}
+
+int validate_and_derive(fxg_ctx* ctx) {
+  if (ctx->rotorder != 1)
+    return fail(ctx, FXG_E_INVALID, "only fringe rotation order 1 is supported by this build");
+  if (ctx->nds < 1) return fail(ctx, FXG_E_INVALID, "no datastreams");
+  // blockspersend from the first datastream's first recorded freq (configuration.cpp:1843-1852)
+  {
+    const fxg_freq& f = ctx->freqs[ctx->dss[0].recfreq_index[0]];
+    const double ffttime = 1000.0 * f.nchan / f.bw_mhz;
+    const double bd = ctx->subint_ns / ffttime;
+    ctx->B = (int)(bd + 0.5);
+    if (std::fabs(bd - ctx->B) > TINY)
+      return fail(ctx, FXG_E_INVALID, "subint is not an integer number of FFTs");
+  }
+  ctx->flaglen = (ctx->B + FLAGS_PER_INT - 1) / FLAGS_PER_INT;
+  for (int d = 0; d < ctx->nds; ++d) {
+    const fxg_datastream& ds = ctx->dss[d];
+    if (ds.format != FXG_FMT_VDIF && ds.format != FXG_FMT_MARK5B)
+      return fail(ctx, FXG_E_INVALID, "unsupported data format");
+    if (!((ds.nbits == 2 && !ds.complex_sampling) || ds.nbits == 8))
+      return fail(ctx, FXG_E_INVALID, "only 2-bit real and 8-bit real/complex sampling are supported");
+    if (ds.complex_sampling && ds.format != FXG_FMT_VDIF)
+      return fail(ctx, FXG_E_INVALID, "complex sampling requires VDIF");
+    if (ds.nrecfreqs < 1 || ds.nrecfreqs > FXG_MAX_FREQS || ds.nrecbands < 1 || ds.nrecbands > FXG_MAX_BANDS)
+      return fail(ctx, FXG_E_INVALID, "bad recorded freq/band count");
+    if (ds.nbits == 2 && !is_pow2(ds.nrecbands))
+      return fail(ctx, FXG_E_INVALID, "2-bit data needs 1, 2, 4, 8 or 16 bands per sample word");
+    if (ds.nbits == 2 && ds.nrecbands > 16)
+      return fail(ctx, FXG_E_INVALID, "at most 16 2-bit bands per sample word");
+    const fxg_freq& f0 = ctx->freqs[ds.recfreq_index[0]];
+    for (int k = 0; k < ds.nrecfreqs; ++k) {
+      int fi = ds.recfreq_index[k];
+      if (fi < 0 || fi >= ctx->nfreqs) return fail(ctx, FXG_E_INVALID, "recorded freq index out of range");
+      const fxg_freq& f = ctx->freqs[fi];
+      if (f.nchan != f0.nchan || std::fabs(f.bw_mhz - f0.bw_mhz) > TINY || f.chanavg != f0.chanavg)
+        return fail(ctx, FXG_E_INVALID, "a datastream's recorded freqs must share nchan, bandwidth, averaging");
+      if (ds.complex_sampling && f.lsb)
+        return fail(ctx, FXG_E_INVALID, "complex lower-sideband data is not supported by this build");
+      if (f.chanavg < 1 || f.nchan % f.chanavg)
+        return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
+    }
+    int L = ds.complex_sampling ? f0.nchan : 2 * f0.nchan;
+    if (d == 0) { ctx->L = L; ctx->iscomplex = ds.complex_sampling; }
+    if (L != ctx->L || ds.complex_sampling != ctx->iscomplex)
+      return fail(ctx, FXG_E_INVALID, "all datastreams must share FFT length and sampling in this build");
+    if (!is_pow2(L) || L < 256 || L > 4096)
+      return fail(ctx, FXG_E_INVALID, "FFT length must be a power of two in [256, 4096]");
+    int S = ds.arraystridelen ? ds.arraystridelen : calcstridelength(f0.nchan);
+    const int C = f0.nchan;
+    if (!is_pow2(S) || (L / 8) % S || S + L / S > L / 4 || S + C / S > L / 4)
+      return fail(ctx, FXG_E_INVALID, "unsupported array stride length");
+    const int pay = ds.framebytes - ds.headerbytes;
+    const int bits = ds.nbits * (ds.complex_sampling ? 2 : 1) * ds.nrecbands;
+    if (pay <= 16 || pay % 8 || (pay * 8) % bits || ds.framebytes % 4 || ds.headerbytes % 4)
+      return fail(ctx, FXG_E_INVALID, "frame size inconsistent with the sample format");
+    int nb_per_freq[FXG_MAX_FREQS] = {0};
+    for (int b = 0; b < ds.nrecbands; ++b) {
+      int lf = ds.recband_localfreq[b];
+      if (lf < 0 || lf >= ds.nrecfreqs) return fail(ctx, FXG_E_INVALID, "band local freq out of range");
+      if (++nb_per_freq[lf] > 4) return fail(ctx, FXG_E_INVALID, "at most 4 bands per recorded freq");
+    }
+    for (int z = 0; z < ds.nzoombands; ++z) {
+      int lz = ds.zoomband_localfreq[z];
+      if (lz < 0 || lz >= ds.nzoomfreqs) return fail(ctx, FXG_E_INVALID, "zoom band freq out of range");
+      const fxg_freq& zf = ctx->freqs[ds.zoomfreq_index[lz]];
+      int parent = ds.zoomfreq_parent_localfreq[lz];
+      if (parent < 0 || parent >= ds.nrecfreqs) return fail(ctx, FXG_E_INVALID, "zoom parent out of range");
+      if (ds.zoomfreq_chanoffset[lz] < 0 || ds.zoomfreq_chanoffset[lz] + zf.nchan > C)
+        return fail(ctx, FXG_E_INVALID, "zoom window outside its parent band");
+      if (zf.chanavg != f0.chanavg || ds.zoomfreq_chanoffset[lz] % f0.chanavg)
+        return fail(ctx, FXG_E_INVALID, "zoom averaging must match its parent's");
+      bool found = false;
+      for (int b = 0; b < ds.nrecbands; ++b)
+        if (ds.recband_localfreq[b] == parent && ds.recband_pol[b] == ds.zoomband_pol[z]) found = true;
+      if (!found) return fail(ctx, FXG_E_INVALID, "zoom band without a parent band");
+    }
+    if (ds.model_order < 0 || ds.model_order >= FXG_MAX_POLY || ds.model_nsamples < 1 || !ds.model_delay_us)
+      return fail(ctx, FXG_E_INVALID, "bad delay model");
+  }
+  for (int j = 0; j < ctx->nbl; ++j) {
+    const fxg_baseline& b = ctx->bls[j];
+    if (b.ds1 < 0 || b.ds1 >= ctx->nds || b.ds2 < 0 || b.ds2 >= ctx->nds)
+      return fail(ctx, FXG_E_INVALID, "baseline datastream out of range");
+    if (b.nfreqs < 0 || b.nfreqs > FXG_MAX_FREQS) return fail(ctx, FXG_E_INVALID, "bad baseline freq count");
+    for (int k = 0; k < b.nfreqs; ++k) {
+      if (b.freq_index[k] < 0 || b.freq_index[k] >= ctx->nfreqs)
+        return fail(ctx, FXG_E_INVALID, "baseline freq out of range");
+      if (b.npolprod[k] < 1 || b.npolprod[k] > FXG_MAX_POLPROD)
+        return fail(ctx, FXG_E_INVALID, "bad polarisation product count");
+      const fxg_freq& f = ctx->freqs[b.freq_index[k]];
+      for (int p = 0; p < b.npolprod[k]; ++p) {
+        const int bands[2] = {b.ds1band[k][p], b.ds2band[k][p]};
+        const int dsi[2] = {b.ds1, b.ds2};
+        for (int s = 0; s < 2; ++s) {
+          const fxg_datastream& ds = ctx->dss[dsi[s]];
+          int bi = bands[s];
+          if (bi < 0 || bi >= ds.nrecbands + ds.nzoombands)
+            return fail(ctx, FXG_E_INVALID, "baseline band index out of range");
+          int fi = bi < ds.nrecbands ? ds.recfreq_index[ds.recband_localfreq[bi]]
+                                     : ds.zoomfreq_index[ds.zoomband_localfreq[bi - ds.nrecbands]];
+          if (fi != b.freq_index[k])
+            return fail(ctx, FXG_E_INVALID, "baseline products must correlate bands of the baseline freq");
+          if (bi >= ds.nrecbands && ctx->xengine != FXG_X_FP32)
+            return fail(ctx, FXG_E_INVALID, "zoom bands need the FP32 X-engine in this build");
+        }
+      }
+      if (f.nchan % f.chanavg) return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
+    }
+  }
+  if (ctx->xengine < FXG_X_FP32 || ctx->xengine > FXG_X_BF16)
+    return fail(ctx, FXG_E_INVALID, "unknown X-engine");
+  return FXG_OK;
+}
+
+// Configuration::getDataBytes (configuration.cpp:794-820)
+long long get_data_bytes(const fxg_ctx* ctx, int d) {
+  const fxg_datastream& ds = ctx->dss[d];
+  const fxg_freq& f = ctx->freqs[ds.recfreq_index[0]];
+  long long valid = ((long long)ctx->B * ds.nrecbands * 2 * ds.nbits * f.nchan) / 8;
+  valid += ((long long)(int)(ctx->guard_ns / (1000.0 / (f.bw_mhz * 2.0)) + 1.0) * ds.nrecbands * ds.nbits) / 8;
+  const int pay = ds.framebytes - ds.headerbytes;
+  long long nframes = valid / pay + 2;
+  if (ds.format == FXG_FMT_MARK5B) nframes += nframes % 2;
+  return nframes * ds.framebytes;
+}
+
+int freq_of_band(const fxg_datastream& ds, int bi) {
+  return bi < ds.nrecbands ? ds.recfreq_index[ds.recband_localfreq[bi]]
+                           : ds.zoomfreq_index[ds.zoomband_localfreq[bi - ds.nrecbands]];
+}
+
+// record band (parent for zooms) and channel offset of band bi
+void band_source(const fxg_datastream& ds, int bi, int* rec, int* choff) {
+  if (bi < ds.nrecbands) { *rec = bi; *choff = 0; return; }
+  int z = bi - ds.nrecbands;
+  int lz = ds.zoomband_localfreq[z];
+  int parent = ds.zoomfreq_parent_localfreq[lz];
+  for (int b = 0; b < ds.nrecbands; ++b)
+    if (ds.recband_localfreq[b] == parent && ds.recband_pol[b] == ds.zoomband_pol[z]) { *rec = b; break; }
+  *choff = ds.zoomfreq_chanoffset[lz];
+}
+
+// Configuration::populateResultLengths for one config, one phase centre, no pulsar bins
+// (configuration.cpp:2348-2630) with the used/equivalent freq flags of :2270-2320.
+void build_layout(fxg_ctx* ctx) {
+  const int nf = ctx->nfreqs, nb = ctx->nbl;
+  ctx->used.assign(nf, 0);
+  ctx->equiv.assign(nf, 0);
+  int maxprod = 0;
+  for (const auto& b : ctx->bls)
+    for (int k = 0; k < b.nfreqs; ++k) {
+      ctx->used[b.freq_index[k]] = 1;
+      maxprod = std::max(maxprod, b.npolprod[k]);
+    }
+  for (int j = 0; j < nf; ++j) {
+    if (ctx->used[j]) continue;
+    for (int k = 0; k < nf; ++k) {
+      const fxg_freq &fj = ctx->freqs[j], &fk = ctx->freqs[k];
+      double bwdiff = fj.bw_mhz - fk.bw_mhz, fdiff = fj.freq_mhz - fk.freq_mhz;
+      if (fj.lsb) fdiff -= fj.bw_mhz;
+      if (fk.lsb) fdiff += fk.bw_mhz;
+      if (std::fabs(bwdiff) < TINY && std::fabs(fdiff) < TINY && fj.nchan == fk.nchan &&
+          fj.chanavg == fk.chanavg && ctx->used[k])
+        ctx->equiv[j] = 1;
+    }
+  }
+  ctx->bpa = maxprod > 2 ? 2 : 1;
+  long long idx = 0;
+  ctx->boff.assign((size_t)nf * nb, -1);
+  ctx->bwoff.assign((size_t)nf * nb, -1);
+  for (int f = 0; f < nf; ++f) {
+    if (!ctx->used[f]) continue;
+    const fxg_freq& fr = ctx->freqs[f];
+    for (int j = 0; j < nb; ++j)
+      for (int k = 0; k < ctx->bls[j].nfreqs; ++k)
+        if (ctx->bls[j].freq_index[k] == f) {
+          ctx->boff[(size_t)f * nb + j] = (int)idx;
+          idx += (long long)ctx->bls[j].npolprod[k] * ((fr.nchan + fr.chanavg - 1) / fr.chanavg);
+        }
+  }
+  for (int f = 0; f < nf; ++f) {
+    if (!ctx->used[f]) continue;
+    for (int j = 0; j < nb; ++j)
+      for (int k = 0; k < ctx->bls[j].nfreqs; ++k)
+        if (ctx->bls[j].freq_index[k] == f) {
+          ctx->bwoff[(size_t)f * nb + j] = (int)idx;
+          idx += ctx->bls[j].npolprod[k] / 2 + ctx->bls[j].npolprod[k] % 2;
+        }
+  }
+  ctx->acoff.assign(ctx->nds, 0);
+  ctx->acwoff.assign(ctx->nds, 0);
+  for (int d = 0; d < ctx->nds; ++d) {
+    const fxg_datastream& ds = ctx->dss[d];
+    ctx->acoff[d] = (int)idx;
+    for (int b = 0; b < ds.nrecbands + ds.nzoombands; ++b) {
+      int fi = freq_of_band(ds, b);
+      if (ctx->used[fi] || ctx->equiv[fi])
+        idx += ctx->bpa * ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
+    }
+  }
+  for (int d = 0; d < ctx->nds; ++d) {
+    const fxg_datastream& ds = ctx->dss[d];
+    ctx->acwoff[d] = (int)idx;
+    int toadd = 0;
+    for (int b = 0; b < ds.nrecbands + ds.nzoombands; ++b) {
+      int fi = freq_of_band(ds, b);
+      if (ctx->used[fi] || ctx->equiv[fi]) toadd += ctx->bpa;
+    }
+    toadd = (toadd + 1) / 2;
+    idx += std::max(toadd, 1);
+  }
+  ctx->coreresultlength = idx;
+}
+
+// For recorded band b of datastream ds: the partner band for cross-polarisation autos
+// (first two matching bands of its freq, mode.cpp:1320-1352), or -1.
+int crosspol_partner(const fxg_datastream& ds, int b) {
+  int lf = ds.recband_localfreq[b];
+  int idx[2] = {-1, -1}, n = 0;
+  for (int k = 0; k < ds.nrecbands && n < 2; ++k)
+    if (ds.recband_localfreq[k] == lf) idx[n++] = k;
+  if (n < 2) return -1;
+  if (b == idx[0]) return idx[1];
+  if (b == idx[1]) return idx[0];
+  return -1;
+}
+
+int build_tables(fxg_ctx* ctx) {
+  const int nds = ctx->nds;
+  ctx->databytes.resize(nds);
+  // models
+  std::vector<double> model_all;
+  std::vector<float> chanfreq;
+  ctx->h_ds.assign(nds, DevDs());
+  ctx->maxframes = 0;
+  long long fvo = 0;
+  for (int d = 0; d < nds; ++d) {
+    const fxg_datastream& s = ctx->dss[d];
+    DevDs& x = ctx->h_ds[d];
+    std::memset(&x, 0, sizeof(x));
+    const fxg_freq& f0 = ctx->freqs[s.recfreq_index[0]];
+    x.format = s.format;
+    x.nbits = s.nbits;
+    x.iscomplex = s.complex_sampling;
+    x.framebytes = s.framebytes;
+    x.headerbytes = s.headerbytes;
+    x.nrecbands = s.nrecbands;
+    x.nrecfreqs = s.nrecfreqs;
+    x.C = f0.nchan;
+    x.L = s.complex_sampling ? f0.nchan : 2 * f0.nchan;
+    x.S = s.arraystridelen ? s.arraystridelen : calcstridelength(f0.nchan);
+    x.NS = x.L / x.S;
+    x.NFS = x.C / x.S;
+    const int nsb = s.nbits * (s.complex_sampling ? 2 : 1);
+    x.spf = (s.framebytes - s.headerbytes) * 8 / (nsb * s.nrecbands);
+    // bytes-per-block ratio (mode.cpp:124-139); samplesperblock is 1 for Mk5Mode
+    x.bpb_num = (s.nrecbands * nsb) / 8;
+    x.bpb_den = 1;
+    if (x.bpb_num == 0) { x.bpb_num = 1; x.bpb_den = 8 / (s.nrecbands * nsb); }
+    x.sampletime_us = s.complex_sampling ? 1.0 / f0.bw_mhz : 1.0 / (2.0 * f0.bw_mhz);
+    x.bw_mhz = f0.bw_mhz;
+    x.intclockseconds = (int)std::floor(s.clock_coeff0_us / 1000000.0 + 0.5);
+    x.model_order = s.model_order;
+    x.model_nsamples = s.model_nsamples;
+    x.model_off = (int)(model_all.size() / FXG_MAX_POLY);
+    for (int k = 0; k < s.model_nsamples * FXG_MAX_POLY; ++k) model_all.push_back(s.model_delay_us[k]);
+    // fractional LO frequency (mode.cpp:83-99)
+    x.fractionallo = 0;
+    for (int k = 0; k < s.nrecfreqs; ++k) {
+      double lo = ctx->freqs[s.recfreq_index[k]].freq_mhz;
+      if (std::fabs(lo - (int)(lo + 0.5)) > TINY) x.fractionallo = 1;
+    }
+    for (int k = 0; k < s.nrecfreqs; ++k) {
+      const fxg_freq& f = ctx->freqs[s.recfreq_index[k]];
+      DevRecFreq& r = x.rf[k];
+      r.lofreq = f.freq_mhz;
+      if (s.complex_sampling && f.lsb) r.lofreq = -r.lofreq;     // mode.cpp:875-880
+      r.lo_offset_hz = s.lo_offset_hz[k];
+      r.looff = (r.lo_offset_hz > 0.0 || r.lo_offset_hz < 0.0);
+      double fl = std::fabs(r.lo_offset_hz) - (int)std::fabs(r.lo_offset_hz);
+      r.isfraclooffset = r.looff && fl > TINY;
+      if (r.lo_offset_hz < 0) fl = -fl;
+      r.fraclooffset = r.looff ? fl : 0.0;
+      r.clockoff = s.clock_offset_us[k];
+      r.halfdelta = s.clock_offset_delta_us[k] / 2;
+      r.lsb = f.lsb;
+      r.nbands = 0;
+      for (int b = 0; b < s.nrecbands; ++b)
+        if (s.recband_localfreq[b] == k) r.band[r.nbands++] = b;
+    }
+    // channel-frequency slopes (mode.cpp:303-324), computed as the reference does
+    x.chanfreq_off = (int)chanfreq.size();
+    const double bw = f0.bw_mhz;
+    const int C = x.C, S = x.S, NFS = x.NFS;
+    for (int i = 0; i < S; ++i) chanfreq.push_back((float)((TWO_PI * (i) * bw) / C));
+    for (int i = 0; i < NFS; ++i) chanfreq.push_back((float)((TWO_PI * i * S * bw) / C));
+    for (int i = 0; i < NFS; ++i) chanfreq.push_back((float)((-TWO_PI * ((NFS - i) * S) * bw) / C));
+    ctx->databytes[d] = get_data_bytes(ctx, d);
+    const int cap_frames = (int)((ctx->databytes[d] + 4LL * s.framebytes) / s.framebytes);
+    x.fvalid_off = (int)fvo;
+    fvo += cap_frames;
+    ctx->maxframes = std::max(ctx->maxframes, cap_frames);
+  }
+  ctx->fvalid_total = fvo;
+  int rc;
+  if ((rc = dev_upload(ctx, &ctx->d_model, model_all))) return rc;
+  if ((rc = dev_upload(ctx, &ctx->d_chanfreq, chanfreq))) return rc;
+  HIPCHK(hipMalloc((void**)&ctx->d_fvalid, std::max<long long>(fvo, 16)));
+  HIPCHK(hipMemset(ctx->d_fvalid, 0, std::max<long long>(fvo, 16)));
+  // FFT twiddles exp(-2 pi i m / L), m < L/2, from FP64
+  {
+    std::vector<float2> tw(ctx->L / 2);
+    for (int m = 0; m < ctx->L / 2; ++m) {
+      double a = -TWO_PI * (double)m / (double)ctx->L;
+      tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    if ((rc = dev_upload(ctx, &ctx->d_twiddle, tw))) return rc;
+  }
+  HIPCHK(hipMalloc((void**)&ctx->d_run, sizeof(DevDsRun) * nds));
+  HIPCHK(hipMalloc((void**)&ctx->d_weights, sizeof(float) * (size_t)nds * ctx->B));
+  HIPCHK(hipMemset(ctx->d_weights, 0, sizeof(float) * (size_t)nds * ctx->B));
+  HIPCHK(hipMalloc((void**)&ctx->d_flags, sizeof(int32_t) * (size_t)nds * ctx->flaglen * 2));
+  return FXG_OK;
+}
+
+long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
+
+int build_engines(fxg_ctx* ctx, int chunk_req) {
+  const int nds = ctx->nds, nb = ctx->nbl;
+  const bool mfma = ctx->xengine != FXG_X_FP32;
+  // ---- MFMA groups: one per recorded freq table entry, rows = station bands ----
+  if (mfma) {
+    std::map<int, int> gof;
+    for (int d = 0; d < nds; ++d) {
+      const fxg_datastream& s = ctx->dss[d];
+      for (int b = 0; b < s.nrecbands; ++b) {
+        int fi = s.recfreq_index[s.recband_localfreq[b]];
+        if (!gof.count(fi)) {
+          gof[fi] = (int)ctx->groups.size();
+          DevXGroup g;
+          std::memset(&g, 0, sizeof(g));
+          g.C = ctx->freqs[fi].nchan;
+          ctx->groups.push_back(g);
+          ctx->group_rows.emplace_back();
+        }
+        int g = gof[fi];
+        ctx->rowof[{d, b}] = {g, (int)ctx->group_rows[g].size()};
+        ctx->group_rows[g].push_back({d, b});
+        ctx->h_ds[d].band_xfreq[b] = g;
+        ctx->h_ds[d].band_xrow[b] = (int)ctx->group_rows[g].size() - 1;
+      }
+    }
+    for (size_t g = 0; g < ctx->groups.size(); ++g) {
+      int nin = (int)ctx->group_rows[g].size();
+      if (nin > 64) return fail(ctx, FXG_E_INVALID, "MFMA X-engine supports at most 64 inputs per freq");
+      ctx->groups[g].nin = nin;
+      ctx->groups[g].ntile = (nin + 31) / 32;
+    }
+  }
+  // ---- chunk capacity: spectra buffer within a budget ----
+  long long per_fft = 0;
+  if (mfma) {
+    for (auto& g : ctx->groups) per_fft += (long long)g.ntile * 32 * g.C * 4;   // bytes
+  } else {
+    for (int d = 0; d < nds; ++d) per_fft += (long long)ctx->h_ds[d].nrecbands * ctx->h_ds[d].C * 8;
+  }
+  const long long budget = 4LL << 30;
+  long long cap = chunk_req > 0 ? chunk_req : std::max<long long>(64, budget / std::max<long long>(per_fft, 1));
+  cap = std::min<long long>(cap, round_up(ctx->B, FXG_XPAD));
+  cap = round_up(cap, FXG_XPAD);
+  ctx->chunkcap = (int)cap;
+  if (mfma) {
+    long long off = 0, aoff = 0, soff = 0;
+    for (auto& g : ctx->groups) {
+      g.spec_off = off;
+      off += (long long)g.ntile * 32 * (cap / FXG_TB) * g.C * FXG_TB;
+      g.acc_off = aoff;
+      aoff += (long long)g.C * (g.ntile * (g.ntile + 1) / 2) * 2 * 1024;
+      g.scale_off = soff;
+      soff += (long long)g.ntile * 32 * g.C;
+    }
+    ctx->spec_elems = off;
+    ctx->vacc_len = aoff;
+    ctx->xscale_len = soff;
+    HIPCHK(hipMalloc((void**)&ctx->d_spec16, sizeof(uint32_t) * std::max<long long>(off, 4)));
+    HIPCHK(hipMemset(ctx->d_spec16, 0, sizeof(uint32_t) * std::max<long long>(off, 4)));
+    HIPCHK(hipMalloc((void**)&ctx->d_vacc, sizeof(float) * std::max<long long>(aoff, 4)));
+    // nominal per-(input, channel) scale: 1/sqrt(L * <x^2>) of the 2-bit (or 8-bit) levels
+    HIPCHK(hipMalloc((void**)&ctx->d_xscale, sizeof(float) * std::max<long long>(soff, 4)));
+    const double p_hi = 0.5 * std::erfc(0.95 / std::sqrt(2.0));
+    double xsq = ctx->iscomplex || ctx->dss[0].nbits == 8 ? 2.0 * (20.0 / 3.3) * (20.0 / 3.3)
+                                                          : 2 * p_hi * 3.3359 * 3.3359 + (1 - 2 * p_hi);
+    float sc = (float)(1.0 / std::sqrt((double)ctx->L * xsq));
+    if (fxg_launch_fill(ctx->d_xscale, soff, sc, ctx->stream)) return fail(ctx, FXG_E_HIP, "fill");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (int rc = dev_upload(ctx, &ctx->d_groups, ctx->groups)) return rc;
+  } else {
+    long long off = 0;
+    for (int d = 0; d < nds; ++d) {
+      ctx->h_ds[d].spec_off = off;
+      off += (long long)ctx->h_ds[d].nrecbands * cap * ctx->h_ds[d].C;
+    }
+    ctx->spec_elems = off;
+    HIPCHK(hipMalloc((void**)&ctx->d_spec32, sizeof(float2) * std::max<long long>(off, 2)));
+    HIPCHK(hipMemset(ctx->d_spec32, 0, sizeof(float2) * std::max<long long>(off, 2)));
+  }
+  // ---- products and accumulation entries ----
+  const int bpa = ctx->bpa;
+  if (!mfma) {
+    long long acc = 0;
+    auto add_prod = [&](int d1, int b1, int c1, int d2, int b2, int c2, int nchan) {
+      DevProd p;
+      p.ds1 = d1; p.band1 = b1; p.choff1 = c1;
+      p.ds2 = d2; p.band2 = b2; p.choff2 = c2;
+      p.nchan = nchan;
+      p.acc_off = (int)acc;
+      acc += nchan;
+      ctx->prods.push_back(p);
+      ctx->maxchan = std::max(ctx->maxchan, nchan);
+      return (long long)p.acc_off;
+    };
+    for (int j = 0; j < nb; ++j) {
+      const fxg_baseline& b = ctx->bls[j];
+      for (int k = 0; k < b.nfreqs; ++k) {
+        const int f = b.freq_index[k];
+        const fxg_freq& fr = ctx->freqs[f];
+        const int nout = (fr.nchan + fr.chanavg - 1) / fr.chanavg;
+        for (int p = 0; p < b.npolprod[k]; ++p) {
+          int r1, c1, r2, c2;
+          band_source(ctx->dss[b.ds1], b.ds1band[k][p], &r1, &c1);
+          band_source(ctx->dss[b.ds2], b.ds2band[k][p], &r2, &c2);
+          long long src = add_prod(b.ds1, r1, c1, b.ds2, r2, c2, fr.nchan);
+          FinPlain e;
+          e.src = src;
+          e.dst = ctx->boff[(size_t)f * nb + j] + p * nout;
+          e.nout = nout;
+          e.avg = fr.chanavg;
+          ctx->finp.push_back(e);
+          ctx->maxoutp = std::max(ctx->maxoutp, nout);
+        }
+      }
+    }
+    // autocorrelations (mode.cpp:1272, 1333-1339) and their placement (core.cpp:1258-1304)
+    for (int d = 0; d < nds; ++d) {
+      const fxg_datastream& s = ctx->dss[d];
+      const int C = ctx->h_ds[d].C;
+      std::vector<long long> aacc(s.nrecbands), xacc(s.nrecbands, -1);
+      for (int b = 0; b < s.nrecbands; ++b) aacc[b] = add_prod(d, b, 0, d, b, 0, C);
+      if (ctx->crossautos)
+        for (int b = 0; b < s.nrecbands; ++b) {
+          int pb = crosspol_partner(s, b);
+          if (pb >= 0) xacc[b] = add_prod(d, b, 0, d, pb, 0, C);
+        }
+      const int cavg = ctx->freqs[s.recfreq_index[0]].chanavg;
+      long long idx = ctx->acoff[d];
+      for (int w = 0; w < bpa; ++w) {
+        if (w == 1 && !ctx->crossautos) break;
+        for (int b = 0; b < s.nrecbands + s.nzoombands; ++b) {
+          const int fi = freq_of_band(s, b);
+          if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
+          const int nout = ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
+          int rec, choff;
+          band_source(s, b, &rec, &choff);
+          long long src = w == 0 ? aacc[rec] : xacc[rec];
+          if (src >= 0) {
+            FinPlain e;
+            e.src = src + choff;
+            e.dst = (int)idx;
+            e.nout = nout;
+            e.avg = cavg;
+            ctx->finp.push_back(e);
+            ctx->maxoutp = std::max(ctx->maxoutp, nout);
+          }
+          idx += nout;
+        }
+      }
+    }
+    ctx->xacc_len = acc;
+    HIPCHK(hipMalloc((void**)&ctx->d_xacc, sizeof(float2) * std::max<long long>(acc, 2)));
+    if (int rc = dev_upload(ctx, &ctx->d_prods, ctx->prods)) return rc;
+    if (int rc = dev_upload(ctx, &ctx->d_finp, ctx->finp)) return rc;
+  } else {
+    for (int j = 0; j < nb; ++j) {
+      const fxg_baseline& b = ctx->bls[j];
+      for (int k = 0; k < b.nfreqs; ++k) {
+        const int f = b.freq_index[k];
+        const fxg_freq& fr = ctx->freqs[f];
+        const int nout = (fr.nchan + fr.chanavg - 1) / fr.chanavg;
+        for (int p = 0; p < b.npolprod[k]; ++p) {
+          auto g1 = ctx->rowof[{b.ds1, b.ds1band[k][p]}];
+          auto g2 = ctx->rowof[{b.ds2, b.ds2band[k][p]}];
+          FinMfma e;
+          e.group = g1.first;
+          e.row1 = g1.second;
+          e.row2 = g2.second;
+          e.chan0 = 0;
+          e.dst = ctx->boff[(size_t)f * nb + j] + p * nout;
+          e.nout = nout;
+          e.avg = fr.chanavg;
+          e.isauto = 0;
+          ctx->finm.push_back(e);
+          ctx->maxoutm = std::max(ctx->maxoutm, nout);
+        }
+      }
+    }
+    for (int d = 0; d < nds; ++d) {
+      const fxg_datastream& s = ctx->dss[d];
+      const int cavg = ctx->freqs[s.recfreq_index[0]].chanavg;
+      long long idx = ctx->acoff[d];
+      for (int w = 0; w < bpa; ++w) {
+        if (w == 1 && !ctx->crossautos) break;
+        for (int b = 0; b < s.nrecbands; ++b) {
+          const int fi = freq_of_band(s, b);
+          if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
+          const int nout = ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
+          int other = w == 0 ? b : crosspol_partner(s, b);
+          if (other >= 0) {
+            auto g1 = ctx->rowof[{d, b}];
+            auto g2 = ctx->rowof[{d, other}];
+            FinMfma e;
+            e.group = g1.first;
+            e.row1 = g1.second;
+            e.row2 = g2.second;
+            e.chan0 = 0;
+            e.dst = (int)idx;
+            e.nout = nout;
+            e.avg = cavg;
+            e.isauto = (w == 0);
+            ctx->finm.push_back(e);
+            ctx->maxoutm = std::max(ctx->maxoutm, nout);
+          }
+          idx += nout;
+        }
+      }
+    }
+    if (int rc = dev_upload(ctx, &ctx->d_finm, ctx->finm)) return rc;
+  }
+  // weights (core.cpp:1005-1052, 1306-1370)
+  for (int f = 0; f < ctx->nfreqs; ++f) {
+    if (!ctx->used[f]) continue;
+    for (int j = 0; j < nb; ++j) {
+      const fxg_baseline& b = ctx->bls[j];
+      for (int k = 0; k < b.nfreqs; ++k)
+        if (b.freq_index[k] == f)
+          for (int p = 0; p < b.npolprod[k]; ++p) {
+            FinWeight w;
+            w.ds1 = b.ds1; w.ds2 = b.ds2;
+            w.dstf = ctx->bwoff[(size_t)f * nb + j] * 2 + p;
+            ctx->finw.push_back(w);
+          }
+    }
+  }
+  for (int d = 0; d < nds; ++d) {
+    const fxg_datastream& s = ctx->dss[d];
+    int widx = ctx->acwoff[d] * 2;
+    for (int w = 0; w < bpa; ++w) {
+      if (w == 1 && !ctx->crossautos) break;
+      for (int b = 0; b < s.nrecbands + s.nzoombands; ++b) {
+        const int fi = freq_of_band(s, b);
+        if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
+        int rec, choff;
+        band_source(s, b, &rec, &choff);
+        if (w == 0 || crosspol_partner(s, rec) >= 0) {
+          FinWeight e;
+          e.ds1 = d; e.ds2 = -1; e.dstf = widx;
+          ctx->finw.push_back(e);
+        }
+        widx++;
+      }
+    }
+  }
+  if (int rc = dev_upload(ctx, &ctx->d_finw, ctx->finw)) return rc;
+  if (int rc = dev_upload(ctx, &ctx->d_ds, ctx->h_ds)) return rc;
+  return FXG_OK;
+}
+
+int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_data, const int32_t* nbytes,
+               const int32_t* const* control, float2* d_results, hipStream_t st) {
+  const int nds = ctx->nds;
+  for (int d = 0; d < nds; ++d) {
+    if (!control || !control[d]) return fail(ctx, FXG_E_INVALID, "missing control buffer");
+    if (nbytes[d] > 1 && !d_data[d]) return fail(ctx, FXG_E_INVALID, "missing data buffer");
+    if (nbytes[d] < 0) return fail(ctx, FXG_E_INVALID, "negative data length");
+    const long long capframes = (ctx->databytes[d] + 4LL * ctx->dss[d].framebytes) / ctx->dss[d].framebytes;
+    if (nbytes[d] / ctx->dss[d].framebytes > capframes)
+      return fail(ctx, FXG_E_INVALID, "data buffer larger than the slot capacity");
+  }
+  // staging of the per-subint datastream state and valid flags (double buffered)
+  Staging& sg = ctx->stage[ctx->stage_next];
+  ctx->stage_next ^= 1;
+  if (sg.pending) {
+    HIPCHK(hipEventSynchronize(sg.used));
+    sg.pending = false;
+  }
+  int maxframes = 0;
+  int32_t* d_flags = ctx->d_flags + (size_t)(&sg - ctx->stage) * nds * ctx->flaglen;
+  for (int d = 0; d < nds; ++d) {
+    DevDsRun& r = sg.h_run[d];
+    std::memset(&r, 0, sizeof(r));
+    r.scan = control[d][0];
+    r.datasec = control[d][1];
+    r.datans = control[d][2];
+    r.datalen = nbytes[d];
+    r.data = d_data[d];
+    r.nframes = nbytes[d] / ctx->dss[d].framebytes;
+    r.valid = (nbytes[d] > 1) && (r.datasec > INVALID_SUBINT) ? 1 : 0;
+    r.vflags = d_flags + (size_t)d * ctx->flaglen;
+    std::memcpy(sg.h_flags + (size_t)d * ctx->flaglen, control[d] + 3, sizeof(int32_t) * ctx->flaglen);
+    maxframes = std::max(maxframes, r.nframes);
+  }
+  HIPCHK(hipMemcpyAsync(ctx->d_run, sg.h_run, sizeof(DevDsRun) * nds, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(d_flags, sg.h_flags, sizeof(int32_t) * nds * ctx->flaglen, hipMemcpyHostToDevice, st));
+  HIPCHK(hipEventRecord(sg.used, st));
+  sg.pending = true;
+
+  const bool mfma = ctx->xengine != FXG_X_FP32;
+  if (fxg_launch_zero(d_results, ctx->coreresultlength * 8, st)) return fail(ctx, FXG_E_HIP, "zero results");
+  if (mfma) {
+    if (fxg_launch_zero(ctx->d_vacc, ctx->vacc_len * 4, st)) return fail(ctx, FXG_E_HIP, "zero acc");
+  } else {
+    if (fxg_launch_zero(ctx->d_xacc, ctx->xacc_len * 8, st)) return fail(ctx, FXG_E_HIP, "zero acc");
+  }
+  SetupArgs sa;
+  sa.ds = ctx->d_ds;
+  sa.run = ctx->d_run;
+  sa.model = ctx->d_model;
+  sa.fvalid = ctx->d_fvalid;
+  sa.nds = nds;
+  sa.offscan = offsets[0];
+  sa.offsec = offsets[1];
+  sa.offns = offsets[2];
+  sa.model_inc_secs = ctx->model_inc;
+  sa.B = ctx->B;
+  if (fxg_launch_setup(sa, std::max(maxframes, 1), st)) return fail(ctx, FXG_E_HIP, "setup launch");
+
+  FengineArgs fa;
+  std::memset(&fa, 0, sizeof(fa));
+  fa.ds = ctx->d_ds;
+  fa.run = ctx->d_run;
+  fa.twiddle = ctx->d_twiddle;
+  fa.weights = ctx->d_weights;
+  fa.B = ctx->B;
+  fa.offsec = offsets[1];
+  fa.offns = offsets[2];
+  fa.scan_start_sec = ctx->scan_start;
+  fa.out_mode = ctx->xengine;
+  fa.spec32 = ctx->d_spec32;
+  fa.spec16 = ctx->d_spec16;
+  fa.xgroups = ctx->d_groups;
+  fa.xscale = ctx->d_xscale;
+  fa.chanfreq = ctx->d_chanfreq;
+  fa.fvalid = ctx->d_fvalid;
+  fa.chunkcap = ctx->chunkcap;
+  fa.ds0 = 0;
+  for (int c0 = 0; c0 < ctx->B; c0 += ctx->chunkcap) {
+    const int n = std::min(ctx->chunkcap, ctx->B - c0);
+    const int npad = (int)round_up(n, FXG_XPAD);
+    fa.chunk0 = c0;
+    fa.nffts = n;
+    fa.nffts_pad = npad;
+    const int nblocks = npad / FXG_TB;
+    // enough workgroups to fill 256 CUs several times over
+    long long wgs_per_ds = std::max(1LL, (long long)(2048 + nds - 1) / nds);
+    int bpw = (int)std::max(1LL, std::min<long long>(16, nblocks / wgs_per_ds));
+    fa.blocks_per_wg = bpw;
+    if (fxg_launch_fengine(fa, ctx->L, ctx->iscomplex, nds, st)) return fail(ctx, FXG_E_HIP, "F-engine launch");
+    if (mfma) {
+      XMfmaArgs xa;
+      xa.groups = ctx->d_groups;
+      xa.spec16 = ctx->d_spec16;
+      xa.acc = ctx->d_vacc;
+      xa.nffts_pad = npad;
+      xa.chunkcap = ctx->chunkcap;
+      xa.mode = ctx->xengine;
+      for (size_t g = 0; g < ctx->groups.size(); ++g)
+        if (fxg_launch_xmfma(xa, 1, (int)g, ctx->groups[g].ntile, ctx->groups[g].C, st))
+          return fail(ctx, FXG_E_HIP, "MFMA X-engine launch");
+    } else {
+      XPlainArgs xa;
+      xa.prods = ctx->d_prods;
+      xa.ds = ctx->d_ds;
+      xa.spec32 = ctx->d_spec32;
+      xa.acc = ctx->d_xacc;
+      xa.nffts = n;
+      xa.chunkcap = ctx->chunkcap;
+      if (fxg_launch_xplain(xa, (int)ctx->prods.size(), ctx->maxchan, st))
+        return fail(ctx, FXG_E_HIP, "X-engine launch");
+    }
+    ctx->last_chunk0 = c0;
+    ctx->last_nffts = n;
+  }
+  if (mfma) {
+    if (fxg_launch_fin_mfma(ctx->d_finm, (int)ctx->finm.size(), ctx->maxoutm, ctx->d_groups, ctx->d_vacc,
+                            ctx->d_xscale, d_results, st))
+      return fail(ctx, FXG_E_HIP, "accumulate launch");
+  } else {
+    if (fxg_launch_fin_plain(ctx->d_finp, (int)ctx->finp.size(), ctx->maxoutp, ctx->d_xacc, d_results, st))
+      return fail(ctx, FXG_E_HIP, "accumulate launch");
+  }
+  if (fxg_launch_fin_weights(ctx->d_finw, (int)ctx->finw.size(), ctx->d_weights, ctx->B, (float*)d_results, st))
+    return fail(ctx, FXG_E_HIP, "weights launch");
+  return FXG_OK;
+}
+
+void free_ctx(fxg_ctx* ctx) {
+  auto f = [](void* p) { if (p) (void)hipFree(p); };
+  f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid);
+  f(ctx->d_twiddle); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
+  f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
+  f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
+  for (auto& sg : ctx->stage) {
+    if (sg.h_run) (void)hipHostFree(sg.h_run);
+    if (sg.h_flags) (void)hipHostFree(sg.h_flags);
+    if (sg.used) (void)hipEventDestroy(sg.used);
+  }
+  for (auto& s : ctx->slots) {
+    for (auto p : s.d_data) f(p);
+    f(s.d_results);
+    if (s.h_results) (void)hipHostFree(s.h_results);
+    if (s.copy) (void)hipStreamDestroy(s.copy);
+    if (s.copied) (void)hipEventDestroy(s.copied);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+}
+
+int ensure_slots(fxg_ctx* ctx) {
+  for (auto& s : ctx->slots) {
+    if (s.copy) continue;
+    HIPCHK(hipStreamCreateWithFlags(&s.copy, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s.copied, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+    s.d_data.assign(ctx->nds, nullptr);
+    s.cap.assign(ctx->nds, 0);
+    for (int d = 0; d < ctx->nds; ++d) {
+      s.cap[d] = ctx->databytes[d] + 4LL * ctx->dss[d].framebytes;
+      HIPCHK(hipMalloc((void**)&s.d_data[d], s.cap[d]));
+    }
+    HIPCHK(hipMalloc((void**)&s.d_results, sizeof(float2) * std::max<long long>(ctx->coreresultlength, 1)));
+    HIPCHK(hipHostMalloc((void**)&s.h_results, sizeof(float2) * std::max<long long>(ctx->coreresultlength, 1), 0));
+  }
+  return FXG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fxg_strerror(int code) {
+  switch (code) {
+    case FXG_OK: return "ok";
+    case FXG_E_INVALID: return "invalid argument or unsupported configuration";
+    case FXG_E_HIP: return "HIP runtime error";
+    case FXG_E_NOMEM: return "out of memory";
+    case FXG_E_STATE: return "call out of order";
+    case FXG_E_NODEVICE: return "no usable gfx950 device";
+  }
+  return "unknown error";
+}
+
+const char* fxg_last_error(const fxg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fxg_create(const fxg_config* cfg, fxg_ctx** out) {
+  if (!cfg || !out) return FXG_E_INVALID;
+  *out = nullptr;
+  if (cfg->nfreqs < 1 || !cfg->freqs || cfg->ndatastreams < 1 || !cfg->datastreams ||
+      cfg->nbaselines < 0 || (cfg->nbaselines > 0 && !cfg->baselines))
+    return FXG_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FXG_E_NODEVICE;
+  fxg_ctx* ctx = new fxg_ctx();
+  (void)hipGetDevice(&ctx->device);
+  ctx->nfreqs = cfg->nfreqs;
+  ctx->nds = cfg->ndatastreams;
+  ctx->nbl = cfg->nbaselines;
+  ctx->freqs.assign(cfg->freqs, cfg->freqs + cfg->nfreqs);
+  ctx->dss.assign(cfg->datastreams, cfg->datastreams + cfg->ndatastreams);
+  if (cfg->nbaselines > 0) ctx->bls.assign(cfg->baselines, cfg->baselines + cfg->nbaselines);
+  ctx->models.resize(ctx->nds);
+  for (int d = 0; d < ctx->nds; ++d) {
+    const fxg_datastream& s = ctx->dss[d];
+    if (s.model_delay_us && s.model_nsamples > 0)
+      ctx->models[d].assign(s.model_delay_us, s.model_delay_us + (size_t)s.model_nsamples * FXG_MAX_POLY);
+    ctx->dss[d].model_delay_us = ctx->models[d].empty() ? nullptr : ctx->models[d].data();
+  }
+  ctx->subint_ns = cfg->subint_ns;
+  ctx->guard_ns = cfg->guard_ns;
+  ctx->rotorder = cfg->fringe_rot_order;
+  ctx->nbuffered = cfg->num_buffered_ffts;
+  ctx->crossautos = cfg->write_cross_autocorrs;
+  ctx->model_inc = cfg->model_inc_secs;
+  ctx->scan_start = cfg->scan_start_sec;
+  ctx->xengine = cfg->xengine;
+  int rc = validate_and_derive(ctx);
+  if (rc == FXG_OK) {
+    build_layout(ctx);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+      rc = fail(ctx, FXG_E_HIP, "stream create");
+  }
+  if (rc == FXG_OK) rc = build_tables(ctx);
+  if (rc == FXG_OK) rc = build_engines(ctx, cfg->chunk_ffts);
+  if (rc == FXG_OK) {
+    for (auto& sg : ctx->stage) {
+      if (hipHostMalloc((void**)&sg.h_run, sizeof(DevDsRun) * ctx->nds, 0) != hipSuccess ||
+          hipHostMalloc((void**)&sg.h_flags, sizeof(int32_t) * ctx->nds * ctx->flaglen + 16, 0) != hipSuccess ||
+          hipEventCreateWithFlags(&sg.used, hipEventDisableTiming) != hipSuccess) {
+        rc = fail(ctx, FXG_E_HIP, "staging allocation");
+        break;
+      }
+    }
+  }
+  if (rc != FXG_OK) {
+    std::fprintf(stderr, "fxg_create: %s\n", ctx->err.c_str());
+    free_ctx(ctx);
+    delete ctx;
+    return rc;
+  }
+  *out = ctx;
+  return FXG_OK;
+}
+
+int fxg_destroy(fxg_ctx* ctx) {
+  if (!ctx) return FXG_E_INVALID;
+  (void)hipDeviceSynchronize();
+  free_ctx(ctx);
+  delete ctx;
+  return FXG_OK;
+}
+
+int fxg_get_layout(const fxg_ctx* ctx, fxg_layout* out) {
+  if (!ctx || !out) return FXG_E_INVALID;
+  out->coreresultlength = ctx->coreresultlength;
+  out->blockspersend = ctx->B;
+  out->nbaselinefreqs = ctx->nfreqs * ctx->nbl;
+  long long m = 0;
+  for (auto v : ctx->databytes) m = std::max(m, v);
+  out->max_databytes = m;
+  return FXG_OK;
+}
+
+int fxg_data_bytes(const fxg_ctx* ctx, int32_t ds, int64_t* bytes) {
+  if (!ctx || !bytes || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
+  *bytes = ctx->databytes[ds];
+  return FXG_OK;
+}
+
+int fxg_result_offsets(const fxg_ctx* ctx, int32_t* baseline, int32_t* bweight, int32_t* autocorr,
+                       int32_t* acweight) {
+  if (!ctx) return FXG_E_INVALID;
+  if (baseline) std::memcpy(baseline, ctx->boff.data(), sizeof(int32_t) * ctx->boff.size());
+  if (bweight) std::memcpy(bweight, ctx->bwoff.data(), sizeof(int32_t) * ctx->bwoff.size());
+  if (autocorr) std::memcpy(autocorr, ctx->acoff.data(), sizeof(int32_t) * ctx->nds);
+  if (acweight) std::memcpy(acweight, ctx->acwoff.data(), sizeof(int32_t) * ctx->nds);
+  return FXG_OK;
+}
+
+int fxg_process_subint_device(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_data,
+                              const int32_t* nbytes, const int32_t* const* control, float* d_results,
+                              void* stream) {
+  if (!ctx || !offsets || !d_data || !nbytes || !control || !d_results) return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return run_subint(ctx, offsets, d_data, nbytes, control, (float2*)d_results, st);
+}
+
+int fxg_ring_length(const fxg_ctx* ctx) { return ctx ? RING : 0; }
+
+int fxg_submit_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* data,
+                      const int32_t* nbytes, const int32_t* const* control, fxg_handle* handle) {
+  if (!ctx || !offsets || !data || !nbytes || !control || !handle) return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  int rc = ensure_slots(ctx);
+  if (rc) return rc;
+  int k = -1;
+  for (int i = 0; i < RING; ++i)
+    if (!ctx->slots[i].busy) { k = i; break; }
+  if (k < 0) return fail(ctx, FXG_E_STATE, "all ring slots are in use");
+  Slot& s = ctx->slots[k];
+  std::vector<const uint8_t*> dd(ctx->nds);
+  for (int d = 0; d < ctx->nds; ++d) {
+    if (nbytes[d] > s.cap[d]) return fail(ctx, FXG_E_INVALID, "data buffer larger than the slot capacity");
+    if (nbytes[d] > 0) HIPCHK(hipMemcpyAsync(s.d_data[d], data[d], nbytes[d], hipMemcpyHostToDevice, s.copy));
+    dd[d] = s.d_data[d];
+  }
+  HIPCHK(hipEventRecord(s.copied, s.copy));
+  HIPCHK(hipStreamWaitEvent(ctx->stream, s.copied, 0));
+  rc = run_subint(ctx, offsets, dd.data(), nbytes, control, s.d_results, ctx->stream);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(s.h_results, s.d_results, sizeof(float2) * ctx->coreresultlength,
+                        hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipEventRecord(s.done, ctx->stream));
+  s.busy = true;
+  *handle = k;
+  return FXG_OK;
+}
+
+int fxg_wait(fxg_ctx* ctx, fxg_handle handle, float* results) {
+  if (!ctx || handle < 0 || handle >= RING) return FXG_E_INVALID;
+  Slot& s = ctx->slots[handle];
+  if (!s.busy) return fail(ctx, FXG_E_STATE, "wait on a slot that was not submitted");
+  HIPCHK(hipEventSynchronize(s.done));
+  if (results) std::memcpy(results, s.h_results, sizeof(float2) * ctx->coreresultlength);
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  s.busy = false;
+  return FXG_OK;
+}
+
+int fxg_process_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* data,
+                       const int32_t* nbytes, const int32_t* const* control, float* results) {
+  fxg_handle h;
+  int rc = fxg_submit_subint(ctx, offsets, data, nbytes, control, &h);
+  if (rc) return rc;
+  return fxg_wait(ctx, h, results);
+}
+
+int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out) {
+  if (!ctx || !out || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
+  fxg_ctx* c = const_cast<fxg_ctx*>(ctx);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return FXG_E_HIP;
+  if (hipMemcpy(out, ctx->d_weights + (size_t)ds * ctx->B, sizeof(float) * ctx->B, hipMemcpyDeviceToHost) != hipSuccess)
+    return FXG_E_HIP;
+  return FXG_OK;
+}
+
+int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, float* out) {
+  if (!ctx || !out || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
+  const DevDs& x = ctx->h_ds[ds];
+  if (band < 0 || band >= x.nrecbands) return FXG_E_INVALID;
+  const int lt = fft - ctx->last_chunk0;
+  if (lt < 0 || lt >= ctx->last_nffts) return FXG_E_STATE;
+  fxg_ctx* c = const_cast<fxg_ctx*>(ctx);
+  if (hipDeviceSynchronize() != hipSuccess) return FXG_E_HIP;
+  const int C = x.C;
+  if (ctx->xengine == FXG_X_FP32) {
+    const float2* src = ctx->d_spec32 + x.spec_off + ((long long)band * ctx->chunkcap + lt) * C;
+    if (hipMemcpy(out, src, sizeof(float2) * C, hipMemcpyDeviceToHost) != hipSuccess) return FXG_E_HIP;
+    return FXG_OK;
+  }
+  const DevXGroup& g = ctx->groups[x.band_xfreq[band]];
+  const int row = x.band_xrow[band];
+  std::vector<uint32_t> blk((size_t)C * FXG_TB);
+  std::vector<float> sc(C);
+  const uint32_t* src = ctx->d_spec16 + g.spec_off + (((long long)row * (ctx->chunkcap / FXG_TB) + lt / FXG_TB) * C) * FXG_TB;
+  if (hipMemcpy(blk.data(), src, sizeof(uint32_t) * blk.size(), hipMemcpyDeviceToHost) != hipSuccess) return FXG_E_HIP;
+  if (hipMemcpy(sc.data(), ctx->d_xscale + g.scale_off + (long long)row * C, sizeof(float) * C,
+                hipMemcpyDeviceToHost) != hipSuccess)
+    return FXG_E_HIP;
+  for (int ch = 0; ch < C; ++ch) {
+    uint32_t u = blk[(size_t)ch * FXG_TB + lt % FXG_TB];
+    float re, im;
+    if (ctx->xengine == FXG_X_FP16) {
+      __half2 h;
+      std::memcpy(&h, &u, 4);
+      float2 f = __half22float2(h);
+      re = f.x; im = f.y;
+    } else {
+      uint32_t lo = (u & 0xFFFFu) << 16, hi = u & 0xFFFF0000u;
+      std::memcpy(&re, &lo, 4);
+      std::memcpy(&im, &hi, 4);
+    }
+    out[2 * ch] = re / sc[ch];
+    out[2 * ch + 1] = im / sc[ch];
+  }
+  (void)c;
+  return FXG_OK;
+}
+
+int fxg_unpack_device(fxg_ctx* ctx, int32_t ds, const uint8_t* d_data, int32_t nbytes, int64_t startsample,
+                      int32_t nsamples, float* d_out, int32_t* d_goodsamples, void* stream) {
+  if (!ctx || ds < 0 || ds >= ctx->nds || !d_data || !d_out || !d_goodsamples || nsamples <= 0 || startsample < 0)
+    return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  const int nframes = nbytes / ctx->dss[ds].framebytes;
+  const long long capframes = (ctx->databytes[ds] + 4LL * ctx->dss[ds].framebytes) / ctx->dss[ds].framebytes;
+  if (nframes > capframes) return fail(ctx, FXG_E_INVALID, "buffer larger than the slot capacity");
+  // frame validity through the setup kernel on a one-datastream run record
+  DevDsRun r;
+  std::memset(&r, 0, sizeof(r));
+  r.data = d_data;
+  r.datalen = nbytes;
+  r.nframes = nframes;
+  r.valid = 1;
+  r.scan = 0;
+  r.datasec = 0;
+  DevDsRun none;
+  std::memset(&none, 0, sizeof(none));          // other datastreams: no frames to scan
+  std::vector<DevDsRun> runs(ctx->nds, none);
+  runs[ds] = r;
+  HIPCHK(hipMemcpyAsync(ctx->d_run, runs.data(), sizeof(DevDsRun) * ctx->nds, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));
+  SetupArgs sa;
+  sa.ds = ctx->d_ds;
+  sa.run = ctx->d_run;
+  sa.model = ctx->d_model;
+  sa.fvalid = ctx->d_fvalid;
+  sa.nds = ctx->nds;
+  sa.offscan = 0; sa.offsec = 0; sa.offns = 0;
+  sa.model_inc_secs = ctx->model_inc;
+  sa.B = ctx->B;
+  if (fxg_launch_setup(sa, std::max(nframes, 1), st)) return fail(ctx, FXG_E_HIP, "setup launch");
+  HIPCHK(hipMemsetAsync(d_goodsamples, 0, sizeof(int32_t), st));
+  UnpackArgs ua;
+  ua.ds = ctx->d_ds;
+  ua.d = ds;
+  ua.data = d_data;
+  ua.fv = ctx->d_fvalid + ctx->h_ds[ds].fvalid_off;
+  ua.nframes = nframes;
+  ua.start = startsample;
+  ua.nsamples = nsamples;
+  ua.out = d_out;
+  ua.good = d_goodsamples;
+  if (fxg_launch_unpack(ua, st)) return fail(ctx, FXG_E_HIP, "unpack launch");
+  HIPCHK(hipStreamSynchronize(st));
+  return FXG_OK;
+}
+
+int fxg_delay_interpolator(fxg_ctx* ctx, int32_t ds, int32_t sec, int32_t ns, double* out) {
+  if (!ctx || !out || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  hipStream_t st = ctx->stream;
+  DevDsRun r;
+  std::memset(&r, 0, sizeof(r));
+  r.valid = 1;
+  r.nframes = 0;
+  std::vector<DevDsRun> runs(ctx->nds, r);
+  HIPCHK(hipMemcpyAsync(ctx->d_run, runs.data(), sizeof(DevDsRun) * ctx->nds, hipMemcpyHostToDevice, st));
+  SetupArgs sa;
+  sa.ds = ctx->d_ds;
+  sa.run = ctx->d_run;
+  sa.model = ctx->d_model;
+  sa.fvalid = ctx->d_fvalid;
+  sa.nds = ctx->nds;
+  sa.offscan = 0; sa.offsec = sec; sa.offns = ns;
+  sa.model_inc_secs = ctx->model_inc;
+  sa.B = ctx->B;
+  if (fxg_launch_setup(sa, 1, st)) return fail(ctx, FXG_E_HIP, "setup launch");
+  DevDsRun back;
+  HIPCHK(hipMemcpyAsync(&back, ctx->d_run + ds, sizeof(DevDsRun), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (!back.valid) return fail(ctx, FXG_E_INVALID, "time outside the delay model");
+  std::memcpy(out, back.interp, sizeof(double) * 3);
+  return FXG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,557 @@
+// F-engine of the FX hot path on CDNA4: unpack, delay / fringe rotation, LDS radix-8 FFT,
+// sideband channel selection and fractional-sample correction, fused in one kernel.
+//
+// What one FFT block computes follows Mode::process (mpifxcorr/src/mode.cpp:634-1374)
+// with Mk5Mode::unpack (mk5mode.cpp:94-174), for fringe rotation order 1:
+//   - the per-FFT delay chain in FP64 (mode.cpp:671-675, 743-757, 781-788),
+//   - the rotator as S sub-phasors times NS step-phasors, FP64 phase wrapped to one turn,
+//     FP32 sin/cos and complex products (mode.cpp:883-955),
+//   - the fractional-sample phasor from FP32 channel slopes (mode.cpp:303-324, 1030-1073),
+//   - an unnormalised forward complex FFT of L = fftchannels points (mode.cpp:1174-1183),
+//   - USB keeps channels [0, C), real LSB keeps [C, 2C) (mode.cpp:1185-1217).
+// The FFT is a Stockham radix-8 (plus one radix-4 or radix-2 pass) exchange through LDS;
+// each of the L/8 threads owns the 8 points t + s*L/8, so the first pass reads the decoded
+// samples straight from registers and the last pass leaves channel t + s*L/8 in register s.
+#include "fxg_internal.h"
+#include "../../include/fxg.h"
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+namespace {
+
+constexpr double TWO_PI_D = 6.283185307179586476925286766559;   // architecture.h.in:430
+constexpr float HIMAG = 3.3359f;                                 // mark5_stream.h:65
+constexpr int FLAGS_PER_INT = 30;                                // mpifxcorr.h:29
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul_negi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
+
+// 2-bit LUTs: VDIF {-H,-1,+1,+H}[field] (format_vdif.c:70); Mark5B {-H,+1,-1,+H}[field]
+// where field = sign bit + 2*magnitude bit (mark5_format_mark5b.c:52-80).
+__device__ __forceinline__ float lut2(int fmt, unsigned field) {
+  const float v0 = -HIMAG, v3 = HIMAG;
+  float v1 = (fmt == FXG_FMT_MARK5B) ? 1.0f : -1.0f;
+  float v2 = -v1;
+  return field == 0 ? v0 : field == 1 ? v1 : field == 2 ? v2 : v3;
+}
+// 8-bit LUT (b - 128)/3.3 computed in double and rounded to float (format_vdif.c:120).
+__device__ __forceinline__ float lut8(unsigned b) { return (float)(((double)b - 128.0) / 3.3); }
+
+// In-register DFTs (forward, sign -1), decimation in frequency.
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a; a = cadd(t, b); b = csub(t, b);
+}
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 b0 = cadd(a0, a2), b2 = csub(a0, a2);
+  float2 b1 = cadd(a1, a3), b3 = mul_negi(csub(a1, a3));
+  a0 = cadd(b0, b1); a2 = csub(b0, b1);
+  a1 = cadd(b2, b3); a3 = csub(b2, b3);
+}
+__device__ __forceinline__ void dft8(float2* v) {
+  const float c = 0.70710678118654752440f;
+  float2 u0 = cadd(v[0], v[4]), u4 = csub(v[0], v[4]);
+  float2 u1 = cadd(v[1], v[5]), u5 = csub(v[1], v[5]);
+  float2 u2 = cadd(v[2], v[6]), u6 = csub(v[2], v[6]);
+  float2 u3 = cadd(v[3], v[7]), u7 = csub(v[3], v[7]);
+  u5 = make_float2(c * (u5.x + u5.y), c * (u5.y - u5.x));      // * W8^1
+  u6 = mul_negi(u6);                                           // * W8^2
+  u7 = make_float2(c * (u7.y - u7.x), -c * (u7.x + u7.y));     // * W8^3
+  float2 w0 = cadd(u0, u2), w2 = csub(u0, u2);
+  float2 w1 = cadd(u1, u3), w3 = mul_negi(csub(u1, u3));
+  float2 w4 = cadd(u4, u6), w6 = csub(u4, u6);
+  float2 w5 = cadd(u5, u7), w7 = mul_negi(csub(u5, u7));
+  v[0] = cadd(w0, w1); v[4] = csub(w0, w1);
+  v[2] = cadd(w2, w3); v[6] = csub(w2, w3);
+  v[1] = cadd(w4, w5); v[5] = csub(w4, w5);
+  v[3] = cadd(w6, w7); v[7] = csub(w6, w7);
+}
+
+// Twiddle exp(-2 pi i m / L) from the half table tw[m], m < L/2 (tw[m + L/2] = -tw[m]).
+template <int L>
+__device__ __forceinline__ float2 twid(const float2* tw, int m) {
+  float2 w = tw[m & (L / 2 - 1)];
+  return (m & (L / 2)) ? make_float2(-w.x, -w.y) : w;
+}
+
+template <int L>
+struct Plan {
+  static constexpr int LOG = (L == 256) ? 8 : (L == 512) ? 9 : (L == 1024) ? 10 : (L == 2048) ? 11
+                           : (L == 4096) ? 12 : (L == 8192) ? 13 : 0;
+  static constexpr int N8 = LOG / 3;      // radix-8 passes
+  static constexpr int REM = LOG % 3;     // 0: none, 1: radix-2 pass, 2: radix-4 pass
+  static constexpr int NT = L / 8;
+};
+
+// Stockham FFT of L points; thread t holds v[s] = x[t + s*L/8] on entry and
+// v[s] = X[t + s*L/8] on exit.  sbuf has L entries; tw is the half twiddle table.
+template <int L>
+__device__ __forceinline__ void fft_stockham(float2* v, float2* sbuf, const float2* tw, int t) {
+  using P = Plan<L>;
+  constexpr int NT = P::NT;
+  int Ns = 1;
+#pragma unroll
+  for (int p = 0; p < P::N8; ++p) {
+    if (p > 0) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) v[s] = sbuf[t + s * NT];
+      const int k = t & (Ns - 1);
+      const int step = L / (Ns * 8);
+#pragma unroll
+      for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twid<L>(tw, k * r * step));
+    }
+    dft8(v);
+    const bool last = (P::REM == 0) && (p == P::N8 - 1);
+    if (!last) {
+      __syncthreads();                       // every thread has read its inputs
+      const int k = t & (Ns - 1);
+      const int idxD = (t / Ns) * Ns * 8 + k;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) sbuf[idxD + r * Ns] = v[r];
+      __syncthreads();
+    }
+    Ns *= 8;
+  }
+  if (P::REM == 2) {                         // radix-4 pass, two butterflies per thread
+#pragma unroll
+    for (int s = 0; s < 8; ++s) v[s] = sbuf[t + s * NT];
+    const int step = L / (Ns * 4);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = t + b * NT;
+      const int k = j & (Ns - 1);
+      float2 a0 = v[b], a1 = v[b + 2], a2 = v[b + 4], a3 = v[b + 6];
+      a1 = cmul(a1, twid<L>(tw, k * 1 * step));
+      a2 = cmul(a2, twid<L>(tw, k * 2 * step));
+      a3 = cmul(a3, twid<L>(tw, k * 3 * step));
+      dft4(a0, a1, a2, a3);
+      v[b] = a0; v[b + 2] = a1; v[b + 4] = a2; v[b + 6] = a3;
+    }
+  } else if (P::REM == 1) {                  // radix-2 pass, four butterflies per thread
+#pragma unroll
+    for (int s = 0; s < 8; ++s) v[s] = sbuf[t + s * NT];
+    const int step = L / (Ns * 2);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int j = t + m * NT;
+      const int k = j & (Ns - 1);
+      float2 a0 = v[m], a1 = cmul(v[m + 4], twid<L>(tw, k * step));
+      dft2(a0, a1);
+      v[m] = a0; v[m + 4] = a1;
+    }
+  }
+}
+
+// Valid samples of [g0, g0+n) given per-frame validity (mk5mode.cpp:94-174: goodsamples is the
+// count of samples from valid frames inside the FFT window).
+__device__ int count_valid(const uint8_t* fv, int nframes, int spf, long long g0, int n) {
+  long long g1 = g0 + n;
+  int f0 = (int)(g0 / spf), f1 = (int)((g1 - 1) / spf);
+  int good = 0;
+  for (int f = f0; f <= f1; ++f) {
+    if (f >= nframes || !fv[f]) continue;
+    long long a = (long long)f * spf, b = a + spf;
+    long long lo = a > g0 ? a : g0, hi = b < g1 ? b : g1;
+    good += (int)(hi - lo);
+  }
+  return good;
+}
+
+// Per-FFT delay chain (mode.cpp:671-675, 743-757, 781-788), all FP64.
+struct FftDelay {
+  int ok;               // 1 = processed, 0 = flagged (zero spectra, weight 0)
+  int n0;               // nearestsample
+  float fse;            // fracsampleerror (us)
+  double a, b;          // linear interpolator across the FFT (FFT-fraction units)
+  int intdelay;         // integerdelay (us)
+  float weight;
+  int intwall;
+  double fracwall;
+};
+
+__device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_t* fv, int i,
+                              int offsec, int offns, double scan_start_sec) {
+  FftDelay r;
+  r.ok = 0; r.weight = 0.0f; r.n0 = 0; r.fse = 0.0f; r.a = 0; r.b = 0; r.intdelay = 0;
+  r.intwall = 0; r.fracwall = 0;
+  if (!run.valid || run.datalen <= 1) return r;
+  if (((run.vflags[i / FLAGS_PER_INT] >> (i % FLAGS_PER_INT)) & 1) == 0) return r;
+  const double* q = run.interp;
+  const double dt = ds.sampletime_us;
+  const int L = ds.L;
+  const double fc = i + 0.5;
+  const double avgdelay = q[0] * fc * fc + q[1] * fc + q[2];
+  const double fftstart = i * L * dt;
+  const double starttime = (offsec - run.datasec) * 1000000.0
+                         + ((long long)offns - (long long)run.datans) / 1000.0 + fftstart - avgdelay;
+  int n0 = (int)(starttime / dt + 0.5);
+  if (n0 < -1 || (((long long)(n0 + L)) * ds.bpb_num) / ds.bpb_den > run.datalen) return r;
+  if (n0 == -1) n0 = 0;
+  const int good = count_valid(fv, run.nframes, ds.spf, n0, L);
+  const float w = (float)good / (float)L;
+  if (!(w > 0.0f)) return r;
+  r.ok = 1;
+  r.n0 = n0;
+  r.weight = w;
+  r.fse = (float)(starttime - n0 * dt);
+  const double di = (double)i;
+  const double d0 = q[0] * di * di + q[1] * di + q[2];
+  const double d1 = q[0] * (di + 0.5) * (di + 0.5) + q[1] * (di + 0.5) + q[2];
+  const double d2 = q[0] * (di + 1) * (di + 1) + q[1] * (di + 1) + q[2];
+  const double a = d2 - d0;
+  double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+  const int intdelay = (int)b;
+  b -= intdelay;
+  r.a = a; r.b = b; r.intdelay = intdelay;
+  const double walltime = scan_start_sec + offsec + offns / 1.0e9 + fftstart / 1.0e6;
+  r.intwall = (int)walltime;
+  r.fracwall = walltime - r.intwall;
+  return r;
+}
+
+// Fringe rotator sub-phasors (j < S) and step-phasors (S <= j < S+NS) for one freq
+// (mode.cpp:883-955).  FP64 phase in turns, wrapped by truncation, FP32 sin/cos.
+__device__ void build_rotator(float2* srot, const DevDs& ds, const DevRecFreq& rf,
+                              const FftDelay& fd, int t, int nt) {
+  const int S = ds.S, NS = ds.NS, L = ds.L;
+  const double lofreq = rf.lofreq;
+  for (int j = t; j < S + NS; j += nt) {
+    double ph;
+    if (j < S) {
+      const double xoff = (double)j / (double)L;                        // subxoff
+      const double xval = __dadd_rn(__dmul_rn(xoff, fd.a), fd.b);       // subxval
+      ph = __dmul_rn(xval, lofreq);
+      if (ds.fractionallo)
+        ph = __dadd_rn(ph, __dmul_rn(lofreq - (double)(int)lofreq, (double)fd.intdelay));
+      if (rf.looff) {
+        const double toff = j * ds.sampletime_us / 1e6;                  // subtoff
+        ph = __dadd_rn(ph, __dmul_rn(toff, -rf.lo_offset_hz));
+        ph = __dadd_rn(ph, -rf.lo_offset_hz * fd.fracwall);
+        if (rf.isfraclooffset) ph = __dadd_rn(ph, -rf.fraclooffset * fd.intwall);
+      }
+    } else {
+      const int s = j - S;
+      const double xoff = (double)(s * S) / (double)L;                  // stepxoff
+      ph = __dmul_rn(__dmul_rn(xoff, fd.a), lofreq);
+      if (rf.looff) {
+        const double toff = s * S * ds.sampletime_us / 1e6;              // steptoff
+        ph = __dadd_rn(ph, __dmul_rn(toff, -rf.lo_offset_hz));
+      }
+    }
+    const float arg = (float)(-TWO_PI_D * (ph - trunc(ph)));
+    float sn, cs;
+    sincosf(arg, &sn, &cs);
+    srot[j] = make_float2(cs, sn);
+  }
+}
+
+// Fractional-sample phasors: sub (j < S) and step (S <= j < S+NFS) (mode.cpp:1030-1073).
+__device__ void build_frac(float2* sfrac, const DevDs& ds, const DevRecFreq& rf, const float* cf,
+                           float fse, int t, int nt) {
+  const int S = ds.S, NFS = ds.NFS;
+  const float e = (float)((double)fse - rf.clockoff + rf.halfdelta);
+  const float* sub = cf;
+  const float* step = rf.lsb && !ds.iscomplex ? cf + S + NFS : cf + S;
+  for (int j = t; j < S + NFS; j += nt) {
+    const float arg = (j < S) ? sub[j] * e : step[j - S] * e;
+    float sn, cs;
+    sincosf(arg, &sn, &cs);
+    sfrac[j] = make_float2(cs, sn);
+  }
+}
+
+__device__ __forceinline__ uint32_t pack16(float2 v, int mode) {
+  if (mode == 1) {
+    __half2 h = __floats2half2_rn(v.x, v.y);
+    return *reinterpret_cast<uint32_t*>(&h);
+  }
+  __hip_bfloat162 h = __float22bfloat162_rn(v);
+  return *reinterpret_cast<uint32_t*>(&h);
+}
+
+}  // namespace
+
+// One workgroup = one datastream x `blocks_per_wg` time blocks of FXG_TB consecutive FFTs.
+template <int L, bool CPLX>
+__global__ void __launch_bounds__(L / 8)
+k_fengine(FengineArgs args) {
+  constexpr int NT = L / 8;
+  __shared__ float2 sbuf[L];
+  __shared__ float2 stw[L / 2];
+  __shared__ float2 srot[L / 4];
+  __shared__ float2 sfrac[L / 4];
+
+  const int t = threadIdx.x;
+  const int d = args.ds0 + blockIdx.y;
+  const DevDs& ds = args.ds[d];
+  const DevDsRun& run = args.run[d];
+  const uint8_t* fv = args.fvalid + ds.fvalid_off;
+  const float* cf = args.chanfreq + ds.chanfreq_off;
+  const int C = ds.C, S = ds.S, nrec = ds.nrecbands;
+  const int fmt = ds.format;
+
+  for (int m = t; m < L / 2; m += NT) stw[m] = args.twiddle[m];
+
+  const int nblocks = args.nffts_pad / FXG_TB;
+  const int blk0 = blockIdx.x * args.blocks_per_wg;
+  const int blk1 = min(nblocks, blk0 + args.blocks_per_wg);
+  for (int blk = blk0; blk < blk1; ++blk) {
+    for (int tt = 0; tt < FXG_TB; ++tt) {
+      const int lt = blk * FXG_TB + tt;           // FFT within the chunk
+      const int i = args.chunk0 + lt;             // FFT within the subint
+      FftDelay fd;
+      fd.ok = 0; fd.weight = 0.0f;
+      if (lt < args.nffts)
+        fd = fft_delay(ds, run, fv, i, args.offsec, args.offns, args.scan_start_sec);
+      if (t == 0 && lt < args.nffts) args.weights[(long long)d * args.B + i] = fd.weight;
+
+      if (!fd.ok) {                               // flagged: zero spectra (mode.cpp:656-669)
+        for (int band = 0; band < nrec; ++band) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            const int c = t + s * NT;
+            if (!CPLX && c >= C) continue;
+            if (args.out_mode == 0) {
+              args.spec32[ds.spec_off + ((long long)band * args.chunkcap + lt) * C + c] = make_float2(0.f, 0.f);
+            } else {
+              const DevXGroup& xg = args.xgroups[ds.band_xfreq[band]];
+              const long long o = xg.spec_off +
+                  (((long long)ds.band_xrow[band] * (args.chunkcap / FXG_TB) + lt / FXG_TB) * C + c) * FXG_TB + (lt % FXG_TB);
+              args.spec16[o] = 0u;
+            }
+          }
+        }
+        continue;
+      }
+
+      // raw sample words of this thread's 8 time samples (shared by all bands)
+      const long long g0 = fd.n0;
+      uint32_t word[8];
+      int fok[8];
+      long long sbase[8];                         // byte offset of the sample's first field
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const long long g = g0 + t + s * NT;
+        const int fr = (int)(g / ds.spf);
+        const int within = (int)(g - (long long)fr * ds.spf);
+        fok[s] = (fr < run.nframes) && fv[fr];
+        const long long fbase = (long long)fr * ds.framebytes + ds.headerbytes;
+        if (!CPLX && ds.nbits == 2) {
+          const long long q = (long long)within * nrec;             // first field of the sample
+          sbase[s] = q;
+          word[s] = fok[s] ? *reinterpret_cast<const uint32_t*>(run.data + fbase + ((q >> 4) << 2)) : 0u;
+        } else {
+          sbase[s] = fbase + (long long)within * nrec * (CPLX ? 2 : 1);
+          word[s] = 0u;
+        }
+      }
+
+      for (int f = 0; f < ds.nrecfreqs; ++f) {
+        const DevRecFreq& rf = ds.rf[f];
+        if (rf.nbands == 0) continue;
+        __syncthreads();                          // previous users of srot/sfrac are done
+        build_rotator(srot, ds, rf, fd, t, NT);
+        build_frac(sfrac, ds, rf, cf, fd.fse, t, NT);
+        __syncthreads();
+        for (int kb = 0; kb < rf.nbands; ++kb) {
+          const int band = rf.band[kb];
+          float2 v[8];
+          const float2 sub = srot[t & (S - 1)];
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            const int n = t + s * NT;
+            const int st = n / S;
+            const float2 rot = st == 0 ? sub : cmul(sub, srot[S + st]);
+            if (!CPLX) {
+              float x = 0.0f;
+              if (fok[s]) {
+                if (ds.nbits == 2) {
+                  const int sh = 2 * (int)((sbase[s] & 15) + band);
+                  x = lut2(fmt, (word[s] >> sh) & 3u);
+                } else {
+                  x = lut8(run.data[sbase[s] + band]);
+                }
+              }
+              v[s] = make_float2(x * rot.x, x * rot.y);          // (x + 0i) * rotator
+            } else {
+              float2 x = make_float2(0.f, 0.f);
+              if (fok[s]) {
+                const uint8_t* p = run.data + sbase[s] + 2 * band;
+                x = make_float2(lut8(p[0]), lut8(p[1]));
+              }
+              v[s] = cmul(rot, x);
+            }
+          }
+          __syncthreads();                        // sbuf free (previous band's FFT done)
+          fft_stockham<L>(v, sbuf, stw, t);
+          // channel selection, fractional-sample correction, output
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            int c;
+            if (CPLX) {
+              c = t + s * NT;
+            } else if (rf.lsb) {
+              if (s < 4) continue;
+              c = t + (s - 4) * NT;
+            } else {
+              if (s >= 4) continue;
+              c = t + s * NT;
+            }
+            const float2 fr = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
+            const float2 F = cmul(v[s], fr);
+            if (args.out_mode == 0) {
+              args.spec32[ds.spec_off + ((long long)band * args.chunkcap + lt) * C + c] = F;
+            } else {
+              const DevXGroup& xg = args.xgroups[ds.band_xfreq[band]];
+              const int row = ds.band_xrow[band];
+              const float sc = args.xscale[xg.scale_off + (long long)row * C + c];
+              const long long o = xg.spec_off +
+                  (((long long)row * (args.chunkcap / FXG_TB) + lt / FXG_TB) * C + c) * FXG_TB + (lt % FXG_TB);
+              args.spec16[o] = pack16(make_float2(F.x * sc, F.y * sc), args.out_mode);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Setup for one subint: the delay interpolator of each datastream in FP64 on the device
+// (Model::calculateDelayInterpolator, model.cpp:219-289, via Mode::setOffsets,
+// mode.cpp:1502-1532) and the per-frame validity of each slot buffer.
+__global__ void k_setup(SetupArgs a) {
+  const int d = blockIdx.y;
+  const DevDs& ds = a.ds[d];
+  DevDsRun& run = a.run[d];
+  // frame validity: VDIF word2 != 0 and invalid bit clear (format_vdif.c:7619-7660),
+  // Mark5B invalid flag (mark5_format_mark5b.c:1723), fill pattern at either end of the
+  // payload blanks the frame (blanker_mark5.c:272-310).
+  for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < run.nframes; f += gridDim.x * blockDim.x) {
+    const uint8_t* fr = run.data + (long long)f * ds.framebytes;
+    int ok;
+    if (ds.format == FXG_FMT_VDIF) {
+      const uint32_t w0 = *reinterpret_cast<const uint32_t*>(fr);
+      const uint32_t w2 = *reinterpret_cast<const uint32_t*>(fr + 8);
+      ok = (w2 != 0u) && (((w0 >> 31) & 1u) == 0u);
+    } else {
+      ok = (fr[5] & 0x80) == 0;
+    }
+    const uint8_t* pay = fr + ds.headerbytes;
+    const int pb = ds.framebytes - ds.headerbytes;
+    const unsigned long long fill = 0x1122334411223344ULL;
+    const unsigned long long first = *reinterpret_cast<const unsigned long long*>(pay);
+    const unsigned long long last = *reinterpret_cast<const unsigned long long*>(pay + pb - 8);
+    if (first == fill || last == fill) ok = 0;
+    a.fvalid[ds.fvalid_off + f] = (uint8_t)ok;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // Mode::setOffsets: scan mismatch or invalid data torch the subint (mode.cpp:1509-1516)
+    if (run.datasec <= -99999999 || run.scan != a.offscan) run.valid = 0;
+    const double offsettime = (double)a.offsec + (double)a.offns / 1000000000.0;
+    const double timespan = (double)a.B * ds.C / ds.bw_mhz / 1e6;   // mode.cpp:1523-1524
+    const double inc = a.model_inc_secs;
+    int scansample = (int)(offsettime / inc);
+    if (scansample == ds.model_nsamples && (offsettime - scansample * inc) < 1e-6) scansample--;
+    if (scansample < 0 || scansample >= ds.model_nsamples) {
+      run.valid = 0;                        // no interpolator: torch the subint (mode.cpp:1528-1531)
+    } else {
+      const double* c = a.model + (long long)(ds.model_off + scansample) * FXG_MAX_POLY;
+      double smp[3];
+      const double base = scansample * inc;
+      const double dts[3] = {offsettime - base, offsettime + timespan / 2.0 - base,
+                             offsettime + timespan - base};
+      for (int k = 0; k < 3; ++k) {
+        double tp = 1.0, acc = 0.0;
+        for (int o = 0; o <= ds.model_order; ++o) { acc += tp * c[o]; tp *= dts[k]; }
+        smp[k] = acc;
+      }
+      const double n = (double)a.B;
+      run.interp[0] = (2.0 * smp[0] - 4.0 * smp[1] + 2.0 * smp[2]) / (n * n);
+      run.interp[1] = (-3.0 * smp[0] + 4.0 * smp[1] - smp[2]) / n;
+      run.interp[2] = smp[0] - 1000000.0 * ds.intclockseconds;
+    }
+  }
+}
+
+// Mk5Mode::unpack as a device call (bit-exact against the LUTs).
+__global__ void k_unpack(UnpackArgs a) {
+  const DevDs& ds = a.ds[a.d];
+  const int nrec = ds.nrecbands;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < a.nsamples; n += gridDim.x * blockDim.x) {
+    const long long g = a.start + n;
+    const int fr = (int)(g / ds.spf);
+    const int within = (int)(g - (long long)fr * ds.spf);
+    const int ok = (fr < a.nframes) && a.fv[fr];
+    const long long fbase = (long long)fr * ds.framebytes + ds.headerbytes;
+    if (ok) atomicAdd(a.good, 1);
+    for (int b = 0; b < nrec; ++b) {
+      if (ds.iscomplex) {
+        float2 x = make_float2(0.f, 0.f);
+        if (ok) {
+          const uint8_t* p = a.data + fbase + ((long long)within * nrec + b) * 2;
+          x = make_float2(lut8(p[0]), lut8(p[1]));
+        }
+        reinterpret_cast<float2*>(a.out)[(long long)b * a.nsamples + n] = x;
+      } else {
+        float x = 0.f;
+        if (ok) {
+          if (ds.nbits == 2) {
+            const long long q = (long long)within * nrec + b;
+            const uint8_t byte = a.data[fbase + (q >> 2)];
+            x = lut2(ds.format, (byte >> (2 * (q & 3))) & 3u);
+          } else {
+            x = lut8(a.data[fbase + (long long)within * nrec + b]);
+          }
+        }
+        a.out[(long long)b * a.nsamples + n] = x;
+      }
+    }
+  }
+}
+
+// ---- host-side launchers -------------------------------------------------------------
+int fxg_launch_setup(const SetupArgs& a, int maxframes, hipStream_t st) {
+  dim3 grid((maxframes + 255) / 256 > 0 ? (maxframes + 255) / 256 : 1, a.nds);
+  hipLaunchKernelGGL(k_setup, grid, dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st) {
+  int blocks = (a.nsamples + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_unpack, dim3(blocks), dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int L, bool CPLX>
+static int launch_f(const FengineArgs& a, int nds, hipStream_t st) {
+  const int nblocks = a.nffts_pad / FXG_TB;
+  const int gx = (nblocks + a.blocks_per_wg - 1) / a.blocks_per_wg;
+  hipLaunchKernelGGL((k_fengine<L, CPLX>), dim3(gx, nds), dim3(L / 8), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, hipStream_t st) {
+  if (iscomplex) {
+    switch (L) {
+      case 256: return launch_f<256, true>(a, nds, st);
+      case 512: return launch_f<512, true>(a, nds, st);
+      case 1024: return launch_f<1024, true>(a, nds, st);
+      case 2048: return launch_f<2048, true>(a, nds, st);
+      case 4096: return launch_f<4096, true>(a, nds, st);
+    }
+  } else {
+    switch (L) {
+      case 256: return launch_f<256, false>(a, nds, st);
+      case 512: return launch_f<512, false>(a, nds, st);
+      case 1024: return launch_f<1024, false>(a, nds, st);
+      case 2048: return launch_f<2048, false>(a, nds, st);
+      case 4096: return launch_f<4096, false>(a, nds, st);
+    }
+  }
+  return -2;
+}

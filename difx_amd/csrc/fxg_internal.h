@@ -1,0 +1,186 @@
+// Internal tables shared by the host layer and the device kernels of libfxg.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/fxg.h"
+
+#define FXG_TB 4            // consecutive FFTs per output time block (MFMA K packing)
+#define FXG_XPAD 8          // FFT count padding of a device chunk (two time blocks)
+
+// Per recorded freq of a datastream: what the rotator and the fractional-sample
+// correction need (mode.cpp:853-931, 1030-1073).
+struct DevRecFreq {
+  double lofreq;          // MHz (band edge; negated for complex LSB)
+  double lo_offset_hz;    // recordedfreqlooffsets
+  double fraclooffset;    // signed fractional part of the LO offset
+  double clockoff;        // recordedfreqclockoffsets (us)
+  double halfdelta;       // recordedfreqclockoffsetsdelta / 2 (us)
+  int    lsb;
+  int    looff;           // LO offset non-zero
+  int    isfraclooffset;
+  int    nbands;          // recorded bands of this freq
+  int    band[4];         // their band indices (sample-word order)
+  int    pad;
+};
+
+// Per datastream (everything the F-engine and the unpacker read).
+struct DevDs {
+  int format;             // FXG_FMT_*
+  int nbits;
+  int iscomplex;
+  int framebytes;
+  int headerbytes;
+  int spf;                // time samples per frame
+  int nrecbands;
+  int nrecfreqs;
+  int L;                  // fftchannels
+  int C;                  // recordedbandchannels
+  int S;                  // arraystridelength
+  int NS;                 // L / S  (rotator steps)
+  int NFS;                // C / S  (fractional-sample steps)
+  int bpb_num, bpb_den;   // bytes-per-block ratio of the bounds check (mode.cpp:124-139)
+  int fractionallo;
+  int intclockseconds;
+  int model_order;
+  int model_nsamples;
+  int model_off;          // offset of this datastream's polynomials in the model table
+  long long spec_off;     // FP32 spectra: float2 offset of (band 0, t 0) in the chunk buffer
+  int chanfreq_off;       // offset of the channel-frequency tables in the chanfreq array
+  int fvalid_off;         // offset of this datastream's frame-valid flags
+  double sampletime_us;
+  double bw_mhz;
+  DevRecFreq rf[32];
+  int band_xfreq[64];     // MFMA: X freq group of each recorded band
+  int band_xrow[64];      // MFMA: input row of each recorded band within its group
+};
+
+// Per subint, per datastream runtime state written by the setup kernel.
+struct DevDsRun {
+  double interp[3];       // quadratic delay interpolator (FFT-index units)
+  int valid;              // 0 = the whole subint is flagged for this datastream
+  int scan, datasec, datans;
+  int datalen;            // bytes
+  int nframes;            // whole frames in the buffer
+  const uint8_t* data;    // device pointer of the slot buffer
+  const int32_t* vflags;  // device copy of the valid flags
+};
+
+// Plain FP32 X-engine product (core.cpp:963 and the autocorrelations of mode.cpp:1272,1334).
+struct DevProd {
+  int ds1, band1, choff1; // first operand: spectra band and channel offset (zoom windows)
+  int ds2, band2, choff2; // second operand (conjugated)
+  int nchan;              // channels of the product
+  int acc_off;            // cf32 offset of its accumulator
+};
+
+// MFMA X-engine group: one recorded frequency with its inputs (station bands).
+struct DevXGroup {
+  int nin;                // inputs (rows)
+  int ntile;              // 32-row tiles
+  int C;                  // channels
+  long long acc_off;      // float offset of its accumulator block [C][npairs][2][32][32]
+  long long spec_off;     // element offset (32-bit units) of its spectra block
+  long long scale_off;    // float offset of its per-(row, channel) scales [rows][C]
+};
+
+// Kernel launch helpers implemented in the .hip files.
+struct FengineArgs {
+  const DevDs* ds;
+  const DevDsRun* run;
+  const float2* twiddle;  // exp(-2 pi i m / L), m in [0, L)
+  float* weights;         // [nds][B]
+  int B;                  // blockspersend
+  int chunk0;             // first FFT index of this chunk
+  int nffts;              // FFTs in this chunk
+  int nffts_pad;          // padded to a multiple of FXG_XPAD
+  int offsec, offns;      // subint offsets
+  double scan_start_sec;
+  int out_mode;           // 0 = FP32 [ds][band][t][C], 1 = FP16, 2 = BF16 MFMA layout
+  float2* spec32;         // FP32 spectra
+  uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row][t/4][C][4]
+  const DevXGroup* xgroups;
+  const float* xscale;    // per group [rows][C] scales applied before 16-bit conversion
+  const float* chanfreq;  // per datastream: sub[S], usb step[NFS], lsb step[NFS] (mode.cpp:303-324)
+  const uint8_t* fvalid;  // frame valid flags
+  int chunkcap;           // FFT capacity of the chunk buffers (multiple of FXG_XPAD)
+  int blocks_per_wg;
+  int ds0;                // first datastream of this launch (grid.y offset)
+};
+
+// Setup for one subint (delay interpolators, frame validity).
+struct SetupArgs {
+  const DevDs* ds;
+  DevDsRun* run;
+  const double* model;        // [..][FXG_MAX_POLY]
+  uint8_t* fvalid;
+  int nds;
+  int offscan, offsec, offns;
+  double model_inc_secs;
+  int B;
+};
+
+// Stand-alone unpack of one datastream's buffer.
+struct UnpackArgs {
+  const DevDs* ds;
+  int d;
+  const uint8_t* data;
+  const uint8_t* fv;
+  int nframes;
+  long long start;
+  int nsamples;
+  float* out;
+  int* good;
+};
+
+struct XPlainArgs {
+  const DevProd* prods;
+  const DevDs* ds;
+  const float2* spec32;
+  float2* acc;
+  int nffts;
+  int chunkcap;
+};
+
+struct XMfmaArgs {
+  const DevXGroup* groups;
+  const uint32_t* spec16;
+  float* acc;
+  int nffts_pad;          // multiple of 8
+  int chunkcap;
+  int mode;               // 1 = FP16, 2 = BF16
+};
+
+// Accumulation of a product accumulator into the results (core.cpp:1819-1853).
+struct FinPlain {
+  long long src;          // cf32 offset in the product accumulators
+  int dst;                // cf32 offset in results
+  int nout;               // output channels
+  int avg;                // channels averaged per output channel
+};
+
+struct FinMfma {
+  int group;
+  int row1, row2;         // V[row1][row2] (row1 is the unconjugated operand)
+  int chan0;              // first channel of the group spectrum (zoom window)
+  int dst;
+  int nout;
+  int avg;
+  int isauto;             // parallel-hand autocorrelation: imaginary part forced to zero
+};
+
+struct FinWeight {
+  int ds1, ds2;           // ds2 < 0: plain sum of ds1's weights
+  int dstf;               // float index in results
+};
+
+int fxg_launch_setup(const SetupArgs& a, int maxframes, hipStream_t st);
+int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st);
+int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, hipStream_t st);
+int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st);
+int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st);
+int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, hipStream_t st);
+int fxg_launch_fin_mfma(const FinMfma* e, int n, int maxout, const DevXGroup* g, const float* acc,
+                        const float* xscale, float2* res, hipStream_t st);
+int fxg_launch_fin_weights(const FinWeight* e, int n, const float* w, int B, float* resf, hipStream_t st);
+int fxg_launch_zero(void* p, long long bytes, hipStream_t st);
+int fxg_launch_fill(float* p, long long n, float v, hipStream_t st);

@@ -1,0 +1,279 @@
+// X-engines and accumulation of the FX hot path on CDNA4.
+//
+// Plain FP32 path: one thread per (product, channel) accumulates
+//   acc += F1[t][c] * conj(F2[t][c])
+// sequentially over the FFTs of the subint, exactly the vectorAddProduct_cf32 of
+// Core::processdata (mpifxcorr/src/core.cpp:884-979, :963) for the cross products and of
+// Mode::process for the autocorrelations (mode.cpp:1272, 1333-1339).
+//
+// MFMA path: per channel the visibilities of all inputs (station bands of one frequency)
+// form the Hermitian matrix V = sum_t x_t x_t^H.  With x stored as interleaved (re, im)
+// 16-bit pairs along time, the row of input n is A_n = (re_0, im_0, re_1, im_1, ...), so
+//   Re V = A A^T            and   Im V = B A^T,  B_n = (im_0, -re_0, im_1, -re_1, ...)
+// and both are 32x32x16 MFMA contractions whose A and B operands come from the same
+// 16-byte load per lane.  FP32 accumulators live in registers for a whole chunk.
+#include "fxg_internal.h"
+#include <hip/hip_fp16.h>
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+
+// Thread per (product, channel).  Parallel-hand autocorrelations keep a zero imaginary part,
+// as F * conj(F) has in the reference's arithmetic.
+__global__ void __launch_bounds__(256) k_xmac_f32(XPlainArgs a) {
+  const DevProd p = a.prods[blockIdx.y];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= p.nchan) return;
+  const DevDs& d1 = a.ds[p.ds1];
+  const DevDs& d2 = a.ds[p.ds2];
+  const int C1 = d1.C, C2 = d2.C;
+  const float2* s1 = a.spec32 + d1.spec_off + (long long)p.band1 * a.chunkcap * C1 + p.choff1 + c;
+  const float2* s2 = a.spec32 + d2.spec_off + (long long)p.band2 * a.chunkcap * C2 + p.choff2 + c;
+  const bool isauto = (p.ds1 == p.ds2) && (p.band1 == p.band2) && (p.choff1 == p.choff2);
+  float2 acc = a.acc[p.acc_off + c];
+  int t = 0;
+  for (; t + 4 <= a.nffts; t += 4) {
+    float2 x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { x[u] = s1[(long long)(t + u) * C1]; y[u] = s2[(long long)(t + u) * C2]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      acc.x += x[u].x * y[u].x + x[u].y * y[u].y;
+      acc.y += x[u].y * y[u].x - x[u].x * y[u].y;
+    }
+  }
+  for (; t < a.nffts; ++t) {
+    const float2 x = s1[(long long)t * C1], y = s2[(long long)t * C2];
+    acc.x += x.x * y.x + x.y * y.y;
+    acc.y += x.y * y.x - x.x * y.y;
+  }
+  if (isauto) acc.y = 0.0f;
+  a.acc[p.acc_off + c] = acc;
+}
+
+
+template <typename V8>
+__device__ __forceinline__ floatx16 mfma32(V8 a, V8 b, floatx16 c);
+template <>
+__device__ __forceinline__ floatx16 mfma32<half8_t>(half8_t a, half8_t b, floatx16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ floatx16 mfma32<bf16x8_t>(bf16x8_t a, bf16x8_t b, floatx16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// (re, im) pairs -> (im, -re) pairs: flip the sign bit of the 16-bit value moved to the
+// high half.  Same bit manipulation for FP16 and BF16 (sign is bit 15).
+__device__ __forceinline__ uint4 rot_negi(uint4 u) {
+  uint4 r;
+  r.x = ((u.x >> 16) | (u.x << 16)) ^ 0x80000000u;
+  r.y = ((u.y >> 16) | (u.y << 16)) ^ 0x80000000u;
+  r.z = ((u.z >> 16) | (u.z << 16)) ^ 0x80000000u;
+  r.w = ((u.w >> 16) | (u.w << 16)) ^ 0x80000000u;
+  return r;
+}
+
+// One wavefront per (group, channel); 4 wavefronts per workgroup on 4 consecutive channels.
+// Tile pairs (ti >= tj) of 32 rows: ntile 1 -> 1 pair, 2 -> 3 pairs.
+template <typename V8, int NTILE>
+__global__ void __launch_bounds__(256) k_xmac_mfma(XMfmaArgs a) {
+  constexpr int NPAIR = NTILE * (NTILE + 1) / 2;
+  const DevXGroup g = a.groups[blockIdx.y];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + wave;
+  if (c >= g.C) return;
+  const int row = lane & 31;
+  const int half = lane >> 5;
+  const long long tbstride = (long long)g.C * 4;              // uint32 per time block
+  const long long rowstride = (long long)(a.chunkcap / 4) * tbstride;
+  const uint32_t* base = a.spec16 + g.spec_off + (long long)c * 4 + (long long)half * tbstride;
+  floatx16 accr[NPAIR], acci[NPAIR];
+  float* accp = a.acc + g.acc_off + (long long)c * NPAIR * 2 * 1024;
+#pragma unroll
+  for (int p = 0; p < NPAIR; ++p) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      accr[p][r] = accp[(p * 2 + 0) * 1024 + r * 64 + lane];
+      acci[p][r] = accp[(p * 2 + 1) * 1024 + r * 64 + lane];
+    }
+  }
+  const int nk = a.nffts_pad / 8;
+  for (int k = 0; k < nk; ++k) {
+    uint4 x[NTILE];
+#pragma unroll
+    for (int ti = 0; ti < NTILE; ++ti) {
+      const uint32_t* p = base + (long long)(row + 32 * ti) * rowstride + (long long)(2 * k) * tbstride;
+      x[ti] = *reinterpret_cast<const uint4*>(p);
+    }
+    int pi = 0;
+#pragma unroll
+    for (int ti = 0; ti < NTILE; ++ti) {
+      const uint4 xr = rot_negi(x[ti]);
+#pragma unroll
+      for (int tj = 0; tj <= ti; ++tj) {
+        const V8 av = __builtin_bit_cast(V8, x[ti]);
+        const V8 bv = __builtin_bit_cast(V8, x[tj]);
+        const V8 arv = __builtin_bit_cast(V8, xr);
+        accr[pi] = mfma32<V8>(av, bv, accr[pi]);
+        acci[pi] = mfma32<V8>(arv, bv, acci[pi]);
+        ++pi;
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NPAIR; ++p) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      accp[(p * 2 + 0) * 1024 + r * 64 + lane] = accr[p][r];
+      accp[(p * 2 + 1) * 1024 + r * 64 + lane] = acci[p][r];
+    }
+  }
+}
+
+// Register r of lane l of a 32x32 MFMA accumulator holds element
+//   (i, j) = (8*(r/4) + 4*(l/32) + r%4, l%32).
+__device__ __forceinline__ int mfma_idx(int i, int j) {
+  const int r = (i >> 3) * 4 + (i & 3);
+  const int l = ((i >> 2) & 1) * 32 + j;
+  return r * 64 + l;
+}
+
+// ---- accumulation into the slot results (core.cpp:1819-1853, 1258-1376) ----------------
+
+__global__ void k_fin_plain(const FinPlain* e, const float2* acc, float2* res) {
+  const FinPlain f = e[blockIdx.y];
+  const int co = blockIdx.x * blockDim.x + threadIdx.x;
+  if (co >= f.nout) return;
+  float2 s = make_float2(0.f, 0.f);
+  for (int k = 0; k < f.avg; ++k) {
+    const float2 v = acc[f.src + (long long)co * f.avg + k];
+    s.x += v.x; s.y += v.y;
+  }
+  if (f.avg > 1) { s.x /= (float)f.avg; s.y /= (float)f.avg; }
+  float2 o = res[f.dst + co];
+  o.x += s.x; o.y += s.y;
+  res[f.dst + co] = o;
+}
+
+
+__global__ void k_fin_mfma(const FinMfma* e, const DevXGroup* groups, const float* acc,
+                           const float* xscale, float2* res) {
+  const FinMfma f = e[blockIdx.y];
+  const int co = blockIdx.x * blockDim.x + threadIdx.x;
+  if (co >= f.nout) return;
+  const DevXGroup g = groups[f.group];
+  const int npair = g.ntile * (g.ntile + 1) / 2;
+  int r1 = f.row1, r2 = f.row2;
+  float sign = 1.0f;
+  int ti = r1 >> 5, tj = r2 >> 5;
+  if (ti < tj) { int tmp = r1; r1 = r2; r2 = tmp; ti = r1 >> 5; tj = r2 >> 5; sign = -1.0f; }
+  const int pair = ti * (ti + 1) / 2 + tj;
+  const int idx = mfma_idx(r1 & 31, r2 & 31);
+  float2 s = make_float2(0.f, 0.f);
+  for (int k = 0; k < f.avg; ++k) {
+    const int c = f.chan0 + co * f.avg + k;
+    const float* base = acc + g.acc_off + ((long long)c * npair + pair) * 2 * 1024;
+    const float sc = xscale[g.scale_off + (long long)f.row1 * g.C + c] *
+                     xscale[g.scale_off + (long long)f.row2 * g.C + c];
+    const float inv = 1.0f / sc;
+    s.x += base[idx] * inv;
+    s.y += sign * base[1024 + idx] * inv;
+  }
+  if (f.avg > 1) { s.x /= (float)f.avg; s.y /= (float)f.avg; }
+  if (f.isauto) s.y = 0.0f;
+  float2 o = res[f.dst + co];
+  o.x += s.x; o.y += s.y;
+  res[f.dst + co] = o;
+}
+
+// Baseline weights (core.cpp:1005-1052, 1066-1112) and autocorrelation weights
+// (core.cpp:1306-1370): sequential FP32 sums over the FFTs of the subint.
+
+__global__ void k_fin_weights(const FinWeight* e, int n, const float* w, int B, float* resf) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const FinWeight f = e[k];
+  const float* w1 = w + (long long)f.ds1 * B;
+  float s = 0.0f;
+  if (f.ds2 < 0) {
+    for (int i = 0; i < B; ++i) s += w1[i];
+  } else {
+    const float* w2 = w + (long long)f.ds2 * B;
+    for (int i = 0; i < B; ++i) s += w1[i] * w2[i];
+  }
+  resf[f.dstf] += s;
+}
+
+__global__ void k_zero(float* p, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = 0.0f;
+}
+
+__global__ void k_fill(float* p, long long n, float v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ---- host-side launchers -------------------------------------------------------------
+int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st) {
+  dim3 grid((maxchan + 255) / 256, nprod);
+  hipLaunchKernelGGL(k_xmac_f32, grid, dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st) {
+  XMfmaArgs b = a;
+  b.groups = a.groups + group0;
+  dim3 grid((C + 3) / 4, ngroup_launch);
+  if (a.mode == 1) {
+    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 1>), grid, dim3(256), 0, st, b);
+    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 2>), grid, dim3(256), 0, st, b);
+    else return -2;
+  } else {
+    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 1>), grid, dim3(256), 0, st, b);
+    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 2>), grid, dim3(256), 0, st, b);
+    else return -2;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fin_plain, dim3((maxout + 255) / 256, n), dim3(256), 0, st, e, acc, res);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_fin_mfma(const FinMfma* e, int n, int maxout, const DevXGroup* g, const float* acc,
+                        const float* xscale, float2* res, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fin_mfma, dim3((maxout + 255) / 256, n), dim3(256), 0, st, e, g, acc, xscale, res);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_fin_weights(const FinWeight* e, int n, const float* w, int B, float* resf, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fin_weights, dim3((n + 255) / 256), dim3(256), 0, st, e, n, w, B, resf);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_zero(void* p, long long bytes, hipStream_t st) {
+  if (bytes <= 0) return 0;
+  const long long n = bytes / 4;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, (float*)p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_fill(float* p, long long n, float v, hipStream_t st) {
+  if (n <= 0) return 0;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}

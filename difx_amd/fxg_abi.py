@@ -1,0 +1,121 @@
+"""ctypes mirror of include/fxg.h (structure layouts and constants).
+
+The structures mirror the flattened Configuration tables that the C ABI takes:
+fxg_freq <- Configuration::freqdata, fxg_datastream <- datastreamdata plus the Model scan
+table of one scan, fxg_baseline <- baselinedata, fxg_config <- configdata
+(mpifxcorr/src/configuration.h:781-950).
+"""
+import ctypes as C
+
+FXG_MAX_FREQS = 32
+FXG_MAX_BANDS = 64
+FXG_MAX_POLPROD = 4
+FXG_MAX_POLY = 6
+
+FXG_OK = 0
+FXG_E_INVALID = -1
+FXG_E_HIP = -2
+FXG_E_NOMEM = -3
+FXG_E_STATE = -4
+FXG_E_NODEVICE = -5
+
+FXG_FMT_VDIF = 0
+FXG_FMT_MARK5B = 1
+
+FXG_X_FP32 = 0
+FXG_X_FP16 = 1
+FXG_X_BF16 = 2
+
+
+class fxg_freq(C.Structure):
+    _fields_ = [
+        ("freq_mhz", C.c_double),
+        ("bw_mhz", C.c_double),
+        ("lsb", C.c_int32),
+        ("nchan", C.c_int32),
+        ("chanavg", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class fxg_datastream(C.Structure):
+    _fields_ = [
+        ("format", C.c_int32),
+        ("nbits", C.c_int32),
+        ("complex_sampling", C.c_int32),
+        ("framebytes", C.c_int32),
+        ("headerbytes", C.c_int32),
+        ("arraystridelen", C.c_int32),
+        ("nrecfreqs", C.c_int32),
+        ("recfreq_index", C.c_int32 * FXG_MAX_FREQS),
+        ("clock_offset_us", C.c_double * FXG_MAX_FREQS),
+        ("clock_offset_delta_us", C.c_double * FXG_MAX_FREQS),
+        ("phase_offset_deg", C.c_double * FXG_MAX_FREQS),
+        ("lo_offset_hz", C.c_double * FXG_MAX_FREQS),
+        ("nrecbands", C.c_int32),
+        ("recband_localfreq", C.c_int32 * FXG_MAX_BANDS),
+        ("recband_pol", C.c_int32 * FXG_MAX_BANDS),
+        ("nzoomfreqs", C.c_int32),
+        ("zoomfreq_index", C.c_int32 * FXG_MAX_FREQS),
+        ("zoomfreq_parent_localfreq", C.c_int32 * FXG_MAX_FREQS),
+        ("zoomfreq_chanoffset", C.c_int32 * FXG_MAX_FREQS),
+        ("nzoombands", C.c_int32),
+        ("zoomband_localfreq", C.c_int32 * FXG_MAX_BANDS),
+        ("zoomband_pol", C.c_int32 * FXG_MAX_BANDS),
+        ("clock_coeff0_us", C.c_double),
+        ("model_order", C.c_int32),
+        ("model_nsamples", C.c_int32),
+        ("model_delay_us", C.POINTER(C.c_double)),
+    ]
+
+
+class fxg_baseline(C.Structure):
+    _fields_ = [
+        ("ds1", C.c_int32),
+        ("ds2", C.c_int32),
+        ("nfreqs", C.c_int32),
+        ("freq_index", C.c_int32 * FXG_MAX_FREQS),
+        ("npolprod", C.c_int32 * FXG_MAX_FREQS),
+        ("ds1band", (C.c_int32 * FXG_MAX_POLPROD) * FXG_MAX_FREQS),
+        ("ds2band", (C.c_int32 * FXG_MAX_POLPROD) * FXG_MAX_FREQS),
+    ]
+
+
+class fxg_config(C.Structure):
+    _fields_ = [
+        ("nfreqs", C.c_int32),
+        ("freqs", C.POINTER(fxg_freq)),
+        ("ndatastreams", C.c_int32),
+        ("datastreams", C.POINTER(fxg_datastream)),
+        ("nbaselines", C.c_int32),
+        ("baselines", C.POINTER(fxg_baseline)),
+        ("subint_ns", C.c_int64),
+        ("guard_ns", C.c_int32),
+        ("fringe_rot_order", C.c_int32),
+        ("xmac_stride_len", C.c_int32),
+        ("num_buffered_ffts", C.c_int32),
+        ("write_cross_autocorrs", C.c_int32),
+        ("model_inc_secs", C.c_double),
+        ("scan_start_sec", C.c_double),
+        ("xengine", C.c_int32),
+        ("chunk_ffts", C.c_int32),
+    ]
+
+
+class fxg_layout(C.Structure):
+    _fields_ = [
+        ("coreresultlength", C.c_int64),
+        ("blockspersend", C.c_int32),
+        ("nbaselinefreqs", C.c_int32),
+        ("max_databytes", C.c_int64),
+    ]
+
+
+# exported symbols of libfxg.so (every function declared in include/fxg.h)
+EXPORTED_SYMBOLS = [
+    "fxg_create", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
+    "fxg_data_bytes", "fxg_result_offsets", "fxg_process_subint",
+    "fxg_process_subint_device", "fxg_ring_length", "fxg_submit_subint", "fxg_wait",
+    "fxg_get_spectrum", "fxg_get_data_weights", "fxg_unpack_device",
+    "fxg_delay_interpolator",
+]

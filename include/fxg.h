@@ -1,0 +1,197 @@
+/*
+ * fxg.h -- C ABI of the MI355X-native FX hot path (per-subintegration F and X engines).
+ *
+ * The library replaces two C++ call surfaces of mpifxcorr:
+ *   - the station-based Mode object (mpifxcorr/src/mode.h:71-222), built by
+ *     Configuration::getMode (mpifxcorr/src/configuration.cpp:932-998), and
+ *   - Core::processdata (mpifxcorr/src/core.cpp:657-1129) with the accumulation
+ *     Core::uvshiftAndAverageBaselineFreq (core.cpp:1605-1938) and
+ *     Core::averageAndSendAutocorrs (core.cpp:1166-1376).
+ * One fxg_ctx corresponds to one Core process with its Modes for one configuration.
+ * A slot in (offsets, data buffers, control buffers) produces a slot out: the cf32
+ * results array in the order of Configuration::populateResultLengths
+ * (configuration.cpp:2348-2630), unnormalised, exactly as Core sends it to FxManager.
+ *
+ * Conventions: every entry point returns an int status (FXG_OK or a negative
+ * FXG_E* code) and never aborts; fxg_last_error() gives the detailed message.
+ * Inputs are borrowed until the call that consumes them returns.  No torch or HIP
+ * types appear in the signatures; streams are passed as void* (a hipStream_t).
+ */
+#ifndef FXG_H
+#define FXG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FXG_MAX_FREQS   32  /* recorded or zoom freqs per datastream; freqs per baseline */
+#define FXG_MAX_BANDS   64  /* recorded or zoom bands per datastream */
+#define FXG_MAX_POLPROD 4   /* polarisation products per baseline freq */
+#define FXG_MAX_POLY    6   /* delay polynomial coefficients per model interval (order <= 5) */
+
+/* status codes */
+#define FXG_OK            0
+#define FXG_E_INVALID    -1  /* invalid argument or unsupported configuration */
+#define FXG_E_HIP        -2  /* HIP runtime error */
+#define FXG_E_NOMEM      -3  /* device or host allocation failed */
+#define FXG_E_STATE      -4  /* call out of order (e.g. wait on an unknown handle) */
+#define FXG_E_NODEVICE   -5  /* no usable gfx950 device */
+
+/* data formats (Configuration::dataformat, configuration.h) that this path decodes */
+#define FXG_FMT_VDIF    0   /* VDIF, 32-byte header, LUT of format_vdif.c:66-145 */
+#define FXG_FMT_MARK5B  1   /* Mark5B, 16-byte header, LUT of mark5_format_mark5b.c:49-86 */
+
+/* X-engine kinds */
+#define FXG_X_FP32  0   /* plain FP32 complex multiply-accumulate per baseline product (core.cpp:963) */
+#define FXG_X_FP16  1   /* per-channel Hermitian station x station contraction on MFMA, FP16 in, FP32 acc */
+#define FXG_X_BF16  2   /* same, BF16 in */
+
+/* One entry of the frequency table (Configuration::freqdata, configuration.h). */
+typedef struct fxg_freq {
+  double  freq_mhz;   /* band-edge sky frequency (the LO for USB/LSB real data), MHz */
+  double  bw_mhz;     /* bandwidth, MHz */
+  int32_t lsb;        /* 1 = lower sideband */
+  int32_t nchan;      /* numchannels */
+  int32_t chanavg;    /* channelstoaverage */
+  int32_t reserved;
+} fxg_freq;
+
+/* One datastream (Configuration::datastreamdata, configuration.h) plus its delay model
+ * (Model scan table for one scan and one phase centre, model.h). */
+typedef struct fxg_datastream {
+  int32_t format;             /* FXG_FMT_* */
+  int32_t nbits;              /* bits per (real or imaginary) sample: 2 or 8 */
+  int32_t complex_sampling;   /* 0 = real, 1 = complex single sideband */
+  int32_t framebytes;         /* frame size including header */
+  int32_t headerbytes;        /* 32 (VDIF) or 16 (Mark5B) */
+  int32_t arraystridelen;     /* ARRAY STRIDE LEN; 0 = automatic (configuration.cpp:38-51) */
+  int32_t nrecfreqs;
+  int32_t recfreq_index[FXG_MAX_FREQS];        /* into the freq table */
+  double  clock_offset_us[FXG_MAX_FREQS];      /* recordedfreqclockoffsets */
+  double  clock_offset_delta_us[FXG_MAX_FREQS];/* recordedfreqclockoffsetsdelta */
+  double  phase_offset_deg[FXG_MAX_FREQS];     /* recordedfreqphaseoffset */
+  double  lo_offset_hz[FXG_MAX_FREQS];         /* recordedfreqlooffsets */
+  int32_t nrecbands;                           /* bands (channels) in the sample word */
+  int32_t recband_localfreq[FXG_MAX_BANDS];    /* local recorded freq index of each band */
+  int32_t recband_pol[FXG_MAX_BANDS];          /* ASCII 'R','L','X','Y' */
+  int32_t nzoomfreqs;
+  int32_t zoomfreq_index[FXG_MAX_FREQS];            /* into the freq table */
+  int32_t zoomfreq_parent_localfreq[FXG_MAX_FREQS]; /* local recorded freq index of the parent */
+  int32_t zoomfreq_chanoffset[FXG_MAX_FREQS];       /* first parent channel of the zoom */
+  int32_t nzoombands;
+  int32_t zoomband_localfreq[FXG_MAX_BANDS];   /* local zoom freq index of each zoom band */
+  int32_t zoomband_pol[FXG_MAX_BANDS];
+  double  clock_coeff0_us;    /* clock offset term 0; intclockseconds = floor(c0/1e6 + 0.5) (mode.cpp:69) */
+  int32_t model_order;        /* delay polynomial order (<= 5) */
+  int32_t model_nsamples;     /* number of model intervals of length model_inc_secs */
+  const double* model_delay_us; /* [model_nsamples][FXG_MAX_POLY]; t in s from interval start */
+} fxg_datastream;
+
+/* One baseline (Configuration::baselinedata, configuration.h): datastream pair and, per
+ * baseline freq, the band index pairs of each polarisation product.  Band indices count
+ * recorded bands first, then zoom bands (mode.cpp:164-196). */
+typedef struct fxg_baseline {
+  int32_t ds1, ds2;                 /* ordered datastream indices */
+  int32_t nfreqs;
+  int32_t freq_index[FXG_MAX_FREQS];
+  int32_t npolprod[FXG_MAX_FREQS];
+  int32_t ds1band[FXG_MAX_FREQS][FXG_MAX_POLPROD];
+  int32_t ds2band[FXG_MAX_FREQS][FXG_MAX_POLPROD];
+} fxg_baseline;
+
+/* Flattened Configuration for one config index. */
+typedef struct fxg_config {
+  int32_t nfreqs;
+  const fxg_freq* freqs;
+  int32_t ndatastreams;
+  const fxg_datastream* datastreams;
+  int32_t nbaselines;
+  const fxg_baseline* baselines;
+  int64_t subint_ns;          /* SUBINT NANOSECONDS */
+  int32_t guard_ns;           /* GUARD NANOSECONDS */
+  int32_t fringe_rot_order;   /* FRINGE ROTN ORDER: 1 (0 and 2 are rejected by this build) */
+  int32_t xmac_stride_len;    /* XMAC STRIDE LEN (the result layout does not depend on it) */
+  int32_t num_buffered_ffts;  /* NUM BUFFERED FFTS */
+  int32_t write_cross_autocorrs; /* cross-polarisation autocorrelations (conf.writeautocorrs) */
+  double  model_inc_secs;     /* model interval length (Model::modelincsecs) */
+  double  scan_start_sec;     /* Model::getScanStartSec of the scan, for wallclock LO-offset terms */
+  int32_t xengine;            /* FXG_X_* */
+  int32_t chunk_ffts;         /* FFTs per device spectra chunk; 0 = automatic */
+} fxg_config;
+
+/* Result layout summary (Configuration::populateResultLengths and getDataBytes). */
+typedef struct fxg_layout {
+  int64_t coreresultlength;   /* cf32 elements of the results array */
+  int32_t blockspersend;      /* FFTs per subint (configuration.cpp:1843-1852) */
+  int32_t nbaselinefreqs;     /* entries of the per (freq, baseline) offset tables */
+  int64_t max_databytes;      /* largest per-datastream slot data size in bytes */
+} fxg_layout;
+
+typedef struct fxg_ctx fxg_ctx;
+typedef int32_t fxg_handle;
+
+/* Create a context: validates the configuration, builds the result layout, allocates the
+ * device-resident state.  Replaces Configuration::getMode + the Core constructor
+ * (configuration.cpp:932-998, core.cpp:23-154). */
+int fxg_create(const fxg_config* cfg, fxg_ctx** out);
+int fxg_destroy(fxg_ctx* ctx);
+const char* fxg_strerror(int code);
+const char* fxg_last_error(const fxg_ctx* ctx);
+
+int fxg_get_layout(const fxg_ctx* ctx, fxg_layout* out);
+/* Configuration::getDataBytes (configuration.cpp:794-820) for datastream ds. */
+int fxg_data_bytes(const fxg_ctx* ctx, int32_t ds, int64_t* bytes);
+/* Offsets (cf32 units) of populateResultLengths: baseline[freq*nbaselines+b] (or -1),
+ * bweight[freq*nbaselines+b] (or -1), autocorr[ds], acweight[ds].  Any pointer may be null. */
+int fxg_result_offsets(const fxg_ctx* ctx, int32_t* baseline, int32_t* bweight,
+                       int32_t* autocorr, int32_t* acweight);
+
+/* Core::processdata for one slot, host buffers (core.cpp:657-1129): offsets = {scan, sec, ns}
+ * of the subint; data[ds] = the slot's data buffer of nbytes[ds] bytes; control[ds] =
+ * {scan, sec, ns, validflags...} (datastream.cpp:380, 581-605).  results receives
+ * coreresultlength cf32 values (interleaved re, im).  Synchronous. */
+int fxg_process_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* data,
+                       const int32_t* nbytes, const int32_t* const* control, float* results);
+
+/* Same, with the data already resident in device memory (d_data[ds]) and the results
+ * written to device memory (d_results, coreresultlength cf32).  Work is enqueued on
+ * `stream` (a hipStream_t, may be null for the context's own stream) and is complete when
+ * the stream is.  Control words are read from host memory before the call returns. */
+int fxg_process_subint_device(fxg_ctx* ctx, const int32_t offsets[3],
+                              const uint8_t* const* d_data, const int32_t* nbytes,
+                              const int32_t* const* control, float* d_results, void* stream);
+
+/* Asynchronous slot ring (Core's RECEIVE_RING_LENGTH slots, core.cpp:156): submit copies the
+ * host inputs into a free slot and enqueues the work; wait blocks until that slot is done
+ * and copies its results out.  At most fxg_ring_length() submissions may be outstanding. */
+int fxg_ring_length(const fxg_ctx* ctx);
+int fxg_submit_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* data,
+                      const int32_t* nbytes, const int32_t* const* control, fxg_handle* handle);
+int fxg_wait(fxg_ctx* ctx, fxg_handle handle, float* results);
+
+/* Mode getters for the last processed subint (mode.h:138-203), for the FFT blocks of the last
+ * device chunk: spectrum of band `band` at FFT `fft` (getFreqs, C cf32 values; zeros if the
+ * block was flagged) and the per-FFT data weights (getDataWeight) of datastream ds. */
+int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, float* out);
+int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out /* [blockspersend] */);
+
+/* Mk5Mode::unpack (mk5mode.cpp:94-174) as a device call: decode nsamples time samples starting
+ * at sample `startsample` of datastream ds from d_data (nbytes) into d_out[nrecbands][nsamples]
+ * (f32, or interleaved cf32 for complex sampling), and the count of valid samples into
+ * d_goodsamples[0] (int32).  Frames that are invalid or blanked decode as zeros. */
+int fxg_unpack_device(fxg_ctx* ctx, int32_t ds, const uint8_t* d_data, int32_t nbytes,
+                      int64_t startsample, int32_t nsamples, float* d_out,
+                      int32_t* d_goodsamples, void* stream);
+
+/* Model::calculateDelayInterpolator (model.cpp:219-289), order 2, as used by Mode::setOffsets
+ * (mode.cpp:1523-1526): evaluated on the device in FP64.  out[3] = quadratic coefficients in
+ * FFT-index units with 1e6*intclockseconds removed from out[2].  Returns FXG_E_INVALID if the
+ * time is outside the model. */
+int fxg_delay_interpolator(fxg_ctx* ctx, int32_t ds, int32_t sec, int32_t ns, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FXG_H */

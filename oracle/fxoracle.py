@@ -1,0 +1,577 @@
+"""FP64 oracle of mpifxcorr's per-subintegration FX path (test infrastructure only).
+
+This module is the CHECKER, not the product: only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import it.  It restates, in numpy with float64/complex128
+arithmetic, what the reference computes for one subint and one Core slot:
+
+  unpack          Mk5Mode::unpack (mk5mode.cpp:94-174) -> mark5_unpack_with_offset
+                  (mark5_stream_unpacker.c:119-138), VDIF LUTs (format_vdif.c:66-145,
+                  834-880, 1621-1680, 5215-5250), Mark5B LUT (mark5_format_mark5b.c:49-86,
+                  1708-1760), frame validation / blanking (format_vdif.c:7619-7660,
+                  blanker_mark5.c:272-310, mark5_stream.c:144-193)
+  delay model     Model::calculateDelayInterpolator (model.cpp:219-289), Mode::setOffsets
+                  (mode.cpp:1502-1532)
+  F-engine        Mode::process (mode.cpp:634-1374), fringe rotation order 1, fractional
+                  sample correction, sideband channel selection, autocorrelations
+  X-engine        Core::processdata XMAC (core.cpp:884-979), baseline weights
+                  (core.cpp:1005-1052)
+  accumulation    Core::uvshiftAndAverageBaselineFreq (core.cpp:1819-1853),
+                  Core::averageAndSendAutocorrs (core.cpp:1258-1376), Mode::averageFrequency
+                  (mode.cpp:1376-1401)
+  layout          Configuration::populateResultLengths (configuration.cpp:2348-2630)
+  normalisation   Visibility::writedata scale (visibility.cpp:526-527)
+
+Where the reference rounds an intermediate to float32 (sin/cos arguments, the fractional
+sample error, channel-frequency slopes) the oracle keeps float64: it is the exact
+mathematics of the reference algorithm, against which both the FP32 and the FP16/BF16
+device paths are measured.  The FFT is numpy's complex128 FFT, itself pinned by brute-force
+DFT checks in tests/test_oracle.py.
+
+Pins (tests/test_oracle.py): LUT closed forms over all 256 byte values; frame skipping and
+blanking on hand-built frames; interpolator exactness for polynomials of degree <= 2;
+per-FFT delay chain special cases (zero and integer-sample delays); rotator identity and
+unit-phase cases; DFT against an O(L^2) brute force; XMAC/auto invariants (Hermitian,
+autos real and non-negative, a station against itself equals its autocorrelation); the
+result layout against hand-counted offsets; end to end against the statistical closed form
+rho*0.88^2 of a 2-bit common signal (mode.cpp:1571).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+TWO_PI = 6.283185307179586476925286766559   # architecture.h.in:430
+HIMAG = 3.3359                                # OPTIMAL_2BIT_HIGH, mark5_stream.h:65
+FLAGS_PER_INT = 30                            # mpifxcorr.h:29
+INVALID_SUBINT = -99999999                    # mode.h:222
+TINY = 1e-9                                   # mode.cpp:27
+FILL_WORD64 = 0x1122334411223344              # blanker_mark5.c:36 (little endian)
+DECORR = [0.63662, 0.88, 0.94, 0.96, 0.98, 0.99, 0.996, 0.998]  # mode.cpp:1571
+
+
+# --------------------------------------------------------------------------------------
+# Look-up tables
+# --------------------------------------------------------------------------------------
+def vdif_lut2bit() -> np.ndarray:
+    """format_vdif.c:66-110: lut2bit[b][i] = {-HiMag,-1,+1,+HiMag}[(b >> 2i) & 3]."""
+    lev = np.array([-HIMAG, -1.0, 1.0, HIMAG], dtype=np.float32)
+    b = np.arange(256)
+    return np.stack([lev[(b >> (2 * i)) & 3] for i in range(4)], axis=1)
+
+
+def mark5b_lut2bit() -> np.ndarray:
+    """mark5_format_mark5b.c:49-86: sign bit at 2i, magnitude bit at 2i+1,
+    lut4level = {-HiMag, 1, -1, HiMag}[sign + 2*mag]."""
+    lev = np.array([-HIMAG, 1.0, -1.0, HIMAG], dtype=np.float32)
+    b = np.arange(256)
+    return np.stack([lev[((b >> (2 * i)) & 1) + (((b >> (2 * i + 1)) & 1) << 1)]
+                     for i in range(4)], axis=1)
+
+
+def lut8bit(offset: float = 128.0) -> np.ndarray:
+    """format_vdif.c:120: (b - 128)/3.3 (127.5 with SYMMETRIC_8_BIT_VDIF), rounded to float."""
+    return ((np.arange(256) - offset) / 3.3).astype(np.float32)
+
+
+# --------------------------------------------------------------------------------------
+# Frames and unpacking
+# --------------------------------------------------------------------------------------
+def frame_valid(buf: np.ndarray, fmt: str, framebytes: int, headerbytes: int) -> np.ndarray:
+    """Per-frame validity of a slot buffer: frames that fail validation or carry the fill
+    pattern are blanked.  VDIF: header word 2 == 0 or invalid bit (word 0 bit 31) fails
+    (format_vdif.c:7619-7660); blanker_vdif blanks a payload whose first or last 64-bit word
+    is the fill pattern (blanker_mark5.c:272-310).  Mark5B: the invalid flag is header byte 5
+    bit 7 (mark5_format_mark5b.c:1723, buf[-11] & 0x80); a payload starting or ending with the
+    fill pattern is blanked whole (the zone blanker's partial-frame zones are not restated).
+    Frames extending past the buffer are invalid."""
+    nframes = len(buf) // framebytes
+    out = np.zeros(nframes, dtype=bool)
+    if nframes == 0:
+        return out
+    fr = buf[: nframes * framebytes].reshape(nframes, framebytes)
+    payload = fr[:, headerbytes:]
+    first = payload[:, :8].copy().view("<u8")[:, 0]
+    last = payload[:, -8:].copy().view("<u8")[:, 0]
+    fill = (first == FILL_WORD64) | (last == FILL_WORD64)
+    if fmt == "VDIF":
+        w0 = fr[:, 0:4].copy().view("<u4")[:, 0]
+        w2 = fr[:, 8:12].copy().view("<u4")[:, 0]
+        ok = (w2 != 0) & (((w0 >> 31) & 1) == 0)
+    else:
+        ok = (fr[:, 5] & 0x80) == 0
+    return ok & ~fill
+
+
+def decode_stream(buf: np.ndarray, ds) -> Tuple[np.ndarray, np.ndarray]:
+    """Decode every sample of a slot buffer: returns (values[nbands, nsamples] float32 or
+    complex64, valid[nsamples] bool).  Sample n of band b sits at sample-field q = n*nbands+b
+    of its frame's payload; 2-bit fields are LSB-first in bytes (format_vdif.c:834-880,
+    1621-1680), 8-bit complex samples are (re, im) byte pairs (format_vdif.c:5215-5250)."""
+    fmt = ds.format.upper()
+    nb = len(ds.recbands)
+    fb, hb = ds.framebytes, ds.headerbytes
+    pay = fb - hb
+    nframes = len(buf) // fb
+    valid_f = frame_valid(buf, fmt, fb, hb)
+    fr = buf[: nframes * fb].reshape(nframes, fb)[:, hb:]
+    if ds.nbits == 2 and not ds.complex_sampling:
+        lut = vdif_lut2bit() if fmt == "VDIF" else mark5b_lut2bit()
+        vals = lut[fr.reshape(-1)].reshape(-1)                 # 4 fields per byte, LSB first
+        spf = pay * 4 // nb                                    # samples per frame
+        vals = vals.reshape(nframes * spf, nb).T
+    elif ds.nbits == 8 and ds.complex_sampling:
+        lut = lut8bit()
+        v = lut[fr.reshape(-1)].reshape(-1, 2)
+        c = (v[:, 0].astype(np.float32) + 1j * v[:, 1].astype(np.float32)).astype(np.complex64)
+        spf = pay // (2 * nb)
+        vals = c.reshape(nframes * spf, nb).T
+    elif ds.nbits == 8 and not ds.complex_sampling:
+        lut = lut8bit()
+        spf = pay // nb
+        vals = lut[fr.reshape(-1)].reshape(nframes * spf, nb).T
+    else:
+        raise NotImplementedError(f"{ds.nbits}-bit complex={ds.complex_sampling}")
+    valid = np.repeat(valid_f, spf)
+    vals = np.where(valid[None, :], vals, np.zeros((), vals.dtype))   # blanked samples are +0.0
+    return vals, valid
+
+
+def samples_per_frame(ds) -> int:
+    pay = ds.framebytes - ds.headerbytes
+    bits = ds.nbits * (2 if ds.complex_sampling else 1) * len(ds.recbands)
+    return pay * 8 // bits
+
+
+# --------------------------------------------------------------------------------------
+# Delay model
+# --------------------------------------------------------------------------------------
+def delay_interpolator(model: np.ndarray, order_poly: int, model_inc_secs: float,
+                       offsettime: float, timespan: float, numincrements: int) -> Optional[np.ndarray]:
+    """Model::calculateDelayInterpolator(order 2) (model.cpp:219-289) for one scan whose
+    polynomials start at the scan start (polyoffset = 0): three evaluations of the station
+    delay polynomial at the start, middle and end of the span, fitted with a quadratic in
+    increment units."""
+    scansample = int(offsettime / float(model_inc_secs))
+    nsamp = model.shape[0]
+    if scansample == nsamp and (offsettime - scansample * model_inc_secs) < 1e-6:
+        scansample -= 1
+    if scansample < 0 or scansample >= nsamp:
+        return None
+    c = model[scansample, : order_poly + 1]
+
+    def ev(dt):
+        tp = dt ** np.arange(order_poly + 1)
+        return float(np.dot(tp, c))
+
+    base = scansample * model_inc_secs
+    d1 = ev(offsettime + timespan / 2.0 - base)
+    d0 = ev(offsettime - base)
+    d2 = ev(offsettime + timespan - base)
+    n = numincrements
+    return np.array([(2.0 * d0 - 4.0 * d1 + 2.0 * d2) / (n * n),
+                     (-3.0 * d0 + 4.0 * d1 - d2) / n,
+                     d0])
+
+
+def intclockseconds(ds) -> int:
+    """mode.cpp:69."""
+    return int(math.floor(ds.clock_coeff0_us / 1000000.0 + 0.5))
+
+
+# --------------------------------------------------------------------------------------
+# Result layout (Configuration::populateResultLengths, single phase centre, no binning)
+# --------------------------------------------------------------------------------------
+@dataclass
+class Layout:
+    coreresultlength: int
+    baseline_off: Dict[Tuple[int, int], int]   # (freq, baseline) -> cf32 offset
+    bweight_off: Dict[Tuple[int, int], int]
+    autocorr_off: List[int]
+    acweight_off: List[int]
+    bandsperautocorr: int
+
+
+def freq_used(cfg) -> Tuple[List[bool], List[bool]]:
+    """configuration.cpp:2270-2320: a freq is used if some baseline correlates it; an
+    'equivalent' freq (same edge, bandwidth, channels, averaging) of a used freq is also
+    kept for its autocorrelations."""
+    nf = len(cfg.freqs)
+    used = [False] * nf
+    for b in cfg.baselines:
+        for fi, _ in b.freqs:
+            used[fi] = True
+    equiv = [False] * nf
+    for j in range(nf):
+        if not used[j]:
+            for k in range(nf):
+                fj, fk = cfg.freqs[j], cfg.freqs[k]
+                bwdiff = fj.bw_mhz - fk.bw_mhz
+                fdiff = fj.freq_mhz - fk.freq_mhz
+                if fj.lsb:
+                    fdiff -= fj.bw_mhz
+                if fk.lsb:
+                    fdiff += fk.bw_mhz
+                if (abs(bwdiff) < TINY and abs(fdiff) < TINY and fj.nchan == fk.nchan
+                        and fj.chanavg == fk.chanavg and used[k]):
+                    equiv[j] = True
+    return used, equiv
+
+
+def result_layout(cfg) -> Layout:
+    nf = len(cfg.freqs)
+    used, equiv = freq_used(cfg)
+    maxprod = max((len(p) for b in cfg.baselines for (_, p) in b.freqs), default=1)
+    bpa = 2 if maxprod > 2 else 1
+    idx = 0
+    boff, bw = {}, {}
+    for f in range(nf):                      # cross-correlations (configuration.cpp:2464-2521)
+        if not used[f]:
+            continue
+        fr = cfg.freqs[f]
+        for j, b in enumerate(cfg.baselines):
+            for fi, prods in b.freqs:
+                if fi == f:
+                    boff[(f, j)] = idx
+                    idx += len(prods) * ((fr.nchan + fr.chanavg - 1) // fr.chanavg)
+    for f in range(nf):                      # baseline weights (:2523-2548)
+        if not used[f]:
+            continue
+        for j, b in enumerate(cfg.baselines):
+            for fi, prods in b.freqs:
+                if fi == f:
+                    bw[(f, j)] = idx
+                    idx += len(prods) // 2 + len(prods) % 2
+    ac = []
+    for d in cfg.datastreams:                # autocorrelations (:2567-2586)
+        ac.append(idx)
+        for lf, _ in d.recbands:
+            fi = d.recfreqs[lf]
+            if used[fi] or equiv[fi]:
+                fr = cfg.freqs[fi]
+                idx += bpa * fr.nchan // fr.chanavg
+        for lz, _ in d.zoombands:
+            fi = d.zoomfreqs[lz][0]
+            if used[fi] or equiv[fi]:
+                fr = cfg.freqs[fi]
+                idx += bpa * fr.nchan // fr.chanavg
+    acw = []
+    for d in cfg.datastreams:                # autocorrelation weights (:2587-2607)
+        acw.append(idx)
+        toadd = 0
+        for lf, _ in d.recbands:
+            fi = d.recfreqs[lf]
+            if used[fi] or equiv[fi]:
+                toadd += bpa
+        for lz, _ in d.zoombands:
+            fi = d.zoomfreqs[lz][0]
+            if used[fi] or equiv[fi]:
+                toadd += bpa
+        toadd = (toadd + 1) // 2
+        idx += max(toadd, 1)
+    return Layout(idx, boff, bw, ac, acw, bpa)
+
+
+# --------------------------------------------------------------------------------------
+# The F-engine (Mode::process) for all FFTs of a subint
+# --------------------------------------------------------------------------------------
+@dataclass
+class StationResult:
+    spectra: np.ndarray        # [nbands_total, B, C] complex128 (zoom bands are parent windows)
+    weight: np.ndarray         # [B] data weights
+    autos: np.ndarray          # [2, nrecbands, C] complex128 (parallel, cross-pol)
+    acweight: np.ndarray       # [2, nrecbands]
+
+
+def fft_params(cfg, d: int, offsets, control, interp, B: int):
+    """Per-FFT delay chain of Mode::process (mode.cpp:671-675, 743-757, 781-788) for all
+    FFT indices: nearestsample, fracsampleerror (us), linear coefficients a, b and the
+    integer delay; plus the validity mask."""
+    ds = cfg.datastreams[d]
+    L = cfg.fftchannels(d)
+    dt = cfg.sampletime_us(d)
+    i = np.arange(B, dtype=np.float64)
+    q = lambda x: interp[0] * x * x + interp[1] * x + interp[2]
+    avgdelay = q(i + 0.5)
+    fftstart = i * L * dt
+    datasec, datans = control[1], control[2]
+    starttime = ((offsets[1] - datasec) * 1000000.0
+                 + (float(offsets[2]) - float(datans)) / 1000.0 + fftstart - avgdelay)
+    ns = np.trunc(starttime / dt + 0.5).astype(np.int64)
+    d0, d1, d2 = q(i), q(i + 0.5), q(i + 1.0)
+    a = d2 - d0
+    b = d0 + (d1 - (a * 0.5 + d0)) / 3.0
+    intdelay = np.trunc(b).astype(np.int64)
+    b = b - intdelay
+    return ns, starttime, a, b, intdelay, avgdelay
+
+
+def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
+                    B: Optional[int] = None) -> StationResult:
+    """Mode::setValidFlags/setData/setOffsets + Mode::process for every FFT of the subint
+    (core.cpp:695-801), with Mk5Mode::unpack, in float64."""
+    ds = cfg.datastreams[d]
+    if B is None:
+        B = cfg.blockspersend()
+    if cfg.fringe_rot_order != 1:
+        raise NotImplementedError("fringe rotation order 1 only")
+    L = cfg.fftchannels(d)
+    dt = cfg.sampletime_us(d)
+    nrec = len(ds.recbands)
+    nzoom = len(ds.zoombands)
+    f0 = cfg.freqs[ds.recfreqs[0]]
+    C = f0.nchan
+    bw = f0.bw_mhz
+    usecomplex = ds.complex_sampling
+    spectra = np.zeros((nrec + nzoom, B, C), dtype=np.complex128)
+    weight = np.zeros(B)
+    autos = np.zeros((2, nrec, C), dtype=np.complex128)
+    acw = np.zeros((2, nrec))
+    datalen = len(data)
+    scan, datasec, datans = int(control[0]), int(control[1]), int(control[2])
+    valid_bits = control[3:]
+
+    # Mode::setOffsets (mode.cpp:1502-1532)
+    if datasec <= INVALID_SUBINT or datalen <= 1:
+        return StationResult(spectra[:nrec + nzoom], weight, autos, acw)
+    if offsets[0] != scan:
+        return StationResult(spectra, weight, autos, acw)
+    # blockspersend*2*nchan*sampletime/1e6, halved for complex data (mode.cpp:1523-1524)
+    timespan = B * C / bw / 1e6
+    interp = delay_interpolator(ds.model, ds.model_order, cfg.model_inc_secs,
+                                offsets[1] + offsets[2] / 1e9, timespan, B)
+    if interp is None:
+        return StationResult(spectra, weight, autos, acw)
+    interp = interp.copy()
+    interp[2] -= 1000000 * intclockseconds(ds)
+
+    vals, valid = decode_stream(data, ds)
+    nsamples_buf = vals.shape[1]
+    ns, starttime, a, b, intdelay, avgdelay = fft_params(cfg, d, offsets, control, interp, B)
+
+    # bytes-per-block ratio of the reference's bounds check (mode.cpp:124-139, 688)
+    nsb = ds.nbits * (2 if usecomplex else 1)
+    bpb_num = (nrec * nsb) // 8
+    bpb_den = 1
+    if bpb_num == 0:
+        bpb_num, bpb_den = 1, 8 // (nrec * nsb)
+    S = ds.arraystridelen or calc_stride(C)
+    nfr = L // S
+    subxoff = np.arange(S) / float(L)
+    stepxoff = (np.arange(nfr) * S) / float(L)
+    sampletime_s = dt / 1e6
+    subtoff = np.arange(S) * sampletime_s
+    steptoff = np.arange(nfr) * S * sampletime_s
+    jj = np.arange(L)
+    fractional_lo = any(abs(cfg.freqs[fi].freq_mhz - int(cfg.freqs[fi].freq_mhz + 0.5)) > TINY
+                        for fi in ds.recfreqs)
+
+    for i in range(B):
+        if not ((int(valid_bits[i // FLAGS_PER_INT]) >> (i % FLAGS_PER_INT)) & 1):
+            continue
+        n0 = int(ns[i])
+        if n0 < -1 or ((n0 + L) * bpb_num) // bpb_den > datalen:
+            continue
+        if n0 == -1:
+            n0 = 0
+        win = slice(n0, n0 + L)
+        if n0 + L > nsamples_buf:
+            # samples past the last whole frame are treated as blanked
+            good = int(valid[n0:].sum()) if n0 < nsamples_buf else 0
+            seg = np.zeros((nrec, L), dtype=vals.dtype)
+            if n0 < nsamples_buf:
+                seg[:, : nsamples_buf - n0] = vals[:, n0:]
+        else:
+            good = int(valid[win].sum())
+            seg = vals[:, win]
+        w = good / float(L)
+        if not w > 0.0:
+            continue
+        weight[i] = w
+        fse = starttime[i] - n0 * dt                        # mode.cpp:756-757 (kept in f64)
+        walltime = cfg.scan_start_sec + offsets[1] + offsets[2] / 1.0e9 + i * L * dt / 1.0e6
+        intwall = int(walltime)
+        fracwall = walltime - intwall
+        for lf, fi in enumerate(ds.recfreqs):
+            fr = cfg.freqs[fi]
+            lofreq = fr.freq_mhz
+            if usecomplex and fr.lsb:
+                lofreq = -lofreq
+            loff = ds.per_freq("lo_offset_hz")[lf]
+            # fringe rotator, order 1 (mode.cpp:883-955): phase in turns, f64
+            subphase = (subxoff * a[i] + b[i]) * lofreq
+            stepphase = (stepxoff * a[i]) * lofreq
+            if fractional_lo:
+                subphase = subphase + (lofreq - int(lofreq)) * float(intdelay[i])
+            if loff != 0.0:
+                fraclo = abs(loff) - int(abs(loff))
+                if loff < 0:
+                    fraclo = -fraclo
+                subphase = subphase + subtoff * (-loff) - loff * fracwall
+                stepphase = stepphase + steptoff * (-loff)
+                if fraclo > TINY or fraclo < -TINY:
+                    subphase = subphase - fraclo * intwall
+            phase = subphase[jj % S] + stepphase[jj // S]
+            rot = np.exp(-1j * TWO_PI * phase)
+            # fractional-sample correction (mode.cpp:1030-1073), f64
+            e = fse - ds.per_freq("clock_offset_us")[lf] + ds.per_freq("clock_offset_delta_us")[lf] / 2
+            k = np.arange(C)
+            if (not usecomplex) and fr.lsb:
+                chanfreq = TWO_PI * (k - C) * bw / C
+            else:
+                chanfreq = TWO_PI * k * bw / C
+            frac = np.exp(1j * chanfreq * e)
+            match = [j for j, (blf, _) in enumerate(ds.recbands) if blf == lf]
+            for j in match:
+                x = seg[j].astype(np.complex128)
+                X = np.fft.fft(x * rot)                                  # mode.cpp:1174-1183
+                if usecomplex:
+                    if fr.lsb:
+                        raise NotImplementedError("complex LSB")
+                    F = X[:C]
+                elif fr.lsb:
+                    F = X[C: 2 * C]                                      # mode.cpp:1205
+                else:
+                    F = X[:C]                                            # mode.cpp:1215
+                F = F * frac                                             # mode.cpp:1257-1261
+                spectra[j, i] = F
+                autos[0, j] += F * np.conj(F)                           # mode.cpp:1272
+                acw[0, j] += w
+            if len(match) > 1 and cfg.write_cross_autocorrs:            # mode.cpp:1333-1352
+                j0, j1 = match[0], match[1]
+                autos[1, j0] += spectra[j0, i] * np.conj(spectra[j1, i])
+                autos[1, j1] += spectra[j1, i] * np.conj(spectra[j0, i])
+                acw[1, j0] += w
+                acw[1, j1] += w
+    # zoom bands are windows of their parent band's spectrum (mode.cpp:184-196)
+    for z, (lz, pol) in enumerate(ds.zoombands):
+        zfi, parent_lf, choff = ds.zoomfreqs[lz]
+        zn = cfg.freqs[zfi].nchan
+        parent = [j for j, (blf, bp) in enumerate(ds.recbands) if blf == parent_lf and bp == pol][0]
+        spectra[nrec + z, :, :zn] = spectra[parent, :, choff: choff + zn]
+    return StationResult(spectra, weight, autos, acw)
+
+
+def calc_stride(n: int) -> int:
+    """configuration.cpp:38-51."""
+    if n <= 16:
+        return n
+    g = int(math.sqrt(n + 0.1))
+    while g > 0:
+        if n % g == 0:
+            return n // g
+        g -= 1
+    return n
+
+
+def band_record_index(ds, band: int) -> int:
+    """Record band of a (possibly zoom) band: the parent recorded band for zooms
+    (getBDataStreamXRecordBandIndex)."""
+    nrec = len(ds.recbands)
+    if band < nrec:
+        return band
+    lz, pol = ds.zoombands[band - nrec]
+    parent_lf = ds.zoomfreqs[lz][1]
+    return [j for j, (blf, bp) in enumerate(ds.recbands) if blf == parent_lf and bp == pol][0]
+
+
+def process_subint(cfg, offsets, data: Sequence[np.ndarray], control: Sequence[np.ndarray],
+                   B: Optional[int] = None, stations: Optional[List[StationResult]] = None):
+    """Core::processdata for one slot (core.cpp:657-1129) with the accumulation of
+    uvshiftAndAverageBaselineFreq (core.cpp:1819-1853) and averageAndSendAutocorrs
+    (core.cpp:1258-1376).  Returns (results complex128[coreresultlength], stations)."""
+    if B is None:
+        B = cfg.blockspersend()
+    lay = result_layout(cfg)
+    res = np.zeros(lay.coreresultlength, dtype=np.complex128)
+    if stations is None:
+        stations = [process_station(cfg, d, offsets, data[d], control[d], B)
+                    for d in range(len(cfg.datastreams))]
+    fres = res.view(np.float64)
+    used, equiv = freq_used(cfg)
+    for j, bl in enumerate(cfg.baselines):
+        s1, s2 = stations[bl.ds1], stations[bl.ds2]
+        d1, d2 = cfg.datastreams[bl.ds1], cfg.datastreams[bl.ds2]
+        for fi, prods in bl.freqs:
+            fr = cfg.freqs[fi]
+            nch = fr.nchan
+            avg = fr.chanavg
+            off = lay.baseline_off[(fi, j)]
+            for p, (b1, b2) in enumerate(prods):
+                v = np.einsum("tc,tc->c", s1.spectra[b1, :, :nch], np.conj(s2.spectra[b2, :, :nch]))
+                nout = (nch + avg - 1) // avg
+                if avg == 1:
+                    res[off + p * nch: off + (p + 1) * nch] += v
+                else:
+                    pad = np.zeros(nout * avg, dtype=np.complex128)
+                    pad[:nch] = v
+                    res[off + p * nout: off + (p + 1) * nout] += pad.reshape(nout, avg).sum(1) / avg
+            # baseline weights (core.cpp:1005-1052, 1066-1112): sum over FFTs of w1*w2
+            bwo = lay.bweight_off[(fi, j)] * 2
+            for p, (b1, b2) in enumerate(prods):
+                fres[bwo + p] += float(np.sum(s1.weight * s2.weight))
+    # autocorrelations and their weights (core.cpp:1258-1376, mode.cpp:1376-1401)
+    for d, ds in enumerate(cfg.datastreams):
+        st = stations[d]
+        nrec = len(ds.recbands)
+        idx = lay.autocorr_off[d]
+        for w in range(lay.bandsperautocorr):
+            if w == 1 and not cfg.write_cross_autocorrs:
+                break
+            for k in range(nrec + len(ds.zoombands)):
+                if k < nrec:
+                    fi = ds.recfreqs[ds.recbands[k][0]]
+                else:
+                    fi = ds.zoomfreqs[ds.zoombands[k - nrec][0]][0]
+                if not (used[fi] or equiv[fi]):
+                    continue
+                fr = cfg.freqs[fi]
+                pf = cfg.freqs[ds.recfreqs[0]]
+                cavg = pf.chanavg                 # Mode's channelstoaverage (first recorded freq)
+                nout = fr.nchan // fr.chanavg
+                if k < nrec:
+                    a = st.autos[w, k]
+                    av = a.reshape(-1, cavg).mean(1) if cavg > 1 else a
+                    res[idx: idx + nout] += av[:nout]
+                else:
+                    lz, pol = ds.zoombands[k - nrec]
+                    zfi, parent_lf, choff = ds.zoomfreqs[lz]
+                    parent = band_record_index(ds, k)
+                    a = st.autos[w, parent]
+                    av = a.reshape(-1, cavg).mean(1) if cavg > 1 else a
+                    res[idx: idx + nout] += av[choff // cavg: choff // cavg + nout]
+                idx += nout
+        widx = lay.acweight_off[d] * 2
+        for w in range(lay.bandsperautocorr):
+            if w == 1 and not cfg.write_cross_autocorrs:
+                break
+            for k in range(nrec + len(ds.zoombands)):
+                if k < nrec:
+                    fi = ds.recfreqs[ds.recbands[k][0]]
+                else:
+                    fi = ds.zoomfreqs[ds.zoombands[k - nrec][0]][0]
+                if not (used[fi] or equiv[fi]):
+                    continue
+                fres[widx] += st.acweight[w, band_record_index(ds, k)]
+                widx += 1
+    return res, stations
+
+
+def normalise(cfg, res: np.ndarray, lay: Layout, meansubints: float = 1.0) -> np.ndarray:
+    """Visibility::writedata's scale without Tsys (visibility.cpp:407-415, 526-527):
+    vis / (bweight/fftsperintegration * meansubints * blockspersend * 2 * nchan * chanavg)
+    with fftsperintegration = blockspersend*meansubints, for cross-correlations."""
+    out = res.copy()
+    B = cfg.blockspersend()
+    fres = res.view(np.float64)
+    for (fi, j), off in lay.baseline_off.items():
+        fr = cfg.freqs[fi]
+        prods = dict(cfg.baselines[j].freqs)[fi]
+        nout = (fr.nchan + fr.chanavg - 1) // fr.chanavg
+        for p in range(len(prods)):
+            bw = fres[lay.bweight_off[(fi, j)] * 2 + p] / (B * meansubints)
+            sc = 0.0 if bw <= 0 else 1.0 / (bw * meansubints * B * 2 * fr.nchan * fr.chanavg)
+            out[off + p * nout: off + (p + 1) * nout] *= sc
+    return out

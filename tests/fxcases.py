@@ -1,0 +1,56 @@
+"""Small seeded cases shaped like the benchmark configurations, and parity metrics."""
+from __future__ import annotations
+
+import numpy as np
+
+from difx_amd import config as cf
+from difx_amd import synth
+
+
+def case_cfg0(B: int = 128, xengine: str = "fp32", lsb: bool = False, seed: int = 1, **kw):
+    """4 stations, single pol, 1 x 32 MHz real 2-bit VDIF, nchan 256 (cfg0 shape)."""
+    return cf.standard_config(4, 1, [4800.0], 32.0, 256, subint_ns=B * 8000, xengine=xengine,
+                              lsb=lsb, seed=seed, **kw)
+
+
+def case_cfg1(B: int = 32, xengine: str = "fp32", nstations: int = 10, nfreq: int = 8, seed: int = 2, **kw):
+    """10 stations, dual pol, 8 x 32 MHz, 16 bands per 32-bit VDIF word, nchan 1024."""
+    freqs = [4800.0 + 40.0 * k for k in range(nfreq)]
+    return cf.standard_config(nstations, 2, freqs, 32.0, 1024, subint_ns=B * 32000,
+                              xengine=xengine, seed=seed, **kw)
+
+
+def case_cfg2(B: int = 16, xengine: str = "fp16", nstations: int = 64, seed: int = 3, **kw):
+    """64 stations, single pol, 16 MHz complex 8-bit, nchan 4096."""
+    return cf.standard_config(nstations, 1, [1400.0], 16.0, 4096, subint_ns=B * 256000,
+                              nbits=8, complex_sampling=True, xengine=xengine, seed=seed, **kw)
+
+
+def slot(cfg, offsets=(0, 1, 0), seed: int = 11):
+    return synth.noise_slot(cfg, list(offsets), seed=seed)
+
+
+def record_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
+    """Per-record errors in the diffDiFX convention (utilities/vis2screen/diffDiFX.py:137-151:
+    mean|got-ref| / mean|ref| over the channels of one record) for every baseline product
+    and every datastream's autocorrelation block."""
+    out = {"cross": [], "auto": []}
+    nb = len(cfg.baselines)
+    for (fi, j), off in lay.baseline_off.items():
+        fr = cfg.freqs[fi]
+        prods = dict(cfg.baselines[j].freqs)[fi]
+        nout = (fr.nchan + fr.chanavg - 1) // fr.chanavg
+        for p in range(len(prods)):
+            g = got[off + p * nout: off + (p + 1) * nout]
+            r = ref[off + p * nout: off + (p + 1) * nout]
+            den = np.mean(np.abs(r))
+            out["cross"].append(float(np.mean(np.abs(g - r)) / den) if den > 0 else float(np.max(np.abs(g))))
+    for d, ds in enumerate(cfg.datastreams):
+        o = lay.autocorr_off[d]
+        end = lay.autocorr_off[d + 1] if d + 1 < len(cfg.datastreams) else lay.acweight_off[0]
+        n = end - o
+        g, r = got[o: o + n], ref[o: o + n]
+        den = np.mean(np.abs(r))
+        if den > 0:
+            out["auto"].append(float(np.mean(np.abs(g - r)) / den))
+    return out

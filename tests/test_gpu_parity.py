@@ -1,0 +1,171 @@
+"""Device parity of the FX path against the FP64 oracle, through the C ABI.
+
+Bars: unpack bit-exact; delay interpolator to FP64 rounding; FP32 path visibilities within
+1e-5 (diffDiFX convention, mean|dV|/mean|V| per record); FP16 MFMA path within 1e-3 and
+BF16 within 1e-2 in the same metric; weights exact to FP32 summation.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import fxoracle as fo
+from difx_amd import synth
+from fxcases import case_cfg0, case_cfg1, case_cfg2, record_errors, slot
+
+pytestmark = pytest.mark.gpu
+
+
+def _core(cfg):
+    from difx_amd.core import FxCore
+    return FxCore(cfg)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("kind", ["vdif1", "vdif16", "mark5b8", "cplx8"])
+def test_unpack_bitexact(gpu, kind):
+    import torch
+    if kind == "vdif1":
+        cfg = case_cfg0(B=128)
+    elif kind == "vdif16":
+        cfg = case_cfg1(B=32, nstations=2)
+    elif kind == "mark5b8":
+        cfg = case_cfg1(B=32, nstations=2, nfreq=4)
+        for d in cfg.datastreams:
+            d.format, d.headerbytes, d.payloadbytes = "MARK5B", 16, 10000
+    else:
+        cfg = case_cfg2(B=16, nstations=2)
+    _say("unpack case", kind)
+    core = _core(cfg)
+    data, ctrl = slot(cfg, seed=5)
+    buf = data[0].copy()
+    ds = cfg.datastreams[0]
+    # invalidate frame 1 (invalid bit / Mark5B flag) and put fill pattern in frame 2's payload
+    fb, hb = ds.framebytes, ds.headerbytes
+    if ds.format == "VDIF":
+        buf[fb + 3] |= 0x80
+    else:
+        buf[fb + 5] |= 0x80
+    fill = np.frombuffer(np.uint64(0x1122334411223344).tobytes(), np.uint8)
+    buf[2 * fb + hb: 2 * fb + hb + 8] = fill
+    vals, valid = fo.decode_stream(buf, ds)
+    nsamp = vals.shape[1]
+    start, n = 3, min(nsamp - 3, 50000)
+    d_buf = torch.from_numpy(buf).to(gpu)
+    nb = len(ds.recbands)
+    out = torch.zeros(nb * n * (2 if ds.complex_sampling else 1), dtype=torch.float32, device=gpu)
+    good = torch.zeros(1, dtype=torch.int32, device=gpu)
+    core.unpack_device(0, d_buf.data_ptr(), len(buf), start, n, out.data_ptr(), good.data_ptr())
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    if ds.complex_sampling:
+        got = got.view(np.complex64).reshape(nb, n)
+    else:
+        got = got.reshape(nb, n)
+    ref = vals[:, start: start + n]
+    assert got.dtype == ref.dtype
+    np.testing.assert_array_equal(got.view(np.uint32) if not ds.complex_sampling else got.view(np.uint64),
+                                  ref.view(np.uint32) if not ds.complex_sampling else ref.view(np.uint64))
+    assert int(good.cpu()[0]) == int(valid[start: start + n].sum())
+    core.close()
+
+
+def test_delay_interpolator(gpu):
+    cfg = case_cfg0(B=128)
+    core = _core(cfg)
+    B = cfg.blockspersend()
+    for d, ds in enumerate(cfg.datastreams):
+        for sec, ns in [(1, 0), (7, 123456789), (60, 500000000)]:
+            got = core.calculateDelayInterpolator(d, sec, ns)
+            C = cfg.freqs[ds.recfreqs[0]].nchan
+            ref = fo.delay_interpolator(ds.model, ds.model_order, cfg.model_inc_secs,
+                                        sec + ns / 1e9, B * C / cfg.freqs[0].bw_mhz / 1e6, B)
+            ref[2] -= 1e6 * fo.intclockseconds(ds)
+            # a0 and a1 come from differences of delays of size |d0|: FP64 rounding of those
+            # differences bounds the agreement (scale-aware absolute tolerance)
+            tol = 1e-13 * max(abs(ref[2]), 1.0) * np.array([1.0 / B ** 2, 1.0 / B, 1.0])
+            assert np.all(np.abs(got - ref) <= tol), (got, ref)
+    core.close()
+
+
+def _run_and_compare(cfg, seed=11, offsets=(0, 1, 0), spectra_check=True):
+    core = _core(cfg)
+    data, ctrl = slot(cfg, offsets, seed=seed)
+    got = core.processdata(offsets, data, ctrl)
+    ref, stations = fo.process_subint(cfg, list(offsets), data, ctrl)
+    lay = fo.result_layout(cfg)
+    assert core.coreresultlength == lay.coreresultlength
+    b, w, a, aw = core.result_offsets()
+    for (fi, j), off in lay.baseline_off.items():
+        assert b[fi, j] == off
+    assert list(a) == lay.autocorr_off and list(aw) == lay.acweight_off
+    # per-FFT data weights
+    for d in range(len(cfg.datastreams)):
+        np.testing.assert_allclose(core.getDataWeights(d), stations[d].weight, rtol=0, atol=1e-7)
+    if spectra_check and cfg.xengine == "fp32":
+        B = cfg.blockspersend()
+        for d in range(min(2, len(cfg.datastreams))):
+            for i in (0, B // 2, B - 1):
+                F = core.getFreqs(d, 0, i)
+                R = stations[d].spectra[0, i]
+                err = np.max(np.abs(F - R)) / max(np.max(np.abs(R)), 1e-30)
+                assert err < 2e-5, (d, i, err)
+    e = record_errors(cfg, lay, got.astype(np.complex128), ref)
+    core.close()
+    return got, ref, e, lay
+
+
+def test_fp32_cfg0(gpu):
+    cfg = case_cfg0(B=128)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say("cfg0 fp32 cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < 1e-5
+    assert max(e["auto"]) < 1e-5
+    # weights: all FFTs valid -> bweight = B, acweight = B
+    fres = got.view(np.float32)
+    B = cfg.blockspersend()
+    for (fi, j), off in lay.bweight_off.items():
+        assert abs(fres[2 * off] - B) < 1e-3
+
+
+def test_fp32_cfg0_lsb(gpu):
+    cfg = case_cfg0(B=64, lsb=True)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say("cfg0 lsb fp32 cross max", max(e["cross"]))
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+
+
+def test_fp32_cfg1(gpu):
+    cfg = case_cfg1(B=32)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say("cfg1 fp32 cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+
+
+@pytest.mark.parametrize("xe,bound", [("fp16", 1e-3), ("bf16", 1e-2)])
+def test_mfma_cfg1(gpu, xe, bound):
+    cfg = case_cfg1(B=32, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"cfg1 {xe} cross max", max(e["cross"]), "median", float(np.median(e["cross"])),
+         "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
+@pytest.mark.parametrize("xe,bound", [("fp16", 1e-3), ("bf16", 1e-2)])
+def test_mfma_cfg2(gpu, xe, bound):
+    cfg = case_cfg2(B=16, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"cfg2 {xe} cross max", max(e["cross"]), "median", float(np.median(e["cross"])),
+         "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
+def test_fp32_cfg2(gpu):
+    cfg = case_cfg2(B=8, xengine="fp32", nstations=8)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say("cfg2 fp32 cross max", max(e["cross"]))
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
