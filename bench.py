@@ -1,0 +1,270 @@
+#!/usr/bin/env python3
+"""Benchmark of the FX hot path on one MI355X: input Gsamples/s and subint latency.
+
+Workload (one "step" = one subintegration through the whole per-subint pipeline):
+10 stations, dual circular polarisation, 8 subbands x 32 MHz, 2-bit real VDIF with 16 bands
+per 32-bit sample word (8000-byte payloads), nchan 1024 (2048-point FFTs), 0.5 s subints
+(15625 FFT blocks per station), fringe rotation order 1, seeded quintic delay models.
+Synthetic 2-bit data is generated on the device before the timed region (inputs resident
+in HBM), so `value` excludes PCIe transfers.
+
+Two paths are timed on the same inputs:
+  headline  fused F-engine + FP16 MFMA Hermitian X-engine   -> `value`
+  plain     fused F-engine (FP32 spectra) + FP32 CMAC X-engine per baseline product
+
+The roofline object describes the headline path's dominant kernel, timed with HIP events
+on its own stream inside the library (fxg_set_timing).  The CPU baseline runs the C
+restatement in oracle/ (when built) or the numpy oracle on a bounded sample of the same
+workload on the host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_config(xengine: str):
+    from difx_amd import config as cf
+    freqs = [4800.0 + 40.0 * k for k in range(8)]
+    return cf.standard_config(10, 2, freqs, 32.0, 1024, subint_ns=500_000_000,
+                              xengine=xengine, seed=20261202)
+
+
+def device_slot(cfg, offsets, torch, dev, seed=7):
+    """VDIF buffers generated on the device: random 2-bit payloads and valid headers."""
+    from difx_amd import synth
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    bufs, ctrl, nbytes = [], [], []
+    for d, ds in enumerate(cfg.datastreams):
+        start_ns = synth.buffer_start_ns(cfg, d, offsets)
+        nb = synth.data_bytes(cfg, d)
+        nframes = nb // ds.framebytes
+        buf = torch.randint(0, 256, (nframes, ds.framebytes), dtype=torch.uint8, device=dev, generator=g)
+        hdr = np.stack([synth.vdif_header(0, f, ds.framebytes, len(ds.recbands), ds.nbits, False)
+                        for f in range(nframes)])
+        buf[:, : ds.headerbytes] = torch.from_numpy(hdr).to(dev)
+        bufs.append(buf.reshape(-1).contiguous())
+        nbytes.append(nframes * ds.framebytes)
+        ctrl.append(synth.control_words(cfg, d, offsets, start_ns))
+    return bufs, nbytes, ctrl
+
+
+def samples_per_subint(cfg) -> float:
+    tot = 0.0
+    for d, ds in enumerate(cfg.datastreams):
+        f = cfg.freqs[ds.recfreqs[0]]
+        rate = (1.0 if ds.complex_sampling else 2.0) * f.bw_mhz * 1e6
+        tot += len(ds.recbands) * rate * cfg.subint_ns * 1e-9
+    return tot
+
+
+def f_engine_bytes_per_launch(cfg, nffts: int, out_bytes_per_chan: int) -> float:
+    """Algorithmic HBM bytes of one fused F-engine launch over `nffts` FFT blocks: the 2-bit
+    payload read once (L samples x nbands x 2 bits) plus the kept channels written once."""
+    tot = 0.0
+    for d, ds in enumerate(cfg.datastreams):
+        C = cfg.freqs[ds.recfreqs[0]].nchan
+        L = C if ds.complex_sampling else 2 * C
+        nb = len(ds.recbands)
+        bits = ds.nbits * (2 if ds.complex_sampling else 1)
+        tot += nffts * (L * nb * bits / 8.0 + nb * C * out_bytes_per_chan)
+    return tot
+
+
+def x_engine_flops(cfg, B: int) -> float:
+    """Useful X-engine flops per subint: 8 per complex multiply-accumulate of every
+    baseline product and autocorrelation product (core.cpp:963, mode.cpp:1272,1334)."""
+    tot = 0
+    for b in cfg.baselines:
+        for fi, prods in b.freqs:
+            tot += len(prods) * cfg.freqs[fi].nchan
+    for ds in cfg.datastreams:
+        C = cfg.freqs[ds.recfreqs[0]].nchan
+        tot += len(ds.recbands) * C * (2 if cfg.write_cross_autocorrs else 1)
+    return 8.0 * tot * B
+
+
+def time_path(cfg, steps, warmup, torch, dev, offsets):
+    from difx_amd.core import FxCore
+    core = FxCore(cfg)
+    bufs, nbytes, ctrl = device_slot(cfg, offsets, torch, dev)
+    res = torch.zeros(2 * core.coreresultlength, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    st = torch.cuda.Stream(device=dev)     # a real stream handle (the null stream is handle 0)
+    sh = st.cuda_stream
+    ptrs = [b.data_ptr() for b in bufs]
+    for _ in range(warmup):
+        core.processdata_device(offsets, ptrs, nbytes, ctrl, res.data_ptr(), sh)
+    torch.cuda.synchronize()
+    core.set_timing(True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        evs[k][0].record(st)
+        core.processdata_device(offsets, ptrs, nbytes, ctrl, res.data_ptr(), sh)
+        evs[k][1].record(st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lat = [a.elapsed_time(b) for a, b in evs]
+    tm = core.timing()
+    out = res.cpu().numpy().view(np.complex64).copy()
+    info = {"wall_s": t1 - t0, "lat_ms": lat, "timing": tm, "chunkcap": None,
+            "B": core.blockspersend, "results": out}
+    core.set_timing(False)
+    core.close()
+    del bufs, res
+    torch.cuda.empty_cache()
+    return info
+
+
+def cpu_baseline(budget_s: float = 15.0):
+    """Time the CPU restatement on a bounded sample of the workload: all 10 stations x 16
+    bands x all 45 baselines, for as many FFT blocks as fit the time budget."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    from difx_amd import config as cf, synth
+    try:
+        import fxport  # C restatement (oracle/fxport.c), multithreaded
+        have_port = fxport.available()
+    except Exception:
+        have_port = False
+    freqs = [4800.0 + 40.0 * k for k in range(8)]
+    offsets = [0, 1, 0]
+    if have_port:
+        nthreads = fxport.default_threads()
+        nblk = 64
+        while True:
+            cfg = cf.standard_config(10, 2, freqs, 32.0, 1024, subint_ns=nblk * 32000, seed=20261202)
+            data, ctrl = synth.noise_slot(cfg, offsets, seed=3)
+            t0 = time.perf_counter()
+            fxport.process_subint(cfg, offsets, data, ctrl, nthreads=nthreads)
+            dt = time.perf_counter() - t0
+            if dt > budget_s / 4 or nblk >= 4096:
+                break
+            nblk *= 2
+        samples = samples_per_subint(cfg)
+        return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": nthreads,
+                "kind": "port",
+                "sample": f"C restatement (oracle/fxport.c), {nblk} FFT blocks of the 10-station "
+                          f"dual-pol 8x32 MHz nchan-1024 workload ({samples/1e6:.0f} Msamples), "
+                          f"{nthreads} threads, {dt:.2f} s"}
+    import fxoracle as fo
+    nblk = 4
+    cfg = cf.standard_config(10, 2, freqs, 32.0, 1024, subint_ns=nblk * 32000, seed=20261202)
+    data, ctrl = synth.noise_slot(cfg, offsets, seed=3)
+    t0 = time.perf_counter()
+    fo.process_subint(cfg, offsets, data, ctrl)
+    dt = time.perf_counter() - t0
+    samples = samples_per_subint(cfg)
+    return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": 1, "kind": "port",
+            "sample": f"numpy FP64 oracle, {nblk} FFT blocks of the workload "
+                      f"({samples/1e6:.1f} Msamples), 1 thread, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the FX path on a single MI355X; "
+              "multi-GPU sharding (whole subints or subband groups per GPU) is not part of it.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("no GPU available", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    offsets = [0, 1, 0]
+    cfg = workload_config("fp16")
+    nsamp = samples_per_subint(cfg)
+    print(f"[bench] headline path (F + FP16 MFMA X), {args.warmup} warmup + {args.steps} steps", file=sys.stderr, flush=True)
+    head = time_path(cfg, args.steps, args.warmup, torch, dev, offsets)
+    B = head["B"]
+    value = nsamp * args.steps / head["wall_s"] / 1e9
+    lat = np.array(head["lat_ms"])
+    tm = head["timing"]
+    f_avg_ms = tm["f_ms"] / max(tm["f_launches"], 1)
+    nchunks_per_step = tm["f_launches"] / args.steps
+    ffts_per_launch = B / nchunks_per_step
+    fbytes = f_engine_bytes_per_launch(cfg, ffts_per_launch, 4)
+    achieved = fbytes / (f_avg_ms / 1e3) / 1e9
+    x_avg_ms = tm["x_ms"] / max(tm["x_launches"], 1)
+    plain = None
+    if not args.no_plain:
+        pcfg = workload_config("fp32")
+        psteps = max(1, min(args.steps, 3))
+        print(f"[bench] plain FP32 path, 1 warmup + {psteps} steps", file=sys.stderr, flush=True)
+        p = time_path(pcfg, psteps, 1, torch, dev, offsets)
+        plat = np.array(p["lat_ms"])
+        pval = nsamp * psteps / p["wall_s"] / 1e9
+        ptm = p["timing"]
+        # agreement of the two paths on identical inputs (diffDiFX metric over the cross records)
+        a, b = head["results"], p["results"]
+        rel = float(np.mean(np.abs(a - b)) / max(np.mean(np.abs(b)), 1e-30))
+        plain = {"value": pval, "unit": "Gsamples/s", "p50_ms": float(np.percentile(plat, 50)),
+                 "p99_ms": float(np.percentile(plat, 99)),
+                 "f_ms_per_step": ptm["f_ms"] / psteps, "x_ms_per_step": ptm["x_ms"] / psteps,
+                 "speedup_headline_vs_plain": value / pval,
+                 "headline_vs_plain_rel_diff": rel}
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+    realtime = nsamp / (cfg.subint_ns * 1e-9) / 1e9
+    line = {
+        "metric": "input Gsamples/s (all stations) + p99 subint latency; MFMA vs plain FP32 path",
+        "value": value,
+        "unit": "Gsamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall_s"] * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 F-engine, fp16 X-engine inputs, f32 accumulate",
+        "data": "synthetic (device-generated 2-bit VDIF, seeded quintic delay models)",
+        "config": {"workload": "10 stations dual-pol, 8 x 32 MHz 2-bit VDIF, nchan 1024, 0.5 s subint",
+                   "stations": 10, "pols": 2, "subbands": 8, "bw_mhz": 32, "nchan": 1024,
+                   "subint_s": 0.5, "fft_blocks_per_subint": B, "parallelism": "single GPU"},
+        "p50_subint_latency_ms": float(np.percentile(lat, 50)),
+        "p99_subint_latency_ms": float(np.percentile(lat, 99)),
+        "subint_length_ms": cfg.subint_ns / 1e6,
+        "realtime_factor": value / realtime,
+        "kernels": {"f_engine_ms_per_step": tm["f_ms"] / args.steps,
+                    "x_engine_ms_per_step": tm["x_ms"] / args.steps,
+                    "other_ms_per_step": tm["other_ms"] / args.steps,
+                    "f_launches_per_step": nchunks_per_step,
+                    "x_engine_useful_tflops": x_engine_flops(cfg, B) / (tm["x_ms"] / args.steps / 1e3) / 1e12},
+        "roofline": {"kernel": "k_fengine<2048,false> (fused unpack+rotate+FFT+post-F)",
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": f_avg_ms, "algorithmic_bytes_per_launch": fbytes},
+        "plain_fp32": plain,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

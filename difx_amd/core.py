@@ -60,6 +60,8 @@ def load_library() -> C.CDLL:
     lib.fxg_unpack_device.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int64,
                                       C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.fxg_delay_interpolator.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]
+    lib.fxg_set_timing.argtypes = [C.c_void_p, C.c_int32]
+    lib.fxg_get_timing.argtypes = [C.c_void_p, P(abi.fxg_timing)]
     _LIB = lib
     return lib
 
@@ -187,6 +189,16 @@ class FxCore:
         out = (C.c_double * 3)()
         self._check(self.lib.fxg_delay_interpolator(self.h, ds, sec, ns, out))
         return np.array(out[:])
+
+    def set_timing(self, enable: bool):
+        self._check(self.lib.fxg_set_timing(self.h, int(enable)))
+
+    def timing(self) -> dict:
+        t = abi.fxg_timing()
+        self._check(self.lib.fxg_get_timing(self.h, C.byref(t)))
+        return {"f_ms": t.f_ms, "f_launches": t.f_launches, "x_ms": t.x_ms,
+                "x_launches": t.x_launches, "other_ms": t.other_ms,
+                "other_launches": t.other_launches}
 
     def unpack_device(self, ds: int, d_data: int, nbytes: int, startsample: int, nsamples: int,
                       d_out: int, d_good: int, stream: int = 0):

@@ -19,6 +19,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -87,6 +88,7 @@ struct fxg_ctx {
   long long fvalid_total = 0;
   int maxframes = 0;
   float2* d_twiddle = nullptr;
+  FftDelay* d_params = nullptr;
   float* d_weights = nullptr;
   int32_t* d_flags = nullptr;
   // chunk buffers
@@ -124,6 +126,10 @@ struct fxg_ctx {
   Slot slots[RING];
   int last_chunk0 = 0, last_nffts = 0;
   std::mutex mu;
+  // optional launch timing: (category, start, stop) event triples
+  bool timing = false;
+  std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> tev;
+  std::vector<hipEvent_t> evpool;
 };
 
 namespace {
@@ -256,6 +262,10 @@ int validate_and_derive(fxg_ctx* ctx) {
       if (f.nchan % f.chanavg) return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
     }
   }
+  if (ctx->xengine != FXG_X_FP32)
+    for (int d = 0; d < ctx->nds; ++d)
+      if (ctx->freqs[ctx->dss[d].recfreq_index[0]].nchan % 16)
+        return fail(ctx, FXG_E_INVALID, "the MFMA X-engine needs nchan to be a multiple of 16");
   if (ctx->xengine < FXG_X_FP32 || ctx->xengine > FXG_X_BF16)
     return fail(ctx, FXG_E_INVALID, "unknown X-engine");
   return FXG_OK;
@@ -456,13 +466,23 @@ int build_tables(fxg_ctx* ctx) {
   if ((rc = dev_upload(ctx, &ctx->d_chanfreq, chanfreq))) return rc;
   HIPCHK(hipMalloc((void**)&ctx->d_fvalid, std::max<long long>(fvo, 16)));
   HIPCHK(hipMemset(ctx->d_fvalid, 0, std::max<long long>(fvo, 16)));
-  // FFT twiddles exp(-2 pi i m / L), m < L/2, from FP64
+  // FFT twiddles of the Stockham passes after the first, per pass (R, Ns): entries
+  // exp(-2 pi i k r / (Ns R)) for r = 1..R-1, k = 0..Ns-1, k fastest; computed in FP64
   {
-    std::vector<float2> tw(ctx->L / 2);
-    for (int m = 0; m < ctx->L / 2; ++m) {
-      double a = -TWO_PI * (double)m / (double)ctx->L;
-      tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
+    int log = 0;
+    while ((1 << log) < ctx->L) ++log;
+    const int n8 = log / 3, rem = log % 3;
+    std::vector<float2> tw;
+    auto add_pass = [&](int R, int Ns) {
+      for (int r = 1; r < R; ++r)
+        for (int k = 0; k < Ns; ++k) {
+          const double a = -TWO_PI * (double)k * (double)r / ((double)Ns * (double)R);
+          tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+        }
+    };
+    int ns = 8;
+    for (int p = 1; p < n8; ++p) { add_pass(8, ns); ns *= 8; }
+    if (rem) add_pass(1 << rem, ns);
     if ((rc = dev_upload(ctx, &ctx->d_twiddle, tw))) return rc;
   }
   HIPCHK(hipMalloc((void**)&ctx->d_run, sizeof(DevDsRun) * nds));
@@ -518,11 +538,13 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
   cap = std::min<long long>(cap, round_up(ctx->B, FXG_XPAD));
   cap = round_up(cap, FXG_XPAD);
   ctx->chunkcap = (int)cap;
+  HIPCHK(hipMalloc((void**)&ctx->d_params, sizeof(FftDelay) * (size_t)nds * cap));
   if (mfma) {
     long long off = 0, aoff = 0, soff = 0;
     for (auto& g : ctx->groups) {
       g.spec_off = off;
-      off += (long long)g.ntile * 32 * (cap / FXG_TB) * g.C * FXG_TB;
+      g.rowstride = cap * g.C + 32;       // one 128-byte line of padding per row
+      off += (long long)g.ntile * 32 * g.rowstride;
       g.acc_off = aoff;
       aoff += (long long)g.C * (g.ntile * (g.ntile + 1) / 2) * 2 * 1024;
       g.scale_off = soff;
@@ -723,6 +745,36 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
   return FXG_OK;
 }
 
+hipEvent_t pool_event(fxg_ctx* ctx) {
+  if (!ctx->evpool.empty()) {
+    hipEvent_t e = ctx->evpool.back();
+    ctx->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch with timing events when timing is enabled.
+struct TimedLaunch {
+  fxg_ctx* ctx;
+  int cat;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimedLaunch(fxg_ctx* c, int category, hipStream_t s) : ctx(c), cat(category), st(s) {
+    if (!ctx->timing) return;
+    a = pool_event(ctx);
+    b = pool_event(ctx);
+    if (a) (void)hipEventRecord(a, st);
+  }
+  ~TimedLaunch() {
+    if (!ctx->timing || !a || !b) return;
+    (void)hipEventRecord(b, st);
+    ctx->tev.emplace_back(cat, a, b);
+  }
+};
+
 int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_data, const int32_t* nbytes,
                const int32_t* const* control, float2* d_results, hipStream_t st) {
   const int nds = ctx->nds;
@@ -730,6 +782,8 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     if (!control || !control[d]) return fail(ctx, FXG_E_INVALID, "missing control buffer");
     if (nbytes[d] > 1 && !d_data[d]) return fail(ctx, FXG_E_INVALID, "missing data buffer");
     if (nbytes[d] < 0) return fail(ctx, FXG_E_INVALID, "negative data length");
+    if ((long long)(nbytes[d] / ctx->dss[d].framebytes) * ctx->h_ds[d].spf >= (1LL << 31))
+      return fail(ctx, FXG_E_INVALID, "slot buffer holds more than 2^31 samples");
     const long long capframes = (ctx->databytes[d] + 4LL * ctx->dss[d].framebytes) / ctx->dss[d].framebytes;
     if (nbytes[d] / ctx->dss[d].framebytes > capframes)
       return fail(ctx, FXG_E_INVALID, "data buffer larger than the slot capacity");
@@ -780,13 +834,17 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   sa.offns = offsets[2];
   sa.model_inc_secs = ctx->model_inc;
   sa.B = ctx->B;
-  if (fxg_launch_setup(sa, std::max(maxframes, 1), st)) return fail(ctx, FXG_E_HIP, "setup launch");
+  {
+    TimedLaunch tl(ctx, 2, st);
+    if (fxg_launch_setup(sa, std::max(maxframes, 1), st)) return fail(ctx, FXG_E_HIP, "setup launch");
+  }
 
   FengineArgs fa;
   std::memset(&fa, 0, sizeof(fa));
   fa.ds = ctx->d_ds;
   fa.run = ctx->d_run;
   fa.twiddle = ctx->d_twiddle;
+  fa.params = ctx->d_params;
   fa.weights = ctx->d_weights;
   fa.B = ctx->B;
   fa.offsec = offsets[1];
@@ -812,7 +870,10 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     long long wgs_per_ds = std::max(1LL, (long long)(2048 + nds - 1) / nds);
     int bpw = (int)std::max(1LL, std::min<long long>(16, nblocks / wgs_per_ds));
     fa.blocks_per_wg = bpw;
-    if (fxg_launch_fengine(fa, ctx->L, ctx->iscomplex, nds, st)) return fail(ctx, FXG_E_HIP, "F-engine launch");
+    {
+      TimedLaunch tl(ctx, 0, st);
+      if (fxg_launch_fengine(fa, ctx->L, ctx->iscomplex, nds, st)) return fail(ctx, FXG_E_HIP, "F-engine launch");
+    }
     if (mfma) {
       XMfmaArgs xa;
       xa.groups = ctx->d_groups;
@@ -821,9 +882,18 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
       xa.nffts_pad = npad;
       xa.chunkcap = ctx->chunkcap;
       xa.mode = ctx->xengine;
-      for (size_t g = 0; g < ctx->groups.size(); ++g)
-        if (fxg_launch_xmfma(xa, 1, (int)g, ctx->groups[g].ntile, ctx->groups[g].C, st))
+      TimedLaunch tl(ctx, 1, st);
+      // one launch per run of groups sharing tile count and channel count
+      size_t g0 = 0;
+      while (g0 < ctx->groups.size()) {
+        size_t g1 = g0 + 1;
+        while (g1 < ctx->groups.size() && ctx->groups[g1].ntile == ctx->groups[g0].ntile &&
+               ctx->groups[g1].C == ctx->groups[g0].C)
+          ++g1;
+        if (fxg_launch_xmfma(xa, (int)(g1 - g0), (int)g0, ctx->groups[g0].ntile, ctx->groups[g0].C, st))
           return fail(ctx, FXG_E_HIP, "MFMA X-engine launch");
+        g0 = g1;
+      }
     } else {
       XPlainArgs xa;
       xa.prods = ctx->d_prods;
@@ -832,12 +902,14 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
       xa.acc = ctx->d_xacc;
       xa.nffts = n;
       xa.chunkcap = ctx->chunkcap;
+      TimedLaunch tl(ctx, 1, st);
       if (fxg_launch_xplain(xa, (int)ctx->prods.size(), ctx->maxchan, st))
         return fail(ctx, FXG_E_HIP, "X-engine launch");
     }
     ctx->last_chunk0 = c0;
     ctx->last_nffts = n;
   }
+  TimedLaunch tl(ctx, 2, st);
   if (mfma) {
     if (fxg_launch_fin_mfma(ctx->d_finm, (int)ctx->finm.size(), ctx->maxoutm, ctx->d_groups, ctx->d_vacc,
                             ctx->d_xscale, d_results, st))
@@ -854,7 +926,7 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
 void free_ctx(fxg_ctx* ctx) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid);
-  f(ctx->d_twiddle); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
+  f(ctx->d_twiddle); f(ctx->d_params); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   for (auto& sg : ctx->stage) {
@@ -870,6 +942,11 @@ void free_ctx(fxg_ctx* ctx) {
     if (s.copied) (void)hipEventDestroy(s.copied);
     if (s.done) (void)hipEventDestroy(s.done);
   }
+  for (auto& t : ctx->tev) {
+    (void)hipEventDestroy(std::get<1>(t));
+    (void)hipEventDestroy(std::get<2>(t));
+  }
+  for (auto e : ctx->evpool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
@@ -1087,15 +1164,15 @@ int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, 
   }
   const DevXGroup& g = ctx->groups[x.band_xfreq[band]];
   const int row = x.band_xrow[band];
-  std::vector<uint32_t> blk((size_t)C * FXG_TB);
+  std::vector<uint32_t> blk((size_t)C);
   std::vector<float> sc(C);
-  const uint32_t* src = ctx->d_spec16 + g.spec_off + (((long long)row * (ctx->chunkcap / FXG_TB) + lt / FXG_TB) * C) * FXG_TB;
+  const uint32_t* src = ctx->d_spec16 + g.spec_off + (long long)row * g.rowstride + (long long)lt * C;
   if (hipMemcpy(blk.data(), src, sizeof(uint32_t) * blk.size(), hipMemcpyDeviceToHost) != hipSuccess) return FXG_E_HIP;
   if (hipMemcpy(sc.data(), ctx->d_xscale + g.scale_off + (long long)row * C, sizeof(float) * C,
                 hipMemcpyDeviceToHost) != hipSuccess)
     return FXG_E_HIP;
   for (int ch = 0; ch < C; ++ch) {
-    uint32_t u = blk[(size_t)ch * FXG_TB + lt % FXG_TB];
+    uint32_t u = blk[(size_t)ch];
     float re, im;
     if (ctx->xengine == FXG_X_FP16) {
       __half2 h;
@@ -1189,6 +1266,36 @@ int fxg_delay_interpolator(fxg_ctx* ctx, int32_t ds, int32_t sec, int32_t ns, do
   HIPCHK(hipStreamSynchronize(st));
   if (!back.valid) return fail(ctx, FXG_E_INVALID, "time outside the delay model");
   std::memcpy(out, back.interp, sizeof(double) * 3);
+  return FXG_OK;
+}
+
+int fxg_set_timing(fxg_ctx* ctx, int32_t enable) {
+  if (!ctx) return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  HIPCHK(hipDeviceSynchronize());
+  for (auto& t : ctx->tev) {
+    ctx->evpool.push_back(std::get<1>(t));
+    ctx->evpool.push_back(std::get<2>(t));
+  }
+  ctx->tev.clear();
+  ctx->timing = enable != 0;
+  return FXG_OK;
+}
+
+int fxg_get_timing(fxg_ctx* ctx, fxg_timing* out) {
+  if (!ctx || !out) return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  std::memset(out, 0, sizeof(*out));
+  for (auto& t : ctx->tev) {
+    HIPCHK(hipEventSynchronize(std::get<2>(t)));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, std::get<1>(t), std::get<2>(t)));
+    switch (std::get<0>(t)) {
+      case 0: out->f_ms += ms; out->f_launches++; break;
+      case 1: out->x_ms += ms; out->x_launches++; break;
+      default: out->other_ms += ms; out->other_launches++; break;
+    }
+  }
   return FXG_OK;
 }
 

@@ -70,12 +70,9 @@ __device__ __forceinline__ void dft8(float2* v) {
   v[3] = cadd(w6, w7); v[7] = csub(w6, w7);
 }
 
-// Twiddle exp(-2 pi i m / L) from the half table tw[m], m < L/2 (tw[m + L/2] = -tw[m]).
-template <int L>
-__device__ __forceinline__ float2 twid(const float2* tw, int m) {
-  float2 w = tw[m & (L / 2 - 1)];
-  return (m & (L / 2)) ? make_float2(-w.x, -w.y) : w;
-}
+// LDS exchange index with one pad element per 8: conflict-free stores for the first two
+// Stockham passes and at most 3-way bank sharing for the strided-by-thread loads.
+__device__ __forceinline__ int padx(int e) { return e + (e >> 3); }
 
 template <int L>
 struct Plan {
@@ -84,10 +81,17 @@ struct Plan {
   static constexpr int N8 = LOG / 3;      // radix-8 passes
   static constexpr int REM = LOG % 3;     // 0: none, 1: radix-2 pass, 2: radix-4 pass
   static constexpr int NT = L / 8;
+  // Per-pass twiddle tables, laid out [pass][r-1][k] with k (the butterfly's position in its
+  // span Ns) fastest, so lanes of a wavefront read consecutive entries.  Pass 0 needs none.
+  static constexpr int pow8(int p) { int v = 1; for (int q = 0; q < p; ++q) v *= 8; return v; }
+  static constexpr int off8(int p) { int o = 0; for (int q = 1; q < p; ++q) o += 7 * pow8(q); return o; }
+  static constexpr int OFFREM = off8(N8);
+  static constexpr int TWN = OFFREM + (REM ? ((1 << REM) - 1) * pow8(N8) : 0);
 };
 
 // Stockham FFT of L points; thread t holds v[s] = x[t + s*L/8] on entry and
-// v[s] = X[t + s*L/8] on exit.  sbuf has L entries; tw is the half twiddle table.
+// v[s] = X[t + s*L/8] on exit.  sbuf has L + L/8 entries (padded); tw holds Plan<L>::TWN
+// twiddles exp(-2 pi i k r / (Ns R)) per pass.
 template <int L>
 __device__ __forceinline__ void fft_stockham(float2* v, float2* sbuf, const float2* tw, int t) {
   using P = Plan<L>;
@@ -97,11 +101,11 @@ __device__ __forceinline__ void fft_stockham(float2* v, float2* sbuf, const floa
   for (int p = 0; p < P::N8; ++p) {
     if (p > 0) {
 #pragma unroll
-      for (int s = 0; s < 8; ++s) v[s] = sbuf[t + s * NT];
+      for (int s = 0; s < 8; ++s) v[s] = sbuf[padx(t + s * NT)];
       const int k = t & (Ns - 1);
-      const int step = L / (Ns * 8);
+      const float2* twp = tw + P::off8(p) + k;
 #pragma unroll
-      for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twid<L>(tw, k * r * step));
+      for (int r = 1; r < 8; ++r) v[r] = cmul(v[r], twp[(r - 1) * Ns]);
     }
     dft8(v);
     const bool last = (P::REM == 0) && (p == P::N8 - 1);
@@ -110,35 +114,34 @@ __device__ __forceinline__ void fft_stockham(float2* v, float2* sbuf, const floa
       const int k = t & (Ns - 1);
       const int idxD = (t / Ns) * Ns * 8 + k;
 #pragma unroll
-      for (int r = 0; r < 8; ++r) sbuf[idxD + r * Ns] = v[r];
+      for (int r = 0; r < 8; ++r) sbuf[padx(idxD + r * Ns)] = v[r];
       __syncthreads();
     }
     Ns *= 8;
   }
   if (P::REM == 2) {                         // radix-4 pass, two butterflies per thread
 #pragma unroll
-    for (int s = 0; s < 8; ++s) v[s] = sbuf[t + s * NT];
-    const int step = L / (Ns * 4);
+    for (int s = 0; s < 8; ++s) v[s] = sbuf[padx(t + s * NT)];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int j = t + b * NT;
       const int k = j & (Ns - 1);
+      const float2* twp = tw + P::OFFREM + k;
       float2 a0 = v[b], a1 = v[b + 2], a2 = v[b + 4], a3 = v[b + 6];
-      a1 = cmul(a1, twid<L>(tw, k * 1 * step));
-      a2 = cmul(a2, twid<L>(tw, k * 2 * step));
-      a3 = cmul(a3, twid<L>(tw, k * 3 * step));
+      a1 = cmul(a1, twp[0]);
+      a2 = cmul(a2, twp[Ns]);
+      a3 = cmul(a3, twp[2 * Ns]);
       dft4(a0, a1, a2, a3);
       v[b] = a0; v[b + 2] = a1; v[b + 4] = a2; v[b + 6] = a3;
     }
   } else if (P::REM == 1) {                  // radix-2 pass, four butterflies per thread
 #pragma unroll
-    for (int s = 0; s < 8; ++s) v[s] = sbuf[t + s * NT];
-    const int step = L / (Ns * 2);
+    for (int s = 0; s < 8; ++s) v[s] = sbuf[padx(t + s * NT)];
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const int j = t + m * NT;
       const int k = j & (Ns - 1);
-      float2 a0 = v[m], a1 = cmul(v[m + 4], twid<L>(tw, k * step));
+      float2 a0 = v[m], a1 = cmul(v[m + 4], tw[P::OFFREM + k]);
       dft2(a0, a1);
       v[m] = a0; v[m + 4] = a1;
     }
@@ -161,17 +164,6 @@ __device__ int count_valid(const uint8_t* fv, int nframes, int spf, long long g0
 }
 
 // Per-FFT delay chain (mode.cpp:671-675, 743-757, 781-788), all FP64.
-struct FftDelay {
-  int ok;               // 1 = processed, 0 = flagged (zero spectra, weight 0)
-  int n0;               // nearestsample
-  float fse;            // fracsampleerror (us)
-  double a, b;          // linear interpolator across the FFT (FFT-fraction units)
-  int intdelay;         // integerdelay (us)
-  float weight;
-  int intwall;
-  double fracwall;
-};
-
 __device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_t* fv, int i,
                               int offsec, int offns, double scan_start_sec) {
   FftDelay r;
@@ -274,13 +266,33 @@ __device__ __forceinline__ uint32_t pack16(float2 v, int mode) {
 
 }  // namespace
 
+// Per-FFT parameters of one chunk, one thread per (datastream, FFT): the FP64 delay chain
+// and the data weight, computed once instead of by every thread of the F-engine.
+__global__ void k_fftparams(FengineArgs args) {
+  const int d = args.ds0 + blockIdx.y;
+  const int lt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lt >= args.nffts_pad) return;
+  const DevDs& ds = args.ds[d];
+  const DevDsRun& run = args.run[d];
+  FftDelay fd;
+  fd.ok = 0; fd.weight = 0.0f; fd.n0 = 0; fd.fse = 0.0f; fd.a = 0; fd.b = 0; fd.intdelay = 0;
+  fd.intwall = 0; fd.fracwall = 0;
+  if (lt < args.nffts) {
+    const int i = args.chunk0 + lt;
+    fd = fft_delay(ds, run, args.fvalid + ds.fvalid_off, i, args.offsec, args.offns, args.scan_start_sec);
+    args.weights[(long long)d * args.B + i] = fd.weight;
+  }
+  args.params[(long long)d * args.chunkcap + lt] = fd;
+}
+
 // One workgroup = one datastream x `blocks_per_wg` time blocks of FXG_TB consecutive FFTs.
 template <int L, bool CPLX>
 __global__ void __launch_bounds__(L / 8)
 k_fengine(FengineArgs args) {
+  using P = Plan<L>;
   constexpr int NT = L / 8;
-  __shared__ float2 sbuf[L];
-  __shared__ float2 stw[L / 2];
+  __shared__ float2 sbuf[L + L / 8];
+  __shared__ float2 stw[P::TWN];
   __shared__ float2 srot[L / 4];
   __shared__ float2 sfrac[L / 4];
 
@@ -292,8 +304,10 @@ k_fengine(FengineArgs args) {
   const float* cf = args.chanfreq + ds.chanfreq_off;
   const int C = ds.C, S = ds.S, nrec = ds.nrecbands;
   const int fmt = ds.format;
+  const int spf = ds.spf;
+  const bool twobit = (ds.nbits == 2);
 
-  for (int m = t; m < L / 2; m += NT) stw[m] = args.twiddle[m];
+  for (int m = t; m < P::TWN; m += NT) stw[m] = args.twiddle[m];
 
   const int nblocks = args.nffts_pad / FXG_TB;
   const int blk0 = blockIdx.x * args.blocks_per_wg;
@@ -301,12 +315,7 @@ k_fengine(FengineArgs args) {
   for (int blk = blk0; blk < blk1; ++blk) {
     for (int tt = 0; tt < FXG_TB; ++tt) {
       const int lt = blk * FXG_TB + tt;           // FFT within the chunk
-      const int i = args.chunk0 + lt;             // FFT within the subint
-      FftDelay fd;
-      fd.ok = 0; fd.weight = 0.0f;
-      if (lt < args.nffts)
-        fd = fft_delay(ds, run, fv, i, args.offsec, args.offns, args.scan_start_sec);
-      if (t == 0 && lt < args.nffts) args.weights[(long long)d * args.B + i] = fd.weight;
+      const FftDelay fd = args.params[(long long)d * args.chunkcap + lt];
 
       if (!fd.ok) {                               // flagged: zero spectra (mode.cpp:656-669)
         for (int band = 0; band < nrec; ++band) {
@@ -318,34 +327,38 @@ k_fengine(FengineArgs args) {
               args.spec32[ds.spec_off + ((long long)band * args.chunkcap + lt) * C + c] = make_float2(0.f, 0.f);
             } else {
               const DevXGroup& xg = args.xgroups[ds.band_xfreq[band]];
-              const long long o = xg.spec_off +
-                  (((long long)ds.band_xrow[band] * (args.chunkcap / FXG_TB) + lt / FXG_TB) * C + c) * FXG_TB + (lt % FXG_TB);
-              args.spec16[o] = 0u;
+              args.spec16[xg.spec_off + (long long)ds.band_xrow[band] * xg.rowstride + (long long)lt * C + c] = 0u;
             }
           }
         }
         continue;
       }
 
-      // raw sample words of this thread's 8 time samples (shared by all bands)
-      const long long g0 = fd.n0;
-      uint32_t word[8];
-      int fok[8];
-      long long sbase[8];                         // byte offset of the sample's first field
+      // raw sample words of this thread's 8 time samples, shared by all bands: for 2-bit data
+      // the 32-bit payload word holding the sample's fields and the field offset in it; for
+      // 8-bit data the byte offset of the sample
+      uint32_t wv[8];
+      uint32_t fok = 0u, qlo = 0u;
+      {
+        const unsigned g0 = (unsigned)fd.n0 + (unsigned)t;
+        unsigned fr = g0 / (unsigned)spf;
+        unsigned within = g0 - fr * (unsigned)spf;
 #pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const long long g = g0 + t + s * NT;
-        const int fr = (int)(g / ds.spf);
-        const int within = (int)(g - (long long)fr * ds.spf);
-        fok[s] = (fr < run.nframes) && fv[fr];
-        const long long fbase = (long long)fr * ds.framebytes + ds.headerbytes;
-        if (!CPLX && ds.nbits == 2) {
-          const long long q = (long long)within * nrec;             // first field of the sample
-          sbase[s] = q;
-          word[s] = fok[s] ? *reinterpret_cast<const uint32_t*>(run.data + fbase + ((q >> 4) << 2)) : 0u;
-        } else {
-          sbase[s] = fbase + (long long)within * nrec * (CPLX ? 2 : 1);
-          word[s] = 0u;
+        for (int s = 0; s < 8; ++s) {
+          if (s > 0) {
+            within += NT;
+            while (within >= (unsigned)spf) { within -= spf; ++fr; }
+          }
+          const bool ok = ((int)fr < run.nframes) && fv[fr];
+          fok |= (ok ? 1u : 0u) << s;
+          const unsigned fbase = fr * (unsigned)ds.framebytes + (unsigned)ds.headerbytes;
+          if (twobit) {
+            const unsigned q = within * (unsigned)nrec;       // first field of the sample
+            qlo |= (q & 15u) << (4 * s);
+            wv[s] = ok ? *reinterpret_cast<const uint32_t*>(run.data + fbase + ((q >> 4) << 2)) : 0u;
+          } else {
+            wv[s] = fbase + within * (unsigned)nrec * (CPLX ? 2u : 1u);
+          }
         }
       }
 
@@ -365,21 +378,22 @@ k_fengine(FengineArgs args) {
             const int n = t + s * NT;
             const int st = n / S;
             const float2 rot = st == 0 ? sub : cmul(sub, srot[S + st]);
+            const bool ok = (fok >> s) & 1u;
             if (!CPLX) {
               float x = 0.0f;
-              if (fok[s]) {
-                if (ds.nbits == 2) {
-                  const int sh = 2 * (int)((sbase[s] & 15) + band);
-                  x = lut2(fmt, (word[s] >> sh) & 3u);
+              if (ok) {
+                if (twobit) {
+                  const int sh = 2 * (int)(((qlo >> (4 * s)) & 15u) + band);
+                  x = lut2(fmt, (wv[s] >> sh) & 3u);
                 } else {
-                  x = lut8(run.data[sbase[s] + band]);
+                  x = lut8(run.data[wv[s] + band]);
                 }
               }
               v[s] = make_float2(x * rot.x, x * rot.y);          // (x + 0i) * rotator
             } else {
               float2 x = make_float2(0.f, 0.f);
-              if (fok[s]) {
-                const uint8_t* p = run.data + sbase[s] + 2 * band;
+              if (ok) {
+                const uint8_t* p = run.data + wv[s] + 2 * band;
                 x = make_float2(lut8(p[0]), lut8(p[1]));
               }
               v[s] = cmul(rot, x);
@@ -408,9 +422,8 @@ k_fengine(FengineArgs args) {
               const DevXGroup& xg = args.xgroups[ds.band_xfreq[band]];
               const int row = ds.band_xrow[band];
               const float sc = args.xscale[xg.scale_off + (long long)row * C + c];
-              const long long o = xg.spec_off +
-                  (((long long)row * (args.chunkcap / FXG_TB) + lt / FXG_TB) * C + c) * FXG_TB + (lt % FXG_TB);
-              args.spec16[o] = pack16(make_float2(F.x * sc, F.y * sc), args.out_mode);
+              args.spec16[xg.spec_off + (long long)row * xg.rowstride + (long long)lt * C + c] =
+                  pack16(make_float2(F.x * sc, F.y * sc), args.out_mode);
             }
           }
         }
@@ -529,6 +542,7 @@ int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st) {
 
 template <int L, bool CPLX>
 static int launch_f(const FengineArgs& a, int nds, hipStream_t st) {
+  hipLaunchKernelGGL(k_fftparams, dim3((a.nffts_pad + 255) / 256, nds), dim3(256), 0, st, a);
   const int nblocks = a.nffts_pad / FXG_TB;
   const int gx = (nblocks + a.blocks_per_wg - 1) / a.blocks_per_wg;
   hipLaunchKernelGGL((k_fengine<L, CPLX>), dim3(gx, nds), dim3(L / 8), 0, st, a);

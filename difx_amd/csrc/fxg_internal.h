@@ -80,14 +80,28 @@ struct DevXGroup {
   int C;                  // channels
   long long acc_off;      // float offset of its accumulator block [C][npairs][2][32][32]
   long long spec_off;     // element offset (32-bit units) of its spectra block
+  long long rowstride;    // 32-bit units per input row: chunk FFTs x C plus one 128-byte line
   long long scale_off;    // float offset of its per-(row, channel) scales [rows][C]
+};
+
+// Per-FFT delay chain of one datastream (mode.cpp:671-675, 743-757, 781-788).
+struct FftDelay {
+  int ok;               // 1 = processed, 0 = flagged (zero spectra, weight 0)
+  int n0;               // nearestsample
+  float fse;            // fracsampleerror (us)
+  int intdelay;         // integerdelay (us)
+  double a, b;          // linear interpolator across the FFT (FFT-fraction units)
+  float weight;
+  int intwall;
+  double fracwall;
 };
 
 // Kernel launch helpers implemented in the .hip files.
 struct FengineArgs {
   const DevDs* ds;
   const DevDsRun* run;
-  const float2* twiddle;  // exp(-2 pi i m / L), m in [0, L)
+  const float2* twiddle;  // per-pass FFT twiddle tables (see Plan<L> in fxg_fengine.hip)
+  FftDelay* params;       // [nds][chunkcap] per-FFT delay chain of the current chunk
   float* weights;         // [nds][B]
   int B;                  // blockspersend
   int chunk0;             // first FFT index of this chunk
@@ -97,7 +111,7 @@ struct FengineArgs {
   double scan_start_sec;
   int out_mode;           // 0 = FP32 [ds][band][t][C], 1 = FP16, 2 = BF16 MFMA layout
   float2* spec32;         // FP32 spectra
-  uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row][t/4][C][4]
+  uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row (rowstride)][t][C]
   const DevXGroup* xgroups;
   const float* xscale;    // per group [rows][C] scales applied before 16-bit conversion
   const float* chanfreq;  // per datastream: sub[S], usb step[NFS], lsb step[NFS] (mode.cpp:303-324)

@@ -76,61 +76,134 @@ __device__ __forceinline__ uint4 rot_negi(uint4 u) {
   return r;
 }
 
-// One wavefront per (group, channel); 4 wavefronts per workgroup on 4 consecutive channels.
-// Tile pairs (ti >= tj) of 32 rows: ntile 1 -> 1 pair, 2 -> 3 pairs.
+// One workgroup = 4 wavefronts on a tile of CT channels of one group; each wavefront owns CPW
+// channels.  Per k-step (8 FFTs) the workgroup streams the live rows' spectra -- 16-byte
+// chunks of 4 channels at one FFT, contiguous in the channel-major layout the F-engine writes
+// -- into LDS as [channel][row][8 FFTs], double buffered, so each lane can read its MFMA
+// operand (one row, 4 consecutive FFTs = 8 halves) with one 16-byte LDS load.  Rows beyond
+// the group's inputs stay zero in LDS and are never fetched.
+template <int NTILE>
+struct XTile {
+  static constexpr int CPW = (NTILE == 1) ? 2 : 1;      // channels per wavefront
+  static constexpr int CT = 4 * CPW;                     // channels per workgroup
+  static constexpr int R = 32 * NTILE;                   // operand rows
+  static constexpr int CSTR = R * 8 + 8;                 // 32-bit words per channel in LDS (padded:
+                                                         // conflict-free chunk stores)
+  static constexpr int QPR = 8 * (CT / 4);               // 16-byte chunks per row per k-step
+  static constexpr int QMAX = (R * QPR + 255) / 256;     // chunks per thread per k-step
+};
+
 template <typename V8, int NTILE>
 __global__ void __launch_bounds__(256) k_xmac_mfma(XMfmaArgs a) {
+  using T = XTile<NTILE>;
   constexpr int NPAIR = NTILE * (NTILE + 1) / 2;
-  const DevXGroup g = a.groups[blockIdx.y];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + wave;
-  if (c >= g.C) return;
-  const int row = lane & 31;
-  const int half = lane >> 5;
-  const long long tbstride = (long long)g.C * 4;              // uint32 per time block
-  const long long rowstride = (long long)(a.chunkcap / 4) * tbstride;
-  const uint32_t* base = a.spec16 + g.spec_off + (long long)c * 4 + (long long)half * tbstride;
-  floatx16 accr[NPAIR], acci[NPAIR];
-  float* accp = a.acc + g.acc_off + (long long)c * NPAIR * 2 * 1024;
+  constexpr int CPW = T::CPW, CT = T::CT, CSTR = T::CSTR, QPR = T::QPR, QMAX = T::QMAX;
+  __shared__ uint4 lds_raw[2][(CT * CSTR) / 4];
+  uint32_t* lds0 = reinterpret_cast<uint32_t*>(lds_raw[0]);
+  uint32_t* lds1 = reinterpret_cast<uint32_t*>(lds_raw[1]);
+  // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Remap so
+  // that every XCD walks a contiguous range of channel tiles: neighbouring tiles share the
+  // 128-byte lines of every input row and should hit the same L2.
+  const int nblk = gridDim.x * gridDim.y;
+  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  int lb = b;
+  if ((nblk & 7) == 0) lb = (b & 7) * (nblk >> 3) + (b >> 3);
+  const DevXGroup g = a.groups[lb / gridDim.x];
+  const int c0 = (lb % gridDim.x) * CT;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int row = lane & 31, half = lane >> 5;
+  const int nq = g.nin * QPR;
+  const uint32_t* gbase = a.spec16 + g.spec_off + c0;
+
+  for (int i = tid; i < 2 * (CT * CSTR) / 4; i += 256)
+    (&lds_raw[0][0])[i] = make_uint4(0u, 0u, 0u, 0u);
+
+  floatx16 accr[CPW][NPAIR], acci[CPW][NPAIR];
 #pragma unroll
-  for (int p = 0; p < NPAIR; ++p) {
+  for (int cw = 0; cw < CPW; ++cw) {
+    const int c = c0 + wave * CPW + cw;
+    const float* accp = a.acc + g.acc_off + (long long)c * NPAIR * 2 * 1024;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      accr[p][r] = accp[(p * 2 + 0) * 1024 + r * 64 + lane];
-      acci[p][r] = accp[(p * 2 + 1) * 1024 + r * 64 + lane];
-    }
+    for (int p = 0; p < NPAIR; ++p)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        accr[cw][p][r] = accp[(p * 2 + 0) * 1024 + r * 64 + lane];
+        acci[cw][p][r] = accp[(p * 2 + 1) * 1024 + r * 64 + lane];
+      }
   }
-  const int nk = a.nffts_pad / 8;
-  for (int k = 0; k < nk; ++k) {
-    uint4 x[NTILE];
+
+  uint4 buf[QMAX];
+  auto gload = [&](int k) {
 #pragma unroll
-    for (int ti = 0; ti < NTILE; ++ti) {
-      const uint32_t* p = base + (long long)(row + 32 * ti) * rowstride + (long long)(2 * k) * tbstride;
-      x[ti] = *reinterpret_cast<const uint4*>(p);
-    }
-    int pi = 0;
-#pragma unroll
-    for (int ti = 0; ti < NTILE; ++ti) {
-      const uint4 xr = rot_negi(x[ti]);
-#pragma unroll
-      for (int tj = 0; tj <= ti; ++tj) {
-        const V8 av = __builtin_bit_cast(V8, x[ti]);
-        const V8 bv = __builtin_bit_cast(V8, x[tj]);
-        const V8 arv = __builtin_bit_cast(V8, xr);
-        accr[pi] = mfma32<V8>(av, bv, accr[pi]);
-        acci[pi] = mfma32<V8>(arv, bv, acci[pi]);
-        ++pi;
+    for (int i = 0; i < QMAX; ++i) {
+      const int q = tid + 256 * i;
+      if (q < nq) {
+        const int r = q / QPR, rem = q - r * QPR;
+        const int t = rem / (CT / 4), cq = rem - t * (CT / 4);
+        buf[i] = *reinterpret_cast<const uint4*>(gbase + (long long)r * g.rowstride +
+                                                 (long long)(8 * k + t) * g.C + 4 * cq);
       }
     }
-  }
+  };
+  auto lstore = [&](uint32_t* L) {
 #pragma unroll
-  for (int p = 0; p < NPAIR; ++p) {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      accp[(p * 2 + 0) * 1024 + r * 64 + lane] = accr[p][r];
-      accp[(p * 2 + 1) * 1024 + r * 64 + lane] = acci[p][r];
+    for (int i = 0; i < QMAX; ++i) {
+      const int q = tid + 256 * i;
+      if (q < nq) {
+        const int r = q / QPR, rem = q - r * QPR;
+        const int t = rem / (CT / 4), cq = rem - t * (CT / 4);
+        uint32_t* d = L + (4 * cq) * CSTR + r * 8 + t;
+        d[0] = buf[i].x; d[CSTR] = buf[i].y; d[2 * CSTR] = buf[i].z; d[3 * CSTR] = buf[i].w;
+      }
     }
+  };
+
+  const int nk = a.nffts_pad / 8;
+  __syncthreads();
+  gload(0);
+  lstore(lds0);
+  __syncthreads();
+  for (int k = 0; k < nk; ++k) {
+    uint32_t* cur = (k & 1) ? lds1 : lds0;
+    uint32_t* nxt = (k & 1) ? lds0 : lds1;
+    if (k + 1 < nk) gload(k + 1);
+#pragma unroll
+    for (int cw = 0; cw < CPW; ++cw) {
+      const uint32_t* cb = cur + (wave * CPW + cw) * CSTR + 4 * half;
+      uint4 x[NTILE];
+#pragma unroll
+      for (int ti = 0; ti < NTILE; ++ti) x[ti] = *reinterpret_cast<const uint4*>(cb + (row + 32 * ti) * 8);
+      int pi = 0;
+#pragma unroll
+      for (int ti = 0; ti < NTILE; ++ti) {
+        const uint4 xr = rot_negi(x[ti]);
+#pragma unroll
+        for (int tj = 0; tj <= ti; ++tj) {
+          const V8 av = __builtin_bit_cast(V8, x[ti]);
+          const V8 bv = __builtin_bit_cast(V8, x[tj]);
+          const V8 arv = __builtin_bit_cast(V8, xr);
+          accr[cw][pi] = mfma32<V8>(av, bv, accr[cw][pi]);
+          acci[cw][pi] = mfma32<V8>(arv, bv, acci[cw][pi]);
+          ++pi;
+        }
+      }
+    }
+    if (k + 1 < nk) lstore(nxt);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int cw = 0; cw < CPW; ++cw) {
+    const int c = c0 + wave * CPW + cw;
+    float* accp = a.acc + g.acc_off + (long long)c * NPAIR * 2 * 1024;
+#pragma unroll
+    for (int p = 0; p < NPAIR; ++p)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        accp[(p * 2 + 0) * 1024 + r * 64 + lane] = accr[cw][p][r];
+        accp[(p * 2 + 1) * 1024 + r * 64 + lane] = acci[cw][p][r];
+      }
   }
 }
 
@@ -193,19 +266,21 @@ __global__ void k_fin_mfma(const FinMfma* e, const DevXGroup* groups, const floa
 // Baseline weights (core.cpp:1005-1052, 1066-1112) and autocorrelation weights
 // (core.cpp:1306-1370): sequential FP32 sums over the FFTs of the subint.
 
-__global__ void k_fin_weights(const FinWeight* e, int n, const float* w, int B, float* resf) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const FinWeight f = e[k];
+// One workgroup per entry: a strided partial sum per thread, then a tree over the block.
+__global__ void __launch_bounds__(256) k_fin_weights(const FinWeight* e, const float* w, int B, float* resf) {
+  __shared__ float part[256];
+  const FinWeight f = e[blockIdx.x];
   const float* w1 = w + (long long)f.ds1 * B;
+  const float* w2 = f.ds2 >= 0 ? w + (long long)f.ds2 * B : nullptr;
   float s = 0.0f;
-  if (f.ds2 < 0) {
-    for (int i = 0; i < B; ++i) s += w1[i];
-  } else {
-    const float* w2 = w + (long long)f.ds2 * B;
-    for (int i = 0; i < B; ++i) s += w1[i] * w2[i];
+  for (int i = threadIdx.x; i < B; i += blockDim.x) s += w2 ? w1[i] * w2[i] : w1[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+    __syncthreads();
   }
-  resf[f.dstf] += s;
+  if (threadIdx.x == 0) resf[f.dstf] += part[0];
 }
 
 __global__ void k_zero(float* p, long long n) {
@@ -228,7 +303,9 @@ int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t s
 int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st) {
   XMfmaArgs b = a;
   b.groups = a.groups + group0;
-  dim3 grid((C + 3) / 4, ngroup_launch);
+  const int ct = ntile == 1 ? XTile<1>::CT : XTile<2>::CT;
+  if (C % ct) return -2;
+  dim3 grid(C / ct, ngroup_launch);
   if (a.mode == 1) {
     if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 1>), grid, dim3(256), 0, st, b);
     else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 2>), grid, dim3(256), 0, st, b);
@@ -256,7 +333,7 @@ int fxg_launch_fin_mfma(const FinMfma* e, int n, int maxout, const DevXGroup* g,
 
 int fxg_launch_fin_weights(const FinWeight* e, int n, const float* w, int B, float* resf, hipStream_t st) {
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_fin_weights, dim3((n + 255) / 256), dim3(256), 0, st, e, n, w, B, resf);
+  hipLaunchKernelGGL(k_fin_weights, dim3(n), dim3(256), 0, st, e, w, B, resf);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

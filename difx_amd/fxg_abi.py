@@ -111,11 +111,19 @@ class fxg_layout(C.Structure):
     ]
 
 
+class fxg_timing(C.Structure):
+    _fields_ = [
+        ("f_ms", C.c_double), ("f_launches", C.c_int32),
+        ("x_ms", C.c_double), ("x_launches", C.c_int32),
+        ("other_ms", C.c_double), ("other_launches", C.c_int32),
+    ]
+
+
 # exported symbols of libfxg.so (every function declared in include/fxg.h)
 EXPORTED_SYMBOLS = [
     "fxg_create", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
     "fxg_data_bytes", "fxg_result_offsets", "fxg_process_subint",
     "fxg_process_subint_device", "fxg_ring_length", "fxg_submit_subint", "fxg_wait",
     "fxg_get_spectrum", "fxg_get_data_weights", "fxg_unpack_device",
-    "fxg_delay_interpolator",
+    "fxg_delay_interpolator", "fxg_set_timing", "fxg_get_timing",
 ]

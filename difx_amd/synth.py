@@ -134,6 +134,21 @@ def buffer_start_ns(cfg, d: int, offsets: Sequence[int]) -> int:
     return int(math.floor(start / frame_ns) - 1) * frame_ns
 
 
+def slot_bytes(cfg, d: int, offsets: Sequence[int], start_ns: int) -> int:
+    """Bytes of a slot buffer starting at start_ns: Configuration::getDataBytes, extended
+    when a short subint needs more frames to cover its last FFT after the start offset."""
+    ds = cfg.datastreams[d]
+    spf = samples_per_frame(ds)
+    dt_ns = cfg.sampletime_us(d) * 1000.0
+    frame_ns = spf * dt_ns
+    t_end = offsets[1] * 1e9 + offsets[2] + cfg.subint_ns
+    tau_end = eval_delay(ds.model, ds.model_order, t_end / 1e9, cfg.model_inc_secs) * 1000.0
+    tau_end -= 1e9 * math.floor(ds.clock_coeff0_us / 1e6 + 0.5)
+    need_ns = t_end - tau_end + cfg.fftchannels(d) * dt_ns - start_ns
+    need_frames = int(math.ceil(need_ns / frame_ns)) + 1
+    return max(data_bytes(cfg, d), need_frames * ds.framebytes)
+
+
 def control_words(cfg, d: int, offsets, start_ns: int, B: Optional[int] = None,
                   invalid_ffts: Sequence[int] = ()) -> np.ndarray:
     """{scan, sec, ns, validflags...} with 30 valid bits per int (mpifxcorr.h:29)."""
@@ -160,7 +175,7 @@ def noise_slot(cfg, offsets, seed: int = 0, nbytes_extra_frames: int = 0):
     data, control = [], []
     for d, ds in enumerate(cfg.datastreams):
         start_ns = buffer_start_ns(cfg, d, offsets)
-        nbytes = data_bytes(cfg, d) + nbytes_extra_frames * ds.framebytes
+        nbytes = slot_bytes(cfg, d, offsets, start_ns) + nbytes_extra_frames * ds.framebytes
         nframes = nbytes // ds.framebytes
         pay = ds.framebytes - ds.headerbytes
         nb = len(ds.recbands)
@@ -190,7 +205,7 @@ def correlated_slot(cfg, offsets, seed: int = 0, amp: float = 0.05, lo_phase: bo
     ds0 = cfg.datastreams[0]
     dt_us = cfg.sampletime_us(0)
     starts = [buffer_start_ns(cfg, d, offsets) for d in range(len(cfg.datastreams))]
-    lens = [data_bytes(cfg, d) for d in range(len(cfg.datastreams))]
+    lens = [slot_bytes(cfg, d, offsets, starts[d]) for d in range(len(cfg.datastreams))]
     nsamps = [(lens[d] // ds.framebytes) * samples_per_frame(ds)
               for d, ds in enumerate(cfg.datastreams)]
     taus = [float(ds.model[0, 0]) for ds in cfg.datastreams]

@@ -191,6 +191,18 @@ int fxg_unpack_device(fxg_ctx* ctx, int32_t ds, const uint8_t* d_data, int32_t n
  * time is outside the model. */
 int fxg_delay_interpolator(fxg_ctx* ctx, int32_t ds, int32_t sec, int32_t ns, double* out);
 
+/* Kernel timing (no counterpart in the reference; for measurement harnesses).  When enabled,
+ * HIP events bracket every F-engine, X-engine and accumulation launch on the stream they are
+ * launched on; fxg_get_timing synchronizes and returns the summed durations since the last
+ * enable. */
+typedef struct fxg_timing {
+  double  f_ms;   int32_t f_launches;    /* fused F-engine (unpack, rotate, FFT, post-F) */
+  double  x_ms;   int32_t x_launches;    /* X-engine */
+  double  other_ms; int32_t other_launches; /* setup, accumulation, weights */
+} fxg_timing;
+int fxg_set_timing(fxg_ctx* ctx, int32_t enable);
+int fxg_get_timing(fxg_ctx* ctx, fxg_timing* out);
+
 #ifdef __cplusplus
 }
 #endif
