@@ -1166,8 +1166,12 @@ int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, 
   const int row = x.band_xrow[band];
   std::vector<uint32_t> blk((size_t)C);
   std::vector<float> sc(C);
-  const uint32_t* src = ctx->d_spec16 + g.spec_off + (long long)row * g.rowstride + (long long)lt * C;
-  if (hipMemcpy(blk.data(), src, sizeof(uint32_t) * blk.size(), hipMemcpyDeviceToHost) != hipSuccess) return FXG_E_HIP;
+  for (int ct = 0; ct < C / 8; ++ct) {
+    const uint32_t* src = ctx->d_spec16 + g.spec_off + (long long)row * g.rowstride +
+                          ((long long)ct * ctx->chunkcap + lt) * 8;
+    if (hipMemcpy(blk.data() + 8 * ct, src, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      return FXG_E_HIP;
+  }
   if (hipMemcpy(sc.data(), ctx->d_xscale + g.scale_off + (long long)row * C, sizeof(float) * C,
                 hipMemcpyDeviceToHost) != hipSuccess)
     return FXG_E_HIP;

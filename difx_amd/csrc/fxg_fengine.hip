@@ -110,7 +110,8 @@ __device__ __forceinline__ void fft_stockham(float2* v, float2* sbuf, const floa
     dft8(v);
     const bool last = (P::REM == 0) && (p == P::N8 - 1);
     if (!last) {
-      __syncthreads();                       // every thread has read its inputs
+      if (p > 0) __syncthreads();            // every thread has read its inputs (pass 0 reads
+                                             // none; the caller's barrier freed sbuf)
       const int k = t & (Ns - 1);
       const int idxD = (t / Ns) * Ns * 8 + k;
 #pragma unroll
@@ -204,11 +205,31 @@ __device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_
   return r;
 }
 
+// Per-datastream constants the F-engine reads, copied to registers once per workgroup.
+struct DsConst {
+  int C, S, NS, NFS, nrec, nrecfreqs, fmt, spf, framebytes, headerbytes, nbits, fractionallo;
+  double sampletime_us;
+};
+
+// Per-freq constants (mode.cpp:853-931).
+struct FreqConst {
+  double lofreq, lo_offset_hz, fraclooffset, clockoff, halfdelta;
+  int lsb, looff, isfraclooffset, nbands;
+};
+
+__device__ __forceinline__ FreqConst load_freq(const DevRecFreq& rf) {
+  FreqConst f;
+  f.lofreq = rf.lofreq; f.lo_offset_hz = rf.lo_offset_hz; f.fraclooffset = rf.fraclooffset;
+  f.clockoff = rf.clockoff; f.halfdelta = rf.halfdelta;
+  f.lsb = rf.lsb; f.looff = rf.looff; f.isfraclooffset = rf.isfraclooffset; f.nbands = rf.nbands;
+  return f;
+}
+
 // Fringe rotator sub-phasors (j < S) and step-phasors (S <= j < S+NS) for one freq
 // (mode.cpp:883-955).  FP64 phase in turns, wrapped by truncation, FP32 sin/cos.
-__device__ void build_rotator(float2* srot, const DevDs& ds, const DevRecFreq& rf,
-                              const FftDelay& fd, int t, int nt) {
-  const int S = ds.S, NS = ds.NS, L = ds.L;
+__device__ void build_rotator(float2* srot, const DsConst& k, const FreqConst& rf,
+                              const FftDelay& fd, int L, int t, int nt) {
+  const int S = k.S, NS = k.NS;
   const double lofreq = rf.lofreq;
   for (int j = t; j < S + NS; j += nt) {
     double ph;
@@ -216,10 +237,10 @@ __device__ void build_rotator(float2* srot, const DevDs& ds, const DevRecFreq& r
       const double xoff = (double)j / (double)L;                        // subxoff
       const double xval = __dadd_rn(__dmul_rn(xoff, fd.a), fd.b);       // subxval
       ph = __dmul_rn(xval, lofreq);
-      if (ds.fractionallo)
+      if (k.fractionallo)
         ph = __dadd_rn(ph, __dmul_rn(lofreq - (double)(int)lofreq, (double)fd.intdelay));
       if (rf.looff) {
-        const double toff = j * ds.sampletime_us / 1e6;                  // subtoff
+        const double toff = j * k.sampletime_us / 1e6;                   // subtoff
         ph = __dadd_rn(ph, __dmul_rn(toff, -rf.lo_offset_hz));
         ph = __dadd_rn(ph, -rf.lo_offset_hz * fd.fracwall);
         if (rf.isfraclooffset) ph = __dadd_rn(ph, -rf.fraclooffset * fd.intwall);
@@ -229,7 +250,7 @@ __device__ void build_rotator(float2* srot, const DevDs& ds, const DevRecFreq& r
       const double xoff = (double)(s * S) / (double)L;                  // stepxoff
       ph = __dmul_rn(__dmul_rn(xoff, fd.a), lofreq);
       if (rf.looff) {
-        const double toff = s * S * ds.sampletime_us / 1e6;              // steptoff
+        const double toff = s * S * k.sampletime_us / 1e6;               // steptoff
         ph = __dadd_rn(ph, __dmul_rn(toff, -rf.lo_offset_hz));
       }
     }
@@ -241,12 +262,12 @@ __device__ void build_rotator(float2* srot, const DevDs& ds, const DevRecFreq& r
 }
 
 // Fractional-sample phasors: sub (j < S) and step (S <= j < S+NFS) (mode.cpp:1030-1073).
-__device__ void build_frac(float2* sfrac, const DevDs& ds, const DevRecFreq& rf, const float* cf,
-                           float fse, int t, int nt) {
-  const int S = ds.S, NFS = ds.NFS;
+__device__ void build_frac(float2* sfrac, const DsConst& k, const FreqConst& rf, bool iscomplex,
+                           const float* __restrict__ cf, float fse, int t, int nt) {
+  const int S = k.S, NFS = k.NFS;
   const float e = (float)((double)fse - rf.clockoff + rf.halfdelta);
   const float* sub = cf;
-  const float* step = rf.lsb && !ds.iscomplex ? cf + S + NFS : cf + S;
+  const float* step = rf.lsb && !iscomplex ? cf + S + NFS : cf + S;
   for (int j = t; j < S + NFS; j += nt) {
     const float arg = (j < S) ? sub[j] * e : step[j - S] * e;
     float sn, cs;
@@ -286,28 +307,46 @@ __global__ void k_fftparams(FengineArgs args) {
 }
 
 // One workgroup = one datastream x `blocks_per_wg` time blocks of FXG_TB consecutive FFTs.
+#ifndef FXG_F_MINWAVES
+#define FXG_F_MINWAVES 1
+#endif
 template <int L, bool CPLX>
-__global__ void __launch_bounds__(L / 8)
+__global__ void __launch_bounds__(L / 8, FXG_F_MINWAVES)
 k_fengine(FengineArgs args) {
   using P = Plan<L>;
   constexpr int NT = L / 8;
   __shared__ float2 sbuf[L + L / 8];
+#ifndef FXG_TW_GLOBAL
   __shared__ float2 stw[P::TWN];
+#endif
   __shared__ float2 srot[L / 4];
   __shared__ float2 sfrac[L / 4];
 
   const int t = threadIdx.x;
   const int d = args.ds0 + blockIdx.y;
-  const DevDs& ds = args.ds[d];
-  const DevDsRun& run = args.run[d];
-  const uint8_t* fv = args.fvalid + ds.fvalid_off;
-  const float* cf = args.chanfreq + ds.chanfreq_off;
-  const int C = ds.C, S = ds.S, nrec = ds.nrecbands;
-  const int fmt = ds.format;
-  const int spf = ds.spf;
-  const bool twobit = (ds.nbits == 2);
+  const DevDs* __restrict__ dsp = args.ds + d;
+  DsConst k;
+  k.C = dsp->C; k.S = dsp->S; k.NS = dsp->NS; k.NFS = dsp->NFS; k.nrec = dsp->nrecbands;
+  k.nrecfreqs = dsp->nrecfreqs; k.fmt = dsp->format; k.spf = dsp->spf;
+  k.framebytes = dsp->framebytes; k.headerbytes = dsp->headerbytes; k.nbits = dsp->nbits;
+  k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us;
+  const long long spec_off32 = dsp->spec_off;
+  const uint8_t* __restrict__ fv = args.fvalid + dsp->fvalid_off;
+  const float* __restrict__ cf = args.chanfreq + dsp->chanfreq_off;
+  const uint8_t* __restrict__ data = args.run[d].data;
+  const int nframes = args.run[d].nframes;
+  const FftDelay* __restrict__ params = args.params + (long long)d * args.chunkcap;
+  const int C = k.C, S = k.S;
+  const bool twobit = (k.nbits == 2);
+  const int out_mode = args.out_mode;
+  const int cap = args.chunkcap;
 
+#ifndef FXG_TW_GLOBAL
   for (int m = t; m < P::TWN; m += NT) stw[m] = args.twiddle[m];
+  const float2* twsrc = stw;
+#else
+  const float2* __restrict__ twsrc = args.twiddle;
+#endif
 
   const int nblocks = args.nffts_pad / FXG_TB;
   const int blk0 = blockIdx.x * args.blocks_per_wg;
@@ -315,19 +354,25 @@ k_fengine(FengineArgs args) {
   for (int blk = blk0; blk < blk1; ++blk) {
     for (int tt = 0; tt < FXG_TB; ++tt) {
       const int lt = blk * FXG_TB + tt;           // FFT within the chunk
-      const FftDelay fd = args.params[(long long)d * args.chunkcap + lt];
+      const FftDelay fd = params[lt];
 
       if (!fd.ok) {                               // flagged: zero spectra (mode.cpp:656-669)
-        for (int band = 0; band < nrec; ++band) {
+        for (int band = 0; band < k.nrec; ++band) {
+          if (out_mode == 0) {
+            float2* __restrict__ o = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            const int c = t + s * NT;
-            if (!CPLX && c >= C) continue;
-            if (args.out_mode == 0) {
-              args.spec32[ds.spec_off + ((long long)band * args.chunkcap + lt) * C + c] = make_float2(0.f, 0.f);
-            } else {
-              const DevXGroup& xg = args.xgroups[ds.band_xfreq[band]];
-              args.spec16[xg.spec_off + (long long)ds.band_xrow[band] * xg.rowstride + (long long)lt * C + c] = 0u;
+            for (int s = 0; s < 8; ++s) {
+              const int c = t + s * NT;
+              if (CPLX || c < C) o[c] = make_float2(0.f, 0.f);
+            }
+          } else {
+            const int g = dsp->band_xfreq[band];
+            uint32_t* __restrict__ o = args.spec16 + args.xgroups[g].spec_off +
+                                       (long long)dsp->band_xrow[band] * args.xgroups[g].rowstride;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+              const int c = t + s * NT;
+              if (CPLX || c < C) o[(c >> 3) * cap * 8 + lt * 8 + (c & 7)] = 0u;
             }
           }
         }
@@ -341,36 +386,36 @@ k_fengine(FengineArgs args) {
       uint32_t fok = 0u, qlo = 0u;
       {
         const unsigned g0 = (unsigned)fd.n0 + (unsigned)t;
-        unsigned fr = g0 / (unsigned)spf;
-        unsigned within = g0 - fr * (unsigned)spf;
+        unsigned fr = g0 / (unsigned)k.spf;
+        unsigned within = g0 - fr * (unsigned)k.spf;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
           if (s > 0) {
             within += NT;
-            while (within >= (unsigned)spf) { within -= spf; ++fr; }
+            while (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
           }
-          const bool ok = ((int)fr < run.nframes) && fv[fr];
+          const bool ok = ((int)fr < nframes) && fv[fr];
           fok |= (ok ? 1u : 0u) << s;
-          const unsigned fbase = fr * (unsigned)ds.framebytes + (unsigned)ds.headerbytes;
+          const unsigned fbase = fr * (unsigned)k.framebytes + (unsigned)k.headerbytes;
           if (twobit) {
-            const unsigned q = within * (unsigned)nrec;       // first field of the sample
+            const unsigned q = within * (unsigned)k.nrec;     // first field of the sample
             qlo |= (q & 15u) << (4 * s);
-            wv[s] = ok ? *reinterpret_cast<const uint32_t*>(run.data + fbase + ((q >> 4) << 2)) : 0u;
+            wv[s] = ok ? *reinterpret_cast<const uint32_t*>(data + fbase + ((q >> 4) << 2)) : 0u;
           } else {
-            wv[s] = fbase + within * (unsigned)nrec * (CPLX ? 2u : 1u);
+            wv[s] = fbase + within * (unsigned)k.nrec * (CPLX ? 2u : 1u);
           }
         }
       }
 
-      for (int f = 0; f < ds.nrecfreqs; ++f) {
-        const DevRecFreq& rf = ds.rf[f];
+      for (int f = 0; f < k.nrecfreqs; ++f) {
+        const FreqConst rf = load_freq(dsp->rf[f]);
         if (rf.nbands == 0) continue;
         __syncthreads();                          // previous users of srot/sfrac are done
-        build_rotator(srot, ds, rf, fd, t, NT);
-        build_frac(sfrac, ds, rf, cf, fd.fse, t, NT);
+        build_rotator(srot, k, rf, fd, L, t, NT);
+        build_frac(sfrac, k, rf, CPLX, cf, fd.fse, t, NT);
         __syncthreads();
         for (int kb = 0; kb < rf.nbands; ++kb) {
-          const int band = rf.band[kb];
+          const int band = dsp->rf[f].band[kb];
           float2 v[8];
           const float2 sub = srot[t & (S - 1)];
 #pragma unroll
@@ -384,46 +429,49 @@ k_fengine(FengineArgs args) {
               if (ok) {
                 if (twobit) {
                   const int sh = 2 * (int)(((qlo >> (4 * s)) & 15u) + band);
-                  x = lut2(fmt, (wv[s] >> sh) & 3u);
+                  x = lut2(k.fmt, (wv[s] >> sh) & 3u);
                 } else {
-                  x = lut8(run.data[wv[s] + band]);
+                  x = lut8(data[wv[s] + band]);
                 }
               }
               v[s] = make_float2(x * rot.x, x * rot.y);          // (x + 0i) * rotator
             } else {
               float2 x = make_float2(0.f, 0.f);
               if (ok) {
-                const uint8_t* p = run.data + wv[s] + 2 * band;
+                const uint8_t* p = data + wv[s] + 2 * band;
                 x = make_float2(lut8(p[0]), lut8(p[1]));
               }
               v[s] = cmul(rot, x);
             }
           }
           __syncthreads();                        // sbuf free (previous band's FFT done)
-          fft_stockham<L>(v, sbuf, stw, t);
+          fft_stockham<L>(v, sbuf, twsrc, t);
+          // output pointers of this band
+          float2* __restrict__ o32 = nullptr;
+          uint32_t* __restrict__ o16 = nullptr;
+          const float* __restrict__ scp = nullptr;
+          if (out_mode == 0) {
+            o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
+          } else {
+            const int g = dsp->band_xfreq[band];
+            const int row = dsp->band_xrow[band];
+            o16 = args.spec16 + args.xgroups[g].spec_off + (long long)row * args.xgroups[g].rowstride + lt * 8;
+            scp = args.xscale + args.xgroups[g].scale_off + (long long)row * C;
+          }
           // channel selection, fractional-sample correction, output
+          constexpr int NKEEP = CPLX ? 8 : 4;
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            int c;
-            if (CPLX) {
-              c = t + s * NT;
-            } else if (rf.lsb) {
-              if (s < 4) continue;
-              c = t + (s - 4) * NT;
+          for (int m = 0; m < NKEEP; ++m) {
+            // USB and complex keep X[t + m*NT]; real LSB keeps X[C + t + m*NT] (mode.cpp:1205)
+            const int s = (!CPLX && rf.lsb) ? m + 4 : m;
+            const int c = t + m * NT;
+            const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
+            const float2 F = cmul(v[s], frq);
+            if (out_mode == 0) {
+              o32[c] = F;
             } else {
-              if (s >= 4) continue;
-              c = t + s * NT;
-            }
-            const float2 fr = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
-            const float2 F = cmul(v[s], fr);
-            if (args.out_mode == 0) {
-              args.spec32[ds.spec_off + ((long long)band * args.chunkcap + lt) * C + c] = F;
-            } else {
-              const DevXGroup& xg = args.xgroups[ds.band_xfreq[band]];
-              const int row = ds.band_xrow[band];
-              const float sc = args.xscale[xg.scale_off + (long long)row * C + c];
-              args.spec16[xg.spec_off + (long long)row * xg.rowstride + (long long)lt * C + c] =
-                  pack16(make_float2(F.x * sc, F.y * sc), args.out_mode);
+              const float sc = scp[c];
+              o16[(c >> 3) * cap * 8 + (c & 7)] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
             }
           }
         }

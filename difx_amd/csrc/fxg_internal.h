@@ -80,7 +80,7 @@ struct DevXGroup {
   int C;                  // channels
   long long acc_off;      // float offset of its accumulator block [C][npairs][2][32][32]
   long long spec_off;     // element offset (32-bit units) of its spectra block
-  long long rowstride;    // 32-bit units per input row: chunk FFTs x C plus one 128-byte line
+  long long rowstride;    // 32-bit units per input row: C x chunk FFTs plus one 128-byte line
   long long scale_off;    // float offset of its per-(row, channel) scales [rows][C]
 };
 
@@ -111,7 +111,7 @@ struct FengineArgs {
   double scan_start_sec;
   int out_mode;           // 0 = FP32 [ds][band][t][C], 1 = FP16, 2 = BF16 MFMA layout
   float2* spec32;         // FP32 spectra
-  uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row (rowstride)][t][C]
+  uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row (rowstride)][C/8][t][8]
   const DevXGroup* xgroups;
   const float* xscale;    // per group [rows][C] scales applied before 16-bit conversion
   const float* chanfreq;  // per datastream: sub[S], usb step[NFS], lsb step[NFS] (mode.cpp:303-324)

@@ -76,47 +76,44 @@ __device__ __forceinline__ uint4 rot_negi(uint4 u) {
   return r;
 }
 
-// One workgroup = 4 wavefronts on a tile of CT channels of one group; each wavefront owns CPW
-// channels.  Per k-step (8 FFTs) the workgroup streams the live rows' spectra -- 16-byte
-// chunks of 4 channels at one FFT, contiguous in the channel-major layout the F-engine writes
-// -- into LDS as [channel][row][8 FFTs], double buffered, so each lane can read its MFMA
-// operand (one row, 4 consecutive FFTs = 8 halves) with one 16-byte LDS load.  Rows beyond
-// the group's inputs stay zero in LDS and are never fetched.
+// One workgroup = one tile of 8 channels of one group; each wavefront owns CPW channels
+// (4 wavefronts x 2 for one 32-row tile, 8 wavefronts x 1 for two).  The F-engine writes the
+// 16-bit spectra channel-tile-blocked ([row][C/8][t][8]), so per k-step (8 FFTs) every input
+// row contributes one contiguous 256-byte run.  The workgroup streams those runs into LDS as
+// [channel][row][8 FFTs], double buffered, so each lane reads its MFMA operand (one row,
+// 4 consecutive FFTs = 8 halves) with one 16-byte LDS load.  Rows beyond the group's inputs
+// stay zero in LDS and are never fetched.
 template <int NTILE>
 struct XTile {
   static constexpr int CPW = (NTILE == 1) ? 2 : 1;      // channels per wavefront
-  static constexpr int CT = 4 * CPW;                     // channels per workgroup
+  static constexpr int CT = 8;                           // channels per workgroup
+  static constexpr int NW = CT / CPW;                    // wavefronts per workgroup
+  static constexpr int NTH = 64 * NW;
   static constexpr int R = 32 * NTILE;                   // operand rows
   static constexpr int CSTR = R * 8 + 8;                 // 32-bit words per channel in LDS (padded:
                                                          // conflict-free chunk stores)
-  static constexpr int QPR = 8 * (CT / 4);               // 16-byte chunks per row per k-step
-  static constexpr int QMAX = (R * QPR + 255) / 256;     // chunks per thread per k-step
+  static constexpr int QPR = 16;                         // 16-byte chunks per row per k-step
+  static constexpr int QMAX = (R * QPR + NTH - 1) / NTH; // chunks per thread per k-step
 };
 
 template <typename V8, int NTILE>
-__global__ void __launch_bounds__(256) k_xmac_mfma(XMfmaArgs a) {
+__global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
   using T = XTile<NTILE>;
   constexpr int NPAIR = NTILE * (NTILE + 1) / 2;
-  constexpr int CPW = T::CPW, CT = T::CT, CSTR = T::CSTR, QPR = T::QPR, QMAX = T::QMAX;
+  constexpr int CPW = T::CPW, CT = T::CT, CSTR = T::CSTR, QPR = T::QPR, QMAX = T::QMAX, NTH = T::NTH;
   __shared__ uint4 lds_raw[2][(CT * CSTR) / 4];
   uint32_t* lds0 = reinterpret_cast<uint32_t*>(lds_raw[0]);
   uint32_t* lds1 = reinterpret_cast<uint32_t*>(lds_raw[1]);
-  // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.  Remap so
-  // that every XCD walks a contiguous range of channel tiles: neighbouring tiles share the
-  // 128-byte lines of every input row and should hit the same L2.
-  const int nblk = gridDim.x * gridDim.y;
-  const int b = blockIdx.y * gridDim.x + blockIdx.x;
-  int lb = b;
-  if ((nblk & 7) == 0) lb = (b & 7) * (nblk >> 3) + (b >> 3);
-  const DevXGroup g = a.groups[lb / gridDim.x];
-  const int c0 = (lb % gridDim.x) * CT;
+  const DevXGroup g = a.groups[blockIdx.y];
+  const int ct = blockIdx.x;
+  const int c0 = ct * CT;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int row = lane & 31, half = lane >> 5;
   const int nq = g.nin * QPR;
-  const uint32_t* gbase = a.spec16 + g.spec_off + c0;
+  const uint32_t* gbase = a.spec16 + g.spec_off + (long long)ct * a.chunkcap * 8;
 
-  for (int i = tid; i < 2 * (CT * CSTR) / 4; i += 256)
+  for (int i = tid; i < 2 * (CT * CSTR) / 4; i += NTH)
     (&lds_raw[0][0])[i] = make_uint4(0u, 0u, 0u, 0u);
 
   floatx16 accr[CPW][NPAIR], acci[CPW][NPAIR];
@@ -133,27 +130,26 @@ __global__ void __launch_bounds__(256) k_xmac_mfma(XMfmaArgs a) {
       }
   }
 
+  // chunk q of a k-step: row r = q / 16, FFT t = (q % 16) / 2, channel quad h = q % 2
   uint4 buf[QMAX];
   auto gload = [&](int k) {
 #pragma unroll
     for (int i = 0; i < QMAX; ++i) {
-      const int q = tid + 256 * i;
+      const int q = tid + NTH * i;
       if (q < nq) {
-        const int r = q / QPR, rem = q - r * QPR;
-        const int t = rem / (CT / 4), cq = rem - t * (CT / 4);
-        buf[i] = *reinterpret_cast<const uint4*>(gbase + (long long)r * g.rowstride +
-                                                 (long long)(8 * k + t) * g.C + 4 * cq);
+        const int r = q >> 4, rem = q & 15;
+        buf[i] = *reinterpret_cast<const uint4*>(gbase + (long long)r * g.rowstride + (long long)(8 * k) * 8 + rem * 4);
       }
     }
   };
   auto lstore = [&](uint32_t* L) {
 #pragma unroll
     for (int i = 0; i < QMAX; ++i) {
-      const int q = tid + 256 * i;
+      const int q = tid + NTH * i;
       if (q < nq) {
-        const int r = q / QPR, rem = q - r * QPR;
-        const int t = rem / (CT / 4), cq = rem - t * (CT / 4);
-        uint32_t* d = L + (4 * cq) * CSTR + r * 8 + t;
+        const int r = q >> 4, rem = q & 15;
+        const int t = rem >> 1, h = rem & 1;
+        uint32_t* d = L + (4 * h) * CSTR + r * 8 + t;
         d[0] = buf[i].x; d[CSTR] = buf[i].y; d[2 * CSTR] = buf[i].z; d[3 * CSTR] = buf[i].w;
       }
     }
@@ -303,16 +299,15 @@ int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t s
 int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st) {
   XMfmaArgs b = a;
   b.groups = a.groups + group0;
-  const int ct = ntile == 1 ? XTile<1>::CT : XTile<2>::CT;
-  if (C % ct) return -2;
-  dim3 grid(C / ct, ngroup_launch);
+  if (C % 8) return -2;
+  dim3 grid(C / 8, ngroup_launch);
   if (a.mode == 1) {
-    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 1>), grid, dim3(256), 0, st, b);
-    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 2>), grid, dim3(256), 0, st, b);
+    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 1>), grid, dim3(XTile<1>::NTH), 0, st, b);
+    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 2>), grid, dim3(XTile<2>::NTH), 0, st, b);
     else return -2;
   } else {
-    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 1>), grid, dim3(256), 0, st, b);
-    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 2>), grid, dim3(256), 0, st, b);
+    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 1>), grid, dim3(XTile<1>::NTH), 0, st, b);
+    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 2>), grid, dim3(XTile<2>::NTH), 0, st, b);
     else return -2;
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
