@@ -160,9 +160,9 @@ def cpu_baseline(budget_s: float = 15.0):
         samples = samples_per_subint(cfg)
         return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": nthreads,
                 "kind": "port",
-                "sample": f"C restatement (oracle/fxport.c), {nblk} FFT blocks of the 10-station "
-                          f"dual-pol 8x32 MHz nchan-1024 workload ({samples/1e6:.0f} Msamples), "
-                          f"{nthreads} threads, {dt:.2f} s"}
+                "sample": f"C restatement (oracle/fxport.c, FFT: {fxport.fft_backend()}), {nblk} FFT "
+                          f"blocks of the 10-station dual-pol 8x32 MHz nchan-1024 workload "
+                          f"({samples/1e6:.0f} Msamples), {nthreads} threads, {dt:.2f} s wall"}
     import fxoracle as fo
     nblk = 4
     cfg = cf.standard_config(10, 2, freqs, 32.0, 1024, subint_ns=nblk * 32000, seed=20261202)
@@ -176,6 +176,42 @@ def cpu_baseline(budget_s: float = 15.0):
                       f"({samples/1e6:.1f} Msamples), 1 thread, {dt:.2f} s"}
 
 
+def measured_traffic(path: Path, ffts_per_launch: float):
+    """HBM bytes per F-engine launch from the committed rocprofv3 --pmc summary: FETCH_SIZE
+    and WRITE_SIZE (KiB per dispatch) each scaled by what the same counter reads for a plain
+    device copy of known size (`bench.py --copy-probe`), rescaled to this launch's FFT count."""
+    if not path.exists():
+        return None
+    kv = {}
+    for line in path.read_text().splitlines():
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            try:
+                kv[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    need = ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "copy_fetch_kib", "copy_write_kib",
+            "copy_bytes", "ffts_per_dispatch")
+    if not all(k in kv for k in need):
+        return None
+    fscale = kv["copy_bytes"] / (kv["copy_fetch_kib"] * 1024.0)
+    wscale = kv["copy_bytes"] / (kv["copy_write_kib"] * 1024.0)
+    per = (kv["fetch_kib_per_dispatch"] * fscale + kv["write_kib_per_dispatch"] * wscale) * 1024.0
+    return per * ffts_per_launch / kv["ffts_per_dispatch"]
+
+
+def copy_probe():
+    """A plain 1 GiB device-to-device copy, the yardstick for scaling FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    x.fill_(1.0)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print("copy probe: 1073741824 bytes copied", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,7 +219,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copy for PMC scaling")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the FX path on a single MI355X; "
               "multi-GPU sharding (whole subints or subband groups per GPU) is not part of it.",
@@ -258,7 +298,8 @@ def main():
                     "x_engine_useful_tflops": x_engine_flops(cfg, B) / (tm["x_ms"] / args.steps / 1e3) / 1e12},
         "roofline": {"kernel": "k_fengine<2048,false> (fused unpack+rotate+FFT+post-F)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(ROOT / "profiles" / "k_fengine_2048_traffic.txt", ffts_per_launch),
                      "avg_launch_ms": f_avg_ms, "algorithmic_bytes_per_launch": fbytes},
         "plain_fp32": plain,
         "cpu_baseline": cpu,

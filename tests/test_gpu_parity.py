@@ -169,3 +169,19 @@ def test_fp32_cfg2(gpu):
     got, ref, e, lay = _run_and_compare(cfg)
     _say("cfg2 fp32 cross max", max(e["cross"]))
     assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+
+
+def test_fp32_device_vs_c_port(gpu):
+    """Both FP32: the device path and the C restatement of the reference's FP32 arithmetic."""
+    import fxport
+    if not fxport.available():
+        pytest.skip("libfxport.so not built")
+    cfg = case_cfg1(B=64, nstations=6)
+    core = _core(cfg)
+    data, ctrl = slot(cfg, seed=21)
+    got = core.processdata((0, 1, 0), data, ctrl)
+    ref, _ = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=4)
+    e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref.astype(np.complex128))
+    _say("device fp32 vs C port: cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+    core.close()

@@ -1,0 +1,37 @@
+"""The C restatement (oracle/fxport.c) against the FP64 oracle on the same inputs (CPU)."""
+import numpy as np
+import pytest
+
+import fxoracle as fo
+from fxcases import case_cfg0, case_cfg1, case_cfg2, record_errors, slot
+
+
+def _port():
+    import fxport
+    if not fxport.available():
+        pytest.skip("libfxport.so not built")
+    return fxport
+
+
+@pytest.mark.parametrize("name", ["cfg0", "cfg0_lsb", "cfg1", "cfg2"])
+def test_port_matches_fp64_oracle(name):
+    fxport = _port()
+    cfg = {"cfg0": lambda: case_cfg0(B=64), "cfg0_lsb": lambda: case_cfg0(B=32, lsb=True),
+           "cfg1": lambda: case_cfg1(B=16, nstations=4, nfreq=4),
+           "cfg2": lambda: case_cfg2(B=8, xengine="fp32", nstations=6)}[name]()
+    data, ctrl = slot(cfg)
+    got, w = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=3)
+    ref, st = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
+    e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+    np.testing.assert_array_equal(w, np.array([s.weight for s in st], dtype=np.float32))
+
+
+def test_port_thread_count_invariance():
+    fxport = _port()
+    cfg = case_cfg0(B=96)
+    data, ctrl = slot(cfg)
+    a, _ = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=1)
+    b, _ = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=5)
+    rel = np.mean(np.abs(a - b)) / np.mean(np.abs(a))
+    assert rel < 1e-6
