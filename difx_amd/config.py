@@ -267,3 +267,76 @@ def standard_config(nstations: int, npol: int, freqs_mhz: Sequence[float], bw_mh
     return Config(freqs=freqs, datastreams=dss, baselines=bls, subint_ns=subint_ns,
                   fringe_rot_order=fringe_rot_order, xengine=xengine, chunk_ffts=chunk_ffts,
                   write_cross_autocorrs=write_cross_autocorrs)
+
+
+def batched_config(nstations: int, groups: Sequence[Tuple[float, int, Sequence[float]]],
+                   zooms: Sequence[Tuple[int, int, float, float]], subint_ns: int, npol: int = 2,
+                   nbits: int = 2, payloadbytes: int = 8000, seed: int = 1, xengine: str = "fp32",
+                   chunk_ffts: int = 0, write_cross_autocorrs: bool = True, max_delay_us: float = 1e4,
+                   max_rate_us_per_s: float = 1.0, model_nsamples: int = 1,
+                   correlate_parents: bool = True) -> Config:
+    """Uneven bands: one datastream per (station, bandwidth group), since a DiFX datastream
+    has one recorded bandwidth and channel count (mode.h:71, configuration.cpp:937-941).
+
+    groups: (bandwidth MHz, nchan, band-edge sky frequencies) per group; every group must
+            give the same FFT duration nchan / bandwidth so that all datastreams run the same
+            blockspersend (configuration.cpp:1843-1852, 3008-3028).
+    zooms:  (group, local parent freq, offset of the zoom's lower edge in MHz, zoom bandwidth
+            MHz); a zoom band is a window of its parent's channels at the parent resolution
+            (configuration.cpp:1733, mode.cpp:184-196), recorded in every polarisation.
+
+    Datastreams are ordered station-major.  Baselines pair the datastreams of one group on two
+    stations and correlate its parent freqs (unless correlate_parents is False) and its zoom
+    freqs, each with npol**2 products (parallel hands first)."""
+    rng = np.random.default_rng(seed)
+    pols = "RL"[:npol]
+    freqs: List[Freq] = []
+    gfreqs: List[List[int]] = []
+    for bw, nchan, edges in groups:
+        gfreqs.append([])
+        for e in edges:
+            gfreqs[-1].append(len(freqs))
+            freqs.append(Freq(float(e), float(bw), int(nchan)))
+    gz: List[List[Tuple[int, int, int]]] = [[] for _ in groups]   # (zoom freq idx, parent lf, choff)
+    for g, lf, off, zbw in zooms:
+        bw, nchan, edges = groups[g]
+        chw = bw / nchan
+        zn = int(round(zbw / chw))
+        choff = int(off / chw + 0.5)
+        if abs(zn * chw - zbw) > TINY or choff + zn > nchan:
+            raise ValueError("zoom band is not a whole channel window of its parent")
+        gz[g].append((len(freqs), lf, choff))
+        freqs.append(Freq(float(edges[lf]) + choff * chw, float(zbw), zn))
+    dss: List[Datastream] = []
+    for s in range(nstations):
+        model = random_delay_model(rng, model_nsamples, max_delay_us, max_rate_us_per_s)
+        for g in range(len(groups)):
+            nf = len(gfreqs[g])
+            recbands = [(lf, p) for lf in range(nf) for p in pols]
+            zb = [(lz, p) for lz in range(len(gz[g])) for p in pols]
+            dss.append(Datastream(recfreqs=list(gfreqs[g]), recbands=recbands, model=model.copy(),
+                                  nbits=nbits, payloadbytes=payloadbytes, zoomfreqs=list(gz[g]),
+                                  zoombands=zb))
+    ng = len(groups)
+    bls: List[Baseline] = []
+    for a in range(nstations):
+        for b in range(a + 1, nstations):
+            for g in range(ng):
+                nrec = len(gfreqs[g]) * npol
+                fl = []
+
+                def prods(first):
+                    idx = {p: first + k for k, p in enumerate(pols)}
+                    pr = [(idx[p], idx[p]) for p in pols]
+                    if npol == 2:
+                        pr += [(idx["R"], idx["L"]), (idx["L"], idx["R"])]
+                    return pr
+
+                if correlate_parents:
+                    for lf, fi in enumerate(gfreqs[g]):
+                        fl.append((fi, prods(lf * npol)))
+                for lz, (zfi, _, _) in enumerate(gz[g]):
+                    fl.append((zfi, prods(nrec + lz * npol)))
+                bls.append(Baseline(a * ng + g, b * ng + g, fl))
+    return Config(freqs=freqs, datastreams=dss, baselines=bls, subint_ns=subint_ns, xengine=xengine,
+                  chunk_ffts=chunk_ffts, write_cross_autocorrs=write_cross_autocorrs)

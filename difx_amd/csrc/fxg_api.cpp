@@ -72,7 +72,16 @@ struct fxg_ctx {
   double model_inc = 120.0, scan_start = 0.0;
   // derived
   int nds = 0, nfreqs = 0, nbl = 0;
-  int B = 0, L = 0, iscomplex = 0, flaglen = 0;
+  int B = 0, flaglen = 0;
+  // FFT-length classes: datastreams sharing (L, sampling) run in one F-engine launch
+  struct FClass {
+    int L = 0, cplx = 0;
+    std::vector<int> dsl;        // datastream indices
+    long long tw_off = 0;        // offset of this L's twiddle tables in d_twiddle
+    long long dsl_off = 0;       // offset of dsl in d_dslist
+  };
+  std::vector<FClass> fclasses;
+  int* d_dslist = nullptr;
   std::vector<long long> databytes;
   int bpa = 1;
   long long coreresultlength = 0;
@@ -155,8 +164,8 @@ int dev_upload(fxg_ctx* ctx, T** dptr, const std::vector<T>& h) {
 }
 
 int validate_and_derive(fxg_ctx* ctx) {
-  if (ctx->rotorder != 1)
-    return fail(ctx, FXG_E_INVALID, "only fringe rotation order 1 is supported by this build");
+  if (ctx->rotorder < 0 || ctx->rotorder > 2)
+    return fail(ctx, FXG_E_INVALID, "fringe rotation order must be 0, 1 or 2");
   if (ctx->nds < 1) return fail(ctx, FXG_E_INVALID, "no datastreams");
   // blockspersend from the first datastream's first recorded freq (configuration.cpp:1843-1852)
   {
@@ -176,6 +185,8 @@ int validate_and_derive(fxg_ctx* ctx) {
       return fail(ctx, FXG_E_INVALID, "only 2-bit real and 8-bit real/complex sampling are supported");
     if (ds.complex_sampling && ds.format != FXG_FMT_VDIF)
       return fail(ctx, FXG_E_INVALID, "complex sampling requires VDIF");
+    if (ds.complex_sampling && ctx->rotorder == 0)       // mode.cpp:1130-1133
+      return fail(ctx, FXG_E_INVALID, "post-F (order 0) fringe rotation of complex data is not supported");
     if (ds.nrecfreqs < 1 || ds.nrecfreqs > FXG_MAX_FREQS || ds.nrecbands < 1 || ds.nrecbands > FXG_MAX_BANDS)
       return fail(ctx, FXG_E_INVALID, "bad recorded freq/band count");
     if (ds.nbits == 2 && !is_pow2(ds.nrecbands))
@@ -194,15 +205,23 @@ int validate_and_derive(fxg_ctx* ctx) {
       if (f.chanavg < 1 || f.nchan % f.chanavg)
         return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
     }
-    int L = ds.complex_sampling ? f0.nchan : 2 * f0.nchan;
-    if (d == 0) { ctx->L = L; ctx->iscomplex = ds.complex_sampling; }
-    if (L != ctx->L || ds.complex_sampling != ctx->iscomplex)
-      return fail(ctx, FXG_E_INVALID, "all datastreams must share FFT length and sampling in this build");
-    if (!is_pow2(L) || L < 256 || L > 4096)
-      return fail(ctx, FXG_E_INVALID, "FFT length must be a power of two in [256, 4096]");
+    const int L = ds.complex_sampling ? f0.nchan : 2 * f0.nchan;
+    if (!is_pow2(L) || L < 256 || L > FXG_MAX_L)
+      return fail(ctx, FXG_E_INVALID, "FFT length must be a power of two in [256, 8192]");
+    // every datastream: a whole number of nanoseconds per FFT and of FFTs per subint
+    // (configuration.cpp:3008-3028)
+    {
+      const double ffttime = 1000.0 * f0.nchan / f0.bw_mhz;
+      const double nffts = ctx->subint_ns / ffttime;
+      if (std::fabs(ffttime - (int)(ffttime + 0.5)) > 1e-8)
+        return fail(ctx, FXG_E_INVALID, "FFT time is not a whole number of nanoseconds");
+      if (std::fabs(nffts - (int)(nffts + 0.5)) > TINY)
+        return fail(ctx, FXG_E_INVALID, "subint is not an integer number of FFTs for a datastream");
+    }
     int S = ds.arraystridelen ? ds.arraystridelen : calcstridelength(f0.nchan);
     const int C = f0.nchan;
-    if (!is_pow2(S) || (L / 8) % S || S + L / S > L / 4 || S + C / S > L / 4)
+    const int phcap = std::min(L / 4, 512);        // LDS phasor tables of the F-engine
+    if (!is_pow2(S) || (L / 8) % S || S + L / S > phcap || S + C / S > phcap)
       return fail(ctx, FXG_E_INVALID, "unsupported array stride length");
     const int pay = ds.framebytes - ds.headerbytes;
     const int bits = ds.nbits * (ds.complex_sampling ? 2 : 1) * ds.nrecbands;
@@ -255,17 +274,20 @@ int validate_and_derive(fxg_ctx* ctx) {
                                      : ds.zoomfreq_index[ds.zoomband_localfreq[bi - ds.nrecbands]];
           if (fi != b.freq_index[k])
             return fail(ctx, FXG_E_INVALID, "baseline products must correlate bands of the baseline freq");
-          if (bi >= ds.nrecbands && ctx->xengine != FXG_X_FP32)
-            return fail(ctx, FXG_E_INVALID, "zoom bands need the FP32 X-engine in this build");
         }
       }
       if (f.nchan % f.chanavg) return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
     }
   }
   if (ctx->xengine != FXG_X_FP32)
-    for (int d = 0; d < ctx->nds; ++d)
-      if (ctx->freqs[ctx->dss[d].recfreq_index[0]].nchan % 16)
-        return fail(ctx, FXG_E_INVALID, "the MFMA X-engine needs nchan to be a multiple of 16");
+    for (int d = 0; d < ctx->nds; ++d) {
+      const fxg_datastream& ds = ctx->dss[d];
+      if (ctx->freqs[ds.recfreq_index[0]].nchan % 8)
+        return fail(ctx, FXG_E_INVALID, "the MFMA X-engine needs nchan to be a multiple of 8");
+      for (int z = 0; z < ds.nzoomfreqs; ++z)
+        if (ctx->freqs[ds.zoomfreq_index[z]].nchan % 8)
+          return fail(ctx, FXG_E_INVALID, "the MFMA X-engine needs zoom nchan to be a multiple of 8");
+    }
   if (ctx->xengine < FXG_X_FP32 || ctx->xengine > FXG_X_BF16)
     return fail(ctx, FXG_E_INVALID, "unknown X-engine");
   return FXG_OK;
@@ -466,24 +488,49 @@ int build_tables(fxg_ctx* ctx) {
   if ((rc = dev_upload(ctx, &ctx->d_chanfreq, chanfreq))) return rc;
   HIPCHK(hipMalloc((void**)&ctx->d_fvalid, std::max<long long>(fvo, 16)));
   HIPCHK(hipMemset(ctx->d_fvalid, 0, std::max<long long>(fvo, 16)));
+  // FFT-length classes, in order of first appearance
+  ctx->fclasses.clear();
+  for (int d = 0; d < nds; ++d) {
+    const int L = ctx->h_ds[d].L, cplx = ctx->h_ds[d].iscomplex;
+    size_t k = 0;
+    while (k < ctx->fclasses.size() && !(ctx->fclasses[k].L == L && ctx->fclasses[k].cplx == cplx)) ++k;
+    if (k == ctx->fclasses.size()) {
+      fxg_ctx::FClass c;
+      c.L = L;
+      c.cplx = cplx;
+      ctx->fclasses.push_back(c);
+    }
+    ctx->fclasses[k].dsl.push_back(d);
+  }
   // FFT twiddles of the Stockham passes after the first, per pass (R, Ns): entries
-  // exp(-2 pi i k r / (Ns R)) for r = 1..R-1, k = 0..Ns-1, k fastest; computed in FP64
+  // exp(-2 pi i k r / (Ns R)) for r = 1..R-1, k = 0..Ns-1, k fastest; computed in FP64.
+  // One table set per FFT length.
   {
-    int log = 0;
-    while ((1 << log) < ctx->L) ++log;
-    const int n8 = log / 3, rem = log % 3;
     std::vector<float2> tw;
-    auto add_pass = [&](int R, int Ns) {
-      for (int r = 1; r < R; ++r)
-        for (int k = 0; k < Ns; ++k) {
-          const double a = -TWO_PI * (double)k * (double)r / ((double)Ns * (double)R);
-          tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
-        }
-    };
-    int ns = 8;
-    for (int p = 1; p < n8; ++p) { add_pass(8, ns); ns *= 8; }
-    if (rem) add_pass(1 << rem, ns);
+    std::vector<int> dsl;
+    std::map<int, long long> tw_of_L;
+    for (auto& c : ctx->fclasses) {
+      c.dsl_off = (long long)dsl.size();
+      dsl.insert(dsl.end(), c.dsl.begin(), c.dsl.end());
+      if (tw_of_L.count(c.L)) { c.tw_off = tw_of_L[c.L]; continue; }
+      c.tw_off = (long long)tw.size();
+      tw_of_L[c.L] = c.tw_off;
+      int log = 0;
+      while ((1 << log) < c.L) ++log;
+      const int n8 = log / 3, rem = log % 3;
+      auto add_pass = [&](int R, int Ns) {
+        for (int r = 1; r < R; ++r)
+          for (int k = 0; k < Ns; ++k) {
+            const double a = -TWO_PI * (double)k * (double)r / ((double)Ns * (double)R);
+            tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+          }
+      };
+      int ns = 8;
+      for (int p = 1; p < n8; ++p) { add_pass(8, ns); ns *= 8; }
+      if (rem) add_pass(1 << rem, ns);
+    }
     if ((rc = dev_upload(ctx, &ctx->d_twiddle, tw))) return rc;
+    if ((rc = dev_upload(ctx, &ctx->d_dslist, dsl))) return rc;
   }
   HIPCHK(hipMalloc((void**)&ctx->d_run, sizeof(DevDsRun) * nds));
   HIPCHK(hipMalloc((void**)&ctx->d_weights, sizeof(float) * (size_t)nds * ctx->B));
@@ -518,6 +565,42 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
         ctx->h_ds[d].band_xfreq[b] = g;
         ctx->h_ds[d].band_xrow[b] = (int)ctx->group_rows[g].size() - 1;
       }
+    }
+    // zoom bands correlated on baselines: rows of their zoom freq's group, written by the
+    // F-engine from the parent band's channel window
+    for (int j = 0; j < nb; ++j) {
+      const fxg_baseline& bl = ctx->bls[j];
+      for (int k = 0; k < bl.nfreqs; ++k)
+        for (int p = 0; p < bl.npolprod[k]; ++p)
+          for (int side = 0; side < 2; ++side) {
+            const int d = side ? bl.ds2 : bl.ds1;
+            const int bi = side ? bl.ds2band[k][p] : bl.ds1band[k][p];
+            const fxg_datastream& s = ctx->dss[d];
+            if (bi < s.nrecbands || ctx->rowof.count({d, bi})) continue;
+            const int fi = freq_of_band(s, bi);
+            if (!gof.count(fi)) {
+              gof[fi] = (int)ctx->groups.size();
+              DevXGroup g;
+              std::memset(&g, 0, sizeof(g));
+              g.C = ctx->freqs[fi].nchan;
+              ctx->groups.push_back(g);
+              ctx->group_rows.emplace_back();
+            }
+            const int g = gof[fi];
+            int rec = 0, choff = 0;
+            band_source(s, bi, &rec, &choff);
+            DevDs& x = ctx->h_ds[d];
+            if (x.nzout >= FXG_MAX_ZOUT)
+              return fail(ctx, FXG_E_INVALID, "too many correlated zoom bands in one datastream");
+            x.zout_band[x.nzout] = rec;
+            x.zout_group[x.nzout] = g;
+            x.zout_row[x.nzout] = (int)ctx->group_rows[g].size();
+            x.zout_c0[x.nzout] = choff;
+            x.zout_n[x.nzout] = ctx->freqs[fi].nchan;
+            x.nzout++;
+            ctx->rowof[{d, bi}] = {g, (int)ctx->group_rows[g].size()};
+            ctx->group_rows[g].push_back({d, bi});
+          }
     }
     for (size_t g = 0; g < ctx->groups.size(); ++g) {
       int nin = (int)ctx->group_rows[g].size();
@@ -556,14 +639,21 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
     HIPCHK(hipMalloc((void**)&ctx->d_spec16, sizeof(uint32_t) * std::max<long long>(off, 4)));
     HIPCHK(hipMemset(ctx->d_spec16, 0, sizeof(uint32_t) * std::max<long long>(off, 4)));
     HIPCHK(hipMalloc((void**)&ctx->d_vacc, sizeof(float) * std::max<long long>(aoff, 4)));
-    // nominal per-(input, channel) scale: 1/sqrt(L * <x^2>) of the 2-bit (or 8-bit) levels
-    HIPCHK(hipMalloc((void**)&ctx->d_xscale, sizeof(float) * std::max<long long>(soff, 4)));
+    // nominal per-(input, channel) scale 1/sqrt(L <x^2>) of the input's sample levels, which
+    // keeps the 16-bit spectra near unit magnitude; the accumulation divides it out again
     const double p_hi = 0.5 * std::erfc(0.95 / std::sqrt(2.0));
-    double xsq = ctx->iscomplex || ctx->dss[0].nbits == 8 ? 2.0 * (20.0 / 3.3) * (20.0 / 3.3)
-                                                          : 2 * p_hi * 3.3359 * 3.3359 + (1 - 2 * p_hi);
-    float sc = (float)(1.0 / std::sqrt((double)ctx->L * xsq));
-    if (fxg_launch_fill(ctx->d_xscale, soff, sc, ctx->stream)) return fail(ctx, FXG_E_HIP, "fill");
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<float> xs((size_t)std::max<long long>(soff, 1), 0.0f);
+    for (size_t g = 0; g < ctx->groups.size(); ++g)
+      for (size_t r = 0; r < ctx->group_rows[g].size(); ++r) {
+        const int d = ctx->group_rows[g][r].first;
+        const DevDs& x = ctx->h_ds[d];
+        const double xsq = x.iscomplex || x.nbits == 8 ? 2.0 * (20.0 / 3.3) * (20.0 / 3.3)
+                                                       : 2 * p_hi * 3.3359 * 3.3359 + (1 - 2 * p_hi);
+        const float sc = (float)(1.0 / std::sqrt((double)x.L * xsq));
+        const DevXGroup& G = ctx->groups[g];
+        for (int c = 0; c < G.C; ++c) xs[(size_t)(G.scale_off + (long long)r * G.C + c)] = sc;
+      }
+    if (int rc = dev_upload(ctx, &ctx->d_xscale, xs)) return rc;
     if (int rc = dev_upload(ctx, &ctx->d_groups, ctx->groups)) return rc;
   } else {
     long long off = 0;
@@ -674,25 +764,29 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
         }
       }
     }
+    // autocorrelations: recorded bands from the diagonal of their group, zoom bands from
+    // the channel window of their parent band (mode.cpp:184-196, 1272, 1333-1339)
     for (int d = 0; d < nds; ++d) {
       const fxg_datastream& s = ctx->dss[d];
       const int cavg = ctx->freqs[s.recfreq_index[0]].chanavg;
       long long idx = ctx->acoff[d];
       for (int w = 0; w < bpa; ++w) {
         if (w == 1 && !ctx->crossautos) break;
-        for (int b = 0; b < s.nrecbands; ++b) {
+        for (int b = 0; b < s.nrecbands + s.nzoombands; ++b) {
           const int fi = freq_of_band(s, b);
           if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
           const int nout = ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
-          int other = w == 0 ? b : crosspol_partner(s, b);
+          int rec = 0, choff = 0;
+          band_source(s, b, &rec, &choff);
+          int other = w == 0 ? rec : crosspol_partner(s, rec);
           if (other >= 0) {
-            auto g1 = ctx->rowof[{d, b}];
+            auto g1 = ctx->rowof[{d, rec}];
             auto g2 = ctx->rowof[{d, other}];
             FinMfma e;
             e.group = g1.first;
             e.row1 = g1.second;
             e.row2 = g2.second;
-            e.chan0 = 0;
+            e.chan0 = choff;
             e.dst = (int)idx;
             e.nout = nout;
             e.avg = cavg;
@@ -858,7 +952,7 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   fa.chanfreq = ctx->d_chanfreq;
   fa.fvalid = ctx->d_fvalid;
   fa.chunkcap = ctx->chunkcap;
-  fa.ds0 = 0;
+  fa.rotorder = ctx->rotorder;
   for (int c0 = 0; c0 < ctx->B; c0 += ctx->chunkcap) {
     const int n = std::min(ctx->chunkcap, ctx->B - c0);
     const int npad = (int)round_up(n, FXG_XPAD);
@@ -866,13 +960,17 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     fa.nffts = n;
     fa.nffts_pad = npad;
     const int nblocks = npad / FXG_TB;
-    // enough workgroups to fill 256 CUs several times over
-    long long wgs_per_ds = std::max(1LL, (long long)(2048 + nds - 1) / nds);
-    int bpw = (int)std::max(1LL, std::min<long long>(16, nblocks / wgs_per_ds));
-    fa.blocks_per_wg = bpw;
     {
       TimedLaunch tl(ctx, 0, st);
-      if (fxg_launch_fengine(fa, ctx->L, ctx->iscomplex, nds, st)) return fail(ctx, FXG_E_HIP, "F-engine launch");
+      for (const auto& fc : ctx->fclasses) {
+        const int ncl = (int)fc.dsl.size();
+        // enough workgroups to fill 256 CUs several times over
+        const long long wgs_per_ds = std::max(1LL, (long long)(2048 + ncl - 1) / ncl);
+        fa.blocks_per_wg = (int)std::max(1LL, std::min<long long>(16, nblocks / wgs_per_ds));
+        fa.twiddle = ctx->d_twiddle + fc.tw_off;
+        fa.dslist = ctx->d_dslist + fc.dsl_off;
+        if (fxg_launch_fengine(fa, fc.L, fc.cplx, ncl, st)) return fail(ctx, FXG_E_HIP, "F-engine launch");
+      }
     }
     if (mfma) {
       XMfmaArgs xa;
@@ -926,7 +1024,7 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
 void free_ctx(fxg_ctx* ctx) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid);
-  f(ctx->d_twiddle); f(ctx->d_params); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
+  f(ctx->d_twiddle); f(ctx->d_dslist); f(ctx->d_params); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   for (auto& sg : ctx->stage) {

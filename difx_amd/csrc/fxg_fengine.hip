@@ -2,13 +2,17 @@
 // sideband channel selection and fractional-sample correction, fused in one kernel.
 //
 // What one FFT block computes follows Mode::process (mpifxcorr/src/mode.cpp:634-1374)
-// with Mk5Mode::unpack (mk5mode.cpp:94-174), for fringe rotation order 1:
-//   - the per-FFT delay chain in FP64 (mode.cpp:671-675, 743-757, 781-788),
+// with Mk5Mode::unpack (mk5mode.cpp:94-174):
+//   - the per-FFT delay chain in FP64 (mode.cpp:671-675, 743-757, 776-825),
 //   - the rotator as S sub-phasors times NS step-phasors, FP64 phase wrapped to one turn,
-//     FP32 sin/cos and complex products (mode.cpp:883-955),
+//     FP32 sin/cos and complex products (order 1: mode.cpp:883-955; order 2 adds a
+//     quadratic sub term raised to the step index, :956-1026; order 0 has none),
 //   - the fractional-sample phasor from FP32 channel slopes (mode.cpp:303-324, 1030-1073),
+//     carrying the post-F fringe phase for order 0 (mode.cpp:1040-1049),
 //   - an unnormalised forward complex FFT of L = fftchannels points (mode.cpp:1174-1183),
-//   - USB keeps channels [0, C), real LSB keeps [C, 2C) (mode.cpp:1185-1217).
+//   - USB keeps channels [0, C), real LSB keeps [C, 2C) (mode.cpp:1185-1217); for order 0
+//     the reference's real FFT with a conj-flip of bins 1..C (mode.cpp:1139-1156) gives the
+//     same channels, conj(X[C-c]) = X[C+c] for real input.
 // The FFT is a Stockham radix-8 (plus one radix-4 or radix-2 pass) exchange through LDS;
 // each of the L/8 threads owns the 8 points t + s*L/8, so the first pass reads the decoded
 // samples straight from registers and the last pass leaves channel t + s*L/8 in register s.
@@ -164,11 +168,14 @@ __device__ int count_valid(const uint8_t* fv, int nframes, int spf, long long g0
   return good;
 }
 
-// Per-FFT delay chain (mode.cpp:671-675, 743-757, 781-788), all FP64.
+// Per-FFT delay chain (mode.cpp:671-675, 743-757, 776-825), all FP64.
+//   order 0: integerdelay = (int)averagedelay, b = averagedelay - integerdelay (post-F phase)
+//   order 1: a = d2 - d0, b = d0 + (d1 - (a/2 + d0))/3 - integerdelay
+//   order 2: a = q0, b = q1 + 2 i q0, c = q2 + i q1 + i^2 q0 - integerdelay
 __device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_t* fv, int i,
-                              int offsec, int offns, double scan_start_sec) {
+                              int offsec, int offns, double scan_start_sec, int order) {
   FftDelay r;
-  r.ok = 0; r.weight = 0.0f; r.n0 = 0; r.fse = 0.0f; r.a = 0; r.b = 0; r.intdelay = 0;
+  r.ok = 0; r.weight = 0.0f; r.n0 = 0; r.fse = 0.0f; r.a = 0; r.b = 0; r.c = 0; r.intdelay = 0;
   r.intwall = 0; r.fracwall = 0;
   if (!run.valid || run.datalen <= 1) return r;
   if (((run.vflags[i / FLAGS_PER_INT] >> (i % FLAGS_PER_INT)) & 1) == 0) return r;
@@ -191,19 +198,38 @@ __device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_
   r.weight = w;
   r.fse = (float)(starttime - n0 * dt);
   const double di = (double)i;
-  const double d0 = q[0] * di * di + q[1] * di + q[2];
-  const double d1 = q[0] * (di + 0.5) * (di + 0.5) + q[1] * (di + 0.5) + q[2];
-  const double d2 = q[0] * (di + 1) * (di + 1) + q[1] * (di + 1) + q[2];
-  const double a = d2 - d0;
-  double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
-  const int intdelay = (int)b;
-  b -= intdelay;
-  r.a = a; r.b = b; r.intdelay = intdelay;
+  if (order == 1) {
+    const double d0 = q[0] * di * di + q[1] * di + q[2];
+    const double d1 = q[0] * (di + 0.5) * (di + 0.5) + q[1] * (di + 0.5) + q[2];
+    const double d2 = q[0] * (di + 1) * (di + 1) + q[1] * (di + 1) + q[2];
+    const double a = d2 - d0;
+    double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+    const int intdelay = (int)b;
+    b -= intdelay;
+    r.a = a; r.b = b; r.intdelay = intdelay;
+  } else if (order == 2) {
+    const double b = q[1] + di * q[0] * 2.0;
+    double c = q[2] + di * q[1] + (double)((long long)i * i) * q[0];
+    const int intdelay = (int)c;
+    c -= intdelay;
+    r.a = q[0]; r.b = b; r.c = c; r.intdelay = intdelay;
+  } else {
+    const int intdelay = (int)avgdelay;
+    r.b = avgdelay - intdelay;
+    r.intdelay = intdelay;
+  }
   const double walltime = scan_start_sec + offsec + offns / 1.0e9 + fftstart / 1.0e6;
   r.intwall = (int)walltime;
   r.fracwall = walltime - r.intwall;
   return r;
 }
+
+// LDS capacity of the rotator and fractional-sample phasor tables: S + NS and S + NFS
+// entries, at most a quarter of the FFT length and at most 512 (checked at set-up).
+template <int L>
+struct PhasorCap {
+  static constexpr int N = (L / 4 < 512) ? L / 4 : 512;
+};
 
 // Per-datastream constants the F-engine reads, copied to registers once per workgroup.
 struct DsConst {
@@ -226,16 +252,28 @@ __device__ __forceinline__ FreqConst load_freq(const DevRecFreq& rf) {
 }
 
 // Fringe rotator sub-phasors (j < S) and step-phasors (S <= j < S+NS) for one freq
-// (mode.cpp:883-955).  FP64 phase in turns, wrapped by truncation, FP32 sin/cos.
-__device__ void build_rotator(float2* srot, const DsConst& k, const FreqConst& rf,
+// (order 1: mode.cpp:883-955; order 2: mode.cpp:956-1026, whose quadratic term is the
+// per-sub argument squad[j], applied as a power of the step index).  FP64 phase in turns,
+// wrapped by truncation, FP32 sin/cos.
+template <int ORDER>
+__device__ void build_rotator(float2* srot, float* squad, const DsConst& k, const FreqConst& rf,
                               const FftDelay& fd, int L, int t, int nt) {
   const int S = k.S, NS = k.NS;
   const double lofreq = rf.lofreq;
+  const double sx1 = (double)S / (double)L;                              // stepxoff[1]
   for (int j = t; j < S + NS; j += nt) {
     double ph;
     if (j < S) {
       const double xoff = (double)j / (double)L;                        // subxoff
-      const double xval = __dadd_rn(__dmul_rn(xoff, fd.a), fd.b);       // subxval
+      double xval;                                                       // subxval
+      if (ORDER == 1) {
+        xval = __dadd_rn(__dmul_rn(xoff, fd.a), fd.b);
+      } else {
+        xval = __dadd_rn(__dmul_rn(xoff, __dadd_rn(fd.b, __dmul_rn(fd.a, sx1))), fd.c);
+        const double qx = __dmul_rn(xoff, __dmul_rn(__dmul_rn(2.0, fd.a), sx1));   // subquadxval
+        const double qph = __dmul_rn(qx, lofreq);
+        squad[j] = (float)(-TWO_PI_D * (qph - trunc(qph)));
+      }
       ph = __dmul_rn(xval, lofreq);
       if (k.fractionallo)
         ph = __dadd_rn(ph, __dmul_rn(lofreq - (double)(int)lofreq, (double)fd.intdelay));
@@ -248,7 +286,10 @@ __device__ void build_rotator(float2* srot, const DsConst& k, const FreqConst& r
     } else {
       const int s = j - S;
       const double xoff = (double)(s * S) / (double)L;                  // stepxoff
-      ph = __dmul_rn(__dmul_rn(xoff, fd.a), lofreq);
+      if (ORDER == 1)
+        ph = __dmul_rn(__dmul_rn(xoff, fd.a), lofreq);
+      else
+        ph = __dmul_rn(__dadd_rn(__dmul_rn(xoff, fd.b), __dmul_rn(__dmul_rn(xoff, xoff), fd.a)), lofreq);
       if (rf.looff) {
         const double toff = s * S * k.sampletime_us / 1e6;               // steptoff
         ph = __dadd_rn(ph, __dmul_rn(toff, -rf.lo_offset_hz));
@@ -262,14 +303,24 @@ __device__ void build_rotator(float2* srot, const DsConst& k, const FreqConst& r
 }
 
 // Fractional-sample phasors: sub (j < S) and step (S <= j < S+NFS) (mode.cpp:1030-1073).
+// Order 0 adds the post-F fringe phase to the sub arguments (mode.cpp:1040-1049).
+template <int ORDER>
 __device__ void build_frac(float2* sfrac, const DsConst& k, const FreqConst& rf, bool iscomplex,
-                           const float* __restrict__ cf, float fse, int t, int nt) {
+                           const float* __restrict__ cf, const FftDelay& fd, int t, int nt) {
   const int S = k.S, NFS = k.NFS;
-  const float e = (float)((double)fse - rf.clockoff + rf.halfdelta);
+  const float e = (float)((double)fd.fse - rf.clockoff + rf.halfdelta);
   const float* sub = cf;
   const float* step = rf.lsb && !iscomplex ? cf + S + NFS : cf + S;
+  float prf = 0.0f;
+  if (ORDER == 0) {
+    const double lofreq = rf.lofreq;
+    double pr = fd.b * lofreq;
+    if (k.fractionallo) pr += fd.intdelay * (lofreq - (int)lofreq);
+    pr -= ((double)fd.intwall + fd.fracwall) * rf.lo_offset_hz;
+    prf = (float)(-TWO_PI_D * (pr - (int)pr));
+  }
   for (int j = t; j < S + NFS; j += nt) {
-    const float arg = (j < S) ? sub[j] * e : step[j - S] * e;
+    const float arg = (j < S) ? (ORDER == 0 ? sub[j] * e + prf : sub[j] * e) : step[j - S] * e;
     float sn, cs;
     sincosf(arg, &sn, &cs);
     sfrac[j] = make_float2(cs, sn);
@@ -290,7 +341,7 @@ __device__ __forceinline__ uint32_t pack16(float2 v, int mode) {
 // Per-FFT parameters of one chunk, one thread per (datastream, FFT): the FP64 delay chain
 // and the data weight, computed once instead of by every thread of the F-engine.
 __global__ void k_fftparams(FengineArgs args) {
-  const int d = args.ds0 + blockIdx.y;
+  const int d = args.dslist[blockIdx.y];
   const int lt = blockIdx.x * blockDim.x + threadIdx.x;
   if (lt >= args.nffts_pad) return;
   const DevDs& ds = args.ds[d];
@@ -300,7 +351,8 @@ __global__ void k_fftparams(FengineArgs args) {
   fd.intwall = 0; fd.fracwall = 0;
   if (lt < args.nffts) {
     const int i = args.chunk0 + lt;
-    fd = fft_delay(ds, run, args.fvalid + ds.fvalid_off, i, args.offsec, args.offns, args.scan_start_sec);
+    fd = fft_delay(ds, run, args.fvalid + ds.fvalid_off, i, args.offsec, args.offns, args.scan_start_sec,
+                   args.rotorder);
     args.weights[(long long)d * args.B + i] = fd.weight;
   }
   args.params[(long long)d * args.chunkcap + lt] = fd;
@@ -310,7 +362,7 @@ __global__ void k_fftparams(FengineArgs args) {
 #ifndef FXG_F_MINWAVES
 #define FXG_F_MINWAVES 1
 #endif
-template <int L, bool CPLX>
+template <int L, bool CPLX, int ORDER>
 __global__ void __launch_bounds__(L / 8, FXG_F_MINWAVES)
 k_fengine(FengineArgs args) {
   using P = Plan<L>;
@@ -319,11 +371,12 @@ k_fengine(FengineArgs args) {
 #ifndef FXG_TW_GLOBAL
   __shared__ float2 stw[P::TWN];
 #endif
-  __shared__ float2 srot[L / 4];
-  __shared__ float2 sfrac[L / 4];
+  __shared__ float2 srot[PhasorCap<L>::N];
+  __shared__ float2 sfrac[PhasorCap<L>::N];
+  __shared__ float squad[ORDER == 2 ? PhasorCap<L>::N : 1];
 
   const int t = threadIdx.x;
-  const int d = args.ds0 + blockIdx.y;
+  const int d = args.dslist[blockIdx.y];
   const DevDs* __restrict__ dsp = args.ds + d;
   DsConst k;
   k.C = dsp->C; k.S = dsp->S; k.NS = dsp->NS; k.NFS = dsp->NFS; k.nrec = dsp->nrecbands;
@@ -376,6 +429,14 @@ k_fengine(FengineArgs args) {
             }
           }
         }
+        if (out_mode != 0) {
+          for (int z = 0; z < dsp->nzout; ++z) {
+            const int g = dsp->zout_group[z], nz = dsp->zout_n[z];
+            uint32_t* __restrict__ o = args.spec16 + args.xgroups[g].spec_off +
+                                       (long long)dsp->zout_row[z] * args.xgroups[g].rowstride;
+            for (int cz = t; cz < nz; cz += NT) o[(cz >> 3) * cap * 8 + lt * 8 + (cz & 7)] = 0u;
+          }
+        }
         continue;
       }
 
@@ -411,18 +472,38 @@ k_fengine(FengineArgs args) {
         const FreqConst rf = load_freq(dsp->rf[f]);
         if (rf.nbands == 0) continue;
         __syncthreads();                          // previous users of srot/sfrac are done
-        build_rotator(srot, k, rf, fd, L, t, NT);
-        build_frac(sfrac, k, rf, CPLX, cf, fd.fse, t, NT);
+        if (ORDER != 0) build_rotator<ORDER>(srot, squad, k, rf, fd, L, t, NT);
+        build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, t, NT);
         __syncthreads();
         for (int kb = 0; kb < rf.nbands; ++kb) {
           const int band = dsp->rf[f].band[kb];
           float2 v[8];
-          const float2 sub = srot[t & (S - 1)];
+          const float2 sub = ORDER != 0 ? srot[t & (S - 1)] : make_float2(1.0f, 0.0f);
+          // order 2: the quadratic sub term to the power of the step index j = t/S + s*NT/S
+          // (mode.cpp:1017-1024), as A * D^s with A = e^{i j0 th}, D = e^{i (NT/S) th}
+          float2 qpow = make_float2(1.0f, 0.0f), qstep = make_float2(1.0f, 0.0f);
+          if (ORDER == 2) {
+            const float th = squad[t & (S - 1)];
+            float sn, cs;
+            sincosf((float)(t / S) * th, &sn, &cs);
+            qpow = make_float2(cs, sn);
+            sincosf((float)(NT / S) * th, &sn, &cs);
+            qstep = make_float2(cs, sn);
+          }
 #pragma unroll
           for (int s = 0; s < 8; ++s) {
             const int n = t + s * NT;
             const int st = n / S;
-            const float2 rot = st == 0 ? sub : cmul(sub, srot[S + st]);
+            float2 rot;
+            if (ORDER == 1) {
+              rot = st == 0 ? sub : cmul(sub, srot[S + st]);
+            } else if (ORDER == 2) {
+              const float2 pq = cmul(sub, qpow);
+              rot = st == 0 ? pq : cmul(pq, srot[S + st]);
+              qpow = cmul(qpow, qstep);
+            } else {
+              rot = sub;                                   // no pre-F rotation
+            }
             const bool ok = (fok >> s) & 1u;
             if (!CPLX) {
               float x = 0.0f;
@@ -434,7 +515,8 @@ k_fengine(FengineArgs args) {
                   x = lut8(data[wv[s] + band]);
                 }
               }
-              v[s] = make_float2(x * rot.x, x * rot.y);          // (x + 0i) * rotator
+              v[s] = ORDER == 0 ? make_float2(x, 0.0f)           // real FFT input
+                                : make_float2(x * rot.x, x * rot.y);  // (x + 0i) * rotator
             } else {
               float2 x = make_float2(0.f, 0.f);
               if (ok) {
@@ -467,11 +549,31 @@ k_fengine(FengineArgs args) {
             const int c = t + m * NT;
             const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
             const float2 F = cmul(v[s], frq);
+            v[m] = F;                                // channel c, kept for the zoom windows
             if (out_mode == 0) {
               o32[c] = F;
             } else {
               const float sc = scp[c];
               o16[(c >> 3) * cap * 8 + (c & 7)] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
+            }
+          }
+          // zoom windows of this band: the same channels again, as rows of the zoom freq's group
+          if (out_mode != 0) {
+            for (int z = 0; z < dsp->nzout; ++z) {
+              if (dsp->zout_band[z] != band) continue;
+              const int g = dsp->zout_group[z], row = dsp->zout_row[z];
+              const int c0 = dsp->zout_c0[z], nz = dsp->zout_n[z];
+              uint32_t* __restrict__ oz = args.spec16 + args.xgroups[g].spec_off +
+                                          (long long)row * args.xgroups[g].rowstride + lt * 8;
+              const float* __restrict__ scz = args.xscale + args.xgroups[g].scale_off + (long long)row * nz;
+#pragma unroll
+              for (int m = 0; m < NKEEP; ++m) {
+                const int cz = t + m * NT - c0;
+                if (cz >= 0 && cz < nz) {
+                  const float sc = scz[cz];
+                  oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16(make_float2(v[m].x * sc, v[m].y * sc), out_mode);
+                }
+              }
             }
           }
         }
@@ -590,10 +692,19 @@ int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st) {
 
 template <int L, bool CPLX>
 static int launch_f(const FengineArgs& a, int nds, hipStream_t st) {
+  static_assert(L <= FXG_MAX_L, "FFT length beyond the fused F-engine");
   hipLaunchKernelGGL(k_fftparams, dim3((a.nffts_pad + 255) / 256, nds), dim3(256), 0, st, a);
   const int nblocks = a.nffts_pad / FXG_TB;
   const int gx = (nblocks + a.blocks_per_wg - 1) / a.blocks_per_wg;
-  hipLaunchKernelGGL((k_fengine<L, CPLX>), dim3(gx, nds), dim3(L / 8), 0, st, a);
+  switch (a.rotorder) {
+    case 1: hipLaunchKernelGGL((k_fengine<L, CPLX, 1>), dim3(gx, nds), dim3(L / 8), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_fengine<L, CPLX, 2>), dim3(gx, nds), dim3(L / 8), 0, st, a); break;
+    case 0:
+      if (CPLX) return -2;                   // post-F rotation of complex data (mode.cpp:1130-1133)
+      hipLaunchKernelGGL((k_fengine<L, false, 0>), dim3(gx, nds), dim3(L / 8), 0, st, a);
+      break;
+    default: return -2;
+  }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -605,6 +716,7 @@ int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, hipS
       case 1024: return launch_f<1024, true>(a, nds, st);
       case 2048: return launch_f<2048, true>(a, nds, st);
       case 4096: return launch_f<4096, true>(a, nds, st);
+      case 8192: return launch_f<8192, true>(a, nds, st);
     }
   } else {
     switch (L) {
@@ -613,6 +725,7 @@ int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, hipS
       case 1024: return launch_f<1024, false>(a, nds, st);
       case 2048: return launch_f<2048, false>(a, nds, st);
       case 4096: return launch_f<4096, false>(a, nds, st);
+      case 8192: return launch_f<8192, false>(a, nds, st);
     }
   }
   return -2;

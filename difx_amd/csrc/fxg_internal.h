@@ -6,6 +6,8 @@
 
 #define FXG_TB 4            // consecutive FFTs per output time block (MFMA K packing)
 #define FXG_XPAD 8          // FFT count padding of a device chunk (two time blocks)
+#define FXG_MAX_ZOUT 16     // zoom-band rows written by the F-engine per datastream
+#define FXG_MAX_L 8192      // longest FFT of the fused F-engine
 
 // Per recorded freq of a datastream: what the rotator and the fractional-sample
 // correction need (mode.cpp:853-931, 1030-1073).
@@ -52,6 +54,14 @@ struct DevDs {
   DevRecFreq rf[32];
   int band_xfreq[64];     // MFMA: X freq group of each recorded band
   int band_xrow[64];      // MFMA: input row of each recorded band within its group
+  // MFMA: zoom bands correlated on baselines are extra rows of their zoom freq's group,
+  // filled with the channel window [c0, c0 + n) of the parent band (mode.cpp:184-196)
+  int nzout;
+  int zout_band[FXG_MAX_ZOUT];   // parent recorded band
+  int zout_group[FXG_MAX_ZOUT];
+  int zout_row[FXG_MAX_ZOUT];
+  int zout_c0[FXG_MAX_ZOUT];
+  int zout_n[FXG_MAX_ZOUT];
 };
 
 // Per subint, per datastream runtime state written by the setup kernel.
@@ -90,7 +100,9 @@ struct FftDelay {
   int n0;               // nearestsample
   float fse;            // fracsampleerror (us)
   int intdelay;         // integerdelay (us)
-  double a, b;          // linear interpolator across the FFT (FFT-fraction units)
+  double a, b;          // interpolator across the FFT (FFT-fraction units); order 0: b is
+                        // averagedelay - integerdelay
+  double c;             // order 2: delay at the FFT start minus integerdelay
   float weight;
   int intwall;
   double fracwall;
@@ -118,7 +130,8 @@ struct FengineArgs {
   const uint8_t* fvalid;  // frame valid flags
   int chunkcap;           // FFT capacity of the chunk buffers (multiple of FXG_XPAD)
   int blocks_per_wg;
-  int ds0;                // first datastream of this launch (grid.y offset)
+  const int* dslist;      // datastreams of this launch, indexed by grid.y (one FFT-length class)
+  int rotorder;           // fringe rotation order 0, 1 or 2
 };
 
 // Setup for one subint (delay interpolators, frame validity).

@@ -286,9 +286,13 @@ class StationResult:
 
 
 def fft_params(cfg, d: int, offsets, control, interp, B: int):
-    """Per-FFT delay chain of Mode::process (mode.cpp:671-675, 743-757, 781-788) for all
-    FFT indices: nearestsample, fracsampleerror (us), linear coefficients a, b and the
-    integer delay; plus the validity mask."""
+    """Per-FFT delay chain of Mode::process (mode.cpp:671-675, 743-757, 776-825) for all
+    FFT indices: nearestsample, starttime, the interpolation coefficients a, b (and c for
+    order 2), the integer delay and the FFT-centre delay.
+
+    order 0: integerdelay = (int)averagedelay (mode.cpp:778-780); b = averagedelay - integerdelay
+    order 1: a = d2 - d0, b = d0 + (d1 - (a/2 + d0))/3 - integerdelay (mode.cpp:781-788)
+    order 2: a = q0, b = q1 + 2 i q0, c = q2 + i q1 + i^2 q0 - integerdelay (mode.cpp:800-805)"""
     ds = cfg.datastreams[d]
     L = cfg.fftchannels(d)
     dt = cfg.sampletime_us(d)
@@ -300,12 +304,77 @@ def fft_params(cfg, d: int, offsets, control, interp, B: int):
     starttime = ((offsets[1] - datasec) * 1000000.0
                  + (float(offsets[2]) - float(datans)) / 1000.0 + fftstart - avgdelay)
     ns = np.trunc(starttime / dt + 0.5).astype(np.int64)
-    d0, d1, d2 = q(i), q(i + 0.5), q(i + 1.0)
-    a = d2 - d0
-    b = d0 + (d1 - (a * 0.5 + d0)) / 3.0
-    intdelay = np.trunc(b).astype(np.int64)
-    b = b - intdelay
-    return ns, starttime, a, b, intdelay, avgdelay
+    order = cfg.fringe_rot_order
+    c = np.zeros(B)
+    if order == 0:
+        a = np.zeros(B)
+        intdelay = np.trunc(avgdelay).astype(np.int64)
+        b = avgdelay - intdelay
+    elif order == 1:
+        d0, d1, d2 = q(i), q(i + 0.5), q(i + 1.0)
+        a = d2 - d0
+        b = d0 + (d1 - (a * 0.5 + d0)) / 3.0
+        intdelay = np.trunc(b).astype(np.int64)
+        b = b - intdelay
+    elif order == 2:
+        a = np.full(B, float(interp[0]))
+        b = interp[1] + i * interp[0] * 2.0
+        c = interp[2] + i * interp[1] + i * i * interp[0]
+        intdelay = np.trunc(c).astype(np.int64)
+        c = c - intdelay
+    else:
+        raise ValueError("fringe rotation order must be 0, 1 or 2")
+    return ns, starttime, a, b, intdelay, avgdelay, c
+
+
+def rotator_phase(order: int, L: int, S: int, a: float, b: float, c: float, lofreq: float,
+                  intdelay: int, fractional_lo: bool, loff: float, sampletime_us: float,
+                  intwall: int, fracwall: float) -> np.ndarray:
+    """Fringe-rotator phase (turns) of every sample of one FFT, FP64
+    (order 1: mode.cpp:883-955; order 2: mode.cpp:956-1026, with the step/sub
+    decomposition of mode.cpp:254-261, 790-825).  The rotator is exp(-2 pi i phase).
+    Order 0 has no pre-F rotator (zeros)."""
+    n = np.arange(L)
+    if order == 0:
+        return np.zeros(L)
+    nfr = L // S
+    subxoff = np.arange(S) / float(L)
+    stepxoff = (np.arange(nfr) * S) / float(L)
+    st = sampletime_us / 1e6
+    subtoff = np.arange(S) * st
+    steptoff = np.arange(nfr) * S * st
+    if order == 1:
+        subphase = (subxoff * a + b) * lofreq
+        stepphase = (stepxoff * a) * lofreq
+        quadphase = np.zeros(S)
+    else:
+        subphase = (subxoff * (b + a * stepxoff[1]) + c) * lofreq
+        quadphase = (subxoff * (2.0 * a * stepxoff[1])) * lofreq
+        stepphase = (stepxoff * b + stepxoff * stepxoff * a) * lofreq
+    if fractional_lo:
+        subphase = subphase + (lofreq - int(lofreq)) * float(intdelay)
+    if loff != 0.0:
+        fraclo = abs(loff) - int(abs(loff))
+        if loff < 0:
+            fraclo = -fraclo
+        subphase = subphase + subtoff * (-loff) - loff * fracwall
+        stepphase = stepphase + steptoff * (-loff)
+        if fraclo > TINY or fraclo < -TINY:
+            subphase = subphase - fraclo * intwall
+    k, j = n % S, n // S
+    # order 2 builds sub x quad^j x step (mode.cpp:1017-1024); the quad term is zero for order 1
+    return subphase[k] + j * quadphase[k] + stepphase[j]
+
+
+def postf_phase(avgdelay: float, intdelay: int, lofreq: float, fractional_lo: bool, loff: float,
+                walltime: float) -> float:
+    """Order 0: the fringe phase applied after the FFT, in turns (mode.cpp:1040-1049); the
+    channel phasors become exp(i (chanfreq e - 2 pi frac(phase)))."""
+    ph = (avgdelay - intdelay) * lofreq
+    if fractional_lo:
+        ph += intdelay * (lofreq - int(lofreq))
+    ph -= walltime * loff
+    return ph
 
 
 def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
@@ -315,8 +384,9 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
     ds = cfg.datastreams[d]
     if B is None:
         B = cfg.blockspersend()
-    if cfg.fringe_rot_order != 1:
-        raise NotImplementedError("fringe rotation order 1 only")
+    order = cfg.fringe_rot_order
+    if order == 0 and ds.complex_sampling:
+        raise ValueError("post-F fringe rotation of complex data is rejected (mode.cpp:1130-1133)")
     L = cfg.fftchannels(d)
     dt = cfg.sampletime_us(d)
     nrec = len(ds.recbands)
@@ -349,7 +419,7 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
 
     vals, valid = decode_stream(data, ds)
     nsamples_buf = vals.shape[1]
-    ns, starttime, a, b, intdelay, avgdelay = fft_params(cfg, d, offsets, control, interp, B)
+    ns, starttime, a, b, intdelay, avgdelay, cq = fft_params(cfg, d, offsets, control, interp, B)
 
     # bytes-per-block ratio of the reference's bounds check (mode.cpp:124-139, 688)
     nsb = ds.nbits * (2 if usecomplex else 1)
@@ -358,13 +428,6 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
     if bpb_num == 0:
         bpb_num, bpb_den = 1, 8 // (nrec * nsb)
     S = ds.arraystridelen or calc_stride(C)
-    nfr = L // S
-    subxoff = np.arange(S) / float(L)
-    stepxoff = (np.arange(nfr) * S) / float(L)
-    sampletime_s = dt / 1e6
-    subtoff = np.arange(S) * sampletime_s
-    steptoff = np.arange(nfr) * S * sampletime_s
-    jj = np.arange(L)
     fractional_lo = any(abs(cfg.freqs[fi].freq_mhz - int(cfg.freqs[fi].freq_mhz + 0.5)) > TINY
                         for fi in ds.recfreqs)
 
@@ -400,20 +463,9 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
             if usecomplex and fr.lsb:
                 lofreq = -lofreq
             loff = ds.per_freq("lo_offset_hz")[lf]
-            # fringe rotator, order 1 (mode.cpp:883-955): phase in turns, f64
-            subphase = (subxoff * a[i] + b[i]) * lofreq
-            stepphase = (stepxoff * a[i]) * lofreq
-            if fractional_lo:
-                subphase = subphase + (lofreq - int(lofreq)) * float(intdelay[i])
-            if loff != 0.0:
-                fraclo = abs(loff) - int(abs(loff))
-                if loff < 0:
-                    fraclo = -fraclo
-                subphase = subphase + subtoff * (-loff) - loff * fracwall
-                stepphase = stepphase + steptoff * (-loff)
-                if fraclo > TINY or fraclo < -TINY:
-                    subphase = subphase - fraclo * intwall
-            phase = subphase[jj % S] + stepphase[jj // S]
+            # fringe rotator (pre-F for orders 1 and 2), phase in turns, f64
+            phase = rotator_phase(order, L, S, a[i], b[i], cq[i], lofreq, int(intdelay[i]), fractional_lo,
+                                  loff, dt, intwall, fracwall)
             rot = np.exp(-1j * TWO_PI * phase)
             # fractional-sample correction (mode.cpp:1030-1073), f64
             e = fse - ds.per_freq("clock_offset_us")[lf] + ds.per_freq("clock_offset_delta_us")[lf] / 2
@@ -423,11 +475,17 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
             else:
                 chanfreq = TWO_PI * k * bw / C
             frac = np.exp(1j * chanfreq * e)
+            if order == 0:                                               # mode.cpp:1040-1049
+                pf = postf_phase(avgdelay[i], int(intdelay[i]), lofreq, fractional_lo, loff, walltime)
+                frac = frac * np.exp(-1j * TWO_PI * pf)
             match = [j for j, (blf, _) in enumerate(ds.recbands) if blf == lf]
             for j in match:
                 x = seg[j].astype(np.complex128)
                 X = np.fft.fft(x * rot)                                  # mode.cpp:1174-1183
-                if usecomplex:
+                if order == 0:
+                    # real FFT; LSB conj-flips bins 1..C (mode.cpp:1139-1156)
+                    F = np.conj(X[C:0:-1]) if fr.lsb else X[:C]
+                elif usecomplex:
                     if fr.lsb:
                         raise NotImplementedError("complex LSB")
                     F = X[:C]

@@ -3,14 +3,19 @@
  * arithmetic (test infrastructure and timed CPU baseline; never part of the product).
  *
  * Per FFT block it follows Mode::process (mpifxcorr/src/mode.cpp:634-1374) for fringe
- * rotation order 1 with the GENERIC architecture primitives (architecture.h.in:399-1063):
+ * rotation orders 0, 1 and 2 with the GENERIC architecture primitives
+ * (architecture.h.in:399-1063):
  *   - Mk5Mode::unpack through the 2-bit / 8-bit LUTs (mk5mode.cpp:94-174, format_vdif.c:66-145,
  *     mark5_format_mark5b.c:49-86) with frame blanking;
- *   - the FP64 per-FFT delay chain (mode.cpp:671-675, 743-757, 781-788);
+ *   - the FP64 per-FFT delay chain (mode.cpp:671-675, 743-757, 776-825);
  *   - the strided FP32 rotator: FP64 phases at S sub points and NS step points, wrapped,
- *     FP32 sin/cos, complexrotator[s*S + j] = sub[j] * step[s] (mode.cpp:883-955);
- *   - the strided FP32 fractional-sample phasors (mode.cpp:1030-1073);
- *   - an unnormalised forward FFT (FFTW3 API via MKL when available, mode.cpp:1174-1183),
+ *     FP32 sin/cos, complexrotator[s*S + j] = sub[j] * step[s] (order 1, mode.cpp:883-955),
+ *     or with the sub phasors multiplied by the quadratic term after every step (order 2,
+ *     mode.cpp:956-1026); order 0 has no pre-F rotator;
+ *   - the strided FP32 fractional-sample phasors (mode.cpp:1030-1073), carrying the post-F
+ *     fringe phase for order 0 (mode.cpp:1040-1049);
+ *   - an unnormalised forward FFT (FFTW3 API via MKL when available, mode.cpp:1174-1183;
+ *     a real-to-complex FFT with the LSB conj-flip for order 0, mode.cpp:1139-1156),
  *     USB / real-LSB channel selection (mode.cpp:1185-1217), frac multiply (:1257-1261);
  * and per buffered batch the cross-multiply-accumulate of Core::processdata
  * (core.cpp:884-979, :963) and the autocorrelation products (mode.cpp:1272, 1333-1339), all
@@ -57,9 +62,11 @@ typedef struct port_prod {
 
 /* ---------------- FFT: FFTW3 API (MKL) when loadable, else a radix-2 restatement ---------- */
 typedef void* (*plan_fn)(int, float*, float*, int, unsigned);
+typedef void* (*rplan_fn)(int, float*, float*, unsigned);
 typedef void (*exec_fn)(void*);
 typedef void (*destroy_fn)(void*);
 static plan_fn p_plan = 0;
+static rplan_fn p_rplan = 0;
 static exec_fn p_exec = 0;
 static destroy_fn p_destroy = 0;
 static int fft_backend = -1;      /* 0 = own radix-2, 1 = FFTW3 API */
@@ -75,6 +82,7 @@ static void load_fftw(void) {
     p_plan = (plan_fn)dlsym(h, "fftwf_plan_dft_1d");
     p_exec = (exec_fn)dlsym(h, "fftwf_execute");
     p_destroy = (destroy_fn)dlsym(h, "fftwf_destroy_plan");
+    p_rplan = (rplan_fn)dlsym(h, "fftwf_plan_dft_r2c_1d");
     if (p_plan && p_exec && p_destroy) { fft_backend = 1; return; }
   }
   fft_backend = 0;
@@ -85,6 +93,8 @@ int fxport_fft_backend(void) { load_fftw(); return fft_backend; }
 typedef struct {
   int L;
   void* plan;
+  void* rplan;  /* real-to-complex plan (order 0) */
+  float* rin;
   cf32* in;
   cf32* out;
   cf32* tw;     /* radix-2 twiddles */
@@ -95,12 +105,15 @@ static void fft_init(fftctx* f, int L) {
   f->L = L;
   f->in = (cf32*)aligned_alloc(64, sizeof(cf32) * L);
   f->out = (cf32*)aligned_alloc(64, sizeof(cf32) * L);
+  f->rin = (float*)aligned_alloc(64, sizeof(float) * L);
   f->plan = 0;
+  f->rplan = 0;
   f->tw = 0;
   f->rev = 0;
   if (fft_backend == 1) {
     pthread_mutex_lock(&plan_mu);
     f->plan = p_plan(L, (float*)f->in, (float*)f->out, -1 /* FFTW_FORWARD */, 1u << 6 /* FFTW_ESTIMATE */);
+    if (p_rplan) f->rplan = p_rplan(L, f->rin, (float*)f->out, 1u << 6);
     pthread_mutex_unlock(&plan_mu);
   }
   if (!f->plan) {
@@ -122,7 +135,11 @@ static void fft_init(fftctx* f, int L) {
 }
 
 static void fft_free(fftctx* f) {
-  if (f->plan) { pthread_mutex_lock(&plan_mu); p_destroy(f->plan); pthread_mutex_unlock(&plan_mu); }
+  pthread_mutex_lock(&plan_mu);
+  if (f->plan) p_destroy(f->plan);
+  if (f->rplan) p_destroy(f->rplan);
+  pthread_mutex_unlock(&plan_mu);
+  free(f->rin);
   free(f->in); free(f->out); free(f->tw); free(f->rev);
 }
 
@@ -143,6 +160,13 @@ static void fft_run(fftctx* f) {   /* out = FFT(in), unnormalised */
   }
 }
 
+/* out[0..L/2] = FFT of the real rin (L/2 + 1 bins, CCS order) */
+static void fft_run_real(fftctx* f) {
+  if (f->rplan) { p_exec(f->rplan); return; }
+  for (int n = 0; n < f->L; ++n) { f->in[n].re = f->rin[n]; f->in[n].im = 0.0f; }
+  fft_run(f);
+}
+
 /* ---------------------------------------------------------------- unpack helpers ---- */
 static float lut2(int fmt, unsigned field) {
   static const float vdif[4] = {-HIMAG, -1.0f, 1.0f, HIMAG};
@@ -157,7 +181,7 @@ static int frame_ok(const port_ds* d, const uint8_t* fv, long long fr) {
 
 /* ------------------------------------------------------------------ worker ---------- */
 typedef struct {
-  int nds, B, offsec, offns, nbuffered;
+  int nds, B, offsec, offns, nbuffered, order;
   double scan_start;
   const port_ds* ds;
   const uint8_t** fvalid;
@@ -194,6 +218,8 @@ static void* worker(void* arg) {
   cf32* rot = (cf32*)malloc(sizeof(cf32) * maxL);
   cf32* frac = (cf32*)malloc(sizeof(cf32) * maxL);
   cf32* tab = (cf32*)malloc(sizeof(cf32) * 2 * maxL);
+  cf32* quad = (cf32*)malloc(sizeof(cf32) * maxL);
+  const int order = J->order;
 
   for (int i0 = J->b0; i0 < J->b1; i0 += NB) {
     const int nsub = (J->b1 - i0) < NB ? (J->b1 - i0) : NB;
@@ -247,25 +273,45 @@ static void* worker(void* arg) {
         if (!(w > 0.0f)) continue;
         dweight[d * NB + sl] = w;
         const float fse = (float)(starttime - n0 * dt);
-        const double d0 = q[0] * i * (double)i + q[1] * i + q[2];
-        const double d1 = q[0] * (i + 0.5) * (i + 0.5) + q[1] * (i + 0.5) + q[2];
-        const double d2 = q[0] * (i + 1.0) * (i + 1.0) + q[1] * (i + 1.0) + q[2];
-        const double a = d2 - d0;
-        double bb = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
-        const int intdelay = (int)bb;
-        bb -= intdelay;
+        double a = 0.0, bb = 0.0, cc = 0.0;
+        int intdelay = 0;
+        if (order == 0) {                                         /* mode.cpp:778-780 */
+          intdelay = (int)avgdelay;
+        } else if (order == 1) {                                  /* mode.cpp:781-799 */
+          const double d0 = q[0] * i * (double)i + q[1] * i + q[2];
+          const double d1 = q[0] * (i + 0.5) * (i + 0.5) + q[1] * (i + 0.5) + q[2];
+          const double d2 = q[0] * (i + 1.0) * (i + 1.0) + q[1] * (i + 1.0) + q[2];
+          a = d2 - d0;
+          bb = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+          intdelay = (int)bb;
+          bb -= intdelay;
+        } else {                                                  /* mode.cpp:800-825 */
+          a = q[0];
+          bb = q[1] + i * q[0] * 2.0;
+          cc = q[2] + i * q[1] + (double)((long long)i * i) * q[0];
+          intdelay = (int)cc;
+          cc -= intdelay;
+        }
         const double walltime = J->scan_start + J->offsec + J->offns / 1.0e9 + fftstart / 1.0e6;
         const int intwall = (int)walltime;
         const double fracwall = walltime - intwall;
         for (int f = 0; f < D->nrecfreqs; ++f) {
           const port_freq* R = &D->rf[f];
           if (R->nbands == 0) continue;
-          /* rotator: sub (tab[0..S)) and step (tab[S..S+NS)) phasors */
-          for (int j = 0; j < S + NS; ++j) {
+          /* rotator: sub (tab[0..S)) and step (tab[S..S+NS)) phasors, quad[0..S) for order 2 */
+          const double sx1 = (double)S / (double)L;
+          for (int j = 0; order != 0 && j < S + NS; ++j) {
             double ph;
             if (j < S) {
-              const double xval = ((double)j / (double)L) * a + bb;
+              const double xoff = (double)j / (double)L;
+              const double xval = order == 1 ? xoff * a + bb : xoff * (bb + a * sx1) + cc;
               ph = xval * R->lofreq;
+              if (order == 2) {
+                const double qph = (xoff * (2 * a * sx1)) * R->lofreq;
+                const float qarg = (float)(-TWO_PI * (qph - (int)qph));
+                quad[j].re = cosf(qarg);
+                quad[j].im = sinf(qarg);
+              }
               if (D->fractionallo) ph += (R->lofreq - (int)R->lofreq) * (double)intdelay;
               if (R->looff) {
                 ph += (j * dt / 1e6) * (-R->lo_offset_hz);
@@ -274,21 +320,40 @@ static void* worker(void* arg) {
               }
             } else {
               const int s = j - S;
-              ph = ((double)(s * S) / (double)L) * a * R->lofreq;
+              const double xoff = (double)(s * S) / (double)L;
+              ph = order == 1 ? xoff * a * R->lofreq : (xoff * bb + (xoff * xoff) * a) * R->lofreq;
               if (R->looff) ph += (s * S * dt / 1e6) * (-R->lo_offset_hz);
             }
             const float arg = (float)(-TWO_PI * (ph - (int)ph));
             tab[j].re = cosf(arg);
             tab[j].im = sinf(arg);
           }
-          for (int j = 0; j < S; ++j) rot[j] = tab[j];
-          for (int s = 1; s < NS; ++s)
-            for (int j = 0; j < S; ++j) rot[s * S + j] = cmulf(tab[j], tab[S + s]);
+          if (order == 1) {
+            for (int j = 0; j < S; ++j) rot[j] = tab[j];
+            for (int s = 1; s < NS; ++s)
+              for (int j = 0; j < S; ++j) rot[s * S + j] = cmulf(tab[j], tab[S + s]);
+          } else if (order == 2) {                    /* mode.cpp:1017-1024 */
+            for (int s = 0; s < NS; ++s) {
+              for (int j = 0; j < S; ++j) rot[s * S + j] = cmulf(tab[j], tab[S + s]);
+              for (int j = 0; j < S; ++j) tab[j] = cmulf(quad[j], tab[j]);
+            }
+          }
           /* fractional-sample phasors */
           const float e = (float)((double)fse - R->clockoff + R->halfdelta);
           const float* sub = D->chanfreq;
           const float* step = (R->lsb && !D->cplx) ? D->chanfreq + S + NFS : D->chanfreq + S;
-          for (int j = 0; j < S; ++j) { const float ar = sub[j] * e; tab[j].re = cosf(ar); tab[j].im = sinf(ar); }
+          float prf = 0.0f;
+          if (order == 0) {                           /* post-F fringe phase, mode.cpp:1040-1049 */
+            double pr = (avgdelay - intdelay) * R->lofreq;
+            if (D->fractionallo) pr += intdelay * (R->lofreq - (int)R->lofreq);
+            pr -= walltime * R->lo_offset_hz;
+            prf = (float)(-TWO_PI * (pr - (int)pr));
+          }
+          for (int j = 0; j < S; ++j) {
+            float ar = sub[j] * e;
+            if (order == 0) ar += prf;
+            tab[j].re = cosf(ar); tab[j].im = sinf(ar);
+          }
           for (int s = 0; s < NFS; ++s) { const float ar = step[s] * e; tab[S + s].re = cosf(ar); tab[S + s].im = sinf(ar); }
           for (int j = 0; j < S; ++j) frac[j] = tab[j];
           for (int s = 1; s < NFS; ++s)
@@ -298,6 +363,19 @@ static void* worker(void* arg) {
             const int b = R->band[kb];
             cf32* in = fc[d].in;
             const cf32* u = unp + (size_t)b * L;
+            cf32* F = spec[d][b * NB + sl];
+            if (order == 0) {                         /* real FFT (mode.cpp:1139-1156) */
+              for (int n = 0; n < L; ++n) fc[d].rin[n] = u[n].re;
+              fft_run_real(&fc[d]);
+              const cf32* X = fc[d].out;
+              if (R->lsb) {                           /* conj-flip of bins 1..C */
+                for (int c = 0; c < C; ++c) { F[c].re = X[C - c].re; F[c].im = -X[C - c].im; }
+              } else {
+                for (int c = 0; c < C; ++c) F[c] = X[c];
+              }
+              for (int c = 0; c < C; ++c) F[c] = cmulf(F[c], frac[c]);
+              continue;
+            }
             if (D->cplx) {
               for (int n = 0; n < L; ++n) in[n] = cmulf(rot[n], u[n]);
             } else {
@@ -305,7 +383,6 @@ static void* worker(void* arg) {
             }
             fft_run(&fc[d]);
             const cf32* X = fc[d].out + ((!D->cplx && R->lsb) ? C : 0);
-            cf32* F = spec[d][b * NB + sl];
             for (int c = 0; c < C; ++c) F[c] = cmulf(X[c], frac[c]);
           }
         }
@@ -337,7 +414,7 @@ static void* worker(void* arg) {
     free(spec[d]);
     fft_free(&fc[d]);
   }
-  free(spec); free(dweight); free(fc); free(unp); free(rot); free(frac); free(tab);
+  free(spec); free(dweight); free(fc); free(unp); free(rot); free(frac); free(tab); free(quad);
   return 0;
 }
 
@@ -345,7 +422,7 @@ static void* worker(void* arg) {
  * weights[nds][B].  Returns the FFT backend used (1 = FFTW3 API, 0 = radix-2). */
 int fxport_run(int nds, const port_ds* ds, int B, int offsec, int offns, double scan_start,
                int nprod, const port_prod* prods, long long acclen, float* acc, float* weights,
-               int nthreads, int nbuffered) {
+               int nthreads, int nbuffered, int order) {
   load_fftw();
   if (nthreads < 1) nthreads = 1;
   if (nthreads > B) nthreads = B;
@@ -373,7 +450,7 @@ int fxport_run(int nds, const port_ds* ds, int B, int offsec, int offns, double 
   for (int t = 0; t < nthreads; ++t) {
     job* J = &jobs[t];
     J->nds = nds; J->B = B; J->offsec = offsec; J->offns = offns; J->scan_start = scan_start;
-    J->nbuffered = nbuffered; J->ds = ds; J->fvalid = (const uint8_t**)fvalid;
+    J->nbuffered = nbuffered; J->order = order; J->ds = ds; J->fvalid = (const uint8_t**)fvalid;
     J->nprod = nprod; J->prods = prods; J->acclen = acclen; J->weights = weights;
     J->b0 = (int)((long long)B * t / nthreads);
     J->b1 = (int)((long long)B * (t + 1) / nthreads);

@@ -57,7 +57,7 @@ def _load():
         lib = C.CDLL(str(LIB))
         lib.fxport_run.argtypes = [C.c_int, C.POINTER(port_ds), C.c_int, C.c_int, C.c_int, C.c_double,
                                    C.c_int, C.POINTER(port_prod), C.c_longlong, C.c_void_p, C.c_void_p,
-                                   C.c_int, C.c_int]
+                                   C.c_int, C.c_int, C.c_int]
         lib.fxport_run.restype = C.c_int
         lib.fxport_fft_backend.restype = C.c_int
         _lib = lib
@@ -214,7 +214,8 @@ def process_subint(cfg, offsets, data, control, nthreads: Optional[int] = None, 
     accbuf = np.zeros(max(acc, 1), np.complex64)
     weights = np.zeros((nds, B), np.float32)
     lib.fxport_run(nds, arr, B, int(offsets[1]), int(offsets[2]), float(cfg.scan_start_sec),
-                   len(prods), parr, acc, accbuf.ctypes.data, weights.ctypes.data, int(nthreads), int(nbuffered))
+                   len(prods), parr, acc, accbuf.ctypes.data, weights.ctypes.data, int(nthreads), int(nbuffered),
+                   int(cfg.fringe_rot_order))
     res = np.zeros(lay.coreresultlength, np.complex64)
     for o, dst, nout, avg in place:
         v = accbuf[o: o + nout * avg]

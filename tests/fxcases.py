@@ -26,6 +26,25 @@ def case_cfg2(B: int = 16, xengine: str = "fp16", nstations: int = 64, seed: int
                               nbits=8, complex_sampling=True, xengine=xengine, seed=seed, **kw)
 
 
+def case_cfg3(B: int = 32, xengine: str = "fp32", nstations: int = 3, seed: int = 4, **kw):
+    """Uneven bands (cfg3 shape, reduced): per station a 32 MHz / 512-channel, an 8 MHz /
+    128-channel and a 64 MHz / 1024-channel datastream (FFT lengths 1024, 256, 2048, one
+    16 us FFT duration), dual pol, plus zoom windows of 2, 1, 0.5 and 2 MHz."""
+    groups = [(32.0, 512, [4800.0, 4832.0]), (8.0, 128, [4900.0, 4908.0]), (64.0, 1024, [5000.0])]
+    zooms = [(0, 1, 4.0, 2.0), (0, 0, 10.0, 1.0), (1, 1, 2.0, 0.5), (2, 0, 20.0, 2.0)]
+    return cf.batched_config(nstations, groups, zooms, subint_ns=B * 16000, xengine=xengine,
+                             seed=seed, **kw)
+
+
+def case_long_ffts(B: int = 8, xengine: str = "fp32", nstations: int = 3, seed: int = 6, **kw):
+    """The longest fused FFTs: 64 MHz / 4096 channels (L = 8192) beside 32 MHz / 2048
+    channels (L = 4096), single pol, with a 1 MHz zoom in each."""
+    groups = [(64.0, 4096, [6000.0]), (32.0, 2048, [6100.0])]
+    zooms = [(0, 0, 5.0, 1.0), (1, 0, 3.0, 1.0)]
+    return cf.batched_config(nstations, groups, zooms, subint_ns=B * 64000, npol=1, xengine=xengine,
+                             seed=seed, **kw)
+
+
 def slot(cfg, offsets=(0, 1, 0), seed: int = 11):
     return synth.noise_slot(cfg, list(offsets), seed=seed)
 

@@ -11,7 +11,7 @@ import pytest
 
 import fxoracle as fo
 from difx_amd import synth
-from fxcases import case_cfg0, case_cfg1, case_cfg2, record_errors, slot
+from fxcases import case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_long_ffts, record_errors, slot
 
 pytestmark = pytest.mark.gpu
 
@@ -185,3 +185,37 @@ def test_fp32_device_vs_c_port(gpu):
     _say("device fp32 vs C port: cross max", max(e["cross"]), "auto max", max(e["auto"]))
     assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
     core.close()
+
+
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
+def test_uneven_bands_and_zooms(gpu, xe, bound):
+    """FFT lengths 256, 1024 and 2048 in one configuration, with zoom windows correlated on
+    every baseline (the MFMA X-engine takes them as rows of the zoom freq's group)."""
+    cfg = case_cfg3(B=32, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"cfg3 {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
+def test_long_ffts(gpu, xe, bound):
+    """FFT lengths 4096 and 8192 (1024-thread workgroups) with zoom windows."""
+    cfg = case_long_ffts(B=8, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"long ffts {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
+@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
+def test_rotation_orders(gpu, order, xe, bound):
+    """Post-F (order 0) and quadratic (order 2) fringe rotation, USB and LSB bands."""
+    for lsb in (False, True):
+        cfg = case_cfg0(B=64, xengine=xe, lsb=lsb, fringe_rot_order=order)
+        got, ref, e, lay = _run_and_compare(cfg)
+        _say(f"order {order} lsb {lsb} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+        assert max(e["cross"]) < bound and max(e["auto"]) < bound
+    cfg = case_cfg1(B=16, nstations=4, nfreq=2, xengine=xe, fringe_rot_order=order)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"order {order} cfg1 {xe} cross max", max(e["cross"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound

@@ -125,7 +125,7 @@ def test_zero_delay_chain():
     cfg = _cfg_small()
     ctrl = np.array([0, 1, 0], np.int32)
     interp = np.zeros(3)
-    ns, st, a, b, idl, _ = fo.fft_params(cfg, 0, [0, 1, 0], ctrl, interp, 16)
+    ns, st, a, b, idl, _, _ = fo.fft_params(cfg, 0, [0, 1, 0], ctrl, interp, 16)
     L = cfg.fftchannels(0)
     np.testing.assert_array_equal(ns, np.arange(16) * L)
     np.testing.assert_allclose(st - ns * cfg.sampletime_us(0), 0, atol=1e-12)
@@ -137,7 +137,7 @@ def test_integer_sample_delay_chain():
     dt = cfg.sampletime_us(0)
     interp = np.array([0.0, 0.0, 37 * dt])        # 37 samples of delay
     ctrl = np.array([0, 0, 999000000], np.int32)  # buffer starts 1 ms before the subint
-    ns, st, a, b, idl, _ = fo.fft_params(cfg, 0, [0, 1, 0], ctrl, interp, 8)
+    ns, st, a, b, idl, _, _ = fo.fft_params(cfg, 0, [0, 1, 0], ctrl, interp, 8)
     L = cfg.fftchannels(0)
     n1ms = int(round(1000.0 / dt))
     np.testing.assert_array_equal(ns, n1ms + np.arange(8) * L - 37)
@@ -158,13 +158,80 @@ def test_rotator_unit_phase_cases():
         st = fo.process_station(cfg, 0, offsets, data[0], ctrl[0], 4)
         # recompute with the delay and the phase terms removed: the rotator must be identity
         interp = np.array([0.0, 0.0, delay_us])
-        ns, stt, a, b, idl, _ = fo.fft_params(cfg, 0, offsets, ctrl[0], interp, 4)
+        ns, stt, a, b, idl, _, _ = fo.fft_params(cfg, 0, offsets, ctrl[0], interp, 4)
         assert np.all(b == 0) and np.all(idl == int(delay_us))
         vals, _ = fo.decode_stream(data[0], cfg.datastreams[0])
         L, C = cfg.fftchannels(0), 256
         for i in range(4):
             X = np.fft.fft(vals[0, ns[i]: ns[i] + L].astype(np.complex128))[:C]
             np.testing.assert_allclose(st.spectra[0, i], X, rtol=1e-9, atol=1e-9)
+
+
+def test_order2_rotator_tracks_quadratic_delay():
+    """Order 2 (mode.cpp:800-825, 956-1026) follows the delay q(i + n/L) of a curved model to
+    within the bound of its sub/step split, a (S/L)^2 / 4 delay units (the sub term uses
+    subxoff * S/L where the exact square has subxoff^2); order 1's straight line misses by
+    far more."""
+    L, S, lofreq = 2048, 32, 4800.0
+    q = np.array([3.0e-7, 2.0e-3, 1234.5678])                 # strongly curved, FFT-index units
+    for i in (0, 17, 500):
+        x = i + np.arange(L) / L
+        truth = q[0] * x * x + q[1] * x + q[2]
+        a2, b2, c2 = q[0], q[1] + 2 * i * q[0], q[2] + i * q[1] + i * i * q[0]
+        idl2 = int(c2)
+        ph2 = fo.rotator_phase(2, L, S, a2, b2, c2 - idl2, lofreq, idl2, False, 0.0, 1 / 64, 0, 0.0)
+        err2 = np.max(np.abs(ph2 - lofreq * (truth - idl2)))
+        bound = q[0] * (S / L) ** 2 / 4 * lofreq
+        rounding = 8 * np.finfo(float).eps * lofreq * abs(truth).max()   # FP64 of ~6e6 turns
+        assert err2 <= bound + rounding
+        d0, d1, d2 = [q[0] * y * y + q[1] * y + q[2] for y in (i, i + 0.5, i + 1.0)]
+        a1 = d2 - d0
+        b1 = d0 + (d1 - (a1 * 0.5 + d0)) / 3.0
+        idl1 = int(b1)
+        ph1 = fo.rotator_phase(1, L, S, a1, b1 - idl1, 0.0, lofreq, idl1, False, 0.0, 1 / 64, 0, 0.0)
+        err1 = np.max(np.abs(ph1 - lofreq * (truth - idl1)))
+        assert err1 > 100 * bound
+
+
+def _constant_delay_cfg(order, lsb, delay_us=2.3456, lofreq=4800.37):
+    cfg = cf.standard_config(2, 1, [lofreq], 32.0, 256, subint_ns=8 * 8000, seed=14, lsb=lsb,
+                             fringe_rot_order=order)
+    for k, d in enumerate(cfg.datastreams):
+        d.model[:] = 0.0
+        d.model[0, 0] = delay_us + 0.5 * k
+    return cfg
+
+
+@pytest.mark.parametrize("lsb", [False, True])
+def test_order0_equals_order1_for_constant_delay(lsb):
+    """A constant delay makes the order-1 rotator a constant phase (b lofreq plus the
+    fractional-LO term), which order 0 applies after the FFT instead (mode.cpp:1040-1049);
+    for real data the LSB conj-flip of bins 1..C (mode.cpp:1139-1156) equals the upper half
+    of the complex FFT, so both orders give the same spectra."""
+    offsets = [0, 1, 0]
+    res = {}
+    for order in (0, 1):
+        cfg = _constant_delay_cfg(order, lsb)
+        data, ctrl = synth.noise_slot(cfg, offsets, seed=15)
+        res[order], _ = fo.process_subint(cfg, offsets, data, ctrl)
+    np.testing.assert_allclose(res[0], res[1], rtol=0, atol=1e-9 * np.max(np.abs(res[1])))
+
+
+def test_order2_equals_order1_for_linear_delay():
+    """With a delay linear across the subint the quadratic coefficient vanishes and order 2
+    (c + b x) and order 1 (b + a x) describe the same line."""
+    offsets = [0, 1, 0]
+    res = {}
+    for order in (1, 2):
+        cfg = cf.standard_config(2, 2, [4800.37, 4832.0], 32.0, 256, subint_ns=8 * 8000, seed=16,
+                                 fringe_rot_order=order)
+        for k, d in enumerate(cfg.datastreams):
+            d.model[:, 2:] = 0.0
+        data, ctrl = synth.noise_slot(cfg, offsets, seed=17)
+        res[order], _ = fo.process_subint(cfg, offsets, data, ctrl)
+    # order 1 takes its slope as d2 - d0 of delays near 1.2e3 us: FP64 cancellation leaves a
+    # relative slope error near 3e-8, hence a 1e-7 bound
+    np.testing.assert_allclose(res[2], res[1], rtol=0, atol=1e-7 * np.max(np.abs(res[1])))
 
 
 # ------------------------------------------------------------------------------ FFT
@@ -194,6 +261,51 @@ def test_layout_dual_pol_hand_count():
     assert lay.bweight_off == {(0, 0): 32}
     assert lay.autocorr_off == [34, 66] and lay.acweight_off == [98, 100]
     assert lay.coreresultlength == 102 and lay.bandsperautocorr == 2
+
+
+def test_layout_zoom_hand_count():
+    """One 8-channel parent (4 MHz channels) with a 2-channel zoom at channel offset 2, both
+    correlated, single pol, 2 stations (configuration.cpp:2464-2621): crosses 8 + 2,
+    bweights 1 + 1, autos (8 + 2) per station, acweights (2 + 1) // 2 = 1 per station."""
+    cfg = cf.batched_config(2, [(32.0, 8, [4800.0])], [(0, 0, 8.0, 8.0)], subint_ns=8 * 250, npol=1)
+    assert cfg.freqs[1].nchan == 2 and cfg.datastreams[0].zoomfreqs == [(1, 0, 2)]
+    lay = fo.result_layout(cfg)
+    assert lay.baseline_off == {(0, 0): 0, (1, 0): 8}
+    assert lay.bweight_off == {(0, 0): 10, (1, 0): 11}
+    assert lay.autocorr_off == [12, 22] and lay.acweight_off == [32, 33]
+    assert lay.coreresultlength == 34
+
+
+def test_zoom_records_are_parent_windows():
+    """A zoom band is a window of its parent's channels (mode.cpp:184-196): every zoom
+    record, cross and auto, equals the matching channels of the parent record."""
+    cfg = cf.batched_config(2, [(32.0, 256, [4800.0]), (8.0, 64, [4900.0])],
+                            [(0, 0, 4.0, 2.0), (1, 0, 1.0, 1.0)], subint_ns=8 * 8000, seed=12)
+    offsets = [0, 1, 0]
+    data, ctrl = synth.noise_slot(cfg, offsets, seed=13)
+    res, _ = fo.process_subint(cfg, offsets, data, ctrl)
+    lay = fo.result_layout(cfg)
+    checked = 0
+    for j, bl in enumerate(cfg.baselines):
+        d1 = cfg.datastreams[bl.ds1]
+        for zfi, plf, choff in d1.zoomfreqs:
+            pfi = d1.recfreqs[plf]
+            n, pn = cfg.freqs[zfi].nchan, cfg.freqs[pfi].nchan
+            for p in range(4):
+                z = res[lay.baseline_off[(zfi, j)] + p * n:][:n]
+                par = res[lay.baseline_off[(pfi, j)] + p * pn + choff:][:n]
+                np.testing.assert_array_equal(z, par)
+                checked += 1
+    # autos: datastream 0 (group 0) holds RR, LL of the parent then of the zoom, then cross-pol
+    d0 = cfg.datastreams[0]
+    a0 = lay.autocorr_off[0]
+    _, _, choff = d0.zoomfreqs[0]
+    n = cfg.freqs[d0.zoomfreqs[0][0]].nchan
+    for w in range(2):                     # parallel block, cross-pol block
+        base = a0 + w * (2 * 256 + 2 * n)
+        for pol in range(2):
+            np.testing.assert_array_equal(res[base + 2 * 256 + pol * n:][:n], res[base + pol * 256 + choff:][:n])
+    assert checked == 8
 
 
 # ---------------------------------------------------------- X-engine / accumulation invariants

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import fxoracle as fo
-from fxcases import case_cfg0, case_cfg1, case_cfg2, record_errors, slot
+from fxcases import case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_long_ffts, record_errors, slot
 
 
 def _port():
@@ -13,12 +13,13 @@ def _port():
     return fxport
 
 
-@pytest.mark.parametrize("name", ["cfg0", "cfg0_lsb", "cfg1", "cfg2"])
+@pytest.mark.parametrize("name", ["cfg0", "cfg0_lsb", "cfg1", "cfg2", "cfg3", "long"])
 def test_port_matches_fp64_oracle(name):
     fxport = _port()
     cfg = {"cfg0": lambda: case_cfg0(B=64), "cfg0_lsb": lambda: case_cfg0(B=32, lsb=True),
            "cfg1": lambda: case_cfg1(B=16, nstations=4, nfreq=4),
-           "cfg2": lambda: case_cfg2(B=8, xengine="fp32", nstations=6)}[name]()
+           "cfg2": lambda: case_cfg2(B=8, xengine="fp32", nstations=6),
+           "cfg3": lambda: case_cfg3(B=16), "long": lambda: case_long_ffts(B=4)}[name]()
     data, ctrl = slot(cfg)
     got, w = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=3)
     ref, st = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
@@ -35,3 +36,17 @@ def test_port_thread_count_invariance():
     b, _ = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=5)
     rel = np.mean(np.abs(a - b)) / np.mean(np.abs(a))
     assert rel < 1e-6
+
+
+@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("lsb", [False, True])
+def test_port_rotation_orders_match_oracle(order, lsb):
+    """Orders 0 (post-F, real FFT with the LSB conj-flip) and 2 (quadratic rotator built by
+    repeated FP32 products) in the C restatement against the FP64 oracle."""
+    fxport = _port()
+    cfg = case_cfg0(B=32, lsb=lsb, fringe_rot_order=order)
+    data, ctrl = slot(cfg)
+    got, _ = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=2)
+    ref, _ = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
+    e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
