@@ -38,7 +38,7 @@ class Datastream:
     model: np.ndarray                        # [nsamples, 6] delay coefficients, microseconds
     format: str = "VDIF"                     # VDIF or MARK5B
     nbits: int = 2
-    complex_sampling: bool = False
+    complex_sampling: int = 0                # 0 real, 1 complex, 2 complex double sideband
     payloadbytes: int = 8000
     headerbytes: int = 32
     arraystridelen: int = 0
@@ -233,7 +233,7 @@ def calcstridelength(n: int) -> int:
 
 
 def standard_config(nstations: int, npol: int, freqs_mhz: Sequence[float], bw_mhz: float,
-                    nchan: int, subint_ns: int, nbits: int = 2, complex_sampling: bool = False,
+                    nchan: int, subint_ns: int, nbits: int = 2, complex_sampling: int = 0,
                     payloadbytes: int = 8000, seed: int = 1, xengine: str = "fp32",
                     lsb: bool = False, fringe_rot_order: int = 1, chunk_ffts: int = 0,
                     write_cross_autocorrs: bool = True, max_delay_us: float = 1e4,

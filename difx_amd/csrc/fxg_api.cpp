@@ -181,6 +181,8 @@ int validate_and_derive(fxg_ctx* ctx) {
     const fxg_datastream& ds = ctx->dss[d];
     if (ds.format != FXG_FMT_VDIF && ds.format != FXG_FMT_MARK5B)
       return fail(ctx, FXG_E_INVALID, "unsupported data format");
+    if (ds.complex_sampling < 0 || ds.complex_sampling > 2)
+      return fail(ctx, FXG_E_INVALID, "complex_sampling must be 0 (real), 1 (complex) or 2 (double sideband)");
     if (!((ds.nbits == 2 && !ds.complex_sampling) || ds.nbits == 8))
       return fail(ctx, FXG_E_INVALID, "only 2-bit real and 8-bit real/complex sampling are supported");
     if (ds.complex_sampling && ds.format != FXG_FMT_VDIF)
@@ -200,8 +202,8 @@ int validate_and_derive(fxg_ctx* ctx) {
       const fxg_freq& f = ctx->freqs[fi];
       if (f.nchan != f0.nchan || std::fabs(f.bw_mhz - f0.bw_mhz) > TINY || f.chanavg != f0.chanavg)
         return fail(ctx, FXG_E_INVALID, "a datastream's recorded freqs must share nchan, bandwidth, averaging");
-      if (ds.complex_sampling && f.lsb)
-        return fail(ctx, FXG_E_INVALID, "complex lower-sideband data is not supported by this build");
+      if (ds.complex_sampling == 2 && f.nchan % 16)
+        return fail(ctx, FXG_E_INVALID, "double-sideband data needs nchan to be a multiple of 16");
       if (f.chanavg < 1 || f.nchan % f.chanavg)
         return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
     }
@@ -422,7 +424,8 @@ int build_tables(fxg_ctx* ctx) {
     const fxg_freq& f0 = ctx->freqs[s.recfreq_index[0]];
     x.format = s.format;
     x.nbits = s.nbits;
-    x.iscomplex = s.complex_sampling;
+    x.iscomplex = s.complex_sampling ? 1 : 0;
+    x.dsb = s.complex_sampling == 2;
     x.framebytes = s.framebytes;
     x.headerbytes = s.headerbytes;
     x.nrecbands = s.nrecbands;
@@ -455,7 +458,8 @@ int build_tables(fxg_ctx* ctx) {
       const fxg_freq& f = ctx->freqs[s.recfreq_index[k]];
       DevRecFreq& r = x.rf[k];
       r.lofreq = f.freq_mhz;
-      if (s.complex_sampling && f.lsb) r.lofreq = -r.lofreq;     // mode.cpp:875-880
+      if (s.complex_sampling && f.lsb) r.lofreq = -r.lofreq;     // mode.cpp:875-877
+      if (s.complex_sampling == 2) r.lofreq += f.bw_mhz / 2.0;   // mode.cpp:878-879
       r.lo_offset_hz = s.lo_offset_hz[k];
       r.looff = (r.lo_offset_hz > 0.0 || r.lo_offset_hz < 0.0);
       double fl = std::fabs(r.lo_offset_hz) - (int)std::fabs(r.lo_offset_hz);
@@ -476,6 +480,7 @@ int build_tables(fxg_ctx* ctx) {
     for (int i = 0; i < S; ++i) chanfreq.push_back((float)((TWO_PI * (i) * bw) / C));
     for (int i = 0; i < NFS; ++i) chanfreq.push_back((float)((TWO_PI * i * S * bw) / C));
     for (int i = 0; i < NFS; ++i) chanfreq.push_back((float)((-TWO_PI * ((NFS - i) * S) * bw) / C));
+    for (int i = 0; i < NFS; ++i) chanfreq.push_back((float)((TWO_PI * i * S * bw) / C - TWO_PI * bw / 2.0));
     ctx->databytes[d] = get_data_bytes(ctx, d);
     const int cap_frames = (int)((ctx->databytes[d] + 4LL * s.framebytes) / s.framebytes);
     x.fvalid_off = (int)fvo;

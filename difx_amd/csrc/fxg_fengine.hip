@@ -233,7 +233,7 @@ struct PhasorCap {
 
 // Per-datastream constants the F-engine reads, copied to registers once per workgroup.
 struct DsConst {
-  int C, S, NS, NFS, nrec, nrecfreqs, fmt, spf, framebytes, headerbytes, nbits, fractionallo;
+  int C, S, NS, NFS, nrec, nrecfreqs, fmt, spf, framebytes, headerbytes, nbits, fractionallo, dsb;
   double sampletime_us;
 };
 
@@ -310,7 +310,8 @@ __device__ void build_frac(float2* sfrac, const DsConst& k, const FreqConst& rf,
   const int S = k.S, NFS = k.NFS;
   const float e = (float)((double)fd.fse - rf.clockoff + rf.halfdelta);
   const float* sub = cf;
-  const float* step = rf.lsb && !iscomplex ? cf + S + NFS : cf + S;
+  // double sideband: dsbstepchannelfreqs; any other LSB: lsbstepchannelfreqs (mode.cpp:838-857)
+  const float* step = k.dsb ? cf + S + 2 * NFS : rf.lsb ? cf + S + NFS : cf + S;
   float prf = 0.0f;
   if (ORDER == 0) {
     const double lofreq = rf.lofreq;
@@ -382,7 +383,7 @@ k_fengine(FengineArgs args) {
   k.C = dsp->C; k.S = dsp->S; k.NS = dsp->NS; k.NFS = dsp->NFS; k.nrec = dsp->nrecbands;
   k.nrecfreqs = dsp->nrecfreqs; k.fmt = dsp->format; k.spf = dsp->spf;
   k.framebytes = dsp->framebytes; k.headerbytes = dsp->headerbytes; k.nbits = dsp->nbits;
-  k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us;
+  k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us; k.dsb = dsp->dsb;
   const long long spec_off32 = dsp->spec_off;
   const uint8_t* __restrict__ fv = args.fvalid + dsp->fvalid_off;
   const float* __restrict__ cf = args.chanfreq + dsp->chanfreq_off;
@@ -524,6 +525,9 @@ k_fengine(FengineArgs args) {
                 x = make_float2(lut8(p[0]), lut8(p[1]));
               }
               v[s] = cmul(rot, x);
+              // complex LSB: the reference conj-flips the spectrum (mode.cpp:1186-1201);
+              // FFT(conj y)[c] = conj(FFT(y)[-c]) gives the same channels without a flip
+              if (rf.lsb) v[s].y = -v[s].y;
             }
           }
           __syncthreads();                        // sbuf free (previous band's FFT done)
@@ -542,14 +546,18 @@ k_fengine(FengineArgs args) {
           }
           // channel selection, fractional-sample correction, output
           constexpr int NKEEP = CPLX ? 8 : 4;
+          float2 Fo[NKEEP];                          // output channels t + m*NT, kept for the zoom windows
 #pragma unroll
           for (int m = 0; m < NKEEP; ++m) {
-            // USB and complex keep X[t + m*NT]; real LSB keeps X[C + t + m*NT] (mode.cpp:1205)
-            const int s = (!CPLX && rf.lsb) ? m + 4 : m;
+            // USB and complex keep X[t + m*NT]; real LSB keeps X[C + t + m*NT] (mode.cpp:1205);
+            // double sideband swaps the halves (mode.cpp:1191-1192, 1211-1213)
+            const int s = (!CPLX && rf.lsb) ? m + 4 : (CPLX && k.dsb) ? (m ^ 4) : m;
             const int c = t + m * NT;
             const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
-            const float2 F = cmul(v[s], frq);
-            v[m] = F;                                // channel c, kept for the zoom windows
+            float2 X = v[s];
+            if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped (mode.cpp:1199-1200)
+            const float2 F = cmul(X, frq);
+            Fo[m] = F;
             if (out_mode == 0) {
               o32[c] = F;
             } else {
@@ -571,7 +579,7 @@ k_fengine(FengineArgs args) {
                 const int cz = t + m * NT - c0;
                 if (cz >= 0 && cz < nz) {
                   const float sc = scz[cz];
-                  oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16(make_float2(v[m].x * sc, v[m].y * sc), out_mode);
+                  oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16(make_float2(Fo[m].x * sc, Fo[m].y * sc), out_mode);
                 }
               }
             }

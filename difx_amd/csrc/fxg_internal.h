@@ -30,6 +30,7 @@ struct DevDs {
   int format;             // FXG_FMT_*
   int nbits;
   int iscomplex;
+  int dsb;                // complex double sideband (mode.cpp:41-49)
   int framebytes;
   int headerbytes;
   int spf;                // time samples per frame
@@ -126,7 +127,7 @@ struct FengineArgs {
   uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row (rowstride)][C/8][t][8]
   const DevXGroup* xgroups;
   const float* xscale;    // per group [rows][C] scales applied before 16-bit conversion
-  const float* chanfreq;  // per datastream: sub[S], usb step[NFS], lsb step[NFS] (mode.cpp:303-324)
+  const float* chanfreq;  // per datastream: sub[S], then usb, lsb and dsb steps [NFS] (mode.cpp:303-324)
   const uint8_t* fvalid;  // frame valid flags
   int chunkcap;           // FFT capacity of the chunk buffers (multiple of FXG_XPAD)
   int blocks_per_wg;

@@ -63,7 +63,7 @@ typedef struct fxg_freq {
 typedef struct fxg_datastream {
   int32_t format;             /* FXG_FMT_* */
   int32_t nbits;              /* bits per (real or imaginary) sample: 2 or 8 */
-  int32_t complex_sampling;   /* 0 = real, 1 = complex single sideband */
+  int32_t complex_sampling;   /* 0 = real, 1 = complex, 2 = complex double sideband (mode.cpp:41-49) */
   int32_t framebytes;         /* frame size including header */
   int32_t headerbytes;        /* 32 (VDIF) or 16 (Mark5B) */
   int32_t arraystridelen;     /* ARRAY STRIDE LEN; 0 = automatic (configuration.cpp:38-51) */

@@ -394,7 +394,8 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
     f0 = cfg.freqs[ds.recfreqs[0]]
     C = f0.nchan
     bw = f0.bw_mhz
-    usecomplex = ds.complex_sampling
+    usecomplex = bool(ds.complex_sampling)
+    usedouble = ds.complex_sampling == 2                                 # mode.cpp:41-49
     spectra = np.zeros((nrec + nzoom, B, C), dtype=np.complex128)
     weight = np.zeros(B)
     autos = np.zeros((2, nrec, C), dtype=np.complex128)
@@ -461,7 +462,9 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
             fr = cfg.freqs[fi]
             lofreq = fr.freq_mhz
             if usecomplex and fr.lsb:
-                lofreq = -lofreq
+                lofreq = -lofreq                                         # mode.cpp:875-877
+            if usedouble:
+                lofreq += bw / 2.0                                       # mode.cpp:878-879
             loff = ds.per_freq("lo_offset_hz")[lf]
             # fringe rotator (pre-F for orders 1 and 2), phase in turns, f64
             phase = rotator_phase(order, L, S, a[i], b[i], cq[i], lofreq, int(intdelay[i]), fractional_lo,
@@ -470,7 +473,9 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
             # fractional-sample correction (mode.cpp:1030-1073), f64
             e = fse - ds.per_freq("clock_offset_us")[lf] + ds.per_freq("clock_offset_delta_us")[lf] / 2
             k = np.arange(C)
-            if (not usecomplex) and fr.lsb:
+            if usedouble:                                                # dsbstepchannelfreqs
+                chanfreq = TWO_PI * (k - C / 2) * bw / C
+            elif fr.lsb:                                                 # lsbstepchannelfreqs
                 chanfreq = TWO_PI * (k - C) * bw / C
             else:
                 chanfreq = TWO_PI * k * bw / C
@@ -486,9 +491,17 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
                     # real FFT; LSB conj-flips bins 1..C (mode.cpp:1139-1156)
                     F = np.conj(X[C:0:-1]) if fr.lsb else X[:C]
                 elif usecomplex:
-                    if fr.lsb:
-                        raise NotImplementedError("complex LSB")
-                    F = X[:C]
+                    # channel 0 = lowest sky frequency (mode.cpp:1186-1222)
+                    if usedouble and fr.lsb:                             # two conj-flips
+                        F = np.empty(C, dtype=np.complex128)
+                        F[: C // 2 + 1] = np.conj(X[C // 2:: -1])
+                        F[C // 2 + 1:] = np.conj(X[C - 1: C // 2: -1])
+                    elif usedouble:                                      # swapped halves
+                        F = np.concatenate([X[C // 2:], X[: C // 2]])
+                    elif fr.lsb:                                         # DC kept, rest flipped
+                        F = np.concatenate([X[:1], np.conj(X[C - 1: 0: -1])])
+                    else:
+                        F = X[:C]
                 elif fr.lsb:
                     F = X[C: 2 * C]                                      # mode.cpp:1205
                 else:

@@ -46,12 +46,13 @@ typedef struct port_freq {
 typedef struct port_ds {
   int fmt, nbits, cplx, framebytes, headerbytes, spf, nrec, nrecfreqs, L, C, S, NS, NFS;
   int bpb_num, bpb_den, fractionallo, valid;
+  int dsb;                        /* complex double sideband (mode.cpp:41-49) */
   double sampletime_us;
   double interp[3];
   int datasec, datans, datalen, nframes;
   const uint8_t* data;
   const int32_t* vflags;
-  const float* chanfreq;          /* sub[S], usb step[NFS], lsb step[NFS] (mode.cpp:303-324) */
+  const float* chanfreq;          /* sub[S], usb, lsb, dsb steps [NFS] each (mode.cpp:303-324) */
   port_freq rf[32];
 } port_ds;
 
@@ -341,7 +342,8 @@ static void* worker(void* arg) {
           /* fractional-sample phasors */
           const float e = (float)((double)fse - R->clockoff + R->halfdelta);
           const float* sub = D->chanfreq;
-          const float* step = (R->lsb && !D->cplx) ? D->chanfreq + S + NFS : D->chanfreq + S;
+          const float* step = D->dsb ? D->chanfreq + S + 2 * NFS
+                                     : R->lsb ? D->chanfreq + S + NFS : D->chanfreq + S;   /* mode.cpp:838-857 */
           float prf = 0.0f;
           if (order == 0) {                           /* post-F fringe phase, mode.cpp:1040-1049 */
             double pr = (avgdelay - intdelay) * R->lofreq;
@@ -382,8 +384,23 @@ static void* worker(void* arg) {
               for (int n = 0; n < L; ++n) { in[n].re = u[n].re * rot[n].re; in[n].im = u[n].re * rot[n].im; }
             }
             fft_run(&fc[d]);
-            const cf32* X = fc[d].out + ((!D->cplx && R->lsb) ? C : 0);
-            for (int c = 0; c < C; ++c) F[c] = cmulf(X[c], frac[c]);
+            const cf32* X = fc[d].out;
+            /* channel 0 = lowest sky frequency (mode.cpp:1186-1222) */
+            if (!D->cplx) {
+              const cf32* Xs = X + (R->lsb ? C : 0);
+              for (int c = 0; c < C; ++c) F[c] = Xs[c];
+            } else if (D->dsb && R->lsb) {
+              for (int c = 0; c <= C / 2; ++c) { F[c].re = X[C / 2 - c].re; F[c].im = -X[C / 2 - c].im; }
+              for (int c = C / 2 + 1; c < C; ++c) { F[c].re = X[3 * C / 2 - c].re; F[c].im = -X[3 * C / 2 - c].im; }
+            } else if (D->dsb) {
+              for (int c = 0; c < C; ++c) F[c] = X[(c + C / 2) % C];
+            } else if (R->lsb) {
+              F[0] = X[0];
+              for (int c = 1; c < C; ++c) { F[c].re = X[C - c].re; F[c].im = -X[C - c].im; }
+            } else {
+              for (int c = 0; c < C; ++c) F[c] = X[c];
+            }
+            for (int c = 0; c < C; ++c) F[c] = cmulf(F[c], frac[c]);
           }
         }
       }

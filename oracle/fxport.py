@@ -29,7 +29,7 @@ class port_freq(C.Structure):
 class port_ds(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("fmt", "nbits", "cplx", "framebytes", "headerbytes", "spf", "nrec",
                                        "nrecfreqs", "L", "C", "S", "NS", "NFS", "bpb_num", "bpb_den",
-                                       "fractionallo", "valid")] + [
+                                       "fractionallo", "valid", "dsb")] + [
         ("sampletime_us", C.c_double), ("interp", C.c_double * 3),
         ("datasec", C.c_int), ("datans", C.c_int), ("datalen", C.c_int), ("nframes", C.c_int),
         ("data", C.c_void_p), ("vflags", C.c_void_p), ("chanfreq", C.c_void_p),
@@ -73,12 +73,14 @@ def fft_backend() -> str:
 
 
 def _chanfreq(C_, S, NFS, bw):
-    """Channel slopes as the reference computes them (mode.cpp:303-324)."""
+    """Channel slopes as the reference computes them (mode.cpp:303-324): sub, USB step,
+    LSB step and double-sideband step tables."""
     two_pi = fo.TWO_PI
     sub = [np.float32((two_pi * i * bw) / C_) for i in range(S)]
     usb = [np.float32((two_pi * i * S * bw) / C_) for i in range(NFS)]
     lsb = [np.float32((-two_pi * ((NFS - i) * S) * bw) / C_) for i in range(NFS)]
-    return np.array(sub + usb + lsb, dtype=np.float32)
+    dsb = [np.float32((two_pi * i * S * bw) / C_ - two_pi * bw / 2.0) for i in range(NFS)]
+    return np.array(sub + usb + lsb + dsb, dtype=np.float32)
 
 
 def process_subint(cfg, offsets, data, control, nthreads: Optional[int] = None, nbuffered: Optional[int] = None):
@@ -101,7 +103,8 @@ def process_subint(cfg, offsets, data, control, nthreads: Optional[int] = None, 
         nrec = len(ds.recbands)
         nsb = ds.nbits * (2 if ds.complex_sampling else 1)
         x.fmt = 1 if ds.format.upper() == "MARK5B" else 0
-        x.nbits, x.cplx = ds.nbits, int(ds.complex_sampling)
+        x.nbits, x.cplx = ds.nbits, int(bool(ds.complex_sampling))
+        x.dsb = int(ds.complex_sampling == 2)
         x.framebytes, x.headerbytes = ds.framebytes, ds.headerbytes
         x.spf = fo.samples_per_frame(ds)
         x.nrec, x.nrecfreqs = nrec, len(ds.recfreqs)
@@ -137,6 +140,8 @@ def process_subint(cfg, offsets, data, control, nthreads: Optional[int] = None, 
             fr = cfg.freqs[fi]
             r = x.rf[lf]
             r.lofreq = -fr.freq_mhz if (ds.complex_sampling and fr.lsb) else fr.freq_mhz
+            if ds.complex_sampling == 2:
+                r.lofreq += fr.bw_mhz / 2.0                       # mode.cpp:878-879
             lo = ds.per_freq("lo_offset_hz")[lf]
             r.lo_offset_hz = lo
             r.looff = int(lo != 0.0)

@@ -20,10 +20,12 @@ def case_cfg1(B: int = 32, xengine: str = "fp32", nstations: int = 10, nfreq: in
                               xengine=xengine, seed=seed, **kw)
 
 
-def case_cfg2(B: int = 16, xengine: str = "fp16", nstations: int = 64, seed: int = 3, **kw):
-    """64 stations, single pol, 16 MHz complex 8-bit, nchan 4096."""
+def case_cfg2(B: int = 16, xengine: str = "fp16", nstations: int = 64, seed: int = 3,
+              complex_sampling: int = 1, **kw):
+    """64 stations, single pol, 16 MHz complex 8-bit, nchan 4096 (complex_sampling 2: double
+    sideband)."""
     return cf.standard_config(nstations, 1, [1400.0], 16.0, 4096, subint_ns=B * 256000,
-                              nbits=8, complex_sampling=True, xengine=xengine, seed=seed, **kw)
+                              nbits=8, complex_sampling=complex_sampling, xengine=xengine, seed=seed, **kw)
 
 
 def case_cfg3(B: int = 32, xengine: str = "fp32", nstations: int = 3, seed: int = 4, **kw):

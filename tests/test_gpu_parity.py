@@ -219,3 +219,14 @@ def test_rotation_orders(gpu, order, xe, bound):
     got, ref, e, lay = _run_and_compare(cfg)
     _say(f"order {order} cfg1 {xe} cross max", max(e["cross"]))
     assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
+@pytest.mark.parametrize("cs,lsb", [(1, True), (2, False), (2, True)])
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
+def test_complex_sidebands(gpu, cs, lsb, xe, bound):
+    """Complex LSB (DC kept, other channels conj-flipped) and complex double sideband (halves
+    swapped; LSB conj-flipped about the band centre), mode.cpp:1186-1222."""
+    cfg = case_cfg2(B=8, xengine=xe, nstations=8, complex_sampling=cs, lsb=lsb)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"complex {cs} lsb {lsb} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound

@@ -234,6 +234,57 @@ def test_order2_equals_order1_for_linear_delay():
     np.testing.assert_allclose(res[2], res[1], rtol=0, atol=1e-7 * np.max(np.abs(res[1])))
 
 
+def _complex_tone_station(sideband: str, f_bb_mhz: float, B: int = 4, conj: bool = False):
+    """One 16 MHz complex 8-bit datastream (nchan 256) with zero delay carrying the baseband
+    tone exp(2 pi i f t) (or its conjugate); returns the oracle's station spectra."""
+    lsb = sideband in ("lsb", "dlsb")
+    cfg = cf.standard_config(1, 1, [1400.0], 16.0, 256, subint_ns=B * 16000, nbits=8,
+                             complex_sampling=2 if sideband in ("dusb", "dlsb") else 1, lsb=lsb)
+    cfg.datastreams[0].model[:] = 0.0
+    offsets = [0, 1, 0]
+    ds = cfg.datastreams[0]
+    start = synth.buffer_start_ns(cfg, 0, offsets)
+    nbytes = synth.slot_bytes(cfg, 0, offsets, start)
+    nsamp = (nbytes // ds.framebytes) * synth.samples_per_frame(ds)
+    t_us = start / 1000.0 + np.arange(nsamp) / 16.0
+    z = np.exp(2j * np.pi * f_bb_mhz * t_us)
+    if conj:
+        z = np.conj(z)
+    pay = np.empty(2 * nsamp, np.uint8)
+    pay[0::2] = np.round(128 + 40 * z.real).astype(np.uint8)
+    pay[1::2] = np.round(128 + 40 * z.imag).astype(np.uint8)
+    spf = synth.samples_per_frame(ds)
+    data = synth.frame_payloads(pay, ds, start // int(round(spf * 1000.0 / 16.0)))
+    ctrl = synth.control_words(cfg, 0, offsets, start)
+    return fo.process_station(cfg, 0, offsets, data, ctrl).spectra[0]
+
+
+@pytest.mark.parametrize("sideband,f_bb,chan", [
+    ("usb", 4.0, 64),          # sky LO + 4 MHz, channel 0 at the LO
+    ("lsb", 4.0, 192),         # sky LO - 4 MHz, channel 0 at LO - 16 MHz
+    ("dusb", 4.0, 192),        # sky LO + 4 MHz, channel 0 at LO - 8 MHz
+    ("dusb", -4.0, 64),
+    ("dlsb", 4.0, 64),         # sky LO - 4 MHz, channel 0 at LO - 8 MHz
+    ("dlsb", -4.0, 192)])
+def test_complex_sideband_channel_order(sideband, f_bb, chan):
+    """Channel 0 is the lowest sky frequency for every complex sideband type
+    (mode.cpp:1224-1235): a baseband tone lands in the channel its sky frequency names."""
+    sp = _complex_tone_station(sideband, f_bb)
+    peak = np.argmax(np.abs(sp).sum(0))
+    assert peak == chan
+    assert np.abs(sp[:, chan]).min() > 50 * np.median(np.abs(sp))
+
+
+def test_complex_lsb_is_usb_of_the_conjugate():
+    """LSB processing conj-flips all channels but DC (mode.cpp:1199-1200), so the LSB spectra
+    of conj(x) equal the USB spectra of x except channel 0, which is conjugated."""
+    for f in (1.3, -5.1):
+        u = _complex_tone_station("usb", f)
+        l = _complex_tone_station("lsb", f, conj=True)
+        np.testing.assert_allclose(l[:, 1:], u[:, 1:], rtol=0, atol=1e-9 * np.abs(u).max())
+        np.testing.assert_allclose(l[:, 0], np.conj(u[:, 0]), rtol=0, atol=1e-9 * np.abs(u).max())
+
+
 # ------------------------------------------------------------------------------ FFT
 @pytest.mark.parametrize("L", [64, 256, 512])
 def test_fft_against_bruteforce_dft(L):

@@ -50,3 +50,14 @@ def test_port_rotation_orders_match_oracle(order, lsb):
     ref, _ = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
     e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
     assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+
+
+@pytest.mark.parametrize("cs,lsb", [(1, True), (2, False), (2, True)])
+def test_port_complex_sidebands_match_oracle(cs, lsb):
+    fxport = _port()
+    cfg = case_cfg2(B=8, xengine="fp32", nstations=4, complex_sampling=cs, lsb=lsb)
+    data, ctrl = slot(cfg)
+    got, _ = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=2)
+    ref, _ = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
+    e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
