@@ -62,8 +62,31 @@ def load_library() -> C.CDLL:
     lib.fxg_delay_interpolator.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]
     lib.fxg_set_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.fxg_get_timing.argtypes = [C.c_void_p, P(abi.fxg_timing)]
+    lib.fxg_footprint.argtypes = [P(abi.fxg_config), P(C.c_int64), P(C.c_int64), P(C.c_int64)]
+    lib.fxg_input_samples.argtypes = [C.c_void_p, P(C.c_int64)]
+    lib.fxg_device_bytes.argtypes = [C.c_void_p, P(C.c_int64)]
+    lib.fxg_server_create.argtypes = [P(abi.fxg_server_config), P(C.c_void_p)]
+    lib.fxg_server_destroy.argtypes = [C.c_void_p]
+    lib.fxg_server_add_job.argtypes = [C.c_void_p, P(abi.fxg_config), C.c_int32, C.c_int32, P(C.c_int32)]
+    lib.fxg_server_push_subint.argtypes = [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_void_p), P(C.c_int32),
+                                           P(C.c_void_p)]
+    lib.fxg_server_wait_job.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_double), P(C.c_double)]
+    lib.fxg_server_get_stats.argtypes = [C.c_void_p, P(abi.fxg_server_stats)]
+    lib.fxg_server_job_error.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
     _LIB = lib
     return lib
+
+
+def footprint(cfg: Config):
+    """(device bytes of a context, coreresultlength, one slot's input bytes) for cfg, computed
+    without a device (fxg_footprint)."""
+    lib = load_library()
+    fc, keep = cfg.to_abi()
+    dev, crl, slot = C.c_int64(), C.c_int64(), C.c_int64()
+    rc = lib.fxg_footprint(C.byref(fc), C.byref(dev), C.byref(crl), C.byref(slot))
+    if rc != abi.FXG_OK:
+        raise FxgError(f"fxg_footprint: {lib.fxg_strerror(rc).decode()}")
+    return dev.value, crl.value, slot.value
 
 
 def _ptr_array(ptrs: Sequence[int]):

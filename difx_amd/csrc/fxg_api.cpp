@@ -62,6 +62,8 @@ struct Staging {
 struct fxg_ctx {
   std::string err;
   int device = 0;
+  bool dry = false;               // size accounting only: no device allocation or upload
+  long long dev_bytes = 0;        // device memory held by this context
   // configuration copy
   std::vector<fxg_freq> freqs;
   std::vector<fxg_datastream> dss;
@@ -155,11 +157,26 @@ int fail(fxg_ctx* c, int code, const std::string& msg) {
       return fail(ctx, FXG_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e));       \
   } while (0)
 
+// Every device allocation of a context goes through here, so that a dry run (fxg_footprint)
+// reports exactly what fxg_create would hold.
+int dev_malloc(fxg_ctx* ctx, void** p, long long bytes) {
+  bytes = (std::max<long long>(bytes, 16) + 255) / 256 * 256;
+  ctx->dev_bytes += bytes;
+  if (ctx->dry) { *p = nullptr; return FXG_OK; }
+  HIPCHK(hipMalloc(p, (size_t)bytes));
+  return FXG_OK;
+}
+
+int dev_zero(fxg_ctx* ctx, void* p, long long bytes) {
+  if (ctx->dry) return FXG_OK;
+  HIPCHK(hipMemset(p, 0, (size_t)bytes));
+  return FXG_OK;
+}
+
 template <typename T>
 int dev_upload(fxg_ctx* ctx, T** dptr, const std::vector<T>& h) {
-  size_t bytes = std::max<size_t>(h.size() * sizeof(T), 16);
-  HIPCHK(hipMalloc((void**)dptr, bytes));
-  if (!h.empty()) HIPCHK(hipMemcpy(*dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (int rc = dev_malloc(ctx, (void**)dptr, (long long)(h.size() * sizeof(T)))) return rc;
+  if (!ctx->dry && !h.empty()) HIPCHK(hipMemcpy(*dptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   return FXG_OK;
 }
 
@@ -491,8 +508,8 @@ int build_tables(fxg_ctx* ctx) {
   int rc;
   if ((rc = dev_upload(ctx, &ctx->d_model, model_all))) return rc;
   if ((rc = dev_upload(ctx, &ctx->d_chanfreq, chanfreq))) return rc;
-  HIPCHK(hipMalloc((void**)&ctx->d_fvalid, std::max<long long>(fvo, 16)));
-  HIPCHK(hipMemset(ctx->d_fvalid, 0, std::max<long long>(fvo, 16)));
+  if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_fvalid, std::max<long long>(fvo, 16))) return rc_;
+  if (int rc_ = dev_zero(ctx, ctx->d_fvalid, std::max<long long>(fvo, 16))) return rc_;
   // FFT-length classes, in order of first appearance
   ctx->fclasses.clear();
   for (int d = 0; d < nds; ++d) {
@@ -537,10 +554,10 @@ int build_tables(fxg_ctx* ctx) {
     if ((rc = dev_upload(ctx, &ctx->d_twiddle, tw))) return rc;
     if ((rc = dev_upload(ctx, &ctx->d_dslist, dsl))) return rc;
   }
-  HIPCHK(hipMalloc((void**)&ctx->d_run, sizeof(DevDsRun) * nds));
-  HIPCHK(hipMalloc((void**)&ctx->d_weights, sizeof(float) * (size_t)nds * ctx->B));
-  HIPCHK(hipMemset(ctx->d_weights, 0, sizeof(float) * (size_t)nds * ctx->B));
-  HIPCHK(hipMalloc((void**)&ctx->d_flags, sizeof(int32_t) * (size_t)nds * ctx->flaglen * 2));
+  if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_run, sizeof(DevDsRun) * nds)) return rc_;
+  if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_weights, sizeof(float) * (size_t)nds * ctx->B)) return rc_;
+  if (int rc_ = dev_zero(ctx, ctx->d_weights, sizeof(float) * (size_t)nds * ctx->B)) return rc_;
+  if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_flags, sizeof(int32_t) * (size_t)nds * ctx->flaglen * 2)) return rc_;
   return FXG_OK;
 }
 
@@ -626,7 +643,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
   cap = std::min<long long>(cap, round_up(ctx->B, FXG_XPAD));
   cap = round_up(cap, FXG_XPAD);
   ctx->chunkcap = (int)cap;
-  HIPCHK(hipMalloc((void**)&ctx->d_params, sizeof(FftDelay) * (size_t)nds * cap));
+  if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_params, sizeof(FftDelay) * (size_t)nds * cap)) return rc_;
   if (mfma) {
     long long off = 0, aoff = 0, soff = 0;
     for (auto& g : ctx->groups) {
@@ -641,9 +658,9 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
     ctx->spec_elems = off;
     ctx->vacc_len = aoff;
     ctx->xscale_len = soff;
-    HIPCHK(hipMalloc((void**)&ctx->d_spec16, sizeof(uint32_t) * std::max<long long>(off, 4)));
-    HIPCHK(hipMemset(ctx->d_spec16, 0, sizeof(uint32_t) * std::max<long long>(off, 4)));
-    HIPCHK(hipMalloc((void**)&ctx->d_vacc, sizeof(float) * std::max<long long>(aoff, 4)));
+    if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_spec16, sizeof(uint32_t) * std::max<long long>(off, 4))) return rc_;
+    if (int rc_ = dev_zero(ctx, ctx->d_spec16, sizeof(uint32_t) * std::max<long long>(off, 4))) return rc_;
+    if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_vacc, sizeof(float) * std::max<long long>(aoff, 4))) return rc_;
     // nominal per-(input, channel) scale 1/sqrt(L <x^2>) of the input's sample levels, which
     // keeps the 16-bit spectra near unit magnitude; the accumulation divides it out again
     const double p_hi = 0.5 * std::erfc(0.95 / std::sqrt(2.0));
@@ -667,8 +684,8 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
       off += (long long)ctx->h_ds[d].nrecbands * cap * ctx->h_ds[d].C;
     }
     ctx->spec_elems = off;
-    HIPCHK(hipMalloc((void**)&ctx->d_spec32, sizeof(float2) * std::max<long long>(off, 2)));
-    HIPCHK(hipMemset(ctx->d_spec32, 0, sizeof(float2) * std::max<long long>(off, 2)));
+    if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_spec32, sizeof(float2) * std::max<long long>(off, 2))) return rc_;
+    if (int rc_ = dev_zero(ctx, ctx->d_spec32, sizeof(float2) * std::max<long long>(off, 2))) return rc_;
   }
   // ---- products and accumulation entries ----
   const int bpa = ctx->bpa;
@@ -742,7 +759,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
       }
     }
     ctx->xacc_len = acc;
-    HIPCHK(hipMalloc((void**)&ctx->d_xacc, sizeof(float2) * std::max<long long>(acc, 2)));
+    if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_xacc, sizeof(float2) * std::max<long long>(acc, 2))) return rc_;
     if (int rc = dev_upload(ctx, &ctx->d_prods, ctx->prods)) return rc;
     if (int rc = dev_upload(ctx, &ctx->d_finp, ctx->finp)) return rc;
   } else {
@@ -1063,9 +1080,9 @@ int ensure_slots(fxg_ctx* ctx) {
     s.cap.assign(ctx->nds, 0);
     for (int d = 0; d < ctx->nds; ++d) {
       s.cap[d] = ctx->databytes[d] + 4LL * ctx->dss[d].framebytes;
-      HIPCHK(hipMalloc((void**)&s.d_data[d], s.cap[d]));
+      if (int rc_ = dev_malloc(ctx, (void**)&s.d_data[d], s.cap[d])) return rc_;
     }
-    HIPCHK(hipMalloc((void**)&s.d_results, sizeof(float2) * std::max<long long>(ctx->coreresultlength, 1)));
+    if (int rc_ = dev_malloc(ctx, (void**)&s.d_results, sizeof(float2) * std::max<long long>(ctx->coreresultlength, 1))) return rc_;
     HIPCHK(hipHostMalloc((void**)&s.h_results, sizeof(float2) * std::max<long long>(ctx->coreresultlength, 1), 0));
   }
   return FXG_OK;
@@ -1089,16 +1106,25 @@ const char* fxg_strerror(int code) {
 
 const char* fxg_last_error(const fxg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int fxg_create(const fxg_config* cfg, fxg_ctx** out) {
+}  // extern "C"
+
+namespace {
+
+// fxg_create, or with dry set a pass that validates and sizes everything without touching
+// the device (fxg_footprint).
+int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
   if (!cfg || !out) return FXG_E_INVALID;
   *out = nullptr;
   if (cfg->nfreqs < 1 || !cfg->freqs || cfg->ndatastreams < 1 || !cfg->datastreams ||
       cfg->nbaselines < 0 || (cfg->nbaselines > 0 && !cfg->baselines))
     return FXG_E_INVALID;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FXG_E_NODEVICE;
+  if (!dry) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FXG_E_NODEVICE;
+  }
   fxg_ctx* ctx = new fxg_ctx();
-  (void)hipGetDevice(&ctx->device);
+  ctx->dry = dry;
+  if (!dry) (void)hipGetDevice(&ctx->device);
   ctx->nfreqs = cfg->nfreqs;
   ctx->nds = cfg->ndatastreams;
   ctx->nbl = cfg->nbaselines;
@@ -1123,12 +1149,12 @@ int fxg_create(const fxg_config* cfg, fxg_ctx** out) {
   int rc = validate_and_derive(ctx);
   if (rc == FXG_OK) {
     build_layout(ctx);
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    if (!dry && hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
       rc = fail(ctx, FXG_E_HIP, "stream create");
   }
   if (rc == FXG_OK) rc = build_tables(ctx);
   if (rc == FXG_OK) rc = build_engines(ctx, cfg->chunk_ffts);
-  if (rc == FXG_OK) {
+  if (rc == FXG_OK && !dry) {
     for (auto& sg : ctx->stage) {
       if (hipHostMalloc((void**)&sg.h_run, sizeof(DevDsRun) * ctx->nds, 0) != hipSuccess ||
           hipHostMalloc((void**)&sg.h_flags, sizeof(int32_t) * ctx->nds * ctx->flaglen + 16, 0) != hipSuccess ||
@@ -1145,6 +1171,41 @@ int fxg_create(const fxg_config* cfg, fxg_ctx** out) {
     return rc;
   }
   *out = ctx;
+  return FXG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fxg_create(const fxg_config* cfg, fxg_ctx** out) { return create_impl(cfg, out, false); }
+
+int fxg_footprint(const fxg_config* cfg, int64_t* device_bytes, int64_t* coreresultlength, int64_t* slot_bytes) {
+  fxg_ctx* ctx = nullptr;
+  int rc = create_impl(cfg, &ctx, true);
+  if (rc != FXG_OK) return rc;
+  if (device_bytes) *device_bytes = ctx->dev_bytes;
+  if (coreresultlength) *coreresultlength = ctx->coreresultlength;
+  if (slot_bytes) {
+    long long sb = 0;
+    for (int d = 0; d < ctx->nds; ++d) sb += ctx->databytes[d] + 4LL * ctx->dss[d].framebytes;
+    *slot_bytes = sb;
+  }
+  delete ctx;
+  return FXG_OK;
+}
+
+int fxg_input_samples(const fxg_ctx* ctx, int64_t* samples) {
+  if (!ctx || !samples) return FXG_E_INVALID;
+  long long n = 0;
+  for (int d = 0; d < ctx->nds; ++d) n += (long long)ctx->h_ds[d].nrecbands * ctx->B * ctx->h_ds[d].L;
+  *samples = n;
+  return FXG_OK;
+}
+
+int fxg_device_bytes(const fxg_ctx* ctx, int64_t* bytes) {
+  if (!ctx || !bytes) return FXG_E_INVALID;
+  *bytes = ctx->dev_bytes;
   return FXG_OK;
 }
 

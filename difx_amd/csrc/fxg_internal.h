@@ -212,3 +212,4 @@ int fxg_launch_fin_mfma(const FinMfma* e, int n, int maxout, const DevXGroup* g,
 int fxg_launch_fin_weights(const FinWeight* e, int n, const float* w, int B, float* resf, hipStream_t st);
 int fxg_launch_zero(void* p, long long bytes, hipStream_t st);
 int fxg_launch_fill(float* p, long long n, float v, hipStream_t st);
+int fxg_launch_add(float* dst, const float* src, long long n, hipStream_t st);

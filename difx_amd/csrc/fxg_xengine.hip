@@ -284,6 +284,12 @@ __global__ void k_zero(float* p, long long n) {
     p[i] = 0.0f;
 }
 
+// Visibility::addData (visibility.cpp:192-205): one subint's results into an integration.
+__global__ void k_add(float* dst, const float* src, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] += src[i];
+}
+
 __global__ void k_fill(float* p, long long n, float v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
@@ -339,6 +345,14 @@ int fxg_launch_zero(void* p, long long bytes, hipStream_t st) {
   if (blocks > 8192) blocks = 8192;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, (float*)p, n);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_add(float* dst, const float* src, long long n, hipStream_t st) {
+  if (n <= 0) return 0;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_add, dim3((unsigned)blocks), dim3(256), 0, st, dst, src, n);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

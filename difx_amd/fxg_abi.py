@@ -119,6 +119,24 @@ class fxg_timing(C.Structure):
     ]
 
 
+class fxg_server_config(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_int64),
+        ("max_jobs", C.c_int32),
+        ("inflight_per_job", C.c_int32),
+    ]
+
+
+class fxg_server_stats(C.Structure):
+    _fields_ = [
+        ("jobs_added", C.c_int32), ("jobs_admitted", C.c_int32), ("jobs_finished", C.c_int32),
+        ("jobs_active", C.c_int32), ("max_active", C.c_int32), ("subints_done", C.c_int32),
+        ("samples_done", C.c_int64), ("admitted_bytes", C.c_int64), ("peak_admitted_bytes", C.c_int64),
+        ("first_launch_s", C.c_double), ("last_done_s", C.c_double),
+        ("latency_p50_ms", C.c_double), ("latency_p99_ms", C.c_double), ("latency_max_ms", C.c_double),
+    ]
+
+
 # exported symbols of libfxg.so (every function declared in include/fxg.h)
 EXPORTED_SYMBOLS = [
     "fxg_create", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
@@ -126,4 +144,7 @@ EXPORTED_SYMBOLS = [
     "fxg_process_subint_device", "fxg_ring_length", "fxg_submit_subint", "fxg_wait",
     "fxg_get_spectrum", "fxg_get_data_weights", "fxg_unpack_device",
     "fxg_delay_interpolator", "fxg_set_timing", "fxg_get_timing",
+    "fxg_footprint", "fxg_input_samples", "fxg_device_bytes",
+    "fxg_server_create", "fxg_server_destroy", "fxg_server_add_job", "fxg_server_push_subint",
+    "fxg_server_wait_job", "fxg_server_get_stats", "fxg_server_job_error",
 ]

@@ -111,7 +111,7 @@ typedef struct fxg_config {
   const fxg_baseline* baselines;
   int64_t subint_ns;          /* SUBINT NANOSECONDS */
   int32_t guard_ns;           /* GUARD NANOSECONDS */
-  int32_t fringe_rot_order;   /* FRINGE ROTN ORDER: 1 (0 and 2 are rejected by this build) */
+  int32_t fringe_rot_order;   /* FRINGE ROTN ORDER: 0 (post-F), 1 (linear) or 2 (quadratic) */
   int32_t xmac_stride_len;    /* XMAC STRIDE LEN (the result layout does not depend on it) */
   int32_t num_buffered_ffts;  /* NUM BUFFERED FFTS */
   int32_t write_cross_autocorrs; /* cross-polarisation autocorrelations (conf.writeautocorrs) */
@@ -202,6 +202,64 @@ typedef struct fxg_timing {
 } fxg_timing;
 int fxg_set_timing(fxg_ctx* ctx, int32_t enable);
 int fxg_get_timing(fxg_ctx* ctx, fxg_timing* out);
+
+/* Sizing without a device (Core::getEstimatedBytes, core.cpp:2197-2210, and
+ * Mode::getEstimatedBytes, mode.h:216): validates cfg and reports the device bytes
+ * fxg_create would hold, coreresultlength, and the bytes of one slot's input data (all
+ * datastreams).  Any output pointer may be null. */
+int fxg_footprint(const fxg_config* cfg, int64_t* device_bytes, int64_t* coreresultlength,
+                  int64_t* slot_bytes);
+/* Input samples one subint processes: sum over datastreams of recorded bands x
+ * blockspersend x fftchannels (a complex sample counts once). */
+int fxg_input_samples(const fxg_ctx* ctx, int64_t* samples);
+/* Device bytes held by a context. */
+int fxg_device_bytes(const fxg_ctx* ctx, int64_t* bytes);
+
+/* ---- Serving: concurrent correlation jobs on one device ------------------------------
+ * No counterpart in the reference, which runs one job per mpifxcorr process; this replaces
+ * the FxManager -> Core subint dispatch (fxmanager.cpp:331-336, core.cpp:205-330) and the
+ * Visibility::addData integration (visibility.cpp:192-205) for many jobs at once.
+ *
+ * A server owns a scheduler thread.  Jobs are admitted in arrival order while the sum of
+ * their device footprints stays within hbm_budget_bytes and at most max_jobs are active;
+ * each admitted job gets its own context and stream, so the subints of different jobs
+ * overlap on the device.  A job keeps its delay-model tables, per-subint delay state and
+ * its integration accumulator (the sum of its subint results, unnormalised) in device
+ * memory until it finishes.  A finished job's memory is released when a waiting job needs
+ * it, which admits that job. */
+typedef struct fxg_server fxg_server;
+typedef struct fxg_server_config {
+  int64_t hbm_budget_bytes;   /* admission limit; 0 = 85% of the device memory free at create */
+  int32_t max_jobs;           /* jobs active at once (0 = 64) */
+  int32_t inflight_per_job;   /* subints of one job enqueued ahead (0 = 2) */
+} fxg_server_config;
+typedef struct fxg_server_stats {
+  int32_t jobs_added, jobs_admitted, jobs_finished, jobs_active;
+  int32_t max_active;         /* most jobs active at once */
+  int32_t subints_done;
+  int64_t samples_done;       /* input samples of the finished subints */
+  int64_t admitted_bytes, peak_admitted_bytes;
+  double  first_launch_s, last_done_s;  /* seconds since server creation */
+  double  latency_p50_ms, latency_p99_ms, latency_max_ms;  /* push -> results integrated */
+} fxg_server_stats;
+int fxg_server_create(const fxg_server_config* cfg, fxg_server** out);
+int fxg_server_destroy(fxg_server* srv);
+/* Queue a job of nsubints subints; returns at once with its id (admission happens later).
+ * With data_on_device the job's subint data will be device pointers used in place; otherwise
+ * host pointers, copied into per-job device input slots (counted in its footprint). */
+int fxg_server_add_job(fxg_server* srv, const fxg_config* cfg, int32_t nsubints,
+                       int32_t data_on_device, int32_t* job);
+/* One subint of a job (arguments as fxg_process_subint).  The data pointers must stay valid
+ * until fxg_server_wait_job returns; control words are copied during the call. */
+int fxg_server_push_subint(fxg_server* srv, int32_t job, const int32_t offsets[3],
+                           const uint8_t* const* data, const int32_t* nbytes,
+                           const int32_t* const* control);
+/* Block until all subints of the job are integrated; copies the integrated results
+ * (coreresultlength cf32) and the job's p50 / p99 subint latency (ms). */
+int fxg_server_wait_job(fxg_server* srv, int32_t job, float* integrated, double* p50_ms, double* p99_ms);
+int fxg_server_get_stats(fxg_server* srv, fxg_server_stats* out);
+/* Why a job failed (empty if it did not). */
+int fxg_server_job_error(fxg_server* srv, int32_t job, char* buf, int32_t len);
 
 #ifdef __cplusplus
 }

@@ -46,3 +46,41 @@ def test_create_rejects_null_config():
     h = ctypes.c_void_p()
     lib.fxg_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
     assert lib.fxg_create(None, ctypes.byref(h)) == abi.FXG_E_INVALID
+
+
+# ------------------------------------------------ sizing and validation without a device
+def test_footprint_without_device():
+    """fxg_footprint validates and sizes a configuration with no device present; the result
+    length agrees with the oracle's populateResultLengths restatement."""
+    _lib()
+    import fxoracle as fo
+    from difx_amd.core import footprint
+    from fxcases import case_cfg1, case_cfg3
+    for cfg in (case_cfg1(B=32, nstations=4, nfreq=2), case_cfg3(B=32, xengine="fp16")):
+        dev, crl, slot = footprint(cfg)
+        assert crl == fo.result_layout(cfg).coreresultlength
+        assert dev > 0 and slot > 0
+    a, _, _ = footprint(case_cfg1(B=64, nstations=4, nfreq=2, chunk_ffts=16))
+    b, _, _ = footprint(case_cfg1(B=64, nstations=4, nfreq=2, chunk_ffts=64))
+    assert b > a                                   # spectra chunk buffers scale with the chunk
+
+
+@pytest.mark.parametrize("mutate,ok", [
+    (lambda c: setattr(c, "fringe_rot_order", 3), False),
+    (lambda c: setattr(c, "fringe_rot_order", 0), True),
+    (lambda c: setattr(c, "fringe_rot_order", 2), True),
+    (lambda c: setattr(c, "subint_ns", c.subint_ns + 1000), False),     # not a whole FFT count
+    (lambda c: (setattr(c.freqs[0], "nchan", 16384), setattr(c, "subint_ns", 2 * 512000)), False),  # L > 8192
+    (lambda c: setattr(c.freqs[0], "chanavg", 3), False),                # does not divide nchan
+])
+def test_validation(mutate, ok):
+    _lib()
+    from difx_amd.core import FxgError, footprint
+    from fxcases import case_cfg0
+    cfg = case_cfg0(B=32)
+    mutate(cfg)
+    if ok:
+        footprint(cfg)
+    else:
+        with pytest.raises(FxgError):
+            footprint(cfg)
