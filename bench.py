@@ -64,6 +64,20 @@ def device_slot(cfg, offsets, torch, dev, seed=7):
     return bufs, nbytes, ctrl
 
 
+def uneven_config(xengine: str, nstations: int = 16):
+    """Uneven bands: per station a 4 x 64 MHz (nchan 4096, 8192-point FFTs), a 4 x 32 MHz
+    (nchan 2048) and an 8 x 8 MHz (nchan 512) datastream, dual pol, 64 us FFTs, plus six zoom
+    windows of 0.5-2 MHz; 0.512 s subints (8000 FFT blocks)."""
+    from difx_amd import config as cf
+    groups = [(64.0, 4096, [4800.0 + 64.0 * k for k in range(4)]),
+              (32.0, 2048, [5200.0 + 32.0 * k for k in range(4)]),
+              (8.0, 512, [5400.0 + 8.0 * k for k in range(8)])]
+    zooms = [(0, 0, 10.0, 2.0), (0, 2, 30.0, 0.5), (1, 1, 4.0, 1.0), (1, 3, 20.0, 2.0),
+             (2, 0, 1.0, 0.5), (2, 5, 3.0, 2.0)]
+    return cf.batched_config(nstations, groups, zooms, subint_ns=512_000_000, xengine=xengine,
+                             seed=20261203)
+
+
 def samples_per_subint(cfg) -> float:
     tot = 0.0
     for d, ds in enumerate(cfg.datastreams):
@@ -97,6 +111,111 @@ def x_engine_flops(cfg, B: int) -> float:
         C = cfg.freqs[ds.recfreqs[0]].nchan
         tot += len(ds.recbands) * C * (2 if cfg.write_cross_autocorrs else 1)
     return 8.0 * tot * B
+
+
+def serving_jobs(rng, window_s: float, rate_per_s: float, subints_per_job: int):
+    """Poisson job stream: arrival times with mean rate `rate_per_s` over `window_s`; each job
+    draws its station count from {4, 8, 16, 32}, its bandwidth as m x 32 MHz with m from
+    {2, 4, 8, 16} (2-bit bands of one sample word must be a power of two) recorded as m/2
+    freqs x 2 pols, nchan from {256, 1024, 4096}; 0.512 s subints (0.5 s is not a whole
+    number of 4096-channel FFTs at 32 MHz).  Delays stay within +-36 us so that the stations
+    of a job can share one input buffer (see serving_inputs)."""
+    from difx_amd import config as cf
+    t, jobs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate_per_s)
+        if t > window_s:
+            break
+        n = int(rng.choice([4, 8, 16, 32]))
+        m = int(rng.choice([2, 4, 8, 16]))
+        nchan = int(rng.choice([256, 1024, 4096]))
+        freqs = [4800.0 + 40.0 * k for k in range(m // 2)]
+        B = 512_000_000 // (nchan * 1000 // 32)
+        # spectra chunk of at most ~0.5 GB per job keeps many jobs resident at once
+        per_fft = 4 * nchan * m * 32 * ((2 * n + 31) // 32)
+        chunk = max(64, min(B, (512 << 20) // per_fft)) // 8 * 8
+        cfg = cf.standard_config(n, 2, freqs, 32.0, nchan, subint_ns=512_000_000, xengine="fp16",
+                                 seed=int(rng.integers(1 << 30)), chunk_ffts=chunk, max_delay_us=40.0)
+        jobs.append((t, cfg, subints_per_job, (n, m, nchan)))
+    return jobs
+
+
+def serving_inputs(cfg, k: int):
+    """Shared-buffer inputs of subint k of a serving job: (buffer bytes, control words) valid
+    for every station.  The buffer holds the slot capacity (getDataBytes plus four frames) and
+    starts one frame before the earliest sample any station needs (its delay at the subint
+    start), which leaves room for the +-36 us spread of the stations' delays."""
+    import math
+    from difx_amd import synth
+    ds = cfg.datastreams[0]
+    frame_ns = int(round(synth.samples_per_frame(ds) * cfg.sampletime_us(0) * 1000.0))
+    t0 = (1 + k) * 1e9
+    first = min(t0 - 1000.0 * synth.eval_delay(d.model, d.model_order, t0 / 1e9, cfg.model_inc_secs)
+                for d in cfg.datastreams)
+    start = int(math.floor(first / frame_ns) - 1) * frame_ns
+    nbytes = synth.data_bytes(cfg, 0) + 4 * ds.framebytes
+    return nbytes, synth.control_words(cfg, 0, [0, 1 + k, 0], start)
+
+
+def serving_bench(torch, dev, window_s=6.0, rate_per_s=4.0, subints_per_job=8, seed=20261204):
+    """Serving run: jobs arrive as a Poisson process, each pushes one 0.512 s subint every
+    0.512 s of wall time from its arrival (real-time pacing), the library's scheduler admits
+    them by device footprint and overlaps them.  Input buffers are device-resident and shared
+    by every station and job of one datastream shape (their content does not affect the
+    timing; every FFT window lies inside them, so no work is skipped)."""
+    from difx_amd.server import FxServer
+    rng = np.random.default_rng(seed)
+    jobs = serving_jobs(rng, window_s, rate_per_s, subints_per_job)
+    pool = {}
+    events = []
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    from difx_amd import synth
+    for j, (t0, cfg, nsub, shape) in enumerate(jobs):
+        key = shape[1:]
+        if key not in pool:
+            ds = cfg.datastreams[0]
+            nbytes, _ = serving_inputs(cfg, 0)
+            nframes = nbytes // ds.framebytes
+            buf = torch.randint(0, 256, (nframes, ds.framebytes), dtype=torch.uint8, device=dev, generator=g)
+            hdr = np.stack([synth.vdif_header(0, f, ds.framebytes, len(ds.recbands), ds.nbits, False)
+                            for f in range(nframes)])
+            buf[:, : ds.headerbytes] = torch.from_numpy(hdr).to(dev)
+            pool[key] = buf.reshape(-1).contiguous()
+        for k in range(nsub):
+            events.append((t0 + k * 0.512, j, k))
+    events.sort()
+    torch.cuda.synchronize()
+    ids = {}
+    total_samples = sum(samples_per_subint(c) * n for _, c, n, _ in jobs)
+    with FxServer() as srv:
+        t_start = time.perf_counter()
+        for t_ev, j, k in events:
+            delay = t_ev - (time.perf_counter() - t_start)
+            if delay > 0:
+                time.sleep(delay)
+            t0, cfg, nsub, shape = jobs[j]
+            if k == 0:
+                ids[j] = srv.add_job(cfg, nsub, data_on_device=True)
+            nbytes, ctrl = serving_inputs(cfg, k)
+            nds = len(cfg.datastreams)
+            srv.push(ids[j], (0, 1 + k, 0), [pool[shape[1:]].data_ptr()] * nds, [nbytes] * nds, [ctrl] * nds)
+        per_job = [srv.wait_job(ids[j]) for j in range(len(jobs))]
+        t_end = time.perf_counter()
+        st = srv.stats()
+    busy = st["last_done_s"] - st["first_launch_s"]
+    offered = total_samples / (events[-1][0] + 0.512) / 1e9 if events else 0.0
+    return {"jobs": len(jobs), "subints": st["subints_done"],
+            "arrival_rate_jobs_per_s": rate_per_s, "window_s": window_s, "subints_per_job": subints_per_job,
+            "offered_gsamples_per_s": offered,
+            "throughput_gsamples_per_s": st["samples_done"] / busy / 1e9 if busy > 0 else 0.0,
+            "p50_subint_latency_ms": st["latency_p50_ms"], "p99_subint_latency_ms": st["latency_p99_ms"],
+            "max_subint_latency_ms": st["latency_max_ms"], "subint_length_ms": 512.0,
+            "max_concurrent_jobs": st["max_active"],
+            "peak_admitted_gb": st["peak_admitted_bytes"] / 1e9,
+            "worst_job_p99_ms": max(p[2] for p in per_job) if per_job else 0.0,
+            "wall_s": t_end - t_start,
+            "job_shapes_n_m_nchan": [list(j[3]) for j in jobs]}
 
 
 def time_path(cfg, steps, warmup, torch, dev, offsets):
@@ -219,6 +338,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copy for PMC scaling")
     args = ap.parse_args()
     if args.copy_probe:
@@ -266,6 +388,41 @@ def main():
                  "f_ms_per_step": ptm["f_ms"] / psteps, "x_ms_per_step": ptm["x_ms"] / psteps,
                  "speedup_headline_vs_plain": value / pval,
                  "headline_vs_plain_rel_diff": rel}
+    uneven = None
+    if not args.no_uneven:
+        ucfg = uneven_config("fp16")
+        print("[bench] uneven bands (FFT lengths 1024/4096/8192 + zooms), 1 warmup + 3 steps",
+              file=sys.stderr, flush=True)
+        u = time_path(ucfg, 3, 1, torch, dev, offsets)
+        usamp = samples_per_subint(ucfg)
+        ulat = np.array(u["lat_ms"])
+        uneven = {"workload": "16 stations dual-pol: 4 x 64 MHz (nchan 4096), 4 x 32 MHz (nchan 2048), "
+                              "8 x 8 MHz (nchan 512) datastreams + 6 zooms of 0.5-2 MHz, 0.512 s subint",
+                  "value": usamp * 3 / u["wall_s"] / 1e9, "unit": "Gsamples/s",
+                  "p50_ms": float(np.percentile(ulat, 50)), "p99_ms": float(np.percentile(ulat, 99)),
+                  "subint_length_ms": 512.0,
+                  "f_ms_per_step": u["timing"]["f_ms"] / 3, "x_ms_per_step": u["timing"]["x_ms"] / 3}
+    target = None
+    if not args.no_target:
+        from difx_amd import config as cf
+        freqs = [4800.0 + 40.0 * k for k in range(8)]
+        tg = {}
+        for xe, steps_t in (("fp16", 3), ("fp32", 2)):
+            tcfg = cf.standard_config(16, 2, freqs, 32.0, 1024, subint_ns=500_000_000, xengine=xe, seed=20261205)
+            print(f"[bench] 16-station target shape, {xe}, 1 warmup + {steps_t} steps", file=sys.stderr, flush=True)
+            r = time_path(tcfg, steps_t, 1, torch, dev, offsets)
+            tl = np.array(r["lat_ms"])
+            tg[xe] = {"value": samples_per_subint(tcfg) * steps_t / r["wall_s"] / 1e9,
+                      "p99_ms": float(np.percentile(tl, 99)),
+                      "f_ms_per_step": r["timing"]["f_ms"] / steps_t, "x_ms_per_step": r["timing"]["x_ms"] / steps_t}
+        target = {"workload": "16 stations dual-pol, 8 x 32 MHz, nchan 1024, 0.5 s subint",
+                  "mfma_fp16": tg["fp16"], "plain_fp32": tg["fp32"],
+                  "speedup_mfma_vs_plain": tg["fp16"]["value"] / tg["fp32"]["value"],
+                  "subint_length_ms": 500.0}
+    serving = None
+    if not args.no_serving:
+        print("[bench] serving: Poisson job stream", file=sys.stderr, flush=True)
+        serving = serving_bench(torch, dev)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
@@ -302,6 +459,9 @@ def main():
                      "traffic": measured_traffic(ROOT / "profiles" / "k_fengine_2048_traffic.txt", ffts_per_launch),
                      "avg_launch_ms": f_avg_ms, "algorithmic_bytes_per_launch": fbytes},
         "plain_fp32": plain,
+        "target_16_stations": target,
+        "uneven_bands": uneven,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -156,9 +156,10 @@ def control_words(cfg, d: int, offsets, start_ns: int, B: Optional[int] = None,
         B = cfg.blockspersend()
     nflag = (B + 29) // 30
     c = np.zeros(3 + nflag, dtype=np.int32)
+    sec = int(start_ns) // 1000000000
     c[0] = offsets[0]
-    c[1] = start_ns // 1000000000
-    c[2] = start_ns - c[1] * 1000000000
+    c[1] = sec
+    c[2] = int(start_ns) - sec * 1000000000
     flags = np.zeros(nflag * 30, dtype=np.int64)
     flags[:B] = 1
     for i in invalid_ffts:
