@@ -81,6 +81,7 @@ struct fxg_ctx {
     std::vector<int> dsl;        // datastream indices
     long long tw_off = 0;        // offset of this L's twiddle tables in d_twiddle
     long long dsl_off = 0;       // offset of dsl in d_dslist
+    int phcap = 0;               // phasor-table entries: max over dsl of S + NS and S + NFS
   };
   std::vector<FClass> fclasses;
   int* d_dslist = nullptr;
@@ -523,6 +524,9 @@ int build_tables(fxg_ctx* ctx) {
       ctx->fclasses.push_back(c);
     }
     ctx->fclasses[k].dsl.push_back(d);
+    const DevDs& x = ctx->h_ds[d];
+    const int need = (std::max(x.S + x.NS, x.S + x.NFS) + 7) / 8 * 8;
+    ctx->fclasses[k].phcap = std::max(ctx->fclasses[k].phcap, need);
   }
   // FFT twiddles of the Stockham passes after the first, per pass (R, Ns): entries
   // exp(-2 pi i k r / (Ns R)) for r = 1..R-1, k = 0..Ns-1, k fastest; computed in FP64.
@@ -991,6 +995,7 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
         fa.blocks_per_wg = (int)std::max(1LL, std::min<long long>(16, nblocks / wgs_per_ds));
         fa.twiddle = ctx->d_twiddle + fc.tw_off;
         fa.dslist = ctx->d_dslist + fc.dsl_off;
+        fa.phcap = fc.phcap;
         if (fxg_launch_fengine(fa, fc.L, fc.cplx, ncl, st)) return fail(ctx, FXG_E_HIP, "F-engine launch");
       }
     }

@@ -224,12 +224,6 @@ __device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_
   return r;
 }
 
-// LDS capacity of the rotator and fractional-sample phasor tables: S + NS and S + NFS
-// entries, at most a quarter of the FFT length and at most 512 (checked at set-up).
-template <int L>
-struct PhasorCap {
-  static constexpr int N = (L / 4 < 512) ? L / 4 : 512;
-};
 
 // Per-datastream constants the F-engine reads, copied to registers once per workgroup.
 struct DsConst {
@@ -368,13 +362,15 @@ __global__ void __launch_bounds__(L / 8, FXG_F_MINWAVES)
 k_fengine(FengineArgs args) {
   using P = Plan<L>;
   constexpr int NT = L / 8;
-  __shared__ float2 sbuf[L + L / 8];
-#ifndef FXG_TW_GLOBAL
-  __shared__ float2 stw[P::TWN];
-#endif
-  __shared__ float2 srot[PhasorCap<L>::N];
-  __shared__ float2 sfrac[PhasorCap<L>::N];
-  __shared__ float squad[ORDER == 2 ? PhasorCap<L>::N : 1];
+  // LDS, sized at launch (fengine_lds_bytes): the FFT exchange buffer, the per-pass twiddles,
+  // and the rotator / fractional-sample phasor tables of `phcap` entries each (the largest
+  // S + NS or S + NFS of the launch's datastreams), plus the order-2 quadratic arguments
+  extern __shared__ float2 fx_smem[];
+  float2* sbuf = fx_smem;
+  float2* stw = sbuf + (L + L / 8);
+  float2* srot = stw + P::TWN;
+  float2* sfrac = srot + args.phcap;
+  float* squad = reinterpret_cast<float*>(sfrac + args.phcap);
 
   const int t = threadIdx.x;
   const int d = args.dslist[blockIdx.y];
@@ -395,12 +391,8 @@ k_fengine(FengineArgs args) {
   const int out_mode = args.out_mode;
   const int cap = args.chunkcap;
 
-#ifndef FXG_TW_GLOBAL
   for (int m = t; m < P::TWN; m += NT) stw[m] = args.twiddle[m];
   const float2* twsrc = stw;
-#else
-  const float2* __restrict__ twsrc = args.twiddle;
-#endif
 
   const int nblocks = args.nffts_pad / FXG_TB;
   const int blk0 = blockIdx.x * args.blocks_per_wg;
@@ -698,21 +690,46 @@ int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+template <int L>
+static size_t fengine_lds_bytes(int phcap, int order) {
+  return sizeof(float2) * ((size_t)(L + L / 8) + Plan<L>::TWN + 2 * (size_t)phcap) +
+         (order == 2 ? sizeof(float) * (size_t)phcap : 0);
+}
+
+// Launch one F-engine instantiation with `lds` bytes of dynamic LDS (raising the kernel's
+// dynamic-LDS limit once where it exceeds the default).
+template <int L, bool CPLX, int ORDER>
+static int launch_k(const FengineArgs& a, dim3 grid, size_t lds, hipStream_t st) {
+  static size_t raised = 0;
+  if (lds > 65536 && lds > raised) {
+    if (hipFuncSetAttribute((const void*)k_fengine<L, CPLX, ORDER>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return -1;
+    raised = lds;
+  }
+  hipLaunchKernelGGL((k_fengine<L, CPLX, ORDER>), grid, dim3(L / 8), lds, st, a);
+  return 0;
+}
+
 template <int L, bool CPLX>
 static int launch_f(const FengineArgs& a, int nds, hipStream_t st) {
   static_assert(L <= FXG_MAX_L, "FFT length beyond the fused F-engine");
   hipLaunchKernelGGL(k_fftparams, dim3((a.nffts_pad + 255) / 256, nds), dim3(256), 0, st, a);
   const int nblocks = a.nffts_pad / FXG_TB;
   const int gx = (nblocks + a.blocks_per_wg - 1) / a.blocks_per_wg;
+  const dim3 grid(gx, nds);
+  const size_t lds = fengine_lds_bytes<L>(a.phcap, a.rotorder);
+  int rc;
   switch (a.rotorder) {
-    case 1: hipLaunchKernelGGL((k_fengine<L, CPLX, 1>), dim3(gx, nds), dim3(L / 8), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_fengine<L, CPLX, 2>), dim3(gx, nds), dim3(L / 8), 0, st, a); break;
+    case 1: rc = launch_k<L, CPLX, 1>(a, grid, lds, st); break;
+    case 2: rc = launch_k<L, CPLX, 2>(a, grid, lds, st); break;
     case 0:
       if (CPLX) return -2;                   // post-F rotation of complex data (mode.cpp:1130-1133)
-      hipLaunchKernelGGL((k_fengine<L, false, 0>), dim3(gx, nds), dim3(L / 8), 0, st, a);
+      rc = launch_k<L, false, 0>(a, grid, lds, st);
       break;
     default: return -2;
   }
+  if (rc) return rc;
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -133,6 +133,7 @@ struct FengineArgs {
   int blocks_per_wg;
   const int* dslist;      // datastreams of this launch, indexed by grid.y (one FFT-length class)
   int rotorder;           // fringe rotation order 0, 1 or 2
+  int phcap;              // LDS entries of each phasor table (max S + NS, S + NFS of the launch)
 };
 
 // Setup for one subint (delay interpolators, frame validity).
