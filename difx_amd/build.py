@@ -11,13 +11,14 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libfxg.so"
-SOURCES = ["fxg_fengine.hip", "fxg_xengine.hip", "fxg_api.cpp", "fxg_server.cpp"]
+SOURCES = ["fxg_fengine.hip", "fxg_fpersist.hip", "fxg_xengine.hip", "fxg_api.cpp", "fxg_server.cpp"]
 ARCH = "gfx950"
 
 
@@ -35,19 +36,20 @@ def build(verbose: bool = False) -> Path:
     flags = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I", str(INCLUDE), "-I", str(CSRC), "-Wno-unused-result"]
-    objs = []
-    for src in SOURCES:
+    def compile_one(src):
         obj = objdir / (src + ".o")
-        cmd = [cc] + flags + ["-c", str(CSRC / src), "-o", str(obj)]
-        if src.endswith(".cpp"):
-            cmd = [cc] + flags + ["-x", "hip", "-c", str(CSRC / src), "-o", str(obj)]
-        r = subprocess.run(cmd, capture_output=True, text=True)
-        if r.returncode != 0:
-            sys.stderr.write(r.stdout + r.stderr)
-            raise RuntimeError(f"hipcc failed on {src}")
-        if verbose:
-            sys.stderr.write(r.stderr)
-        objs.append(str(obj))
+        cmd = [cc] + flags + (["-x", "hip"] if src.endswith(".cpp") else []) + ["-c", str(CSRC / src), "-o", str(obj)]
+        return src, obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    objs = []
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 1)) as pool:
+        for src, obj, r in pool.map(compile_one, SOURCES):
+            if r.returncode != 0:
+                sys.stderr.write(r.stdout + r.stderr)
+                raise RuntimeError(f"hipcc failed on {src}")
+            if verbose:
+                sys.stderr.write(r.stderr)
+            objs.append(str(obj))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cc, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp)] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)

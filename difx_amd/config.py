@@ -82,6 +82,9 @@ class Config:
     scan_start_sec: float = 0.0
     xengine: str = "fp32"
     chunk_ffts: int = 0
+    fengine_mode: int = 0       # 0 automatic, 1 one launch per FFT length, 2 one persistent launch
+    fengine_threads: int = 0    # F-engine workgroup size 256 or 512 (0: by FFT length)
+    autotune: int = 0           # 1: time the launch candidates at create and keep the fastest
 
     # ---- derived quantities (configuration.cpp) ----
     def blockspersend(self) -> int:
@@ -194,6 +197,9 @@ class Config:
         cfg.xengine = {"fp32": abi.FXG_X_FP32, "fp16": abi.FXG_X_FP16,
                        "bf16": abi.FXG_X_BF16}[self.xengine]
         cfg.chunk_ffts = self.chunk_ffts
+        cfg.fengine_mode = self.fengine_mode
+        cfg.fengine_threads = self.fengine_threads
+        cfg.autotune = self.autotune
         return cfg, keep
 
 

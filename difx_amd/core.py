@@ -62,6 +62,7 @@ def load_library() -> C.CDLL:
     lib.fxg_delay_interpolator.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]
     lib.fxg_set_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.fxg_get_timing.argtypes = [C.c_void_p, P(abi.fxg_timing)]
+    lib.fxg_autotune_report.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
     lib.fxg_footprint.argtypes = [P(abi.fxg_config), P(C.c_int64), P(C.c_int64), P(C.c_int64)]
     lib.fxg_input_samples.argtypes = [C.c_void_p, P(C.c_int64)]
     lib.fxg_device_bytes.argtypes = [C.c_void_p, P(C.c_int64)]
@@ -212,6 +213,12 @@ class FxCore:
         out = (C.c_double * 3)()
         self._check(self.lib.fxg_delay_interpolator(self.h, ds, sec, ns, out))
         return np.array(out[:])
+
+    def autotune_report(self) -> str:
+        """The F-engine launch form in use and the autotuner's candidate timings."""
+        buf = C.create_string_buffer(4096)
+        self._check(self.lib.fxg_autotune_report(self.h, buf, 4096))
+        return buf.value.decode()
 
     def set_timing(self, enable: bool):
         self._check(self.lib.fxg_set_timing(self.h, int(enable)))

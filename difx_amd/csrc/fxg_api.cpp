@@ -85,6 +85,19 @@ struct fxg_ctx {
   };
   std::vector<FClass> fclasses;
   int* d_dslist = nullptr;
+  // F-engine launch form (per FFT length, or one persistent launch per sampling kind)
+  struct FChoice {
+    int mode = FXG_F_PER_LENGTH;
+    int ntb = 0;                 // workgroup size; 0 = 256 below L = 4096, 512 from there
+    int grid_mult = 1;           // persistent: workgroups per CU
+  };
+  FChoice fsel;
+  int cfg_fmode = 0, cfg_fthreads = 0, cfg_autotune = 0;
+  std::vector<std::pair<std::string, double>> tune_log;   // candidate, ms per subint (< 0: failed)
+  std::vector<FItem> items[2];   // persistent work items: [0] real, [1] complex sampling
+  FItem* d_items[2] = {nullptr, nullptr};
+  int* d_counter = nullptr;
+  int ncu = 256;
   std::vector<long long> databytes;
   int bpa = 1;
   long long coreresultlength = 0;
@@ -310,6 +323,10 @@ int validate_and_derive(fxg_ctx* ctx) {
     }
   if (ctx->xengine < FXG_X_FP32 || ctx->xengine > FXG_X_BF16)
     return fail(ctx, FXG_E_INVALID, "unknown X-engine");
+  if (ctx->cfg_fmode < FXG_F_AUTO || ctx->cfg_fmode > FXG_F_PERSISTENT)
+    return fail(ctx, FXG_E_INVALID, "unknown F-engine launch form");
+  if (ctx->cfg_fthreads != 0 && ctx->cfg_fthreads != 256 && ctx->cfg_fthreads != 512)
+    return fail(ctx, FXG_E_INVALID, "F-engine workgroup size must be 256 or 512");
   return FXG_OK;
 }
 
@@ -895,6 +912,87 @@ struct TimedLaunch {
   }
 };
 
+int ntb_for(const fxg_ctx* ctx, int L) {
+  if (ctx->fsel.ntb) return ctx->fsel.ntb;
+  return L >= 4096 ? 512 : 256;
+}
+
+int log2i(int v) {
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return l;
+}
+
+// Work items of the persistent launch: every datastream's chunk in `parts` parts, longest FFT
+// first (the items of one length run back to back, so twiddles reload rarely).
+int build_items(fxg_ctx* ctx) {
+  int nds_kind[2] = {0, 0};
+  for (const auto& fc : ctx->fclasses) nds_kind[fc.cplx] += (int)fc.dsl.size();
+  for (int kind = 0; kind < 2; ++kind) {
+    ctx->items[kind].clear();
+    if (!nds_kind[kind]) continue;
+    const int parts = std::max(1, std::min(64, (4 * ctx->ncu + nds_kind[kind] - 1) / nds_kind[kind]));
+    std::vector<const fxg_ctx::FClass*> cls;
+    for (const auto& fc : ctx->fclasses) if (fc.cplx == kind) cls.push_back(&fc);
+    std::sort(cls.begin(), cls.end(), [](const fxg_ctx::FClass* a, const fxg_ctx::FClass* b) { return a->L > b->L; });
+    for (const auto* fc : cls)
+      for (int d : fc->dsl)
+        for (int pt = 0; pt < parts; ++pt) {
+          FItem it;
+          it.L = fc->L;
+          it.log2L = log2i(fc->L);
+          it.d = d;
+          it.part = pt;
+          it.nparts = parts;
+          ctx->items[kind].push_back(it);
+        }
+    if (int rc = dev_upload(ctx, &ctx->d_items[kind], ctx->items[kind])) return rc;
+  }
+  return dev_malloc(ctx, (void**)&ctx->d_counter, sizeof(int));
+}
+
+// The F-engine of one chunk in the selected launch form.
+int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
+  const int nunits = fa.nffts_pad / 8;
+  if (ctx->fsel.mode == FXG_F_PERSISTENT) {
+    for (int kind = 0; kind < 2; ++kind) {
+      if (ctx->items[kind].empty()) continue;
+      int phmax = 0;
+      for (const auto& fc : ctx->fclasses) {
+        if (fc.cplx != kind) continue;
+        fa.twiddle = ctx->d_twiddle + fc.tw_off;
+        fa.dslist = ctx->d_dslist + fc.dsl_off;
+        fa.phcap = fc.phcap;
+        if (fxg_launch_fftparams(fa, (int)fc.dsl.size(), st)) return fail(ctx, FXG_E_HIP, "FFT parameter launch");
+        fa.tw_by_log[log2i(fc.L) - 8] = ctx->d_twiddle + fc.tw_off;
+        phmax = std::max(phmax, fc.phcap);
+      }
+      const int ntb = ctx->fsel.ntb ? ctx->fsel.ntb : 256;
+      size_t lds = 0;
+      for (const auto& fc : ctx->fclasses)
+        if (fc.cplx == kind) lds = std::max(lds, fxg_fengine_lds(fc.L, ntb, phmax, ctx->rotorder));
+      fa.phcap = phmax;
+      const int nitems = (int)ctx->items[kind].size();
+      const int nwg = std::max(1, std::min(nitems, ctx->ncu * ctx->fsel.grid_mult));
+      if (fxg_launch_fpersist(fa, ctx->d_items[kind], nitems, ctx->d_counter, nwg, ntb, kind, lds, st))
+        return fail(ctx, FXG_E_HIP, "persistent F-engine launch");
+    }
+    return FXG_OK;
+  }
+  for (const auto& fc : ctx->fclasses) {
+    const int ncl = (int)fc.dsl.size();
+    // enough workgroups to fill the CUs several times over
+    const long long wgs_per_ds = std::max(1LL, (long long)(8 * ctx->ncu + ncl - 1) / ncl);
+    fa.units_per_wg = (int)std::max(1LL, std::min<long long>(8, nunits / wgs_per_ds));
+    fa.twiddle = ctx->d_twiddle + fc.tw_off;
+    fa.dslist = ctx->d_dslist + fc.dsl_off;
+    fa.phcap = fc.phcap;
+    if (fxg_launch_fengine(fa, fc.L, fc.cplx, ncl, ntb_for(ctx, fc.L), st))
+      return fail(ctx, FXG_E_HIP, "F-engine launch");
+  }
+  return FXG_OK;
+}
+
 int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_data, const int32_t* nbytes,
                const int32_t* const* control, float2* d_results, hipStream_t st) {
   const int nds = ctx->nds;
@@ -985,19 +1083,9 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     fa.chunk0 = c0;
     fa.nffts = n;
     fa.nffts_pad = npad;
-    const int nblocks = npad / FXG_TB;
     {
       TimedLaunch tl(ctx, 0, st);
-      for (const auto& fc : ctx->fclasses) {
-        const int ncl = (int)fc.dsl.size();
-        // enough workgroups to fill 256 CUs several times over
-        const long long wgs_per_ds = std::max(1LL, (long long)(2048 + ncl - 1) / ncl);
-        fa.blocks_per_wg = (int)std::max(1LL, std::min<long long>(16, nblocks / wgs_per_ds));
-        fa.twiddle = ctx->d_twiddle + fc.tw_off;
-        fa.dslist = ctx->d_dslist + fc.dsl_off;
-        fa.phcap = fc.phcap;
-        if (fxg_launch_fengine(fa, fc.L, fc.cplx, ncl, st)) return fail(ctx, FXG_E_HIP, "F-engine launch");
-      }
+      if (int rc = launch_fengines(ctx, fa, st)) return rc;
     }
     if (mfma) {
       XMfmaArgs xa;
@@ -1048,10 +1136,90 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   return FXG_OK;
 }
 
+// Autotuner: times one synthetic subint (zeroed slot buffers: the F-engine's control flow and
+// arithmetic do not depend on the data, only its loads of valid frames do) in every F-engine
+// launch form and workgroup size, and keeps the fastest.  The forms give bit-identical
+// results, so the choice only affects speed.
+int autotune(fxg_ctx* ctx) {
+  std::vector<std::pair<std::string, fxg_ctx::FChoice>> cand;
+  auto add = [&](const char* name, int mode, int ntb, int mult) {
+    if (ctx->cfg_fmode != FXG_F_AUTO && ctx->cfg_fmode != mode) return;
+    if (ctx->cfg_fthreads && ntb && ctx->cfg_fthreads != ntb) return;
+    fxg_ctx::FChoice c;
+    c.mode = mode;
+    c.ntb = ntb;
+    c.grid_mult = mult;
+    cand.emplace_back(name, c);
+  };
+  add("per-length, 256/512 threads by length", FXG_F_PER_LENGTH, 0, 1);
+  add("per-length, 256 threads", FXG_F_PER_LENGTH, 256, 1);
+  add("per-length, 512 threads", FXG_F_PER_LENGTH, 512, 1);
+  add("persistent, 256 threads, 1 workgroup/CU", FXG_F_PERSISTENT, 256, 1);
+  add("persistent, 256 threads, 2 workgroups/CU", FXG_F_PERSISTENT, 256, 2);
+  add("persistent, 512 threads, 1 workgroup/CU", FXG_F_PERSISTENT, 512, 1);
+  if (cand.empty()) return FXG_OK;
+  // synthetic slot: zeroed data, every FFT flagged valid, subint at the model's start
+  const int nds = ctx->nds;
+  std::vector<uint8_t*> bufs(nds, nullptr);
+  std::vector<std::vector<int32_t>> ctl(nds, std::vector<int32_t>(3 + ctx->flaglen, 0));
+  std::vector<const uint8_t*> ptr(nds);
+  std::vector<const int32_t*> cptr(nds);
+  std::vector<int32_t> nb(nds);
+  float2* d_res = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FXG_OK;
+  for (int d = 0; d < nds && rc == FXG_OK; ++d) {
+    nb[d] = (int32_t)ctx->databytes[d];
+    if (hipMalloc((void**)&bufs[d], (size_t)nb[d]) != hipSuccess || hipMemset(bufs[d], 0, (size_t)nb[d]) != hipSuccess)
+      rc = fail(ctx, FXG_E_NOMEM, "autotune buffers");
+    for (int k = 0; k < ctx->flaglen; ++k) ctl[d][3 + k] = (1 << FLAGS_PER_INT) - 1;
+    ptr[d] = bufs[d];
+    cptr[d] = ctl[d].data();
+  }
+  if (rc == FXG_OK && (hipMalloc((void**)&d_res, sizeof(float2) * std::max<long long>(ctx->coreresultlength, 1)) != hipSuccess ||
+                       hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(ctx, FXG_E_NOMEM, "autotune buffers");
+  const int32_t offs[3] = {0, 0, 0};
+  double best = 1e30;
+  fxg_ctx::FChoice bestc = ctx->fsel;
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  for (auto& c : cand) {
+    if (rc != FXG_OK) break;
+    ctx->fsel = c.second;
+    double ms = 1e30;
+    bool ok = run_subint(ctx, offs, ptr.data(), nb.data(), cptr.data(), d_res, ctx->stream) == FXG_OK;   // warm
+    for (int rep = 0; ok && rep < 2; ++rep) {
+      ok = hipEventRecord(e0, ctx->stream) == hipSuccess &&
+           run_subint(ctx, offs, ptr.data(), nb.data(), cptr.data(), d_res, ctx->stream) == FXG_OK &&
+           hipEventRecord(e1, ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+      float t = 0.0f;
+      if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = std::min(ms, (double)t);
+    }
+    if (!ok) {
+      (void)hipGetLastError();
+      (void)hipStreamSynchronize(ctx->stream);
+      ctx->err.clear();
+    }
+    ctx->tune_log.emplace_back(c.first, ok ? ms : -1.0);
+    if (ok && ms < best) { best = ms; bestc = c.second; }
+  }
+  ctx->timing = timing;
+  ctx->fsel = bestc;
+  if (best >= 1e30 && rc == FXG_OK) rc = fail(ctx, FXG_E_HIP, "autotune: no F-engine launch form ran");
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto b : bufs) if (b) (void)hipFree(b);
+  if (d_res) (void)hipFree(d_res);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  return rc;
+}
+
 void free_ctx(fxg_ctx* ctx) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid);
-  f(ctx->d_twiddle); f(ctx->d_dslist); f(ctx->d_params); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
+  f(ctx->d_twiddle); f(ctx->d_dslist); f(ctx->d_params); f(ctx->d_items[0]); f(ctx->d_items[1]);
+  f(ctx->d_counter); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   for (auto& sg : ctx->stage) {
@@ -1151,6 +1319,12 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
   ctx->model_inc = cfg->model_inc_secs;
   ctx->scan_start = cfg->scan_start_sec;
   ctx->xengine = cfg->xengine;
+  ctx->cfg_fmode = cfg->fengine_mode;
+  ctx->cfg_fthreads = cfg->fengine_threads;
+  ctx->cfg_autotune = cfg->autotune;
+  ctx->fsel.mode = cfg->fengine_mode == FXG_F_PERSISTENT ? FXG_F_PERSISTENT : FXG_F_PER_LENGTH;
+  ctx->fsel.ntb = cfg->fengine_threads;
+  if (!dry) (void)hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device);
   int rc = validate_and_derive(ctx);
   if (rc == FXG_OK) {
     build_layout(ctx);
@@ -1159,6 +1333,7 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
   }
   if (rc == FXG_OK) rc = build_tables(ctx);
   if (rc == FXG_OK) rc = build_engines(ctx, cfg->chunk_ffts);
+  if (rc == FXG_OK) rc = build_items(ctx);
   if (rc == FXG_OK && !dry) {
     for (auto& sg : ctx->stage) {
       if (hipHostMalloc((void**)&sg.h_run, sizeof(DevDsRun) * ctx->nds, 0) != hipSuccess ||
@@ -1169,6 +1344,7 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
       }
     }
   }
+  if (rc == FXG_OK && !dry && ctx->cfg_autotune) rc = autotune(ctx);
   if (rc != FXG_OK) {
     std::fprintf(stderr, "fxg_create: %s\n", ctx->err.c_str());
     free_ctx(ctx);
@@ -1439,6 +1615,22 @@ int fxg_delay_interpolator(fxg_ctx* ctx, int32_t ds, int32_t sec, int32_t ns, do
   HIPCHK(hipStreamSynchronize(st));
   if (!back.valid) return fail(ctx, FXG_E_INVALID, "time outside the delay model");
   std::memcpy(out, back.interp, sizeof(double) * 3);
+  return FXG_OK;
+}
+
+int fxg_autotune_report(const fxg_ctx* ctx, char* buf, int32_t len) {
+  if (!ctx || !buf || len < 1) return FXG_E_INVALID;
+  std::string r = "F-engine: ";
+  r += ctx->fsel.mode == FXG_F_PERSISTENT ? "persistent" : "per-length";
+  r += ", threads " + (ctx->fsel.ntb ? std::to_string(ctx->fsel.ntb) : std::string("by length"));
+  if (ctx->fsel.mode == FXG_F_PERSISTENT) r += ", " + std::to_string(ctx->fsel.grid_mult) + " workgroup(s)/CU";
+  for (const auto& e : ctx->tune_log) {
+    char line[160];
+    std::snprintf(line, sizeof(line), "\n  %s: %.3f ms per subint", e.first.c_str(), e.second);
+    r += line;
+  }
+  std::strncpy(buf, r.c_str(), (size_t)len - 1);
+  buf[len - 1] = 0;
   return FXG_OK;
 }
 

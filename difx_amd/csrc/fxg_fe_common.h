@@ -1,0 +1,586 @@
+// Device code shared by the F-engine translation units (fxg_fengine.hip: one launch per FFT
+// length; fxg_fpersist.hip: every length in one persistent launch).
+#pragma once
+#include "fxg_internal.h"
+#include "../../include/fxg.h"
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+namespace {
+
+constexpr double TWO_PI_D = 6.283185307179586476925286766559;   // architecture.h.in:430
+constexpr float HIMAG = 3.3359f;                                 // mark5_stream.h:65
+constexpr int FLAGS_PER_INT = 30;                                // mpifxcorr.h:29
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul_negi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
+
+// 2-bit LUTs: VDIF {-H,-1,+1,+H}[field] (format_vdif.c:70); Mark5B {-H,+1,-1,+H}[field]
+// where field = sign bit + 2*magnitude bit (mark5_format_mark5b.c:52-80).
+__device__ __forceinline__ float lut2(int fmt, unsigned field) {
+  const float v0 = -HIMAG, v3 = HIMAG;
+  float v1 = (fmt == FXG_FMT_MARK5B) ? 1.0f : -1.0f;
+  float v2 = -v1;
+  return field == 0 ? v0 : field == 1 ? v1 : field == 2 ? v2 : v3;
+}
+// 8-bit LUT (b - 128)/3.3 computed in double and rounded to float (format_vdif.c:120).
+__device__ __forceinline__ float lut8(unsigned b) { return (float)(((double)b - 128.0) / 3.3); }
+
+// In-register DFTs (forward, sign -1), decimation in frequency.
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a; a = cadd(t, b); b = csub(t, b);
+}
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 b0 = cadd(a0, a2), b2 = csub(a0, a2);
+  float2 b1 = cadd(a1, a3), b3 = mul_negi(csub(a1, a3));
+  a0 = cadd(b0, b1); a2 = csub(b0, b1);
+  a1 = cadd(b2, b3); a3 = csub(b2, b3);
+}
+__device__ __forceinline__ void dft8(float2* v) {
+  const float c = 0.70710678118654752440f;
+  float2 u0 = cadd(v[0], v[4]), u4 = csub(v[0], v[4]);
+  float2 u1 = cadd(v[1], v[5]), u5 = csub(v[1], v[5]);
+  float2 u2 = cadd(v[2], v[6]), u6 = csub(v[2], v[6]);
+  float2 u3 = cadd(v[3], v[7]), u7 = csub(v[3], v[7]);
+  u5 = make_float2(c * (u5.x + u5.y), c * (u5.y - u5.x));      // * W8^1
+  u6 = mul_negi(u6);                                           // * W8^2
+  u7 = make_float2(c * (u7.y - u7.x), -c * (u7.x + u7.y));     // * W8^3
+  float2 w0 = cadd(u0, u2), w2 = csub(u0, u2);
+  float2 w1 = cadd(u1, u3), w3 = mul_negi(csub(u1, u3));
+  float2 w4 = cadd(u4, u6), w6 = csub(u4, u6);
+  float2 w5 = cadd(u5, u7), w7 = mul_negi(csub(u5, u7));
+  v[0] = cadd(w0, w1); v[4] = csub(w0, w1);
+  v[2] = cadd(w2, w3); v[6] = csub(w2, w3);
+  v[1] = cadd(w4, w5); v[5] = csub(w4, w5);
+  v[3] = cadd(w6, w7); v[7] = csub(w6, w7);
+}
+
+// LDS exchange index with one pad element per 8: conflict-free stores for the first two
+// Stockham passes and at most 3-way bank sharing for the strided-by-thread loads.
+__device__ __forceinline__ int padx(int e) { return e + (e >> 3); }
+
+template <int L>
+struct Plan {
+  static constexpr int LOG = (L == 256) ? 8 : (L == 512) ? 9 : (L == 1024) ? 10 : (L == 2048) ? 11
+                           : (L == 4096) ? 12 : (L == 8192) ? 13 : 0;
+  static constexpr int N8 = LOG / 3;      // radix-8 passes
+  static constexpr int REM = LOG % 3;     // 0: none, 1: radix-2 pass, 2: radix-4 pass
+  static constexpr int NT = L / 8;
+  // Per-pass twiddle tables, laid out [pass][r-1][k] with k (the butterfly's position in its
+  // span Ns) fastest, so lanes of a wavefront read consecutive entries.  Pass 0 needs none.
+  static constexpr int pow8(int p) { int v = 1; for (int q = 0; q < p; ++q) v *= 8; return v; }
+  static constexpr int off8(int p) { int o = 0; for (int q = 1; q < p; ++q) o += 7 * pow8(q); return o; }
+  static constexpr int OFFREM = off8(N8);
+  static constexpr int TWN = OFFREM + (REM ? ((1 << REM) - 1) * pow8(N8) : 0);
+};
+
+// Stockham FFT of L points over NT = L/8 butterfly positions, held V per thread: position
+// vt = tl + u*TPF (u < V) holds v[u][s] = x[vt + s*NT] on entry and X[vt + s*NT] on exit.
+// sbuf has L + L/8 entries (padded); tw holds Plan<L>::TWN twiddles exp(-2 pi i k r / (Ns R))
+// per pass.  Every thread of the workgroup calls it (the barriers are workgroup-wide); with
+// several FFTs per workgroup each group passes its own sbuf.
+template <int L, int V, int TPF>
+__device__ __forceinline__ void fft_stockham(float2 (*v)[8], float2* sbuf, const float2* tw, int tl) {
+  using P = Plan<L>;
+  constexpr int NT = P::NT;
+  static_assert(V * TPF == NT, "butterfly positions");
+  int Ns = 1;
+#pragma unroll
+  for (int p = 0; p < P::N8; ++p) {
+    if (p > 0) {
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const int vt = tl + u * TPF;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) v[u][s] = sbuf[padx(vt + s * NT)];
+        const int k = vt & (Ns - 1);
+        const float2* twp = tw + P::off8(p) + k;
+#pragma unroll
+        for (int r = 1; r < 8; ++r) v[u][r] = cmul(v[u][r], twp[(r - 1) * Ns]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < V; ++u) dft8(v[u]);
+    const bool last = (P::REM == 0) && (p == P::N8 - 1);
+    if (!last) {
+      if (p > 0) __syncthreads();            // every thread has read its inputs (pass 0 reads
+                                             // none; the caller's barrier freed sbuf)
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const int vt = tl + u * TPF;
+        const int k = vt & (Ns - 1);
+        const int idxD = (vt / Ns) * Ns * 8 + k;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) sbuf[padx(idxD + r * Ns)] = v[u][r];
+      }
+      __syncthreads();
+    }
+    Ns *= 8;
+  }
+  if (P::REM == 2) {                         // radix-4 pass, two butterflies per position
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const int vt = tl + u * TPF;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) v[u][s] = sbuf[padx(vt + s * NT)];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int j = vt + b * NT;
+        const int k = j & (Ns - 1);
+        const float2* twp = tw + P::OFFREM + k;
+        float2 a0 = v[u][b], a1 = v[u][b + 2], a2 = v[u][b + 4], a3 = v[u][b + 6];
+        a1 = cmul(a1, twp[0]);
+        a2 = cmul(a2, twp[Ns]);
+        a3 = cmul(a3, twp[2 * Ns]);
+        dft4(a0, a1, a2, a3);
+        v[u][b] = a0; v[u][b + 2] = a1; v[u][b + 4] = a2; v[u][b + 6] = a3;
+      }
+    }
+  } else if (P::REM == 1) {                  // radix-2 pass, four butterflies per position
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const int vt = tl + u * TPF;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) v[u][s] = sbuf[padx(vt + s * NT)];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int j = vt + m * NT;
+        const int k = j & (Ns - 1);
+        float2 a0 = v[u][m], a1 = cmul(v[u][m + 4], tw[P::OFFREM + k]);
+        dft2(a0, a1);
+        v[u][m] = a0; v[u][m + 4] = a1;
+      }
+    }
+  }
+}
+
+// Valid samples of [g0, g0+n) given per-frame validity (mk5mode.cpp:94-174: goodsamples is the
+// count of samples from valid frames inside the FFT window).
+__device__ int count_valid(const uint8_t* fv, int nframes, int spf, long long g0, int n) {
+  long long g1 = g0 + n;
+  int f0 = (int)(g0 / spf), f1 = (int)((g1 - 1) / spf);
+  int good = 0;
+  for (int f = f0; f <= f1; ++f) {
+    if (f >= nframes || !fv[f]) continue;
+    long long a = (long long)f * spf, b = a + spf;
+    long long lo = a > g0 ? a : g0, hi = b < g1 ? b : g1;
+    good += (int)(hi - lo);
+  }
+  return good;
+}
+
+// Per-FFT delay chain (mode.cpp:671-675, 743-757, 776-825), all FP64.
+//   order 0: integerdelay = (int)averagedelay, b = averagedelay - integerdelay (post-F phase)
+//   order 1: a = d2 - d0, b = d0 + (d1 - (a/2 + d0))/3 - integerdelay
+//   order 2: a = q0, b = q1 + 2 i q0, c = q2 + i q1 + i^2 q0 - integerdelay
+__device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_t* fv, int i,
+                              int offsec, int offns, double scan_start_sec, int order) {
+  FftDelay r;
+  r.ok = 0; r.weight = 0.0f; r.n0 = 0; r.fse = 0.0f; r.a = 0; r.b = 0; r.c = 0; r.intdelay = 0;
+  r.intwall = 0; r.fracwall = 0;
+  if (!run.valid || run.datalen <= 1) return r;
+  if (((run.vflags[i / FLAGS_PER_INT] >> (i % FLAGS_PER_INT)) & 1) == 0) return r;
+  const double* q = run.interp;
+  const double dt = ds.sampletime_us;
+  const int L = ds.L;
+  const double fc = i + 0.5;
+  const double avgdelay = q[0] * fc * fc + q[1] * fc + q[2];
+  const double fftstart = i * L * dt;
+  const double starttime = (offsec - run.datasec) * 1000000.0
+                         + ((long long)offns - (long long)run.datans) / 1000.0 + fftstart - avgdelay;
+  int n0 = (int)(starttime / dt + 0.5);
+  if (n0 < -1 || (((long long)(n0 + L)) * ds.bpb_num) / ds.bpb_den > run.datalen) return r;
+  if (n0 == -1) n0 = 0;
+  const int good = count_valid(fv, run.nframes, ds.spf, n0, L);
+  const float w = (float)good / (float)L;
+  if (!(w > 0.0f)) return r;
+  r.ok = 1;
+  r.n0 = n0;
+  r.weight = w;
+  r.fse = (float)(starttime - n0 * dt);
+  const double di = (double)i;
+  if (order == 1) {
+    const double d0 = q[0] * di * di + q[1] * di + q[2];
+    const double d1 = q[0] * (di + 0.5) * (di + 0.5) + q[1] * (di + 0.5) + q[2];
+    const double d2 = q[0] * (di + 1) * (di + 1) + q[1] * (di + 1) + q[2];
+    const double a = d2 - d0;
+    double b = d0 + (d1 - (a * 0.5 + d0)) / 3.0;
+    const int intdelay = (int)b;
+    b -= intdelay;
+    r.a = a; r.b = b; r.intdelay = intdelay;
+  } else if (order == 2) {
+    const double b = q[1] + di * q[0] * 2.0;
+    double c = q[2] + di * q[1] + (double)((long long)i * i) * q[0];
+    const int intdelay = (int)c;
+    c -= intdelay;
+    r.a = q[0]; r.b = b; r.c = c; r.intdelay = intdelay;
+  } else {
+    const int intdelay = (int)avgdelay;
+    r.b = avgdelay - intdelay;
+    r.intdelay = intdelay;
+  }
+  const double walltime = scan_start_sec + offsec + offns / 1.0e9 + fftstart / 1.0e6;
+  r.intwall = (int)walltime;
+  r.fracwall = walltime - r.intwall;
+  return r;
+}
+
+
+// Per-datastream constants the F-engine reads, copied to registers once per workgroup.
+struct DsConst {
+  int C, S, NS, NFS, nrec, nrecfreqs, fmt, spf, framebytes, headerbytes, nbits, fractionallo, dsb;
+  double sampletime_us;
+};
+
+// Per-freq constants (mode.cpp:853-931).
+struct FreqConst {
+  double lofreq, lo_offset_hz, fraclooffset, clockoff, halfdelta;
+  int lsb, looff, isfraclooffset, nbands;
+};
+
+__device__ __forceinline__ FreqConst load_freq(const DevRecFreq& rf) {
+  FreqConst f;
+  f.lofreq = rf.lofreq; f.lo_offset_hz = rf.lo_offset_hz; f.fraclooffset = rf.fraclooffset;
+  f.clockoff = rf.clockoff; f.halfdelta = rf.halfdelta;
+  f.lsb = rf.lsb; f.looff = rf.looff; f.isfraclooffset = rf.isfraclooffset; f.nbands = rf.nbands;
+  return f;
+}
+
+// Fringe rotator sub-phasors (j < S) and step-phasors (S <= j < S+NS) for one freq
+// (order 1: mode.cpp:883-955; order 2: mode.cpp:956-1026, whose quadratic term is the
+// per-sub argument squad[j], applied as a power of the step index).  FP64 phase in turns,
+// wrapped by truncation, FP32 sin/cos.
+template <int ORDER>
+__device__ void build_rotator(float2* srot, float* squad, const DsConst& k, const FreqConst& rf,
+                              const FftDelay& fd, int L, int t, int nt) {
+  const int S = k.S, NS = k.NS;
+  const double lofreq = rf.lofreq;
+  const double sx1 = (double)S / (double)L;                              // stepxoff[1]
+  for (int j = t; j < S + NS; j += nt) {
+    double ph;
+    if (j < S) {
+      const double xoff = (double)j / (double)L;                        // subxoff
+      double xval;                                                       // subxval
+      if (ORDER == 1) {
+        xval = __dadd_rn(__dmul_rn(xoff, fd.a), fd.b);
+      } else {
+        xval = __dadd_rn(__dmul_rn(xoff, __dadd_rn(fd.b, __dmul_rn(fd.a, sx1))), fd.c);
+        const double qx = __dmul_rn(xoff, __dmul_rn(__dmul_rn(2.0, fd.a), sx1));   // subquadxval
+        const double qph = __dmul_rn(qx, lofreq);
+        squad[j] = (float)(-TWO_PI_D * (qph - trunc(qph)));
+      }
+      ph = __dmul_rn(xval, lofreq);
+      if (k.fractionallo)
+        ph = __dadd_rn(ph, __dmul_rn(lofreq - (double)(int)lofreq, (double)fd.intdelay));
+      if (rf.looff) {
+        const double toff = j * k.sampletime_us / 1e6;                   // subtoff
+        ph = __dadd_rn(ph, __dmul_rn(toff, -rf.lo_offset_hz));
+        ph = __dadd_rn(ph, -rf.lo_offset_hz * fd.fracwall);
+        if (rf.isfraclooffset) ph = __dadd_rn(ph, -rf.fraclooffset * fd.intwall);
+      }
+    } else {
+      const int s = j - S;
+      const double xoff = (double)(s * S) / (double)L;                  // stepxoff
+      if (ORDER == 1)
+        ph = __dmul_rn(__dmul_rn(xoff, fd.a), lofreq);
+      else
+        ph = __dmul_rn(__dadd_rn(__dmul_rn(xoff, fd.b), __dmul_rn(__dmul_rn(xoff, xoff), fd.a)), lofreq);
+      if (rf.looff) {
+        const double toff = s * S * k.sampletime_us / 1e6;               // steptoff
+        ph = __dadd_rn(ph, __dmul_rn(toff, -rf.lo_offset_hz));
+      }
+    }
+    const float arg = (float)(-TWO_PI_D * (ph - trunc(ph)));
+    float sn, cs;
+    sincosf(arg, &sn, &cs);
+    srot[j] = make_float2(cs, sn);
+  }
+}
+
+// Fractional-sample phasors: sub (j < S) and step (S <= j < S+NFS) (mode.cpp:1030-1073).
+// Order 0 adds the post-F fringe phase to the sub arguments (mode.cpp:1040-1049).
+template <int ORDER>
+__device__ void build_frac(float2* sfrac, const DsConst& k, const FreqConst& rf, bool iscomplex,
+                           const float* __restrict__ cf, const FftDelay& fd, int t, int nt) {
+  const int S = k.S, NFS = k.NFS;
+  const float e = (float)((double)fd.fse - rf.clockoff + rf.halfdelta);
+  const float* sub = cf;
+  // double sideband: dsbstepchannelfreqs; any other LSB: lsbstepchannelfreqs (mode.cpp:838-857)
+  const float* step = k.dsb ? cf + S + 2 * NFS : rf.lsb ? cf + S + NFS : cf + S;
+  float prf = 0.0f;
+  if (ORDER == 0) {
+    const double lofreq = rf.lofreq;
+    double pr = fd.b * lofreq;
+    if (k.fractionallo) pr += fd.intdelay * (lofreq - (int)lofreq);
+    pr -= ((double)fd.intwall + fd.fracwall) * rf.lo_offset_hz;
+    prf = (float)(-TWO_PI_D * (pr - (int)pr));
+  }
+  for (int j = t; j < S + NFS; j += nt) {
+    const float arg = (j < S) ? (ORDER == 0 ? sub[j] * e + prf : sub[j] * e) : step[j - S] * e;
+    float sn, cs;
+    sincosf(arg, &sn, &cs);
+    sfrac[j] = make_float2(cs, sn);
+  }
+}
+
+__device__ __forceinline__ uint32_t pack16(float2 v, int mode) {
+  if (mode == 1) {
+    __half2 h = __floats2half2_rn(v.x, v.y);
+    return *reinterpret_cast<uint32_t*>(&h);
+  }
+  __hip_bfloat162 h = __float22bfloat162_rn(v);
+  return *reinterpret_cast<uint32_t*>(&h);
+}
+
+
+// Geometry of FFT length L on a workgroup of NTB threads.  An FFT has NT = L/8 butterfly
+// positions.  With NT < NTB the workgroup runs G = NTB/NT FFTs side by side, TPF = NT threads
+// each (consecutive FFTs of one datastream); with NT > NTB each thread holds V = NT/NTB
+// positions of the one FFT.
+template <int L, int NTB>
+struct Geo {
+  static constexpr int NT = L / 8;
+  static constexpr int TPF = NT < NTB ? NT : NTB;
+  static constexpr int G = NTB / TPF;
+  static constexpr int V = NT / TPF;
+  static constexpr int SBUF = L + L / 8;       // padded exchange entries of one FFT
+};
+
+// LDS bytes of one (L, NTB) geometry: G exchange buffers, the twiddles, G rotator and G
+// fractional-sample phasor tables of phcap entries, and for order 2 G quadratic-argument tables.
+template <int L, int NTB>
+size_t fe_lds_bytes(int phcap, int order) {
+  using GE = Geo<L, NTB>;
+  return sizeof(float2) * ((size_t)GE::G * GE::SBUF + Plan<L>::TWN + 2 * (size_t)GE::G * phcap) +
+         (order == 2 ? sizeof(float) * (size_t)GE::G * phcap : 0);
+}
+
+// The F-engine proper: FFT units [u0, u1) (8 consecutive FFTs each) of datastream d.
+// Per FFT and recorded freq: the rotator and fractional-sample tables, then per band the
+// decode + rotation, the FFT, channel selection, fractional-sample correction and the store
+// (FP32 spectra, or scaled 16-bit MFMA operands including zoom rows).  The control flow is the
+// same for every thread of the workgroup (flagged FFTs compute on zeros and store zeros), so
+// the workgroup barriers are uniform with several FFTs in flight.  With load_tw the twiddles
+// of L are (re)loaded into LDS first.
+template <int L, bool CPLX, int ORDER, int NTB>
+__device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twiddle, int d, int u0, int u1,
+                         float2* smem, bool load_tw) {
+  using P = Plan<L>;
+  using GE = Geo<L, NTB>;
+  constexpr int NT = GE::NT, TPF = GE::TPF, G = GE::G, V = GE::V;
+  const int t = threadIdx.x;
+  const int g = t / TPF, tl = t - g * TPF;
+  float2* sbuf = smem + g * GE::SBUF;
+  float2* stw = smem + G * GE::SBUF;
+  float2* srot = stw + P::TWN + g * args.phcap;
+  float2* sfrac = stw + P::TWN + (G + g) * args.phcap;
+  float* squad = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + g * args.phcap;
+
+  if (load_tw) {
+    __syncthreads();                              // earlier users of the twiddle tables are done
+    for (int m = t; m < P::TWN; m += NTB) stw[m] = twiddle[m];
+  }
+
+  const DevDs* __restrict__ dsp = args.ds + d;
+  DsConst k;
+  k.C = dsp->C; k.S = dsp->S; k.NS = dsp->NS; k.NFS = dsp->NFS; k.nrec = dsp->nrecbands;
+  k.nrecfreqs = dsp->nrecfreqs; k.fmt = dsp->format; k.spf = dsp->spf;
+  k.framebytes = dsp->framebytes; k.headerbytes = dsp->headerbytes; k.nbits = dsp->nbits;
+  k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us; k.dsb = dsp->dsb;
+  const long long spec_off32 = dsp->spec_off;
+  const uint8_t* __restrict__ fv = args.fvalid + dsp->fvalid_off;
+  const float* __restrict__ cf = args.chanfreq + dsp->chanfreq_off;
+  const uint8_t* __restrict__ data = args.run[d].data;
+  const int nframes = args.run[d].nframes;
+  const FftDelay* __restrict__ params = args.params + (long long)d * args.chunkcap;
+  const int C = k.C, S = k.S;
+  const bool twobit = (k.nbits == 2);
+  const int out_mode = args.out_mode;
+  const int cap = args.chunkcap;
+
+  for (int lt0 = u0 * 8; lt0 < u1 * 8; lt0 += G) {
+    const int lt = lt0 + g;                        // this group's FFT within the chunk
+    const FftDelay fd = params[lt];
+    const bool fok_all = fd.ok != 0;               // flagged FFTs: zero spectra (mode.cpp:656-669)
+
+    // raw sample words of this thread's time samples, shared by all bands: for 2-bit data
+    // the 32-bit payload word holding the sample's fields and the field offset in it; for
+    // 8-bit data the byte offset of the sample
+    uint32_t wv[V][8];
+    uint32_t fok[V], qlo[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      fok[u] = 0u;
+      qlo[u] = 0u;
+      const unsigned g0 = (unsigned)fd.n0 + (unsigned)(tl + u * TPF);
+      unsigned fr = g0 / (unsigned)k.spf;
+      unsigned within = g0 - fr * (unsigned)k.spf;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        if (s > 0) {
+          within += NT;
+          while (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
+        }
+        const bool ok = fok_all && ((int)fr < nframes) && fv[fr];
+        fok[u] |= (ok ? 1u : 0u) << s;
+        const unsigned fbase = fr * (unsigned)k.framebytes + (unsigned)k.headerbytes;
+        if (twobit) {
+          const unsigned q = within * (unsigned)k.nrec;     // first field of the sample
+          qlo[u] |= (q & 15u) << (4 * s);
+          wv[u][s] = ok ? *reinterpret_cast<const uint32_t*>(data + fbase + ((q >> 4) << 2)) : 0u;
+        } else {
+          wv[u][s] = fbase + within * (unsigned)k.nrec * (CPLX ? 2u : 1u);
+        }
+      }
+    }
+
+    for (int f = 0; f < k.nrecfreqs; ++f) {
+      const FreqConst rf = load_freq(dsp->rf[f]);
+      if (rf.nbands == 0) continue;
+      __syncthreads();                            // previous users of srot/sfrac are done
+      if (ORDER != 0) build_rotator<ORDER>(srot, squad, k, rf, fd, L, tl, TPF);
+      build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, tl, TPF);
+      __syncthreads();
+      for (int kb = 0; kb < rf.nbands; ++kb) {
+        const int band = dsp->rf[f].band[kb];
+        float2 v[V][8];
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          const int vt = tl + u * TPF;
+          const float2 sub = ORDER != 0 ? srot[vt & (S - 1)] : make_float2(1.0f, 0.0f);
+          // order 2: the quadratic sub term to the power of the step index j = vt/S + s*NT/S
+          // (mode.cpp:1017-1024), as A * D^s with A = e^{i j0 th}, D = e^{i (NT/S) th}
+          float2 qpow = make_float2(1.0f, 0.0f), qstep = make_float2(1.0f, 0.0f);
+          if (ORDER == 2) {
+            const float th = squad[vt & (S - 1)];
+            float sn, cs;
+            sincosf((float)(vt / S) * th, &sn, &cs);
+            qpow = make_float2(cs, sn);
+            sincosf((float)(NT / S) * th, &sn, &cs);
+            qstep = make_float2(cs, sn);
+          }
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            const int n = vt + s * NT;
+            const int st = n / S;
+            float2 rot;
+            if (ORDER == 1) {
+              rot = st == 0 ? sub : cmul(sub, srot[S + st]);
+            } else if (ORDER == 2) {
+              const float2 pq = cmul(sub, qpow);
+              rot = st == 0 ? pq : cmul(pq, srot[S + st]);
+              qpow = cmul(qpow, qstep);
+            } else {
+              rot = sub;                                     // no pre-F rotation
+            }
+            const bool ok = (fok[u] >> s) & 1u;
+            if (!CPLX) {
+              float x = 0.0f;
+              if (ok) {
+                if (twobit) {
+                  const int sh = 2 * (int)(((qlo[u] >> (4 * s)) & 15u) + band);
+                  x = lut2(k.fmt, (wv[u][s] >> sh) & 3u);
+                } else {
+                  x = lut8(data[wv[u][s] + band]);
+                }
+              }
+              v[u][s] = ORDER == 0 ? make_float2(x, 0.0f)           // real FFT input
+                                   : make_float2(x * rot.x, x * rot.y);  // (x + 0i) * rotator
+            } else {
+              float2 x = make_float2(0.f, 0.f);
+              if (ok) {
+                const uint8_t* pp = data + wv[u][s] + 2 * band;
+                x = make_float2(lut8(pp[0]), lut8(pp[1]));
+              }
+              v[u][s] = cmul(rot, x);
+              // complex LSB: the reference conj-flips the spectrum (mode.cpp:1186-1201);
+              // FFT(conj y)[c] = conj(FFT(y)[-c]) gives the same channels without a flip
+              if (rf.lsb) v[u][s].y = -v[u][s].y;
+            }
+          }
+        }
+        __syncthreads();                          // sbuf free (previous band's FFT done)
+        fft_stockham<L, V, TPF>(v, sbuf, stw, tl);
+        // output pointers of this band
+        float2* __restrict__ o32 = nullptr;
+        uint32_t* __restrict__ o16 = nullptr;
+        const float* __restrict__ scp = nullptr;
+        if (out_mode == 0) {
+          o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
+        } else {
+          const int xg = dsp->band_xfreq[band];
+          const int row = dsp->band_xrow[band];
+          o16 = args.spec16 + args.xgroups[xg].spec_off + (long long)row * args.xgroups[xg].rowstride + lt * 8;
+          scp = args.xscale + args.xgroups[xg].scale_off + (long long)row * C;
+        }
+        // channel selection, fractional-sample correction, output
+        constexpr int NKEEP = CPLX ? 8 : 4;
+        float2 Fo[V][NKEEP];                      // output channels vt + m*NT, kept for the zoom windows
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          const int vt = tl + u * TPF;
+#pragma unroll
+          for (int m = 0; m < NKEEP; ++m) {
+            // USB and complex keep X[vt + m*NT]; real LSB keeps X[C + vt + m*NT] (mode.cpp:1205);
+            // double sideband swaps the halves (mode.cpp:1191-1192, 1211-1213).  Register
+            // indices stay compile-time constants (a run-time index would move v to scratch).
+            const bool upper = CPLX ? (k.dsb != 0) : (rf.lsb != 0);
+            const int c = vt + m * NT;
+            const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
+            float2 X = upper ? v[u][(m + 4) & 7] : v[u][m];
+            if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped (mode.cpp:1199-1200)
+            float2 F = cmul(X, frq);
+            if (!fok_all) F = make_float2(0.0f, 0.0f);
+            Fo[u][m] = F;
+            if (out_mode == 0) {
+              o32[c] = F;
+            } else {
+              const float sc = scp[c];
+              o16[(c >> 3) * cap * 8 + (c & 7)] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
+            }
+          }
+        }
+        // zoom windows of this band: the same channels again, as rows of the zoom freq's group
+        if (out_mode != 0) {
+          for (int z = 0; z < dsp->nzout; ++z) {
+            if (dsp->zout_band[z] != band) continue;
+            const int xg = dsp->zout_group[z], row = dsp->zout_row[z];
+            const int c0 = dsp->zout_c0[z], nz = dsp->zout_n[z];
+            uint32_t* __restrict__ oz = args.spec16 + args.xgroups[xg].spec_off +
+                                        (long long)row * args.xgroups[xg].rowstride + lt * 8;
+            const float* __restrict__ scz = args.xscale + args.xgroups[xg].scale_off + (long long)row * nz;
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+#pragma unroll
+              for (int m = 0; m < NKEEP; ++m) {
+                const int cz = tl + u * TPF + m * NT - c0;
+                if (cz >= 0 && cz < nz) {
+                  const float sc = scz[cz];
+                  oz[(cz >> 3) * cap * 8 + (cz & 7)] =
+                      pack16(make_float2(Fo[u][m].x * sc, Fo[u][m].y * sc), out_mode);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// Raise a kernel's dynamic-LDS limit once where a launch needs more than the default.
+template <typename K>
+static int raise_lds(K kern, size_t lds, size_t* raised) {
+  if (lds > 65536 && lds > *raised) {
+    if (hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+      return -1;
+    *raised = lds;
+  }
+  return 0;
+}
+
+}  // namespace

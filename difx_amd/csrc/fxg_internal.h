@@ -130,10 +130,17 @@ struct FengineArgs {
   const float* chanfreq;  // per datastream: sub[S], then usb, lsb and dsb steps [NFS] (mode.cpp:303-324)
   const uint8_t* fvalid;  // frame valid flags
   int chunkcap;           // FFT capacity of the chunk buffers (multiple of FXG_XPAD)
-  int blocks_per_wg;
+  int units_per_wg;       // per-length launch: units of 8 FFTs per workgroup
   const int* dslist;      // datastreams of this launch, indexed by grid.y (one FFT-length class)
   int rotorder;           // fringe rotation order 0, 1 or 2
   int phcap;              // LDS entries of each phasor table (max S + NS, S + NFS of the launch)
+  const float2* tw_by_log[6];  // persistent launch: twiddle tables of L = 256 << i
+};
+
+// Work item of the persistent F-engine launch: part `part` of `nparts` of the chunk's FFT
+// units of datastream d, FFT length L.
+struct FItem {
+  int L, log2L, d, part, nparts;
 };
 
 // Setup for one subint (delay interpolators, frame validity).
@@ -204,7 +211,11 @@ struct FinWeight {
 
 int fxg_launch_setup(const SetupArgs& a, int maxframes, hipStream_t st);
 int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st);
-int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, hipStream_t st);
+int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, int ntb, hipStream_t st);
+int fxg_launch_fftparams(const FengineArgs& a, int nds, hipStream_t st);
+int fxg_launch_fpersist(const FengineArgs& a, const FItem* items, int nitems, int* counter, int nwg, int ntb,
+                        int iscomplex, size_t lds, hipStream_t st);
+size_t fxg_fengine_lds(int L, int ntb, int phcap, int order);
 int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st);
 int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st);
 int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, hipStream_t st);

@@ -26,6 +26,10 @@ FXG_X_FP32 = 0
 FXG_X_FP16 = 1
 FXG_X_BF16 = 2
 
+FXG_F_AUTO = 0
+FXG_F_PER_LENGTH = 1
+FXG_F_PERSISTENT = 2
+
 
 class fxg_freq(C.Structure):
     _fields_ = [
@@ -99,6 +103,9 @@ class fxg_config(C.Structure):
         ("scan_start_sec", C.c_double),
         ("xengine", C.c_int32),
         ("chunk_ffts", C.c_int32),
+        ("fengine_mode", C.c_int32),
+        ("fengine_threads", C.c_int32),
+        ("autotune", C.c_int32),
     ]
 
 
@@ -143,7 +150,7 @@ EXPORTED_SYMBOLS = [
     "fxg_data_bytes", "fxg_result_offsets", "fxg_process_subint",
     "fxg_process_subint_device", "fxg_ring_length", "fxg_submit_subint", "fxg_wait",
     "fxg_get_spectrum", "fxg_get_data_weights", "fxg_unpack_device",
-    "fxg_delay_interpolator", "fxg_set_timing", "fxg_get_timing",
+    "fxg_delay_interpolator", "fxg_set_timing", "fxg_get_timing", "fxg_autotune_report",
     "fxg_footprint", "fxg_input_samples", "fxg_device_bytes",
     "fxg_server_create", "fxg_server_destroy", "fxg_server_add_job", "fxg_server_push_subint",
     "fxg_server_wait_job", "fxg_server_get_stats", "fxg_server_job_error",

@@ -119,7 +119,16 @@ typedef struct fxg_config {
   double  scan_start_sec;     /* Model::getScanStartSec of the scan, for wallclock LO-offset terms */
   int32_t xengine;            /* FXG_X_* */
   int32_t chunk_ffts;         /* FFTs per device spectra chunk; 0 = automatic */
+  int32_t fengine_mode;       /* FXG_F_AUTO, FXG_F_PER_LENGTH or FXG_F_PERSISTENT */
+  int32_t fengine_threads;    /* workgroup size 256 or 512; 0 = by FFT length */
+  int32_t autotune;           /* 1: time the F-engine launch candidates at create and keep the fastest */
 } fxg_config;
+
+/* F-engine launch forms: one launch per FFT length (datastreams grouped by length), or every
+ * length in one persistent launch whose workgroups take (length, datastream, part) work items. */
+#define FXG_F_AUTO 0
+#define FXG_F_PER_LENGTH 1
+#define FXG_F_PERSISTENT 2
 
 /* Result layout summary (Configuration::populateResultLengths and getDataBytes). */
 typedef struct fxg_layout {
@@ -202,6 +211,10 @@ typedef struct fxg_timing {
 } fxg_timing;
 int fxg_set_timing(fxg_ctx* ctx, int32_t enable);
 int fxg_get_timing(fxg_ctx* ctx, fxg_timing* out);
+
+/* The F-engine launch form in use and, after an autotuned create, the candidates with their
+ * measured subint times, as text (no counterpart in the reference). */
+int fxg_autotune_report(const fxg_ctx* ctx, char* buf, int32_t len);
 
 /* Sizing without a device (Core::getEstimatedBytes, core.cpp:2197-2210, and
  * Mode::getEstimatedBytes, mode.h:216): validates cfg and reports the device bytes

@@ -230,3 +230,40 @@ def test_complex_sidebands(gpu, cs, lsb, xe, bound):
     got, ref, e, lay = _run_and_compare(cfg)
     _say(f"complex {cs} lsb {lsb} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
     assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
+@pytest.mark.parametrize("xe", ["fp32", "fp16"])
+def test_fengine_launch_forms_bit_identical(gpu, xe):
+    """One launch per FFT length with 256- or 512-thread workgroups, and every length in one
+    persistent launch, compute the same arithmetic per FFT: identical results."""
+    offsets = (0, 1, 0)
+    out = {}
+    for mode, threads in ((1, 256), (1, 512), (2, 256), (2, 512)):
+        cfg = case_cfg3(B=32, xengine=xe, fengine_mode=mode, fengine_threads=threads)
+        core = _core(cfg)
+        data, ctrl = slot(cfg, offsets, seed=31)
+        out[(mode, threads)] = core.processdata(offsets, data, ctrl)
+        core.close()
+    ref = out[(1, 256)]
+    for key, v in out.items():
+        _say("launch form", key, "max |diff|", float(np.max(np.abs(v - ref))))
+        np.testing.assert_array_equal(v.view(np.uint64), ref.view(np.uint64))
+    lay = fo.result_layout(case_cfg3(B=32, xengine=xe))
+    data, ctrl = slot(case_cfg3(B=32, xengine=xe), offsets, seed=31)
+    oref, _ = fo.process_subint(case_cfg3(B=32, xengine=xe), list(offsets), data, ctrl)
+    e = record_errors(case_cfg3(B=32, xengine=xe), lay, ref.astype(np.complex128), oref)
+    assert max(e["cross"]) < (1e-5 if xe == "fp32" else 1e-3)
+
+
+def test_autotune_picks_a_form(gpu):
+    cfg = case_cfg3(B=64, xengine="fp16", autotune=1)
+    core = _core(cfg)
+    rep = core.autotune_report()
+    _say(rep)
+    assert "F-engine:" in rep and "ms per subint" in rep
+    data, ctrl = slot(cfg, (0, 1, 0), seed=32)
+    got = core.processdata((0, 1, 0), data, ctrl)
+    core.close()
+    ref, _ = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
+    e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
+    assert max(e["cross"]) < 1e-3
