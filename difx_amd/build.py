@@ -18,7 +18,8 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libfxg.so"
-SOURCES = ["fxg_fengine.hip", "fxg_fpersist.hip", "fxg_xengine.hip", "fxg_api.cpp", "fxg_server.cpp"]
+SOURCES = ["fxg_fengine.hip", "fxg_fpersist.hip", "fxg_fbig.hip", "fxg_xengine.hip", "fxg_api.cpp",
+           "fxg_server.cpp"]
 ARCH = "gfx950"
 
 

@@ -82,7 +82,11 @@ struct fxg_ctx {
     long long tw_off = 0;        // offset of this L's twiddle tables in d_twiddle
     long long dsl_off = 0;       // offset of dsl in d_dslist
     int phcap = 0;               // phasor-table entries: max over dsl of S + NS and S + NFS
+    bool big = false;            // L > FXG_MAX_L: four-step F-engine
+    long long twL_off = 0;       // big: exp(-2 pi i j / L), j < L
+    int wgx = 1;                 // big: workgroups per datastream (scratch rows)
   };
+  float2* d_bigscratch = nullptr;
   std::vector<FClass> fclasses;
   int* d_dslist = nullptr;
   // F-engine launch form (per FFT length, or one persistent launch per sampling kind)
@@ -239,8 +243,8 @@ int validate_and_derive(fxg_ctx* ctx) {
         return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
     }
     const int L = ds.complex_sampling ? f0.nchan : 2 * f0.nchan;
-    if (!is_pow2(L) || L < 256 || L > FXG_MAX_L)
-      return fail(ctx, FXG_E_INVALID, "FFT length must be a power of two in [256, 8192]");
+    if (!is_pow2(L) || L < 256 || L > FXG_MAX_BIG_L)
+      return fail(ctx, FXG_E_INVALID, "FFT length must be a power of two in [256, 32768]");
     // every datastream: a whole number of nanoseconds per FFT and of FFTs per subint
     // (configuration.cpp:3008-3028)
     {
@@ -538,6 +542,7 @@ int build_tables(fxg_ctx* ctx) {
       fxg_ctx::FClass c;
       c.L = L;
       c.cplx = cplx;
+      c.big = L > FXG_MAX_L;
       ctx->fclasses.push_back(c);
     }
     ctx->fclasses[k].dsl.push_back(d);
@@ -555,11 +560,19 @@ int build_tables(fxg_ctx* ctx) {
     for (auto& c : ctx->fclasses) {
       c.dsl_off = (long long)dsl.size();
       dsl.insert(dsl.end(), c.dsl.begin(), c.dsl.end());
-      if (tw_of_L.count(c.L)) { c.tw_off = tw_of_L[c.L]; continue; }
+      if (c.big) {                 // sub-FFT tables of 2048 points plus the L-point twiddles
+        c.twL_off = (long long)tw.size();
+        for (int j = 0; j < c.L; ++j) {
+          const double a = -TWO_PI * (double)j / (double)c.L;
+          tw.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+        }
+      }
+      const int Lt = c.big ? 2048 : c.L;
+      if (tw_of_L.count(Lt)) { c.tw_off = tw_of_L[Lt]; continue; }
       c.tw_off = (long long)tw.size();
-      tw_of_L[c.L] = c.tw_off;
+      tw_of_L[Lt] = c.tw_off;
       int log = 0;
-      while ((1 << log) < c.L) ++log;
+      while ((1 << log) < Lt) ++log;
       const int n8 = log / 3, rem = log % 3;
       auto add_pass = [&](int R, int Ns) {
         for (int r = 1; r < R; ++r)
@@ -927,13 +940,13 @@ int log2i(int v) {
 // first (the items of one length run back to back, so twiddles reload rarely).
 int build_items(fxg_ctx* ctx) {
   int nds_kind[2] = {0, 0};
-  for (const auto& fc : ctx->fclasses) nds_kind[fc.cplx] += (int)fc.dsl.size();
+  for (const auto& fc : ctx->fclasses) if (!fc.big) nds_kind[fc.cplx] += (int)fc.dsl.size();
   for (int kind = 0; kind < 2; ++kind) {
     ctx->items[kind].clear();
     if (!nds_kind[kind]) continue;
     const int parts = std::max(1, std::min(64, (4 * ctx->ncu + nds_kind[kind] - 1) / nds_kind[kind]));
     std::vector<const fxg_ctx::FClass*> cls;
-    for (const auto& fc : ctx->fclasses) if (fc.cplx == kind) cls.push_back(&fc);
+    for (const auto& fc : ctx->fclasses) if (fc.cplx == kind && !fc.big) cls.push_back(&fc);
     std::sort(cls.begin(), cls.end(), [](const fxg_ctx::FClass* a, const fxg_ctx::FClass* b) { return a->L > b->L; });
     for (const auto* fc : cls)
       for (int d : fc->dsl)
@@ -948,18 +961,38 @@ int build_items(fxg_ctx* ctx) {
         }
     if (int rc = dev_upload(ctx, &ctx->d_items[kind], ctx->items[kind])) return rc;
   }
+  // four-step classes: scratch rows of L cf32 per workgroup, shared by the classes (launched
+  // one after another on the context's stream)
+  long long rows = 0;
+  for (auto& fc : ctx->fclasses) {
+    if (!fc.big) continue;
+    const int ncl = (int)fc.dsl.size();
+    fc.wgx = std::max(1, std::min(ctx->chunkcap, (2 * ctx->ncu + ncl - 1) / ncl));
+    rows = std::max(rows, (long long)fc.wgx * ncl * fc.L);
+  }
+  if (rows > 0)
+    if (int rc = dev_malloc(ctx, (void**)&ctx->d_bigscratch, (long long)sizeof(float2) * rows)) return rc;
   return dev_malloc(ctx, (void**)&ctx->d_counter, sizeof(int));
 }
 
 // The F-engine of one chunk in the selected launch form.
 int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
   const int nunits = fa.nffts_pad / 8;
+  for (const auto& fc : ctx->fclasses) {
+    if (!fc.big) continue;
+    fa.twiddle = ctx->d_twiddle + fc.tw_off;
+    fa.dslist = ctx->d_dslist + fc.dsl_off;
+    fa.phcap = fc.phcap;
+    if (fxg_launch_fbig(fa, fc.L, fc.cplx, (int)fc.dsl.size(), fc.wgx, ctx->d_bigscratch,
+                        ctx->d_twiddle + fc.twL_off, st))
+      return fail(ctx, FXG_E_HIP, "four-step F-engine launch");
+  }
   if (ctx->fsel.mode == FXG_F_PERSISTENT) {
     for (int kind = 0; kind < 2; ++kind) {
       if (ctx->items[kind].empty()) continue;
       int phmax = 0;
       for (const auto& fc : ctx->fclasses) {
-        if (fc.cplx != kind) continue;
+        if (fc.cplx != kind || fc.big) continue;
         fa.twiddle = ctx->d_twiddle + fc.tw_off;
         fa.dslist = ctx->d_dslist + fc.dsl_off;
         fa.phcap = fc.phcap;
@@ -970,7 +1003,7 @@ int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
       const int ntb = ctx->fsel.ntb ? ctx->fsel.ntb : 256;
       size_t lds = 0;
       for (const auto& fc : ctx->fclasses)
-        if (fc.cplx == kind) lds = std::max(lds, fxg_fengine_lds(fc.L, ntb, phmax, ctx->rotorder));
+        if (fc.cplx == kind && !fc.big) lds = std::max(lds, fxg_fengine_lds(fc.L, ntb, phmax, ctx->rotorder));
       fa.phcap = phmax;
       const int nitems = (int)ctx->items[kind].size();
       const int nwg = std::max(1, std::min(nitems, ctx->ncu * ctx->fsel.grid_mult));
@@ -980,6 +1013,7 @@ int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
     return FXG_OK;
   }
   for (const auto& fc : ctx->fclasses) {
+    if (fc.big) continue;
     const int ncl = (int)fc.dsl.size();
     // enough workgroups to fill the CUs several times over
     const long long wgs_per_ds = std::max(1LL, (long long)(8 * ctx->ncu + ncl - 1) / ncl);
@@ -1219,7 +1253,7 @@ void free_ctx(fxg_ctx* ctx) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid);
   f(ctx->d_twiddle); f(ctx->d_dslist); f(ctx->d_params); f(ctx->d_items[0]); f(ctx->d_items[1]);
-  f(ctx->d_counter); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
+  f(ctx->d_counter); f(ctx->d_bigscratch); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   for (auto& sg : ctx->stage) {

@@ -7,7 +7,8 @@
 #define FXG_TB 4            // consecutive FFTs per output time block (MFMA K packing)
 #define FXG_XPAD 8          // FFT count padding of a device chunk (two time blocks)
 #define FXG_MAX_ZOUT 16     // zoom-band rows written by the F-engine per datastream
-#define FXG_MAX_L 8192      // longest FFT of the fused F-engine
+#define FXG_MAX_L 8192      // longest FFT of the fused (single-workgroup LDS) F-engine
+#define FXG_MAX_BIG_L 32768 // longest FFT of the four-step F-engine (fxg_fbig.hip)
 
 // Per recorded freq of a datastream: what the rotator and the fractional-sample
 // correction need (mode.cpp:853-931, 1030-1073).
@@ -216,6 +217,8 @@ int fxg_launch_fftparams(const FengineArgs& a, int nds, hipStream_t st);
 int fxg_launch_fpersist(const FengineArgs& a, const FItem* items, int nitems, int* counter, int nwg, int ntb,
                         int iscomplex, size_t lds, hipStream_t st);
 size_t fxg_fengine_lds(int L, int ntb, int phcap, int order);
+int fxg_launch_fbig(const FengineArgs& a, int L, int iscomplex, int nds, int wgx, float2* scratch,
+                    const float2* twL, hipStream_t st);
 int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st);
 int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st);
 int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, hipStream_t st);

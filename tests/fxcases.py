@@ -47,6 +47,21 @@ def case_long_ffts(B: int = 8, xengine: str = "fp32", nstations: int = 3, seed: 
                              seed=seed, **kw)
 
 
+def case_fourstep_ffts(B: int = 4, xengine: str = "fp32", nstations: int = 3, seed: int = 7, **kw):
+    """FFT lengths beyond one workgroup: 64 MHz / 16384 channels (L = 32768) beside
+    32 MHz / 8192 channels (L = 16384), single pol, with a 2 MHz zoom (512 channels)."""
+    groups = [(64.0, 16384, [6000.0]), (32.0, 8192, [6100.0])]
+    zooms = [(0, 0, 10.0, 2.0), (1, 0, 3.0, 0.5)]
+    return cf.batched_config(nstations, groups, zooms, subint_ns=B * 256000, npol=1, xengine=xengine,
+                             seed=seed, **kw)
+
+
+def case_complex_fourstep(B: int = 4, xengine: str = "fp32", nstations: int = 4, seed: int = 8, **kw):
+    """Complex 8-bit 16 MHz with nchan 16384 (a 16384-point complex FFT)."""
+    return cf.standard_config(nstations, 1, [1400.0], 16.0, 16384, subint_ns=B * 1024000,
+                              nbits=8, complex_sampling=1, xengine=xengine, seed=seed, **kw)
+
+
 def slot(cfg, offsets=(0, 1, 0), seed: int = 11):
     return synth.noise_slot(cfg, list(offsets), seed=seed)
 

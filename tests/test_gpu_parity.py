@@ -11,7 +11,8 @@ import pytest
 
 import fxoracle as fo
 from difx_amd import synth
-from fxcases import case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_long_ffts, record_errors, slot
+from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_complex_fourstep,
+                     case_fourstep_ffts, case_long_ffts, record_errors, slot)
 
 pytestmark = pytest.mark.gpu
 
@@ -267,3 +268,14 @@ def test_autotune_picks_a_form(gpu):
     ref, _ = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
     e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
     assert max(e["cross"]) < 1e-3
+
+
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
+@pytest.mark.parametrize("kind", ["real", "complex"])
+def test_fourstep_ffts(gpu, kind, xe, bound):
+    """FFT lengths 16384 and 32768 (four-step F-engine: 2048-point LDS sub-FFTs, twiddles,
+    R-point DFTs) with zoom windows; a 16384-point complex FFT."""
+    cfg = case_fourstep_ffts(B=4, xengine=xe) if kind == "real" else case_complex_fourstep(B=4, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"four-step {kind} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound

@@ -70,7 +70,8 @@ def test_footprint_without_device():
     (lambda c: setattr(c, "fringe_rot_order", 0), True),
     (lambda c: setattr(c, "fringe_rot_order", 2), True),
     (lambda c: setattr(c, "subint_ns", c.subint_ns + 1000), False),     # not a whole FFT count
-    (lambda c: (setattr(c.freqs[0], "nchan", 16384), setattr(c, "subint_ns", 2 * 512000)), False),  # L > 8192
+    (lambda c: (setattr(c.freqs[0], "nchan", 16384), setattr(c, "subint_ns", 2 * 512000)), True),   # L 32768
+    (lambda c: (setattr(c.freqs[0], "nchan", 32768), setattr(c, "subint_ns", 2 * 1024000)), False),  # L > 32768
     (lambda c: setattr(c.freqs[0], "chanavg", 3), False),                # does not divide nchan
 ])
 def test_validation(mutate, ok):

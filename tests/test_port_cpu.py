@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import fxoracle as fo
-from fxcases import case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_long_ffts, record_errors, slot
+from fxcases import case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_fourstep_ffts, case_long_ffts, record_errors, slot
 
 
 def _port():
@@ -13,13 +13,14 @@ def _port():
     return fxport
 
 
-@pytest.mark.parametrize("name", ["cfg0", "cfg0_lsb", "cfg1", "cfg2", "cfg3", "long"])
+@pytest.mark.parametrize("name", ["cfg0", "cfg0_lsb", "cfg1", "cfg2", "cfg3", "long", "fourstep"])
 def test_port_matches_fp64_oracle(name):
     fxport = _port()
     cfg = {"cfg0": lambda: case_cfg0(B=64), "cfg0_lsb": lambda: case_cfg0(B=32, lsb=True),
            "cfg1": lambda: case_cfg1(B=16, nstations=4, nfreq=4),
            "cfg2": lambda: case_cfg2(B=8, xengine="fp32", nstations=6),
-           "cfg3": lambda: case_cfg3(B=16), "long": lambda: case_long_ffts(B=4)}[name]()
+           "cfg3": lambda: case_cfg3(B=16), "long": lambda: case_long_ffts(B=4),
+           "fourstep": lambda: case_fourstep_ffts(B=2)}[name]()
     data, ctrl = slot(cfg)
     got, w = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=3)
     ref, st = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
