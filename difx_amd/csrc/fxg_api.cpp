@@ -87,6 +87,7 @@ struct fxg_ctx {
     int wgx = 1;                 // big: workgroups per datastream (scratch rows)
   };
   float2* d_bigscratch = nullptr;
+  int bigL = 0;
   std::vector<FClass> fclasses;
   int* d_dslist = nullptr;
   // F-engine launch form (per FFT length, or one persistent launch per sampling kind)
@@ -940,13 +941,13 @@ int log2i(int v) {
 // first (the items of one length run back to back, so twiddles reload rarely).
 int build_items(fxg_ctx* ctx) {
   int nds_kind[2] = {0, 0};
-  for (const auto& fc : ctx->fclasses) if (!fc.big) nds_kind[fc.cplx] += (int)fc.dsl.size();
+  for (const auto& fc : ctx->fclasses) nds_kind[fc.cplx] += (int)fc.dsl.size();
   for (int kind = 0; kind < 2; ++kind) {
     ctx->items[kind].clear();
     if (!nds_kind[kind]) continue;
     const int parts = std::max(1, std::min(64, (4 * ctx->ncu + nds_kind[kind] - 1) / nds_kind[kind]));
     std::vector<const fxg_ctx::FClass*> cls;
-    for (const auto& fc : ctx->fclasses) if (fc.cplx == kind && !fc.big) cls.push_back(&fc);
+    for (const auto& fc : ctx->fclasses) if (fc.cplx == kind) cls.push_back(&fc);
     std::sort(cls.begin(), cls.end(), [](const fxg_ctx::FClass* a, const fxg_ctx::FClass* b) { return a->L > b->L; });
     for (const auto* fc : cls)
       for (int d : fc->dsl)
@@ -964,12 +965,17 @@ int build_items(fxg_ctx* ctx) {
   // four-step classes: scratch rows of L cf32 per workgroup, shared by the classes (launched
   // one after another on the context's stream)
   long long rows = 0;
+  int bigL = 0;
   for (auto& fc : ctx->fclasses) {
     if (!fc.big) continue;
     const int ncl = (int)fc.dsl.size();
     fc.wgx = std::max(1, std::min(ctx->chunkcap, (2 * ctx->ncu + ncl - 1) / ncl));
     rows = std::max(rows, (long long)fc.wgx * ncl * fc.L);
+    bigL = std::max(bigL, fc.L);
   }
+  // the persistent launch: up to 2 workgroups per CU of up to 512 threads (2 rows each)
+  if (bigL) rows = std::max(rows, 4LL * ctx->ncu * bigL);
+  ctx->bigL = bigL;
   if (rows > 0)
     if (int rc = dev_malloc(ctx, (void**)&ctx->d_bigscratch, (long long)sizeof(float2) * rows)) return rc;
   return dev_malloc(ctx, (void**)&ctx->d_counter, sizeof(int));
@@ -979,7 +985,7 @@ int build_items(fxg_ctx* ctx) {
 int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
   const int nunits = fa.nffts_pad / 8;
   for (const auto& fc : ctx->fclasses) {
-    if (!fc.big) continue;
+    if (!fc.big || ctx->fsel.mode == FXG_F_PERSISTENT) continue;
     fa.twiddle = ctx->d_twiddle + fc.tw_off;
     fa.dslist = ctx->d_dslist + fc.dsl_off;
     fa.phcap = fc.phcap;
@@ -991,19 +997,24 @@ int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
     for (int kind = 0; kind < 2; ++kind) {
       if (ctx->items[kind].empty()) continue;
       int phmax = 0;
+      bool anybig = false;
       for (const auto& fc : ctx->fclasses) {
-        if (fc.cplx != kind || fc.big) continue;
+        if (fc.cplx != kind) continue;
         fa.twiddle = ctx->d_twiddle + fc.tw_off;
         fa.dslist = ctx->d_dslist + fc.dsl_off;
         fa.phcap = fc.phcap;
         if (fxg_launch_fftparams(fa, (int)fc.dsl.size(), st)) return fail(ctx, FXG_E_HIP, "FFT parameter launch");
-        fa.tw_by_log[log2i(fc.L) - 8] = ctx->d_twiddle + fc.tw_off;
+        fa.tw_by_log[log2i(fc.L) - 8] = ctx->d_twiddle + (fc.big ? fc.twL_off : fc.tw_off);
+        if (fc.big) { fa.tw2048 = ctx->d_twiddle + fc.tw_off; anybig = true; }
         phmax = std::max(phmax, fc.phcap);
       }
+      fa.bigscratch = ctx->d_bigscratch;
+      fa.bigL = ctx->bigL;
       const int ntb = ctx->fsel.ntb ? ctx->fsel.ntb : 256;
       size_t lds = 0;
       for (const auto& fc : ctx->fclasses)
         if (fc.cplx == kind && !fc.big) lds = std::max(lds, fxg_fengine_lds(fc.L, ntb, phmax, ctx->rotorder));
+      if (anybig) lds = std::max(lds, fxg_fbig_lds(ntb, phmax));
       fa.phcap = phmax;
       const int nitems = (int)ctx->items[kind].size();
       const int nwg = std::max(1, std::min(nitems, ctx->ncu * ctx->fsel.grid_mult));

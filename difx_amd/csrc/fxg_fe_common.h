@@ -584,3 +584,244 @@ static int raise_lds(K kern, size_t lds, size_t* raised) {
 }
 
 }  // namespace
+
+namespace {
+
+// ---- four-step F-engine (FFT lengths 16384 and 32768) -------------------------------------
+
+constexpr int BM = 2048;          // sub-FFT length
+constexpr int BNT = BM / 8;       // threads of the workgroup
+
+// 16-point forward DFT as 4 x 4: A[n2][k1] = DFT4 over n1 of x[4 n1 + n2], times
+// exp(-2 pi i n2 k1 / 16), then DFT4 over n2 gives X[k1 + 4 k2].
+__device__ __forceinline__ void dft16(float2* x) {
+  // exp(-2 pi i m / 16) for m = 1, 2, 3, 4, 6, 9
+  const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+  const float c2 = 0.70710678118654752440f;
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) dft4(x[n2], x[4 + n2], x[8 + n2], x[12 + n2]);
+  // x[4 k1 + n2] now holds A[n2][k1]; twiddles W16^(n2 k1)
+  x[4 * 1 + 1] = cmul(x[4 * 1 + 1], make_float2(c1, -s1));    // W^1
+  x[4 * 2 + 1] = cmul(x[4 * 2 + 1], make_float2(c2, -c2));    // W^2
+  x[4 * 3 + 1] = cmul(x[4 * 3 + 1], make_float2(s1, -c1));    // W^3
+  x[4 * 1 + 2] = cmul(x[4 * 1 + 2], make_float2(c2, -c2));    // W^2
+  x[4 * 2 + 2] = mul_negi(x[4 * 2 + 2]);                      // W^4
+  x[4 * 3 + 2] = cmul(x[4 * 3 + 2], make_float2(-c2, -c2));   // W^6
+  x[4 * 1 + 3] = cmul(x[4 * 1 + 3], make_float2(s1, -c1));    // W^3
+  x[4 * 2 + 3] = cmul(x[4 * 2 + 3], make_float2(-c2, -c2));   // W^6
+  x[4 * 3 + 3] = cmul(x[4 * 3 + 3], make_float2(-c1, s1));    // W^9
+  float2 y[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    float2 a0 = x[4 * k1], a1 = x[4 * k1 + 1], a2 = x[4 * k1 + 2], a3 = x[4 * k1 + 3];
+    dft4(a0, a1, a2, a3);
+    y[k1] = a0; y[k1 + 4] = a1; y[k1 + 8] = a2; y[k1 + 12] = a3;
+  }
+#pragma unroll
+  for (int k = 0; k < 16; ++k) x[k] = y[k];
+}
+
+template <int R>
+__device__ __forceinline__ void dftR(float2* x) {
+  if (R == 8) dft8(x);
+  else dft16(x);
+}
+
+// Four-step FFTs of datastream d: FFTs lt0 + g, lt0 + g + step, ... below lt_end, where g is
+// the thread's group of 256 (NTB / 256 groups per workgroup, each with its own exchange buffer,
+// phasor tables and scratch row `scr + g * L`).  Twiddles of the 2048-point sub-FFTs are loaded
+// into LDS first when load_tw is set.
+template <int L, bool CPLX, int ORDER, int NTB>
+__device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2048, const float2* __restrict__ twL,
+                         int d, int lt0, int lt_end, int step, float2* smem, float2* __restrict__ scr_base,
+                         bool load_tw) {
+  constexpr int R = L / BM;
+  constexpr int KEEP = CPLX ? R : R / 2;            // k2 values kept per k1
+  constexpr int G = NTB / BNT;
+  using P = Plan<BM>;
+  const int tid = threadIdx.x;
+  const int g = tid / BNT, t = tid - g * BNT;
+  float2* sbuf = smem + g * (BM + BM / 8);
+  float2* stw = smem + G * (BM + BM / 8);
+  float2* srot = stw + P::TWN + g * args.phcap;
+  float2* sfrac = stw + P::TWN + (G + g) * args.phcap;
+  float* squad = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + g * args.phcap;
+  float2* __restrict__ scr = scr_base + (long long)g * L;
+  if (load_tw) {
+    __syncthreads();                              // earlier users of the twiddle tables are done
+    for (int m = tid; m < P::TWN; m += NTB) stw[m] = tw2048[m];
+  }
+  const DevDs* __restrict__ dsp = args.ds + d;
+  DsConst k;
+  k.C = dsp->C; k.S = dsp->S; k.NS = dsp->NS; k.NFS = dsp->NFS; k.nrec = dsp->nrecbands;
+  k.nrecfreqs = dsp->nrecfreqs; k.fmt = dsp->format; k.spf = dsp->spf;
+  k.framebytes = dsp->framebytes; k.headerbytes = dsp->headerbytes; k.nbits = dsp->nbits;
+  k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us; k.dsb = dsp->dsb;
+  const long long spec_off32 = dsp->spec_off;
+  const uint8_t* __restrict__ fv = args.fvalid + dsp->fvalid_off;
+  const float* __restrict__ cf = args.chanfreq + dsp->chanfreq_off;
+  const uint8_t* __restrict__ data = args.run[d].data;
+  const int nframes = args.run[d].nframes;
+  const FftDelay* __restrict__ params = args.params + (long long)d * args.chunkcap;
+  const int C = k.C, S = k.S;
+  const bool twobit = (k.nbits == 2);
+  const int out_mode = args.out_mode;
+  const int cap = args.chunkcap;
+
+  for (int ltb = lt0; ltb < lt_end; ltb += step * G) {
+    const int lt = ltb + g * step;
+    const bool active = lt < lt_end;              // a group past the end computes but stores nothing
+    FftDelay fd;
+    if (active) {
+      fd = params[lt];
+    } else {
+      fd.ok = 0; fd.n0 = 0; fd.fse = 0.0f; fd.intdelay = 0; fd.a = 0; fd.b = 0; fd.c = 0;
+      fd.weight = 0.0f; fd.intwall = 0; fd.fracwall = 0;
+    }
+    const bool fok_all = fd.ok != 0;               // flagged FFTs: zero spectra (mode.cpp:656-669)
+    for (int f = 0; f < k.nrecfreqs; ++f) {
+      const FreqConst rf = load_freq(dsp->rf[f]);
+      if (rf.nbands == 0) continue;
+      __syncthreads();                            // previous users of srot/sfrac are done
+      if (ORDER != 0) build_rotator<ORDER>(srot, squad, k, rf, fd, L, t, BNT);
+      build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, t, BNT);
+      __syncthreads();
+      for (int kb = 0; kb < rf.nbands; ++kb) {
+        const int band = dsp->rf[f].band[kb];
+        for (int r = 0; r < R; ++r) {
+          // samples n = R*(t + 256 s) + r of the FFT window
+          float2 v[1][8];
+          const int nb0 = R * t + r;
+          const float2 sub = ORDER != 0 ? srot[nb0 & (S - 1)] : make_float2(1.0f, 0.0f);
+          float2 qpow = make_float2(1.0f, 0.0f), qstep = make_float2(1.0f, 0.0f);
+          if (ORDER == 2) {                      // quadratic sub term^j, j = n/S (mode.cpp:1017-1024)
+            const float th = squad[nb0 & (S - 1)];
+            float sn, cs;
+            sincosf((float)(nb0 / S) * th, &sn, &cs);
+            qpow = make_float2(cs, sn);
+            sincosf((float)(R * BNT / S) * th, &sn, &cs);
+            qstep = make_float2(cs, sn);
+          }
+          const unsigned g0 = (unsigned)fd.n0 + (unsigned)nb0;
+          unsigned fr = g0 / (unsigned)k.spf;
+          unsigned within = g0 - fr * (unsigned)k.spf;
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            if (s > 0) {
+              within += R * BNT;
+              while (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
+            }
+            const int n = nb0 + s * R * BNT;
+            const int st = n / S;
+            float2 rot;
+            if (ORDER == 1) {
+              rot = st == 0 ? sub : cmul(sub, srot[S + st]);
+            } else if (ORDER == 2) {
+              const float2 pq = cmul(sub, qpow);
+              rot = st == 0 ? pq : cmul(pq, srot[S + st]);
+              qpow = cmul(qpow, qstep);
+            } else {
+              rot = sub;
+            }
+            const bool ok = fok_all && ((int)fr < nframes) && fv[fr];
+            const unsigned fbase = fr * (unsigned)k.framebytes + (unsigned)k.headerbytes;
+            if (!CPLX) {
+              float x = 0.0f;
+              if (ok) {
+                if (twobit) {
+                  const unsigned q = within * (unsigned)k.nrec + (unsigned)band;
+                  const uint32_t w = *reinterpret_cast<const uint32_t*>(data + fbase + ((q >> 4) << 2));
+                  x = lut2(k.fmt, (w >> (2 * (q & 15u))) & 3u);
+                } else {
+                  x = lut8(data[fbase + within * (unsigned)k.nrec + band]);
+                }
+              }
+              v[0][s] = ORDER == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
+            } else {
+              float2 x = make_float2(0.f, 0.f);
+              if (ok) {
+                const uint8_t* pp = data + fbase + (within * (unsigned)k.nrec + band) * 2u;
+                x = make_float2(lut8(pp[0]), lut8(pp[1]));
+              }
+              v[0][s] = cmul(rot, x);
+              if (rf.lsb) v[0][s].y = -v[0][s].y;   // conj input (see fe_range)
+            }
+          }
+          __syncthreads();                        // sbuf free
+          fft_stockham<BM, 1, BNT>(v, sbuf, stw, t);
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {           // Z_r[k1] = Y_r[k1] W_L^(r k1)
+            const int k1 = t + s * BNT;
+            scr[r * BM + k1] = cmul(v[0][s], twL[r * k1]);
+          }
+        }
+        __syncthreads();                          // every Z_r of this band is in the scratch row
+        // output pointers of this band
+        float2* __restrict__ o32 = nullptr;
+        uint32_t* __restrict__ o16 = nullptr;
+        const float* __restrict__ scp = nullptr;
+        if (out_mode == 0) {
+          o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
+        } else {
+          const int xg = dsp->band_xfreq[band];
+          const int row = dsp->band_xrow[band];
+          o16 = args.spec16 + args.xgroups[xg].spec_off + (long long)row * args.xgroups[xg].rowstride + lt * 8;
+          scp = args.xscale + args.xgroups[xg].scale_off + (long long)row * C;
+        }
+#pragma unroll 1
+        for (int s = 0; s < 8; ++s) {
+          const int k1 = t + s * BNT;
+          float2 z[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) z[r] = scr[r * BM + k1];
+          dftR<R>(z);                             // z[k2] = X[k1 + 2048 k2]
+#pragma unroll
+          for (int j = 0; j < KEEP; ++j) {
+            // channel c = k1 + 2048 j; real LSB keeps X[C + c]; double sideband swaps halves
+            int k2 = j;
+            if (!CPLX && rf.lsb) k2 = j + R / 2;
+            if (CPLX && k.dsb) k2 = (j + R / 2) % R;
+            const int c = k1 + BM * j;
+            float2 X = z[0];
+#pragma unroll
+            for (int q = 1; q < R; ++q) X = (k2 == q) ? z[q] : X;
+            if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped
+            const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
+            float2 F = cmul(X, frq);
+            if (!fok_all) F = make_float2(0.0f, 0.0f);
+            if (!active) continue;
+            if (out_mode == 0) {
+              o32[c] = F;
+            } else {
+              const float sc = scp[c];
+              o16[(c >> 3) * cap * 8 + (c & 7)] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
+              for (int zz = 0; zz < dsp->nzout; ++zz) {          // zoom windows of this band
+                if (dsp->zout_band[zz] != band) continue;
+                const int cz = c - dsp->zout_c0[zz];
+                if (cz < 0 || cz >= dsp->zout_n[zz]) continue;
+                const int xg = dsp->zout_group[zz], row = dsp->zout_row[zz];
+                uint32_t* __restrict__ oz = args.spec16 + args.xgroups[xg].spec_off +
+                                            (long long)row * args.xgroups[xg].rowstride + lt * 8;
+                const float scz = args.xscale[args.xgroups[xg].scale_off + (long long)row * dsp->zout_n[zz] + cz];
+                oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16(make_float2(F.x * scz, F.y * scz), out_mode);
+              }
+            }
+          }
+        }
+        __syncthreads();                          // scratch row free for the next band
+      }
+    }
+  }
+}
+
+// LDS bytes of the four-step F-engine with NTB threads (NTB / 256 FFTs in flight).
+template <int NTB>
+size_t fb_lds_bytes(int phcap) {
+  constexpr int G = NTB / BNT;
+  return sizeof(float2) * ((size_t)G * (BM + BM / 8) + Plan<BM>::TWN + 2 * (size_t)G * phcap) +
+         sizeof(float) * (size_t)G * phcap;
+}
+
+}  // namespace
+
+

@@ -12,11 +12,18 @@ __device__ __attribute__((noinline)) void fe_range_call(const FengineArgs& args,
   fe_range<L, CPLX, ORDER, NTB>(args, twiddle, d, u0, u1, smem, load_tw);
 }
 
+template <int L, bool CPLX, int ORDER, int NTB>
+__device__ __attribute__((noinline)) void fb_range_call(const FengineArgs& args, const float2* twL, int d,
+                                                        int lt0, int lt1, float2* smem, float2* scr, bool load_tw) {
+  fb_range<L, CPLX, ORDER, NTB>(args, args.tw2048, twL, d, lt0, lt1, 1, smem, scr, load_tw);
+}
+
 }  // namespace
 
 // Every FFT length of a configuration in one persistent launch: workgroups take work items
 // (FFT length, datastream, part of the chunk) from a device counter until none are left; the
-// twiddles are reloaded when the length changes.  Items are ordered longest FFT first.
+// twiddles are reloaded when the length changes.  Items are ordered longest FFT first.  Items
+// of length 16384 and 32768 run the four-step body with this workgroup's scratch rows.
 template <bool CPLX, int ORDER, int NTB>
 __global__ void __launch_bounds__(NTB, 1) k_fpersist(FengineArgs args, const FItem* __restrict__ items,
                                                        int nitems, int* counter) {
@@ -43,6 +50,13 @@ __global__ void __launch_bounds__(NTB, 1) k_fpersist(FengineArgs args, const FIt
       case 2048: fe_range_call<2048, CPLX, ORDER, NTB>(args, tw, w.d, u0, u1, fx_smem, load); break;
       case 4096: fe_range_call<4096, CPLX, ORDER, NTB>(args, tw, w.d, u0, u1, fx_smem, load); break;
       case 8192: fe_range_call<8192, CPLX, ORDER, NTB>(args, tw, w.d, u0, u1, fx_smem, load); break;
+      case 16384:
+      case 32768: {
+        float2* scr = args.bigscratch + (long long)blockIdx.x * (NTB / 256) * args.bigL;
+        if (w.L == 16384) fb_range_call<16384, CPLX, ORDER, NTB>(args, tw, w.d, u0 * 8, u1 * 8, fx_smem, scr, load);
+        else fb_range_call<32768, CPLX, ORDER, NTB>(args, tw, w.d, u0 * 8, u1 * 8, fx_smem, scr, load);
+        break;
+      }
       default: break;
     }
   }

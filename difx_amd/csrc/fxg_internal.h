@@ -135,7 +135,11 @@ struct FengineArgs {
   const int* dslist;      // datastreams of this launch, indexed by grid.y (one FFT-length class)
   int rotorder;           // fringe rotation order 0, 1 or 2
   int phcap;              // LDS entries of each phasor table (max S + NS, S + NFS of the launch)
-  const float2* tw_by_log[6];  // persistent launch: twiddle tables of L = 256 << i
+  const float2* tw_by_log[8];  // persistent launch: twiddle tables of L = 256 << i (for
+                               // L > 8192 the L-point exp(-2 pi i j / L) table)
+  const float2* tw2048;        // persistent launch: 2048-point sub-FFT tables of the four-step F-engine
+  float2* bigscratch;          // persistent launch: four-step scratch rows, NTB/256 per workgroup
+  int bigL;                    // persistent launch: row length of bigscratch (largest four-step L)
 };
 
 // Work item of the persistent F-engine launch: part `part` of `nparts` of the chunk's FFT
@@ -219,6 +223,7 @@ int fxg_launch_fpersist(const FengineArgs& a, const FItem* items, int nitems, in
 size_t fxg_fengine_lds(int L, int ntb, int phcap, int order);
 int fxg_launch_fbig(const FengineArgs& a, int L, int iscomplex, int nds, int wgx, float2* scratch,
                     const float2* twL, hipStream_t st);
+size_t fxg_fbig_lds(int ntb, int phcap);
 int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st);
 int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st);
 int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, hipStream_t st);
