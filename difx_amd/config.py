@@ -203,6 +203,15 @@ class Config:
         return cfg, keep
 
 
+def _with(cfg: Config, extra: dict) -> Config:
+    """Set further Config fields (fringe_rot_order, fengine_mode, fengine_threads, autotune, ...)."""
+    for k, v in extra.items():
+        if not hasattr(cfg, k):
+            raise TypeError(f"Config has no field {k!r}")
+        setattr(cfg, k, v)
+    return cfg
+
+
 def random_delay_model(rng: np.random.Generator, nsamples: int, max_delay_us: float = 1e4,
                        max_rate_us_per_s: float = 1.0, model_inc_secs: float = 120.0) -> np.ndarray:
     """Seeded quintic delay polynomials (microseconds, t in seconds from interval start):
@@ -244,7 +253,7 @@ def standard_config(nstations: int, npol: int, freqs_mhz: Sequence[float], bw_mh
                     lsb: bool = False, fringe_rot_order: int = 1, chunk_ffts: int = 0,
                     write_cross_autocorrs: bool = True, max_delay_us: float = 1e4,
                     max_rate_us_per_s: float = 1.0, model_nsamples: int = 1,
-                    start_offset_s: int = 0) -> Config:
+                    start_offset_s: int = 0, **extra) -> Config:
     """One datastream per station, all npol x len(freqs) bands in one sample word
     (band order: freq-major, pol-minor, as vex2difx records them), and every baseline
     correlating every freq with all npol**2 products (parallel hands first)."""
@@ -270,9 +279,9 @@ def standard_config(nstations: int, npol: int, freqs_mhz: Sequence[float], bw_mh
                     prods += [(idx["R"], idx["L"]), (idx["L"], idx["R"])]
                 fl.append((fi, prods))
             bls.append(Baseline(a, b, fl))
-    return Config(freqs=freqs, datastreams=dss, baselines=bls, subint_ns=subint_ns,
-                  fringe_rot_order=fringe_rot_order, xengine=xengine, chunk_ffts=chunk_ffts,
-                  write_cross_autocorrs=write_cross_autocorrs)
+    return _with(Config(freqs=freqs, datastreams=dss, baselines=bls, subint_ns=subint_ns,
+                        fringe_rot_order=fringe_rot_order, xengine=xengine, chunk_ffts=chunk_ffts,
+                        write_cross_autocorrs=write_cross_autocorrs), extra)
 
 
 def batched_config(nstations: int, groups: Sequence[Tuple[float, int, Sequence[float]]],
@@ -280,7 +289,7 @@ def batched_config(nstations: int, groups: Sequence[Tuple[float, int, Sequence[f
                    nbits: int = 2, payloadbytes: int = 8000, seed: int = 1, xengine: str = "fp32",
                    chunk_ffts: int = 0, write_cross_autocorrs: bool = True, max_delay_us: float = 1e4,
                    max_rate_us_per_s: float = 1.0, model_nsamples: int = 1,
-                   correlate_parents: bool = True) -> Config:
+                   correlate_parents: bool = True, **extra) -> Config:
     """Uneven bands: one datastream per (station, bandwidth group), since a DiFX datastream
     has one recorded bandwidth and channel count (mode.h:71, configuration.cpp:937-941).
 
@@ -344,5 +353,5 @@ def batched_config(nstations: int, groups: Sequence[Tuple[float, int, Sequence[f
                 for lz, (zfi, _, _) in enumerate(gz[g]):
                     fl.append((zfi, prods(nrec + lz * npol)))
                 bls.append(Baseline(a * ng + g, b * ng + g, fl))
-    return Config(freqs=freqs, datastreams=dss, baselines=bls, subint_ns=subint_ns, xengine=xengine,
-                  chunk_ffts=chunk_ffts, write_cross_autocorrs=write_cross_autocorrs)
+    return _with(Config(freqs=freqs, datastreams=dss, baselines=bls, subint_ns=subint_ns, xengine=xengine,
+                        chunk_ffts=chunk_ffts, write_cross_autocorrs=write_cross_autocorrs), extra)

@@ -279,3 +279,18 @@ def test_fourstep_ffts(gpu, kind, xe, bound):
     got, ref, e, lay = _run_and_compare(cfg)
     _say(f"four-step {kind} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
     assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
+def test_fourstep_in_persistent_launch_bit_identical(gpu):
+    """The persistent launch runs the 16384/32768-point items with the same four-step body."""
+    offsets = (0, 1, 0)
+    out = {}
+    for mode, threads in ((1, 256), (2, 256), (2, 512)):
+        cfg = case_fourstep_ffts(B=8, xengine="fp16", fengine_mode=mode, fengine_threads=threads)
+        core = _core(cfg)
+        data, ctrl = slot(cfg, offsets, seed=33)
+        out[(mode, threads)] = core.processdata(offsets, data, ctrl)
+        core.close()
+    ref = out[(1, 256)]
+    for key, v in out.items():
+        np.testing.assert_array_equal(v.view(np.uint64), ref.view(np.uint64))
