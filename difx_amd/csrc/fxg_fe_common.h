@@ -66,6 +66,12 @@ __device__ __forceinline__ void dft8(float2* v) {
 // costs 1.3-1.6x in bank conflicts on the reads and the span-64 writes).
 __device__ __forceinline__ int padx(int e) { return e ^ ((e >> 3) & 31); }
 
+// Which per-freq quantities fe_range keeps in registers across the bands of a freq (bit 0:
+// the rotator of the thread's samples, bit 1: the fractional-sample phasors of its channels).
+#ifndef FXG_F_HOIST
+#define FXG_F_HOIST 0
+#endif
+
 template <int L>
 struct Plan {
   static constexpr int LOG = (L == 256) ? 8 : (L == 512) ? 9 : (L == 1024) ? 10 : (L == 2048) ? 11
@@ -448,38 +454,61 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
       if (ORDER != 0) build_rotator<ORDER>(srot, squad, k, rf, fd, L, tl, TPF);
       build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, tl, TPF);
       __syncthreads();
+      // rotator (orders 1, 2) of this thread's samples and fractional-sample phasors of its
+      // channels: shared by the bands of the freq, so they can be computed once per freq
+      // (FXG_F_HOIST) when a thread holds one butterfly position, at the cost of registers
+      constexpr bool HOIST_ROT = (V == 1) && (FXG_F_HOIST & 1);
+      constexpr bool HOIST = (V == 1) && (FXG_F_HOIST & 2);
+      constexpr int NKEEP = CPLX ? 8 : 4;
+      auto make_rot = [&](int u, float2* rot8) {
+        const int vt = tl + u * TPF;
+        const float2 sub = ORDER != 0 ? srot[vt & (S - 1)] : make_float2(1.0f, 0.0f);
+        // order 2: the quadratic sub term to the power of the step index j = vt/S + s*NT/S
+        // (mode.cpp:1017-1024), as A * D^s with A = e^{i j0 th}, D = e^{i (NT/S) th}
+        float2 qpow = make_float2(1.0f, 0.0f), qstep = make_float2(1.0f, 0.0f);
+        if (ORDER == 2) {
+          const float th = squad[vt & (S - 1)];
+          float sn, cs;
+          sincosf((float)(vt / S) * th, &sn, &cs);
+          qpow = make_float2(cs, sn);
+          sincosf((float)(NT / S) * th, &sn, &cs);
+          qstep = make_float2(cs, sn);
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const int st = (vt + s * NT) / S;
+          if (ORDER == 1) {
+            rot8[s] = st == 0 ? sub : cmul(sub, srot[S + st]);
+          } else if (ORDER == 2) {
+            const float2 pq = cmul(sub, qpow);
+            rot8[s] = st == 0 ? pq : cmul(pq, srot[S + st]);
+            qpow = cmul(qpow, qstep);
+          } else {
+            rot8[s] = sub;                                   // no pre-F rotation
+          }
+        }
+      };
+      auto make_frq = [&](int u, float2* frqk) {
+        const int vt = tl + u * TPF;
+#pragma unroll
+        for (int m = 0; m < NKEEP; ++m) {
+          const int c = vt + m * NT;
+          frqk[m] = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
+        }
+      };
+      float2 rot_h[HOIST_ROT ? 8 : 1], frq_h[HOIST ? NKEEP : 1];
+      if (HOIST_ROT) make_rot(0, rot_h);
+      if (HOIST) make_frq(0, frq_h);
       for (int kb = 0; kb < rf.nbands; ++kb) {
         const int band = dsp->rf[f].band[kb];
         float2 v[V][8];
 #pragma unroll
         for (int u = 0; u < V; ++u) {
-          const int vt = tl + u * TPF;
-          const float2 sub = ORDER != 0 ? srot[vt & (S - 1)] : make_float2(1.0f, 0.0f);
-          // order 2: the quadratic sub term to the power of the step index j = vt/S + s*NT/S
-          // (mode.cpp:1017-1024), as A * D^s with A = e^{i j0 th}, D = e^{i (NT/S) th}
-          float2 qpow = make_float2(1.0f, 0.0f), qstep = make_float2(1.0f, 0.0f);
-          if (ORDER == 2) {
-            const float th = squad[vt & (S - 1)];
-            float sn, cs;
-            sincosf((float)(vt / S) * th, &sn, &cs);
-            qpow = make_float2(cs, sn);
-            sincosf((float)(NT / S) * th, &sn, &cs);
-            qstep = make_float2(cs, sn);
-          }
+          float2 rot_l[HOIST_ROT ? 1 : 8];
+          if (!HOIST_ROT) make_rot(u, rot_l);
 #pragma unroll
           for (int s = 0; s < 8; ++s) {
-            const int n = vt + s * NT;
-            const int st = n / S;
-            float2 rot;
-            if (ORDER == 1) {
-              rot = st == 0 ? sub : cmul(sub, srot[S + st]);
-            } else if (ORDER == 2) {
-              const float2 pq = cmul(sub, qpow);
-              rot = st == 0 ? pq : cmul(pq, srot[S + st]);
-              qpow = cmul(qpow, qstep);
-            } else {
-              rot = sub;                                     // no pre-F rotation
-            }
+            const float2 rot = HOIST_ROT ? rot_h[s] : rot_l[HOIST_ROT ? 0 : s];
             const bool ok = (fok[u] >> s) & 1u;
             if (!CPLX) {
               float x = 0.0f;
@@ -521,11 +550,12 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           scp = args.xscale + args.xgroups[xg].scale_off + (long long)row * C;
         }
         // channel selection, fractional-sample correction, output
-        constexpr int NKEEP = CPLX ? 8 : 4;
         float2 Fo[V][NKEEP];                      // output channels vt + m*NT, kept for the zoom windows
 #pragma unroll
         for (int u = 0; u < V; ++u) {
           const int vt = tl + u * TPF;
+          float2 frq_l[HOIST ? 1 : NKEEP];
+          if (!HOIST) make_frq(u, frq_l);
 #pragma unroll
           for (int m = 0; m < NKEEP; ++m) {
             // USB and complex keep X[vt + m*NT]; real LSB keeps X[C + vt + m*NT] (mode.cpp:1205);
@@ -533,7 +563,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             // indices stay compile-time constants (a run-time index would move v to scratch).
             const bool upper = CPLX ? (k.dsb != 0) : (rf.lsb != 0);
             const int c = vt + m * NT;
-            const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
+            const float2 frq = HOIST ? frq_h[m] : frq_l[HOIST ? 0 : m];
             float2 X = upper ? v[u][(m + 4) & 7] : v[u][m];
             if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped (mode.cpp:1199-1200)
             float2 F = cmul(X, frq);
