@@ -413,7 +413,16 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
 
   for (int lt0 = u0 * 8; lt0 < u1 * 8; lt0 += G) {
     const int lt = lt0 + g;                        // this group's FFT within the chunk
-    const FftDelay fd = params[lt];
+    // with more groups than FFTs left (G = 16 > 8 FFTs per unit) the extra groups compute on
+    // zeros, read no parameters and store nothing, keeping the barriers uniform
+    const bool active = lt < u1 * 8;
+    FftDelay fd;
+    if (active) {
+      fd = params[lt];
+    } else {
+      fd.ok = 0; fd.n0 = 0; fd.fse = 0.0f; fd.intdelay = 0; fd.a = 0; fd.b = 0; fd.c = 0;
+      fd.weight = 0.0f; fd.intwall = 0; fd.fracwall = 0;
+    }
     const bool fok_all = fd.ok != 0;               // flagged FFTs: zero spectra (mode.cpp:656-669)
 
     // raw sample words of this thread's time samples, shared by all bands: for 2-bit data
@@ -569,6 +578,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             float2 F = cmul(X, frq);
             if (!fok_all) F = make_float2(0.0f, 0.0f);
             Fo[u][m] = F;
+            if (!active) continue;
             if (out_mode == 0) {
               o32[c] = F;
             } else {
@@ -578,7 +588,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           }
         }
         // zoom windows of this band: the same channels again, as rows of the zoom freq's group
-        if (out_mode != 0) {
+        if (out_mode != 0 && active) {
           for (int z = 0; z < dsp->nzout; ++z) {
             if (dsp->zout_band[z] != band) continue;
             const int xg = dsp->zout_group[z], row = dsp->zout_row[z];
