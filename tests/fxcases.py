@@ -62,8 +62,57 @@ def case_complex_fourstep(B: int = 4, xengine: str = "fp32", nstations: int = 4,
                               nbits=8, complex_sampling=1, xengine=xengine, seed=seed, **kw)
 
 
+def case_offsets(B: int = 32, xengine: str = "fp32", seed: int = 9, **kw):
+    """4 stations dual pol, 2 x 32 MHz with a fractional-MHz band edge, per-freq LO offsets
+    (integer and fractional Hz), clock offsets and pol deltas, channel averaging 4."""
+    cfg = cf.standard_config(4, 2, [4800.37, 4840.0], 32.0, 512, subint_ns=B * 16000, xengine=xengine,
+                             seed=seed, **kw)
+    rng = np.random.default_rng(seed)
+    for d in cfg.datastreams:
+        d.lo_offset_hz = [float(rng.choice([0.0, 10.0, -2.5, 37.25])) for _ in d.recfreqs]
+        d.clock_offset_us = [float(rng.uniform(-0.02, 0.02)) for _ in d.recfreqs]
+        d.clock_offset_delta_us = [float(rng.uniform(-0.005, 0.005)) for _ in d.recfreqs]
+    for f in cfg.freqs:
+        f.chanavg = 4
+    return cfg
+
+
+def case_mark5b(B: int = 32, xengine: str = "fp32", **kw):
+    """cfg1 shape (4 stations, 4 freqs dual pol) recorded as Mark5B frames."""
+    cfg = case_cfg1(B=B, xengine=xengine, nstations=4, nfreq=4, **kw)
+    for d in cfg.datastreams:
+        d.format, d.headerbytes, d.payloadbytes = "MARK5B", 16, 10000
+    return cfg
+
+
+def case_real8(B: int = 32, xengine: str = "fp32", **kw):
+    """8-bit real VDIF, 4 stations, 2 freqs dual pol, 16 MHz, nchan 256."""
+    return cf.standard_config(4, 2, [1400.0, 1420.0], 16.0, 256, subint_ns=B * 16000, nbits=8,
+                              xengine=xengine, seed=5, **kw)
+
+
 def slot(cfg, offsets=(0, 1, 0), seed: int = 11):
     return synth.noise_slot(cfg, list(offsets), seed=seed)
+
+
+def damaged_slot(cfg, offsets=(0, 1, 0), seed: int = 12):
+    """A noise slot with flagged FFTs (valid bits cleared) on station 0 and an invalid frame
+    and a fill-pattern frame on station 1."""
+    data, ctrl = synth.noise_slot(cfg, list(offsets), seed=seed)
+    B = cfg.blockspersend()
+    start = synth.buffer_start_ns(cfg, 0, list(offsets))
+    ctrl[0] = synth.control_words(cfg, 0, list(offsets), start, invalid_ffts=[1, 2, B // 2, B - 1])
+    ds = cfg.datastreams[1]
+    buf = data[1].copy()
+    fb, hb = ds.framebytes, ds.headerbytes
+    if ds.format.upper() == "VDIF":
+        buf[2 * fb + 3] |= 0x80                          # invalid bit of frame 2
+    else:
+        buf[2 * fb + 5] |= 0x80
+    fill = np.frombuffer(np.uint64(0x1122334411223344).tobytes(), np.uint8)
+    buf[5 * fb + hb: 5 * fb + hb + 8] = fill             # fill pattern opens frame 5
+    data[1] = buf
+    return data, ctrl
 
 
 def record_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):

@@ -12,7 +12,8 @@ import pytest
 import fxoracle as fo
 from difx_amd import synth
 from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_complex_fourstep,
-                     case_fourstep_ffts, case_long_ffts, record_errors, slot)
+                     case_fourstep_ffts, case_long_ffts, case_mark5b, case_offsets, case_real8,
+                     damaged_slot, record_errors, slot)
 
 pytestmark = pytest.mark.gpu
 
@@ -93,9 +94,9 @@ def test_delay_interpolator(gpu):
     core.close()
 
 
-def _run_and_compare(cfg, seed=11, offsets=(0, 1, 0), spectra_check=True):
+def _run_and_compare(cfg, seed=11, offsets=(0, 1, 0), spectra_check=True, damaged=False):
     core = _core(cfg)
-    data, ctrl = slot(cfg, offsets, seed=seed)
+    data, ctrl = damaged_slot(cfg, offsets, seed=seed) if damaged else slot(cfg, offsets, seed=seed)
     got = core.processdata(offsets, data, ctrl)
     ref, stations = fo.process_subint(cfg, list(offsets), data, ctrl)
     lay = fo.result_layout(cfg)
@@ -294,3 +295,22 @@ def test_fourstep_in_persistent_launch_bit_identical(gpu):
     ref = out[(1, 256)]
     for key, v in out.items():
         np.testing.assert_array_equal(v.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("name", ["offsets", "mark5b", "real8", "damaged"])
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
+def test_more_cases(gpu, name, xe, bound):
+    """Per-freq LO offsets (integer and fractional Hz), a fractional-MHz band edge, clock
+    offsets and pol deltas with channel averaging 4; Mark5B frames; 8-bit real samples; a slot
+    with flagged FFTs, an invalid frame and a fill-pattern frame."""
+    if name == "offsets":
+        cfg = case_offsets(B=32, xengine=xe)
+    elif name == "mark5b":
+        cfg = case_mark5b(B=32, xengine=xe)
+    elif name == "real8":
+        cfg = case_real8(B=32, xengine=xe)
+    else:
+        cfg = case_cfg1(B=32, nstations=3, nfreq=2, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg, damaged=(name == "damaged"), spectra_check=(name != "offsets"))
+    _say(f"{name} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound

@@ -3,7 +3,8 @@ import numpy as np
 import pytest
 
 import fxoracle as fo
-from fxcases import case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_fourstep_ffts, case_long_ffts, record_errors, slot
+from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_fourstep_ffts, case_long_ffts, case_mark5b,
+                     case_offsets, case_real8, damaged_slot, record_errors, slot)
 
 
 def _port():
@@ -62,3 +63,26 @@ def test_port_complex_sidebands_match_oracle(cs, lsb):
     ref, _ = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
     e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
     assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+
+
+@pytest.mark.parametrize("name", ["offsets", "mark5b", "real8", "damaged"])
+def test_port_more_cases_match_oracle(name):
+    """LO / clock offsets with channel averaging, Mark5B, 8-bit real data, and a slot with
+    flagged FFTs, an invalid frame and a fill-pattern frame."""
+    fxport = _port()
+    if name == "offsets":
+        cfg = case_offsets(B=16)
+    elif name == "mark5b":
+        cfg = case_mark5b(B=16)
+    elif name == "real8":
+        cfg = case_real8(B=16)
+    else:
+        cfg = case_cfg1(B=32, nstations=3, nfreq=2)
+    data, ctrl = damaged_slot(cfg) if name == "damaged" else slot(cfg)
+    got, w = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=2)
+    ref, st = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
+    e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+    np.testing.assert_array_equal(w, np.array([s.weight for s in st], dtype=np.float32))
+    if name == "damaged":
+        assert (w[0] == 0).sum() >= 4 and (w[1] < 1).sum() >= 1
