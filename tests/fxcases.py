@@ -58,8 +58,11 @@ def case_fourstep_ffts(B: int = 4, xengine: str = "fp32", nstations: int = 3, se
 
 def case_complex_fourstep(B: int = 4, xengine: str = "fp32", nstations: int = 4, seed: int = 8, **kw):
     """Complex 8-bit 16 MHz with nchan 16384 (a 16384-point complex FFT)."""
+    # delays within +-200 us: a 4-FFT subint is 4.1 ms, shorter than the default +-10 ms delays,
+    # and the slot buffer may hold only getDataBytes plus four frames
     return cf.standard_config(nstations, 1, [1400.0], 16.0, 16384, subint_ns=B * 1024000,
-                              nbits=8, complex_sampling=1, xengine=xengine, seed=seed, **kw)
+                              nbits=8, complex_sampling=1, xengine=xengine, seed=seed,
+                              max_delay_us=200.0, **kw)
 
 
 def case_offsets(B: int = 32, xengine: str = "fp32", seed: int = 9, **kw):
