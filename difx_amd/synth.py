@@ -144,7 +144,8 @@ def slot_bytes(cfg, d: int, offsets: Sequence[int], start_ns: int) -> int:
     t_end = offsets[1] * 1e9 + offsets[2] + cfg.subint_ns
     tau_end = eval_delay(ds.model, ds.model_order, t_end / 1e9, cfg.model_inc_secs) * 1000.0
     tau_end -= 1e9 * math.floor(ds.clock_coeff0_us / 1e6 + 0.5)
-    need_ns = t_end - tau_end + cfg.fftchannels(d) * dt_ns - start_ns
+    # the last FFT ends at the subint end minus the delay there; one frame of margin
+    need_ns = t_end - tau_end - start_ns
     need_frames = int(math.ceil(need_ns / frame_ns)) + 1
     return max(data_bytes(cfg, d), need_frames * ds.framebytes)
 
