@@ -969,7 +969,7 @@ int build_items(fxg_ctx* ctx) {
   for (auto& fc : ctx->fclasses) {
     if (!fc.big) continue;
     const int ncl = (int)fc.dsl.size();
-    fc.wgx = std::max(1, std::min(ctx->chunkcap, (2 * ctx->ncu + ncl - 1) / ncl));
+    fc.wgx = std::max(1, std::min(ctx->chunkcap, (3 * ctx->ncu + ncl - 1) / ncl));   // 3 per CU
     rows = std::max(rows, (long long)fc.wgx * ncl * fc.L);
     bigL = std::max(bigL, fc.L);
   }

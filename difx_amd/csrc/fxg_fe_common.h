@@ -365,7 +365,7 @@ template <int L, int NTB>
 size_t fe_lds_bytes(int phcap, int order) {
   using GE = Geo<L, NTB>;
   return sizeof(float2) * ((size_t)GE::G * GE::SBUF + Plan<L>::TWN + 2 * (size_t)GE::G * phcap) +
-         (order == 2 ? sizeof(float) * (size_t)GE::G * phcap : 0);
+         sizeof(float) * ((size_t)GE::G * phcap + 256);      // quadratic arguments, 8-bit levels
 }
 
 // The F-engine proper: FFT units [u0, u1) (8 consecutive FFTs each) of datastream d.
@@ -388,10 +388,12 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
   float2* srot = stw + P::TWN + g * args.phcap;
   float2* sfrac = stw + P::TWN + (G + g) * args.phcap;
   float* squad = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + g * args.phcap;
+  float* slut8 = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + G * args.phcap;
 
   if (load_tw) {
     __syncthreads();                              // earlier users of the twiddle tables are done
     for (int m = t; m < P::TWN; m += NTB) stw[m] = twiddle[m];
+    for (int m = t; m < 256; m += NTB) slut8[m] = lut8(m);      // 8-bit levels, once per workgroup
   }
 
   const DevDs* __restrict__ dsp = args.ds + d;
@@ -483,9 +485,10 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           sincosf((float)(NT / S) * th, &sn, &cs);
           qstep = make_float2(cs, sn);
         }
+        const int st0 = vt / S;                          // NT is a multiple of S (checked at set-up)
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-          const int st = (vt + s * NT) / S;
+          const int st = st0 + s * (NT / S);
           if (ORDER == 1) {
             rot8[s] = st == 0 ? sub : cmul(sub, srot[S + st]);
           } else if (ORDER == 2) {
@@ -499,11 +502,10 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
       };
       auto make_frq = [&](int u, float2* frqk) {
         const int vt = tl + u * TPF;
+        const float2 fsub = sfrac[vt & (S - 1)];          // the same sub-phasor for every m
+        const int f0 = S + vt / S;
 #pragma unroll
-        for (int m = 0; m < NKEEP; ++m) {
-          const int c = vt + m * NT;
-          frqk[m] = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
-        }
+        for (int m = 0; m < NKEEP; ++m) frqk[m] = cmul(fsub, sfrac[f0 + m * (NT / S)]);
       };
       float2 rot_h[HOIST_ROT ? 8 : 1], frq_h[HOIST ? NKEEP : 1];
       if (HOIST_ROT) make_rot(0, rot_h);
@@ -520,14 +522,13 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             const float2 rot = HOIST_ROT ? rot_h[s] : rot_l[HOIST_ROT ? 0 : s];
             const bool ok = (fok[u] >> s) & 1u;
             if (!CPLX) {
-              float x = 0.0f;
-              if (ok) {
-                if (twobit) {
-                  const int sh = 2 * (int)(((qlo[u] >> (4 * s)) & 15u) + band);
-                  x = lut2(k.fmt, (wv[u][s] >> sh) & 3u);
-                } else {
-                  x = lut8(data[wv[u][s] + band]);
-                }
+              float x;
+              if (twobit) {                          // branch-free: the word is 0 when !ok
+                const int sh = 2 * (int)(((qlo[u] >> (4 * s)) & 15u) + band);
+                x = ok ? lut2(k.fmt, (wv[u][s] >> sh) & 3u) : 0.0f;
+              } else {
+                x = 0.0f;
+                if (ok) x = slut8[data[wv[u][s] + band]];
               }
               v[u][s] = ORDER == 0 ? make_float2(x, 0.0f)           // real FFT input
                                    : make_float2(x * rot.x, x * rot.y);  // (x + 0i) * rotator
@@ -535,7 +536,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
               float2 x = make_float2(0.f, 0.f);
               if (ok) {
                 const uint8_t* pp = data + wv[u][s] + 2 * band;
-                x = make_float2(lut8(pp[0]), lut8(pp[1]));
+                x = make_float2(slut8[pp[0]], slut8[pp[1]]);
               }
               v[u][s] = cmul(rot, x);
               // complex LSB: the reference conj-flips the spectrum (mode.cpp:1186-1201);
@@ -565,6 +566,12 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           const int vt = tl + u * TPF;
           float2 frq_l[HOIST ? 1 : NKEEP];
           if (!HOIST) make_frq(u, frq_l);
+          // channel c = vt + m*NT; NT is a multiple of 8, so c >> 3 = (vt >> 3) + m*NT/8 and
+          // c & 7 = vt & 7: per-thread bases with uniform strides
+          const long long cs16 = (long long)(NT / 8) * cap * 8;
+          uint32_t* __restrict__ o16t = out_mode ? o16 + (long long)(vt >> 3) * cap * 8 + (vt & 7) : nullptr;
+          const float* __restrict__ scpt = out_mode ? scp + vt : nullptr;
+          float2* __restrict__ o32t = out_mode ? nullptr : o32 + vt;
 #pragma unroll
           for (int m = 0; m < NKEEP; ++m) {
             // USB and complex keep X[vt + m*NT]; real LSB keeps X[C + vt + m*NT] (mode.cpp:1205);
@@ -580,10 +587,10 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             Fo[u][m] = F;
             if (!active) continue;
             if (out_mode == 0) {
-              o32[c] = F;
+              o32t[m * NT] = F;
             } else {
-              const float sc = scp[c];
-              o16[(c >> 3) * cap * 8 + (c & 7)] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
+              const float sc = scpt[m * NT];
+              o16t[m * cs16] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
             }
           }
         }
@@ -689,10 +696,12 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
   float2* srot = stw + P::TWN + g * args.phcap;
   float2* sfrac = stw + P::TWN + (G + g) * args.phcap;
   float* squad = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + g * args.phcap;
+  float* slut8 = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + G * args.phcap;
   float2* __restrict__ scr = scr_base + (long long)g * L;
   if (load_tw) {
     __syncthreads();                              // earlier users of the twiddle tables are done
     for (int m = tid; m < P::TWN; m += NTB) stw[m] = tw2048[m];
+    for (int m = tid; m < 256; m += NTB) slut8[m] = lut8(m);    // 8-bit levels, once per workgroup
   }
   const DevDs* __restrict__ dsp = args.ds + d;
   DsConst k;
@@ -776,7 +785,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
                   const uint32_t w = *reinterpret_cast<const uint32_t*>(data + fbase + ((q >> 4) << 2));
                   x = lut2(k.fmt, (w >> (2 * (q & 15u))) & 3u);
                 } else {
-                  x = lut8(data[fbase + within * (unsigned)k.nrec + band]);
+                  x = slut8[data[fbase + within * (unsigned)k.nrec + band]];
                 }
               }
               v[0][s] = ORDER == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
@@ -784,7 +793,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
               float2 x = make_float2(0.f, 0.f);
               if (ok) {
                 const uint8_t* pp = data + fbase + (within * (unsigned)k.nrec + band) * 2u;
-                x = make_float2(lut8(pp[0]), lut8(pp[1]));
+                x = make_float2(slut8[pp[0]], slut8[pp[1]]);
               }
               v[0][s] = cmul(rot, x);
               if (rf.lsb) v[0][s].y = -v[0][s].y;   // conj input (see fe_range)
@@ -821,13 +830,10 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
 #pragma unroll
           for (int j = 0; j < KEEP; ++j) {
             // channel c = k1 + 2048 j; real LSB keeps X[C + c]; double sideband swaps halves
-            int k2 = j;
-            if (!CPLX && rf.lsb) k2 = j + R / 2;
-            if (CPLX && k.dsb) k2 = (j + R / 2) % R;
+            // (compile-time register indices: one select between two candidates)
+            const bool upper = CPLX ? (k.dsb != 0) : (rf.lsb != 0);
             const int c = k1 + BM * j;
-            float2 X = z[0];
-#pragma unroll
-            for (int q = 1; q < R; ++q) X = (k2 == q) ? z[q] : X;
+            float2 X = upper ? z[(j + R / 2) % R] : z[j];
             if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped
             const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
             float2 F = cmul(X, frq);
@@ -862,7 +868,7 @@ template <int NTB>
 size_t fb_lds_bytes(int phcap) {
   constexpr int G = NTB / BNT;
   return sizeof(float2) * ((size_t)G * BM + Plan<BM>::TWN + 2 * (size_t)G * phcap) +
-         sizeof(float) * (size_t)G * phcap;
+         sizeof(float) * ((size_t)G * phcap + 256);
 }
 
 }  // namespace
