@@ -330,8 +330,10 @@ int validate_and_derive(fxg_ctx* ctx) {
     return fail(ctx, FXG_E_INVALID, "unknown X-engine");
   if (ctx->cfg_fmode < FXG_F_AUTO || ctx->cfg_fmode > FXG_F_PERSISTENT)
     return fail(ctx, FXG_E_INVALID, "unknown F-engine launch form");
-  if (ctx->cfg_fthreads != 0 && ctx->cfg_fthreads != 256 && ctx->cfg_fthreads != 512)
-    return fail(ctx, FXG_E_INVALID, "F-engine workgroup size must be 256 or 512");
+  if (ctx->cfg_fthreads != 0 && ctx->cfg_fthreads != 128 && ctx->cfg_fthreads != 256 && ctx->cfg_fthreads != 512)
+    return fail(ctx, FXG_E_INVALID, "F-engine workgroup size must be 128, 256 or 512");
+  if (ctx->cfg_fthreads == 128 && ctx->cfg_fmode == FXG_F_PERSISTENT)
+    return fail(ctx, FXG_E_INVALID, "the persistent F-engine launch runs 256 or 512 threads");
   return FXG_OK;
 }
 
@@ -1199,6 +1201,7 @@ int autotune(fxg_ctx* ctx) {
   add("per-length, 256/512 threads by length", FXG_F_PER_LENGTH, 0, 1);
   add("per-length, 256 threads", FXG_F_PER_LENGTH, 256, 1);
   add("per-length, 512 threads", FXG_F_PER_LENGTH, 512, 1);
+  add("per-length, 128 threads", FXG_F_PER_LENGTH, 128, 1);
   add("persistent, 256 threads, 1 workgroup/CU", FXG_F_PERSISTENT, 256, 1);
   add("persistent, 256 threads, 2 workgroups/CU", FXG_F_PERSISTENT, 256, 2);
   add("persistent, 512 threads, 1 workgroup/CU", FXG_F_PERSISTENT, 512, 1);

@@ -207,6 +207,7 @@ static int launch_len(const FengineArgs& a, int L, int nds, hipStream_t st) {
 int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, int ntb, hipStream_t st) {
   if (ntb == 512) return iscomplex ? launch_len<true, 512>(a, L, nds, st) : launch_len<false, 512>(a, L, nds, st);
   if (ntb == 256) return iscomplex ? launch_len<true, 256>(a, L, nds, st) : launch_len<false, 256>(a, L, nds, st);
+  if (ntb == 128) return iscomplex ? launch_len<true, 128>(a, L, nds, st) : launch_len<false, 128>(a, L, nds, st);
   return -2;
 }
 
@@ -219,6 +220,15 @@ size_t fxg_fengine_lds(int L, int ntb, int phcap, int order) {
       case 2048: return fe_lds_bytes<2048, 256>(phcap, order);
       case 4096: return fe_lds_bytes<4096, 256>(phcap, order);
       case 8192: return fe_lds_bytes<8192, 256>(phcap, order);
+    }
+  } else if (ntb == 128) {
+    switch (L) {
+      case 256: return fe_lds_bytes<256, 128>(phcap, order);
+      case 512: return fe_lds_bytes<512, 128>(phcap, order);
+      case 1024: return fe_lds_bytes<1024, 128>(phcap, order);
+      case 2048: return fe_lds_bytes<2048, 128>(phcap, order);
+      case 4096: return fe_lds_bytes<4096, 128>(phcap, order);
+      case 8192: return fe_lds_bytes<8192, 128>(phcap, order);
     }
   } else if (ntb == 512) {
     switch (L) {
