@@ -690,7 +690,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
       g.acc_off = aoff;
       aoff += (long long)g.C * (g.ntile * (g.ntile + 1) / 2) * 2 * 1024;
       g.scale_off = soff;
-      soff += (long long)g.ntile * 32 * g.C;
+      soff += (long long)g.ntile * 32;                 // one scale per input row
     }
     ctx->spec_elems = off;
     ctx->vacc_len = aoff;
@@ -698,8 +698,8 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
     if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_spec16, sizeof(uint32_t) * std::max<long long>(off, 4))) return rc_;
     if (int rc_ = dev_zero(ctx, ctx->d_spec16, sizeof(uint32_t) * std::max<long long>(off, 4))) return rc_;
     if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_vacc, sizeof(float) * std::max<long long>(aoff, 4))) return rc_;
-    // nominal per-(input, channel) scale 1/sqrt(L <x^2>) of the input's sample levels, which
-    // keeps the 16-bit spectra near unit magnitude; the accumulation divides it out again
+    // nominal per-input scale 1/sqrt(L <x^2>) of the input's sample levels, which keeps the
+    // 16-bit spectra near unit magnitude; the accumulation divides it out again
     const double p_hi = 0.5 * std::erfc(0.95 / std::sqrt(2.0));
     std::vector<float> xs((size_t)std::max<long long>(soff, 1), 0.0f);
     for (size_t g = 0; g < ctx->groups.size(); ++g)
@@ -708,9 +708,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
         const DevDs& x = ctx->h_ds[d];
         const double xsq = x.iscomplex || x.nbits == 8 ? 2.0 * (20.0 / 3.3) * (20.0 / 3.3)
                                                        : 2 * p_hi * 3.3359 * 3.3359 + (1 - 2 * p_hi);
-        const float sc = (float)(1.0 / std::sqrt((double)x.L * xsq));
-        const DevXGroup& G = ctx->groups[g];
-        for (int c = 0; c < G.C; ++c) xs[(size_t)(G.scale_off + (long long)r * G.C + c)] = sc;
+        xs[(size_t)(ctx->groups[g].scale_off + (long long)r)] = (float)(1.0 / std::sqrt((double)x.L * xsq));
       }
     if (int rc = dev_upload(ctx, &ctx->d_xscale, xs)) return rc;
     if (int rc = dev_upload(ctx, &ctx->d_groups, ctx->groups)) return rc;
@@ -1558,15 +1556,14 @@ int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, 
   const DevXGroup& g = ctx->groups[x.band_xfreq[band]];
   const int row = x.band_xrow[band];
   std::vector<uint32_t> blk((size_t)C);
-  std::vector<float> sc(C);
+  float sc = 1.0f;
   for (int ct = 0; ct < C / 8; ++ct) {
     const uint32_t* src = ctx->d_spec16 + g.spec_off + (long long)row * g.rowstride +
                           ((long long)ct * ctx->chunkcap + lt) * 8;
     if (hipMemcpy(blk.data() + 8 * ct, src, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost) != hipSuccess)
       return FXG_E_HIP;
   }
-  if (hipMemcpy(sc.data(), ctx->d_xscale + g.scale_off + (long long)row * C, sizeof(float) * C,
-                hipMemcpyDeviceToHost) != hipSuccess)
+  if (hipMemcpy(&sc, ctx->d_xscale + g.scale_off + row, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
     return FXG_E_HIP;
   for (int ch = 0; ch < C; ++ch) {
     uint32_t u = blk[(size_t)ch];
@@ -1581,8 +1578,8 @@ int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, 
       std::memcpy(&re, &lo, 4);
       std::memcpy(&im, &hi, 4);
     }
-    out[2 * ch] = re / sc[ch];
-    out[2 * ch + 1] = im / sc[ch];
+    out[2 * ch] = re / sc;
+    out[2 * ch + 1] = im / sc;
   }
   (void)c;
   return FXG_OK;

@@ -27,6 +27,13 @@ __device__ __forceinline__ float lut2(int fmt, unsigned field) {
   float v2 = -v1;
   return field == 0 ? v0 : field == 1 ? v1 : field == 2 ? v2 : v3;
 }
+// The same levels without a table or branches: magnitude 1 for fields 1 and 2, HIMAG for 0 and
+// 3; negative when field bit `sbit` is clear (VDIF: bit 1, Mark5B: bit 0).
+__device__ __forceinline__ float lut2s(unsigned field, unsigned sbit) {
+  const float mag = (field - 1u) < 2u ? 1.0f : HIMAG;
+  const unsigned neg = (~(field >> sbit) & 1u) << 31;
+  return __uint_as_float(__float_as_uint(mag) | neg);
+}
 // 8-bit LUT (b - 128)/3.3 computed in double and rounded to float (format_vdif.c:120).
 __device__ __forceinline__ float lut8(unsigned b) { return (float)(((double)b - 128.0) / 3.3); }
 
@@ -336,14 +343,18 @@ __device__ void build_frac(float2* sfrac, const DsConst& k, const FreqConst& rf,
   }
 }
 
-__device__ __forceinline__ uint32_t pack16(float2 v, int mode) {
-  if (mode == 1) {
+// A complex value as two 16-bit floats in one word (MODE 1: FP16, 2: BF16), round to nearest.
+template <int MODE>
+__device__ __forceinline__ uint32_t pack16m(float2 v) {
+  if (MODE == 1) {
     __half2 h = __floats2half2_rn(v.x, v.y);
     return *reinterpret_cast<uint32_t*>(&h);
   }
   __hip_bfloat162 h = __float22bfloat162_rn(v);
   return *reinterpret_cast<uint32_t*>(&h);
 }
+template <int N>
+struct IntC { static constexpr int value = N; };
 
 
 // Geometry of FFT length L on a workgroup of NTB threads.  An FFT has NT = L/8 butterfly
@@ -410,6 +421,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
   const FftDelay* __restrict__ params = args.params + (long long)d * args.chunkcap;
   const int C = k.C, S = k.S;
   const bool twobit = (k.nbits == 2);
+  const unsigned sbit = k.fmt == FXG_FMT_MARK5B ? 0u : 1u;   // sign bit of a 2-bit field
   const int out_mode = args.out_mode;
   const int cap = args.chunkcap;
 
@@ -486,14 +498,16 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           qstep = make_float2(cs, sn);
         }
         const int st0 = vt / S;                          // NT is a multiple of S (checked at set-up)
+        // step 0's phasor is exp(-0 i) = (1, -0), so every step multiplies: no branch, and the
+        // eight table reads are in flight together
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
           const int st = st0 + s * (NT / S);
           if (ORDER == 1) {
-            rot8[s] = st == 0 ? sub : cmul(sub, srot[S + st]);
+            rot8[s] = cmul(sub, srot[S + st]);
           } else if (ORDER == 2) {
             const float2 pq = cmul(sub, qpow);
-            rot8[s] = st == 0 ? pq : cmul(pq, srot[S + st]);
+            rot8[s] = cmul(pq, srot[S + st]);
             qpow = cmul(qpow, qstep);
           } else {
             rot8[s] = sub;                                   // no pre-F rotation
@@ -517,61 +531,55 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
         for (int u = 0; u < V; ++u) {
           float2 rot_l[HOIST_ROT ? 1 : 8];
           if (!HOIST_ROT) make_rot(u, rot_l);
+          if (!CPLX) {
+            // the sample width is tested once per band, not per sample, so the eight decodes
+            // are straight-line code
+            float x[8];
+            if (twobit) {                            // branch-free: the word is 0 when !ok
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            const float2 rot = HOIST_ROT ? rot_h[s] : rot_l[HOIST_ROT ? 0 : s];
-            const bool ok = (fok[u] >> s) & 1u;
-            if (!CPLX) {
-              float x;
-              if (twobit) {                          // branch-free: the word is 0 when !ok
+              for (int s = 0; s < 8; ++s) {
                 const int sh = 2 * (int)(((qlo[u] >> (4 * s)) & 15u) + band);
-                x = ok ? lut2(k.fmt, (wv[u][s] >> sh) & 3u) : 0.0f;
-              } else {
-                x = 0.0f;
-                if (ok) x = slut8[data[wv[u][s] + band]];
+                const float lv = lut2s((wv[u][s] >> sh) & 3u, sbit);
+                x[s] = ((fok[u] >> s) & 1u) ? lv : 0.0f;
               }
-              v[u][s] = ORDER == 0 ? make_float2(x, 0.0f)           // real FFT input
-                                   : make_float2(x * rot.x, x * rot.y);  // (x + 0i) * rotator
             } else {
-              float2 x = make_float2(0.f, 0.f);
-              if (ok) {
-                const uint8_t* pp = data + wv[u][s] + 2 * band;
-                x = make_float2(slut8[pp[0]], slut8[pp[1]]);
+#pragma unroll
+              for (int s = 0; s < 8; ++s) {
+                x[s] = 0.0f;
+                if ((fok[u] >> s) & 1u) x[s] = slut8[data[wv[u][s] + band]];
               }
-              v[u][s] = cmul(rot, x);
-              // complex LSB: the reference conj-flips the spectrum (mode.cpp:1186-1201);
-              // FFT(conj y)[c] = conj(FFT(y)[-c]) gives the same channels without a flip
-              if (rf.lsb) v[u][s].y = -v[u][s].y;
             }
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+              const float2 rot = HOIST_ROT ? rot_h[s] : rot_l[HOIST_ROT ? 0 : s];
+              v[u][s] = ORDER == 0 ? make_float2(x[s], 0.0f)              // real FFT input
+                                   : make_float2(x[s] * rot.x, x[s] * rot.y);  // (x + 0i) * rotator
+            }
+            continue;
+          }
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {             // complex 8-bit samples
+            const float2 rot = HOIST_ROT ? rot_h[s] : rot_l[HOIST_ROT ? 0 : s];
+            float2 x = make_float2(0.f, 0.f);
+            if ((fok[u] >> s) & 1u) {
+              const uint8_t* pp = data + wv[u][s] + 2 * band;
+              x = make_float2(slut8[pp[0]], slut8[pp[1]]);
+            }
+            v[u][s] = cmul(rot, x);
+            // complex LSB: the reference conj-flips the spectrum (mode.cpp:1186-1201);
+            // FFT(conj y)[c] = conj(FFT(y)[-c]) gives the same channels without a flip
+            if (rf.lsb) v[u][s].y = -v[u][s].y;
           }
         }
         __syncthreads();                          // sbuf free (previous band's FFT done)
         fft_stockham<L, V, TPF>(v, sbuf, stw, tl);
-        // output pointers of this band
-        float2* __restrict__ o32 = nullptr;
-        uint32_t* __restrict__ o16 = nullptr;
-        const float* __restrict__ scp = nullptr;
-        if (out_mode == 0) {
-          o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
-        } else {
-          const int xg = dsp->band_xfreq[band];
-          const int row = dsp->band_xrow[band];
-          o16 = args.spec16 + args.xgroups[xg].spec_off + (long long)row * args.xgroups[xg].rowstride + lt * 8;
-          scp = args.xscale + args.xgroups[xg].scale_off + (long long)row * C;
-        }
-        // channel selection, fractional-sample correction, output
-        float2 Fo[V][NKEEP];                      // output channels vt + m*NT, kept for the zoom windows
+        // channel selection and fractional-sample correction
+        float2 Fo[V][NKEEP];                      // output channels vt + m*NT
 #pragma unroll
         for (int u = 0; u < V; ++u) {
           const int vt = tl + u * TPF;
           float2 frq_l[HOIST ? 1 : NKEEP];
           if (!HOIST) make_frq(u, frq_l);
-          // channel c = vt + m*NT; NT is a multiple of 8, so c >> 3 = (vt >> 3) + m*NT/8 and
-          // c & 7 = vt & 7: per-thread bases with uniform strides
-          const long long cs16 = (long long)(NT / 8) * cap * 8;
-          uint32_t* __restrict__ o16t = out_mode ? o16 + (long long)(vt >> 3) * cap * 8 + (vt & 7) : nullptr;
-          const float* __restrict__ scpt = out_mode ? scp + vt : nullptr;
-          float2* __restrict__ o32t = out_mode ? nullptr : o32 + vt;
 #pragma unroll
           for (int m = 0; m < NKEEP; ++m) {
             // USB and complex keep X[vt + m*NT]; real LSB keeps X[C + vt + m*NT] (mode.cpp:1205);
@@ -582,41 +590,62 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             const float2 frq = HOIST ? frq_h[m] : frq_l[HOIST ? 0 : m];
             float2 X = upper ? v[u][(m + 4) & 7] : v[u][m];
             if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped (mode.cpp:1199-1200)
-            float2 F = cmul(X, frq);
-            if (!fok_all) F = make_float2(0.0f, 0.0f);
-            Fo[u][m] = F;
-            if (!active) continue;
-            if (out_mode == 0) {
-              o32t[m * NT] = F;
-            } else {
-              const float sc = scpt[m * NT];
-              o16t[m * cs16] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
-            }
+            const float2 F = cmul(X, frq);
+            Fo[u][m] = fok_all ? F : make_float2(0.0f, 0.0f);
           }
         }
-        // zoom windows of this band: the same channels again, as rows of the zoom freq's group
-        if (out_mode != 0 && active) {
+        if (!active) continue;
+        // the stores: the output form is tested once per band, so each form's stores are
+        // straight-line code.  Channel c = vt + m*NT; NT is a multiple of 8, so c >> 3 =
+        // (vt >> 3) + m*NT/8 and c & 7 = vt & 7: per-thread bases with uniform strides.
+        if (out_mode == 0) {
+          float2* __restrict__ o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
+#pragma unroll
+          for (int u = 0; u < V; ++u) {
+            float2* __restrict__ o32t = o32 + tl + u * TPF;
+#pragma unroll
+            for (int m = 0; m < NKEEP; ++m) o32t[m * NT] = Fo[u][m];
+          }
+          continue;
+        }
+        auto store16 = [&](auto mode_c) {
+          constexpr int OM = decltype(mode_c)::value;
+          const int xg = dsp->band_xfreq[band];
+          const int row = dsp->band_xrow[band];
+          const DevXGroup& xgr = args.xgroups[xg];
+          uint32_t* __restrict__ o16 = args.spec16 + xgr.spec_off + (long long)row * xgr.rowstride + lt * 8;
+          const float sc = args.xscale[xgr.scale_off + row];          // one scale per input row
+          const long long cs16 = (long long)(NT / 8) * cap * 8;
+#pragma unroll
+          for (int u = 0; u < V; ++u) {
+            const int vt = tl + u * TPF;
+            uint32_t* __restrict__ o16t = o16 + (long long)(vt >> 3) * cap * 8 + (vt & 7);
+#pragma unroll
+            for (int m = 0; m < NKEEP; ++m)
+              o16t[m * cs16] = pack16m<OM>(make_float2(Fo[u][m].x * sc, Fo[u][m].y * sc));
+          }
+          // zoom windows of this band: the same channels again, as rows of the zoom freq's group
           for (int z = 0; z < dsp->nzout; ++z) {
             if (dsp->zout_band[z] != band) continue;
-            const int xg = dsp->zout_group[z], row = dsp->zout_row[z];
+            const DevXGroup& zg = args.xgroups[dsp->zout_group[z]];
+            const int zrow = dsp->zout_row[z];
             const int c0 = dsp->zout_c0[z], nz = dsp->zout_n[z];
-            uint32_t* __restrict__ oz = args.spec16 + args.xgroups[xg].spec_off +
-                                        (long long)row * args.xgroups[xg].rowstride + lt * 8;
-            const float* __restrict__ scz = args.xscale + args.xgroups[xg].scale_off + (long long)row * nz;
+            uint32_t* __restrict__ oz = args.spec16 + zg.spec_off + (long long)zrow * zg.rowstride + lt * 8;
+            const float scz = args.xscale[zg.scale_off + zrow];
 #pragma unroll
             for (int u = 0; u < V; ++u) {
 #pragma unroll
               for (int m = 0; m < NKEEP; ++m) {
                 const int cz = tl + u * TPF + m * NT - c0;
-                if (cz >= 0 && cz < nz) {
-                  const float sc = scz[cz];
+                if (cz >= 0 && cz < nz)
                   oz[(cz >> 3) * cap * 8 + (cz & 7)] =
-                      pack16(make_float2(Fo[u][m].x * sc, Fo[u][m].y * sc), out_mode);
-                }
+                      pack16m<OM>(make_float2(Fo[u][m].x * scz, Fo[u][m].y * scz));
               }
             }
           }
-        }
+        };
+        if (out_mode == 1) store16(IntC<1>());
+        else store16(IntC<2>());
       }
     }
   }
@@ -717,6 +746,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
   const FftDelay* __restrict__ params = args.params + (long long)d * args.chunkcap;
   const int C = k.C, S = k.S;
   const bool twobit = (k.nbits == 2);
+  const unsigned sbit = k.fmt == FXG_FMT_MARK5B ? 0u : 1u;   // sign bit of a 2-bit field
   const int out_mode = args.out_mode;
   const int cap = args.chunkcap;
 
@@ -766,11 +796,11 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
             const int n = nb0 + s * R * BNT;
             const int st = n / S;
             float2 rot;
-            if (ORDER == 1) {
-              rot = st == 0 ? sub : cmul(sub, srot[S + st]);
+            if (ORDER == 1) {                    // step 0's phasor is (1, -0): no branch
+              rot = cmul(sub, srot[S + st]);
             } else if (ORDER == 2) {
               const float2 pq = cmul(sub, qpow);
-              rot = st == 0 ? pq : cmul(pq, srot[S + st]);
+              rot = cmul(pq, srot[S + st]);
               qpow = cmul(qpow, qstep);
             } else {
               rot = sub;
@@ -783,7 +813,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
                 if (twobit) {
                   const unsigned q = within * (unsigned)k.nrec + (unsigned)band;
                   const uint32_t w = *reinterpret_cast<const uint32_t*>(data + fbase + ((q >> 4) << 2));
-                  x = lut2(k.fmt, (w >> (2 * (q & 15u))) & 3u);
+                  x = lut2s((w >> (2 * (q & 15u))) & 3u, sbit);
                 } else {
                   x = slut8[data[fbase + within * (unsigned)k.nrec + band]];
                 }
@@ -808,55 +838,60 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
           }
         }
         __syncthreads();                          // every Z_r of this band is in the scratch row
-        // output pointers of this band
-        float2* __restrict__ o32 = nullptr;
-        uint32_t* __restrict__ o16 = nullptr;
-        const float* __restrict__ scp = nullptr;
-        if (out_mode == 0) {
-          o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
-        } else {
-          const int xg = dsp->band_xfreq[band];
-          const int row = dsp->band_xrow[band];
-          o16 = args.spec16 + args.xgroups[xg].spec_off + (long long)row * args.xgroups[xg].rowstride + lt * 8;
-          scp = args.xscale + args.xgroups[xg].scale_off + (long long)row * C;
-        }
+        // the second step and the stores; the output form is tested once per band
+        auto finish = [&](auto mode_c) {
+          constexpr int OM = decltype(mode_c)::value;   // 0: FP32, 1: FP16, 2: BF16
+          float2* __restrict__ o32 = nullptr;
+          uint32_t* __restrict__ o16 = nullptr;
+          float sc = 1.0f;
+          if (OM == 0) {
+            o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
+          } else {
+            const DevXGroup& xgr = args.xgroups[dsp->band_xfreq[band]];
+            const int row = dsp->band_xrow[band];
+            o16 = args.spec16 + xgr.spec_off + (long long)row * xgr.rowstride + lt * 8;
+            sc = args.xscale[xgr.scale_off + row];                    // one scale per input row
+          }
 #pragma unroll 1
-        for (int s = 0; s < 8; ++s) {
-          const int k1 = t + s * BNT;
-          float2 z[R];
+          for (int s = 0; s < 8; ++s) {
+            const int k1 = t + s * BNT;
+            float2 z[R];
 #pragma unroll
-          for (int r = 0; r < R; ++r) z[r] = scr[r * BM + k1];
-          dftR<R>(z);                             // z[k2] = X[k1 + 2048 k2]
+            for (int r = 0; r < R; ++r) z[r] = scr[r * BM + k1];
+            dftR<R>(z);                           // z[k2] = X[k1 + 2048 k2]
 #pragma unroll
-          for (int j = 0; j < KEEP; ++j) {
-            // channel c = k1 + 2048 j; real LSB keeps X[C + c]; double sideband swaps halves
-            // (compile-time register indices: one select between two candidates)
-            const bool upper = CPLX ? (k.dsb != 0) : (rf.lsb != 0);
-            const int c = k1 + BM * j;
-            float2 X = upper ? z[(j + R / 2) % R] : z[j];
-            if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped
-            const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
-            float2 F = cmul(X, frq);
-            if (!fok_all) F = make_float2(0.0f, 0.0f);
-            if (!active) continue;
-            if (out_mode == 0) {
-              o32[c] = F;
-            } else {
-              const float sc = scp[c];
-              o16[(c >> 3) * cap * 8 + (c & 7)] = pack16(make_float2(F.x * sc, F.y * sc), out_mode);
-              for (int zz = 0; zz < dsp->nzout; ++zz) {          // zoom windows of this band
-                if (dsp->zout_band[zz] != band) continue;
-                const int cz = c - dsp->zout_c0[zz];
-                if (cz < 0 || cz >= dsp->zout_n[zz]) continue;
-                const int xg = dsp->zout_group[zz], row = dsp->zout_row[zz];
-                uint32_t* __restrict__ oz = args.spec16 + args.xgroups[xg].spec_off +
-                                            (long long)row * args.xgroups[xg].rowstride + lt * 8;
-                const float scz = args.xscale[args.xgroups[xg].scale_off + (long long)row * dsp->zout_n[zz] + cz];
-                oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16(make_float2(F.x * scz, F.y * scz), out_mode);
+            for (int j = 0; j < KEEP; ++j) {
+              // channel c = k1 + 2048 j; real LSB keeps X[C + c]; double sideband swaps halves
+              // (compile-time register indices: one select between two candidates)
+              const bool upper = CPLX ? (k.dsb != 0) : (rf.lsb != 0);
+              const int c = k1 + BM * j;
+              float2 X = upper ? z[(j + R / 2) % R] : z[j];
+              if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped
+              const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
+              float2 F = cmul(X, frq);
+              if (!fok_all) F = make_float2(0.0f, 0.0f);
+              if (!active) continue;
+              if (OM == 0) {
+                o32[c] = F;
+              } else {
+                o16[(c >> 3) * cap * 8 + (c & 7)] = pack16m<OM>(make_float2(F.x * sc, F.y * sc));
+                for (int zz = 0; zz < dsp->nzout; ++zz) {          // zoom windows of this band
+                  if (dsp->zout_band[zz] != band) continue;
+                  const int cz = c - dsp->zout_c0[zz];
+                  if (cz < 0 || cz >= dsp->zout_n[zz]) continue;
+                  const DevXGroup& zg = args.xgroups[dsp->zout_group[zz]];
+                  const int zrow = dsp->zout_row[zz];
+                  uint32_t* __restrict__ oz = args.spec16 + zg.spec_off + (long long)zrow * zg.rowstride + lt * 8;
+                  const float scz = args.xscale[zg.scale_off + zrow];
+                  oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16m<OM>(make_float2(F.x * scz, F.y * scz));
+                }
               }
             }
           }
-        }
+        };
+        if (out_mode == 0) finish(IntC<0>());
+        else if (out_mode == 1) finish(IntC<1>());
+        else finish(IntC<2>());
         __syncthreads();                          // scratch row free for the next band
       }
     }

@@ -93,7 +93,7 @@ struct DevXGroup {
   long long acc_off;      // float offset of its accumulator block [C][npairs][2][32][32]
   long long spec_off;     // element offset (32-bit units) of its spectra block
   long long rowstride;    // 32-bit units per input row: C x chunk FFTs plus one 128-byte line
-  long long scale_off;    // float offset of its per-(row, channel) scales [rows][C]
+  long long scale_off;    // float offset of its per-row scales [rows]
 };
 
 // Per-FFT delay chain of one datastream (mode.cpp:671-675, 743-757, 781-788).
@@ -127,7 +127,7 @@ struct FengineArgs {
   float2* spec32;         // FP32 spectra
   uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row (rowstride)][C/8][t][8]
   const DevXGroup* xgroups;
-  const float* xscale;    // per group [rows][C] scales applied before 16-bit conversion
+  const float* xscale;    // per group [rows] scales applied before 16-bit conversion
   const float* chanfreq;  // per datastream: sub[S], then usb, lsb and dsb steps [NFS] (mode.cpp:303-324)
   const uint8_t* fvalid;  // frame valid flags
   int chunkcap;           // FFT capacity of the chunk buffers (multiple of FXG_XPAD)

@@ -246,8 +246,7 @@ __global__ void k_fin_mfma(const FinMfma* e, const DevXGroup* groups, const floa
   for (int k = 0; k < f.avg; ++k) {
     const int c = f.chan0 + co * f.avg + k;
     const float* base = acc + g.acc_off + ((long long)c * npair + pair) * 2 * 1024;
-    const float sc = xscale[g.scale_off + (long long)f.row1 * g.C + c] *
-                     xscale[g.scale_off + (long long)f.row2 * g.C + c];
+    const float sc = xscale[g.scale_off + f.row1] * xscale[g.scale_off + f.row2];
     const float inv = 1.0f / sc;
     s.x += base[idx] * inv;
     s.y += sign * base[1024 + idx] * inv;
