@@ -265,6 +265,8 @@ int validate_and_derive(fxg_ctx* ctx) {
     const int bits = ds.nbits * (ds.complex_sampling ? 2 : 1) * ds.nrecbands;
     if (pay <= 16 || pay % 8 || (pay * 8) % bits || ds.framebytes % 4 || ds.headerbytes % 4)
       return fail(ctx, FXG_E_INVALID, "frame size inconsistent with the sample format");
+    if ((long long)pay * 8 / bits > (1LL << 20))      // the F-engine's frame index arithmetic
+      return fail(ctx, FXG_E_INVALID, "more than 2^20 samples per frame");
     int nb_per_freq[FXG_MAX_FREQS] = {0};
     for (int b = 0; b < ds.nrecbands; ++b) {
       int lf = ds.recband_localfreq[b];
@@ -468,6 +470,7 @@ int build_tables(fxg_ctx* ctx) {
     x.nbits = s.nbits;
     x.iscomplex = s.complex_sampling ? 1 : 0;
     x.dsb = s.complex_sampling == 2;
+    x.xsc = 1.0f;
     x.framebytes = s.framebytes;
     x.headerbytes = s.headerbytes;
     x.nrecbands = s.nrecbands;
@@ -708,7 +711,9 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
         const DevDs& x = ctx->h_ds[d];
         const double xsq = x.iscomplex || x.nbits == 8 ? 2.0 * (20.0 / 3.3) * (20.0 / 3.3)
                                                        : 2 * p_hi * 3.3359 * 3.3359 + (1 - 2 * p_hi);
-        xs[(size_t)(ctx->groups[g].scale_off + (long long)r)] = (float)(1.0 / std::sqrt((double)x.L * xsq));
+        const float sc = (float)(1.0 / std::sqrt((double)x.L * xsq));
+        xs[(size_t)(ctx->groups[g].scale_off + (long long)r)] = sc;
+        ctx->h_ds[d].xsc = sc;                  // the F-engine folds it into its phasors
       }
     if (int rc = dev_upload(ctx, &ctx->d_xscale, xs)) return rc;
     if (int rc = dev_upload(ctx, &ctx->d_groups, ctx->groups)) return rc;

@@ -8,6 +8,18 @@
 
 namespace {
 
+// Read-only tables through the constant address space: with a uniform address the compiler
+// loads them with scalar loads.  A vector-memory load between the bands would make its wait
+// also wait for every store issued before it (the vector-memory counter counts both), so each
+// band would stall on the previous band's spectra stores.
+#define FXG_CONST __attribute__((address_space(4)))
+typedef const FXG_CONST DevDs* CDevDs;
+typedef const FXG_CONST DevXGroup* CDevXGroup;
+// Sample data and frame flags as global-address-space pointers: their loads are then vector
+// global loads (a flat load would also count against the LDS wait counter).
+typedef const __attribute__((address_space(1))) uint8_t* GBytes;
+typedef const __attribute__((address_space(1))) uint32_t* GWords;
+
 constexpr double TWO_PI_D = 6.283185307179586476925286766559;   // architecture.h.in:430
 constexpr float HIMAG = 3.3359f;                                 // mark5_stream.h:65
 constexpr int FLAGS_PER_INT = 30;                                // mpifxcorr.h:29
@@ -26,13 +38,6 @@ __device__ __forceinline__ float lut2(int fmt, unsigned field) {
   float v1 = (fmt == FXG_FMT_MARK5B) ? 1.0f : -1.0f;
   float v2 = -v1;
   return field == 0 ? v0 : field == 1 ? v1 : field == 2 ? v2 : v3;
-}
-// The same levels without a table or branches: magnitude 1 for fields 1 and 2, HIMAG for 0 and
-// 3; negative when field bit `sbit` is clear (VDIF: bit 1, Mark5B: bit 0).
-__device__ __forceinline__ float lut2s(unsigned field, unsigned sbit) {
-  const float mag = (field - 1u) < 2u ? 1.0f : HIMAG;
-  const unsigned neg = (~(field >> sbit) & 1u) << 31;
-  return __uint_as_float(__float_as_uint(mag) | neg);
 }
 // 8-bit LUT (b - 128)/3.3 computed in double and rounded to float (format_vdif.c:120).
 __device__ __forceinline__ float lut8(unsigned b) { return (float)(((double)b - 128.0) / 3.3); }
@@ -258,7 +263,7 @@ struct FreqConst {
   int lsb, looff, isfraclooffset, nbands;
 };
 
-__device__ __forceinline__ FreqConst load_freq(const DevRecFreq& rf) {
+__device__ __forceinline__ FreqConst load_freq(const FXG_CONST DevRecFreq& rf) {
   FreqConst f;
   f.lofreq = rf.lofreq; f.lo_offset_hz = rf.lo_offset_hz; f.fraclooffset = rf.fraclooffset;
   f.clockoff = rf.clockoff; f.halfdelta = rf.halfdelta;
@@ -357,6 +362,20 @@ template <int N>
 struct IntC { static constexpr int value = N; };
 
 
+
+// LDS floats of the staged channel-frequency tables (S + 3 NFS <= 3 phcap).
+__host__ __device__ constexpr size_t stage_lds_bytes(int phcap) { return sizeof(float) * 3 * (size_t)phcap; }
+
+// Copy datastream d's channel-frequency tables (read per lane by build_frac) into LDS; all
+// threads of the workgroup call it (the barriers are workgroup-wide).
+__device__ void stage_cf(const FengineArgs& args, CDevDs ds, float* scf, int tid, int nt) {
+  __syncthreads();                                  // earlier readers of the tables are done
+  const float* cf = args.chanfreq + ds->chanfreq_off;
+  const int ncf = ds->S + 3 * ds->NFS;
+  for (int i = tid; i < ncf; i += nt) scf[i] = cf[i];
+  __syncthreads();
+}
+
 // Geometry of FFT length L on a workgroup of NTB threads.  An FFT has NT = L/8 butterfly
 // positions.  With NT < NTB the workgroup runs G = NTB/NT FFTs side by side, TPF = NT threads
 // each (consecutive FFTs of one datastream); with NT > NTB each thread holds V = NT/NTB
@@ -376,7 +395,8 @@ template <int L, int NTB>
 size_t fe_lds_bytes(int phcap, int order) {
   using GE = Geo<L, NTB>;
   return sizeof(float2) * ((size_t)GE::G * GE::SBUF + Plan<L>::TWN + 2 * (size_t)GE::G * phcap) +
-         sizeof(float) * ((size_t)GE::G * phcap + 256);      // quadratic arguments, 8-bit levels
+         sizeof(float) * ((size_t)GE::G * phcap + 264) +     // quadratic arguments, 8- and 2-bit levels
+         stage_lds_bytes(phcap);
 }
 
 // The F-engine proper: FFT units [u0, u1) (8 consecutive FFTs each) of datastream d.
@@ -405,23 +425,28 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
     __syncthreads();                              // earlier users of the twiddle tables are done
     for (int m = t; m < P::TWN; m += NTB) stw[m] = twiddle[m];
     for (int m = t; m < 256; m += NTB) slut8[m] = lut8(m);      // 8-bit levels, once per workgroup
+    for (int m = t; m < 8; m += NTB) slut8[256 + m] = lut2(m < 4 ? FXG_FMT_VDIF : FXG_FMT_MARK5B, m & 3);
   }
+  const CDevDs dsp = (CDevDs)(args.ds + d);
+  float* scf = slut8 + 264;                       // staged channel-frequency tables
+  stage_cf(args, dsp, scf, t, NTB);
 
-  const DevDs* __restrict__ dsp = args.ds + d;
   DsConst k;
   k.C = dsp->C; k.S = dsp->S; k.NS = dsp->NS; k.NFS = dsp->NFS; k.nrec = dsp->nrecbands;
   k.nrecfreqs = dsp->nrecfreqs; k.fmt = dsp->format; k.spf = dsp->spf;
   k.framebytes = dsp->framebytes; k.headerbytes = dsp->headerbytes; k.nbits = dsp->nbits;
   k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us; k.dsb = dsp->dsb;
   const long long spec_off32 = dsp->spec_off;
-  const uint8_t* __restrict__ fv = args.fvalid + dsp->fvalid_off;
-  const float* __restrict__ cf = args.chanfreq + dsp->chanfreq_off;
-  const uint8_t* __restrict__ data = args.run[d].data;
+  const GBytes fv = (GBytes)(args.fvalid + dsp->fvalid_off);
+  const float* __restrict__ cf = scf;
+  const GBytes data = (GBytes)args.run[d].data;
   const int nframes = args.run[d].nframes;
   const FftDelay* __restrict__ params = args.params + (long long)d * args.chunkcap;
   const int C = k.C, S = k.S;
   const bool twobit = (k.nbits == 2);
-  const unsigned sbit = k.fmt == FXG_FMT_MARK5B ? 0u : 1u;   // sign bit of a 2-bit field
+  const float xsc = dsp->xsc;
+  const double inv_spf = 1.0 / (double)k.spf;
+  const float* __restrict__ slev2 = slut8 + 256 + (k.fmt == FXG_FMT_MARK5B ? 4 : 0);   // 2-bit levels
   const int out_mode = args.out_mode;
   const int cap = args.chunkcap;
 
@@ -438,37 +463,68 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
       fd.weight = 0.0f; fd.intwall = 0; fd.fracwall = 0;
     }
     const bool fok_all = fd.ok != 0;               // flagged FFTs: zero spectra (mode.cpp:656-669)
+    const float fs = fok_all ? xsc : 0.0f;         // output scale of this FFT's spectra
 
     // raw sample words of this thread's time samples, shared by all bands: for 2-bit data
     // the 32-bit payload word holding the sample's fields and the field offset in it; for
     // 8-bit data the byte offset of the sample
     uint32_t wv[V][8];
     uint32_t fok[V], qlo[V];
+    // (all sixteen loads of a position are issued before any result is used: the frame index
+    // is clamped to a frame of the buffer, and a sample past the data is masked afterwards)
+    const bool anyload = fok_all && nframes > 0;
 #pragma unroll
     for (int u = 0; u < V; ++u) {
-      fok[u] = 0u;
-      qlo[u] = 0u;
       const unsigned g0 = (unsigned)fd.n0 + (unsigned)(tl + u * TPF);
-      unsigned fr = g0 / (unsigned)k.spf;
-      unsigned within = g0 - fr * (unsigned)k.spf;
+      unsigned frs[8], wis[8];
 #pragma unroll
       for (int s = 0; s < 8; ++s) {
-        if (s > 0) {
-          within += NT;
-          while (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
-        }
-        const bool ok = fok_all && ((int)fr < nframes) && fv[fr];
-        fok[u] |= (ok ? 1u : 0u) << s;
-        const unsigned fbase = fr * (unsigned)k.framebytes + (unsigned)k.headerbytes;
-        if (twobit) {
-          const unsigned q = within * (unsigned)k.nrec;     // first field of the sample
-          qlo[u] |= (q & 15u) << (4 * s);
-          wv[u][s] = ok ? *reinterpret_cast<const uint32_t*>(data + fbase + ((q >> 4) << 2)) : 0u;
-        } else {
-          wv[u][s] = fbase + within * (unsigned)k.nrec * (CPLX ? 2u : 1u);
+        // frame and sample within it: the quotient through the reciprocal of spf (exact after
+        // the one correction: sample indices and spf stay below 2^31 and 2^20)
+        const unsigned gs = g0 + (unsigned)(s * NT);
+        unsigned fr = (unsigned)((double)gs * inv_spf);
+        unsigned within = gs - fr * (unsigned)k.spf;
+        if (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
+        frs[s] = fr;
+        wis[s] = within;
+      }
+      uint32_t fvb[8], w8[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) { fvb[s] = 0u; w8[s] = 0u; }
+      if (anyload) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const unsigned frc = min(frs[s], (unsigned)(nframes - 1));
+          fvb[s] = fv[frc];
+          if (twobit) {
+            const unsigned q = wis[s] * (unsigned)k.nrec;   // first field of the sample
+            w8[s] = *(GWords)(data + frc * (unsigned)k.framebytes + (unsigned)k.headerbytes + ((q >> 4) << 2));
+          }
         }
       }
+      fok[u] = 0u;
+      qlo[u] = 0u;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const bool ok = fvb[s] != 0u && frs[s] < (unsigned)nframes;
+        fok[u] |= (ok ? 1u : 0u) << s;
+        const unsigned q = wis[s] * (unsigned)k.nrec;
+        qlo[u] |= (q & 15u) << (4 * s);
+        wv[u][s] = twobit ? w8[s]
+                          : frs[s] * (unsigned)k.framebytes + (unsigned)k.headerbytes +
+                                wis[s] * (unsigned)k.nrec * (CPLX ? 2u : 1u);
+      }
     }
+    // Wait for this FFT's loads here, once: left to its first use inside the freq and band
+    // loops, the wait would run in every band and, since the vector-memory counter also counts
+    // stores, wait for the previous band's spectra stores each time.
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+#pragma unroll
+      for (int s = 0; s < 8; ++s) asm volatile("" ::"v"(wv[u][s]));
+    }
+    asm volatile("" ::"v"(fd.a), "v"(fd.b), "v"(fd.c), "v"(fd.fse), "v"(fd.n0), "v"(fd.intdelay),
+                 "v"(fd.intwall), "v"(fd.fracwall));
 
     for (int f = 0; f < k.nrecfreqs; ++f) {
       const FreqConst rf = load_freq(dsp->rf[f]);
@@ -514,15 +570,24 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           }
         }
       };
+      // the fractional-sample phasors times the output scale `fs` (the 16-bit input scale, 1
+      // for FP32 spectra, 0 for a flagged FFT, whose spectra are zero: mode.cpp:656-669)
       auto make_frq = [&](int u, float2* frqk) {
         const int vt = tl + u * TPF;
-        const float2 fsub = sfrac[vt & (S - 1)];          // the same sub-phasor for every m
+        const float2 fsub0 = sfrac[vt & (S - 1)];         // the same sub-phasor for every m
+        const float2 fsub = make_float2(fsub0.x * fs, fsub0.y * fs);
         const int f0 = S + vt / S;
 #pragma unroll
         for (int m = 0; m < NKEEP; ++m) frqk[m] = cmul(fsub, sfrac[f0 + m * (NT / S)]);
       };
       float2 rot_h[HOIST_ROT ? 8 : 1], frq_h[HOIST ? NKEEP : 1];
-      if (HOIST_ROT) make_rot(0, rot_h);
+      if (HOIST_ROT) {
+        make_rot(0, rot_h);
+        // a sample of an invalid frame decodes to zero: a zero rotator gives x * 0 = 0
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          if (!((fok[0] >> s) & 1u)) rot_h[s] = make_float2(0.0f, 0.0f);
+      }
       if (HOIST) make_frq(0, frq_h);
       for (int kb = 0; kb < rf.nbands; ++kb) {
         const int band = dsp->rf[f].band[kb];
@@ -535,12 +600,13 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             // the sample width is tested once per band, not per sample, so the eight decodes
             // are straight-line code
             float x[8];
-            if (twobit) {                            // branch-free: the word is 0 when !ok
+            if (twobit) {                            // levels from the LDS table of the format
 #pragma unroll
               for (int s = 0; s < 8; ++s) {
                 const int sh = 2 * (int)(((qlo[u] >> (4 * s)) & 15u) + band);
-                const float lv = lut2s((wv[u][s] >> sh) & 3u, sbit);
-                x[s] = ((fok[u] >> s) & 1u) ? lv : 0.0f;
+                const float lv = slev2[(wv[u][s] >> sh) & 3u];
+                // invalid samples: zeroed through the rotator when it is kept per freq
+                x[s] = (HOIST_ROT && ORDER != 0) || ((fok[u] >> s) & 1u) ? lv : 0.0f;
               }
             } else {
 #pragma unroll
@@ -562,7 +628,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             const float2 rot = HOIST_ROT ? rot_h[s] : rot_l[HOIST_ROT ? 0 : s];
             float2 x = make_float2(0.f, 0.f);
             if ((fok[u] >> s) & 1u) {
-              const uint8_t* pp = data + wv[u][s] + 2 * band;
+              const GBytes pp = data + wv[u][s] + 2 * band;
               x = make_float2(slut8[pp[0]], slut8[pp[1]]);
             }
             v[u][s] = cmul(rot, x);
@@ -590,8 +656,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             const float2 frq = HOIST ? frq_h[m] : frq_l[HOIST ? 0 : m];
             float2 X = upper ? v[u][(m + 4) & 7] : v[u][m];
             if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped (mode.cpp:1199-1200)
-            const float2 F = cmul(X, frq);
-            Fo[u][m] = fok_all ? F : make_float2(0.0f, 0.0f);
+            Fo[u][m] = cmul(X, frq);
           }
         }
         if (!active) continue;
@@ -610,36 +675,28 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
         }
         auto store16 = [&](auto mode_c) {
           constexpr int OM = decltype(mode_c)::value;
-          const int xg = dsp->band_xfreq[band];
-          const int row = dsp->band_xrow[band];
-          const DevXGroup& xgr = args.xgroups[xg];
-          uint32_t* __restrict__ o16 = args.spec16 + xgr.spec_off + (long long)row * xgr.rowstride + lt * 8;
-          const float sc = args.xscale[xgr.scale_off + row];          // one scale per input row
+          const CDevXGroup xgr = ((CDevXGroup)args.xgroups) + dsp->band_xfreq[band];
+          uint32_t* __restrict__ o16 = args.spec16 + xgr->spec_off + (long long)dsp->band_xrow[band] * xgr->rowstride + lt * 8;
           const long long cs16 = (long long)(NT / 8) * cap * 8;
 #pragma unroll
           for (int u = 0; u < V; ++u) {
             const int vt = tl + u * TPF;
             uint32_t* __restrict__ o16t = o16 + (long long)(vt >> 3) * cap * 8 + (vt & 7);
 #pragma unroll
-            for (int m = 0; m < NKEEP; ++m)
-              o16t[m * cs16] = pack16m<OM>(make_float2(Fo[u][m].x * sc, Fo[u][m].y * sc));
+            for (int m = 0; m < NKEEP; ++m) o16t[m * cs16] = pack16m<OM>(Fo[u][m]);
           }
           // zoom windows of this band: the same channels again, as rows of the zoom freq's group
           for (int z = 0; z < dsp->nzout; ++z) {
             if (dsp->zout_band[z] != band) continue;
-            const DevXGroup& zg = args.xgroups[dsp->zout_group[z]];
-            const int zrow = dsp->zout_row[z];
             const int c0 = dsp->zout_c0[z], nz = dsp->zout_n[z];
-            uint32_t* __restrict__ oz = args.spec16 + zg.spec_off + (long long)zrow * zg.rowstride + lt * 8;
-            const float scz = args.xscale[zg.scale_off + zrow];
+            const CDevXGroup zg = ((CDevXGroup)args.xgroups) + dsp->zout_group[z];
+            uint32_t* __restrict__ oz = args.spec16 + zg->spec_off + (long long)dsp->zout_row[z] * zg->rowstride + lt * 8;
 #pragma unroll
             for (int u = 0; u < V; ++u) {
 #pragma unroll
               for (int m = 0; m < NKEEP; ++m) {
                 const int cz = tl + u * TPF + m * NT - c0;
-                if (cz >= 0 && cz < nz)
-                  oz[(cz >> 3) * cap * 8 + (cz & 7)] =
-                      pack16m<OM>(make_float2(Fo[u][m].x * scz, Fo[u][m].y * scz));
+                if (cz >= 0 && cz < nz) oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16m<OM>(Fo[u][m]);
               }
             }
           }
@@ -731,22 +788,26 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
     __syncthreads();                              // earlier users of the twiddle tables are done
     for (int m = tid; m < P::TWN; m += NTB) stw[m] = tw2048[m];
     for (int m = tid; m < 256; m += NTB) slut8[m] = lut8(m);    // 8-bit levels, once per workgroup
+    for (int m = tid; m < 8; m += NTB) slut8[256 + m] = lut2(m < 4 ? FXG_FMT_VDIF : FXG_FMT_MARK5B, m & 3);
   }
-  const DevDs* __restrict__ dsp = args.ds + d;
+  const CDevDs dsp = (CDevDs)(args.ds + d);
+  float* scf = slut8 + 264;                       // staged channel-frequency tables
+  stage_cf(args, dsp, scf, tid, NTB);
   DsConst k;
   k.C = dsp->C; k.S = dsp->S; k.NS = dsp->NS; k.NFS = dsp->NFS; k.nrec = dsp->nrecbands;
   k.nrecfreqs = dsp->nrecfreqs; k.fmt = dsp->format; k.spf = dsp->spf;
   k.framebytes = dsp->framebytes; k.headerbytes = dsp->headerbytes; k.nbits = dsp->nbits;
   k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us; k.dsb = dsp->dsb;
   const long long spec_off32 = dsp->spec_off;
-  const uint8_t* __restrict__ fv = args.fvalid + dsp->fvalid_off;
-  const float* __restrict__ cf = args.chanfreq + dsp->chanfreq_off;
-  const uint8_t* __restrict__ data = args.run[d].data;
+  const GBytes fv = (GBytes)(args.fvalid + dsp->fvalid_off);
+  const float* __restrict__ cf = scf;
+  const GBytes data = (GBytes)args.run[d].data;
   const int nframes = args.run[d].nframes;
   const FftDelay* __restrict__ params = args.params + (long long)d * args.chunkcap;
   const int C = k.C, S = k.S;
   const bool twobit = (k.nbits == 2);
-  const unsigned sbit = k.fmt == FXG_FMT_MARK5B ? 0u : 1u;   // sign bit of a 2-bit field
+  const float xsc = dsp->xsc;
+  const float* __restrict__ slev2 = slut8 + 256 + (k.fmt == FXG_FMT_MARK5B ? 4 : 0);   // 2-bit levels
   const int out_mode = args.out_mode;
   const int cap = args.chunkcap;
 
@@ -761,6 +822,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
       fd.weight = 0.0f; fd.intwall = 0; fd.fracwall = 0;
     }
     const bool fok_all = fd.ok != 0;               // flagged FFTs: zero spectra (mode.cpp:656-669)
+    const float fs = fok_all ? xsc : 0.0f;         // output scale of this FFT's spectra
     for (int f = 0; f < k.nrecfreqs; ++f) {
       const FreqConst rf = load_freq(dsp->rf[f]);
       if (rf.nbands == 0) continue;
@@ -812,8 +874,8 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
               if (ok) {
                 if (twobit) {
                   const unsigned q = within * (unsigned)k.nrec + (unsigned)band;
-                  const uint32_t w = *reinterpret_cast<const uint32_t*>(data + fbase + ((q >> 4) << 2));
-                  x = lut2s((w >> (2 * (q & 15u))) & 3u, sbit);
+                  const uint32_t w = *(GWords)(data + fbase + ((q >> 4) << 2));
+                  x = slev2[(w >> (2 * (q & 15u))) & 3u];
                 } else {
                   x = slut8[data[fbase + within * (unsigned)k.nrec + band]];
                 }
@@ -822,7 +884,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
             } else {
               float2 x = make_float2(0.f, 0.f);
               if (ok) {
-                const uint8_t* pp = data + fbase + (within * (unsigned)k.nrec + band) * 2u;
+                const GBytes pp = data + fbase + (within * (unsigned)k.nrec + band) * 2u;
                 x = make_float2(slut8[pp[0]], slut8[pp[1]]);
               }
               v[0][s] = cmul(rot, x);
@@ -843,14 +905,11 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
           constexpr int OM = decltype(mode_c)::value;   // 0: FP32, 1: FP16, 2: BF16
           float2* __restrict__ o32 = nullptr;
           uint32_t* __restrict__ o16 = nullptr;
-          float sc = 1.0f;
           if (OM == 0) {
             o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
           } else {
-            const DevXGroup& xgr = args.xgroups[dsp->band_xfreq[band]];
-            const int row = dsp->band_xrow[band];
-            o16 = args.spec16 + xgr.spec_off + (long long)row * xgr.rowstride + lt * 8;
-            sc = args.xscale[xgr.scale_off + row];                    // one scale per input row
+            const CDevXGroup xgr = ((CDevXGroup)args.xgroups) + dsp->band_xfreq[band];
+            o16 = args.spec16 + xgr->spec_off + (long long)dsp->band_xrow[band] * xgr->rowstride + lt * 8;
           }
 #pragma unroll 1
           for (int s = 0; s < 8; ++s) {
@@ -867,23 +926,22 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
               const int c = k1 + BM * j;
               float2 X = upper ? z[(j + R / 2) % R] : z[j];
               if (CPLX && rf.lsb && !k.dsb && c == 0) X.y = -X.y;   // DC is not flipped
-              const float2 frq = cmul(sfrac[c & (S - 1)], sfrac[S + c / S]);
-              float2 F = cmul(X, frq);
-              if (!fok_all) F = make_float2(0.0f, 0.0f);
+              // fractional-sample phasor times the output scale (0 for a flagged FFT)
+              const float2 fsub0 = sfrac[c & (S - 1)];
+              const float2 frq = cmul(make_float2(fsub0.x * fs, fsub0.y * fs), sfrac[S + c / S]);
+              const float2 F = cmul(X, frq);
               if (!active) continue;
               if (OM == 0) {
                 o32[c] = F;
               } else {
-                o16[(c >> 3) * cap * 8 + (c & 7)] = pack16m<OM>(make_float2(F.x * sc, F.y * sc));
+                o16[(c >> 3) * cap * 8 + (c & 7)] = pack16m<OM>(F);
                 for (int zz = 0; zz < dsp->nzout; ++zz) {          // zoom windows of this band
                   if (dsp->zout_band[zz] != band) continue;
                   const int cz = c - dsp->zout_c0[zz];
                   if (cz < 0 || cz >= dsp->zout_n[zz]) continue;
-                  const DevXGroup& zg = args.xgroups[dsp->zout_group[zz]];
-                  const int zrow = dsp->zout_row[zz];
-                  uint32_t* __restrict__ oz = args.spec16 + zg.spec_off + (long long)zrow * zg.rowstride + lt * 8;
-                  const float scz = args.xscale[zg.scale_off + zrow];
-                  oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16m<OM>(make_float2(F.x * scz, F.y * scz));
+                  const CDevXGroup zg = ((CDevXGroup)args.xgroups) + dsp->zout_group[zz];
+                  uint32_t* __restrict__ oz = args.spec16 + zg->spec_off + (long long)dsp->zout_row[zz] * zg->rowstride + lt * 8;
+                  oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16m<OM>(F);
                 }
               }
             }
@@ -903,7 +961,7 @@ template <int NTB>
 size_t fb_lds_bytes(int phcap) {
   constexpr int G = NTB / BNT;
   return sizeof(float2) * ((size_t)G * BM + Plan<BM>::TWN + 2 * (size_t)G * phcap) +
-         sizeof(float) * ((size_t)G * phcap + 256);
+         sizeof(float) * ((size_t)G * phcap + 264) + stage_lds_bytes(phcap);
 }
 
 }  // namespace

@@ -53,6 +53,7 @@ struct DevDs {
   int fvalid_off;         // offset of this datastream's frame-valid flags
   double sampletime_us;
   double bw_mhz;
+  float xsc;              // 16-bit spectra: this datastream's input scale (1 for FP32 spectra)
   DevRecFreq rf[32];
   int band_xfreq[64];     // MFMA: X freq group of each recorded band
   int band_xrow[64];      // MFMA: input row of each recorded band within its group
