@@ -16,8 +16,14 @@
 
 // One FFT length per launch: workgroup (x, y) processes FFTs x, x + gridDim.x, ... of datastream
 // dslist[y]; its scratch row is scratch[(y * gridDim.x + x) * L].
+// Waves per SIMD the register allocation is held to: 3 (at most 168 VGPRs, a few spilled)
+// measured faster than 2 (up to 256, no spills) on 16384- and 32768-point FFTs.
+#ifndef FXG_FB_MINWAVES
+#define FXG_FB_MINWAVES 3
+#endif
+
 template <int L, bool CPLX, int ORDER>
-__global__ void __launch_bounds__(BNT, 1) k_fbig(FengineArgs args, float2* __restrict__ scratch,
+__global__ void __launch_bounds__(BNT, FXG_FB_MINWAVES) k_fbig(FengineArgs args, float2* __restrict__ scratch,
                                                   const float2* __restrict__ twL) {
   extern __shared__ float2 fx_smem[];
   const int d = args.dslist[blockIdx.y];

@@ -727,6 +727,7 @@ namespace {
 
 constexpr int BM = 2048;          // sub-FFT length
 constexpr int BNT = BM / 8;       // threads of the workgroup
+constexpr int BTWL = FXG_MAX_BIG_L / 64 + 64;   // LDS entries of the W_L factor tables
 
 // 16-point forward DFT as 4 x 4: A[n2][k1] = DFT4 over n1 of x[4 n1 + n2], times
 // exp(-2 pi i n2 k1 / 16), then DFT4 over n2 gives X[k1 + 4 k2].
@@ -779,14 +780,17 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
   const int g = tid / BNT, t = tid - g * BNT;
   float2* sbuf = smem + g * BM;
   float2* stw = smem + G * BM;
-  float2* srot = stw + P::TWN + g * args.phcap;
-  float2* sfrac = stw + P::TWN + (G + g) * args.phcap;
-  float* squad = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + g * args.phcap;
-  float* slut8 = reinterpret_cast<float*>(stw + P::TWN + 2 * G * args.phcap) + G * args.phcap;
+  float2* stl = stw + P::TWN;                     // W_L^(64 jh), jh < L/64, then W_L^jl, jl < 64
+  float2* srot = stl + BTWL + g * args.phcap;
+  float2* sfrac = stl + BTWL + (G + g) * args.phcap;
+  float* squad = reinterpret_cast<float*>(stl + BTWL + 2 * G * args.phcap) + g * args.phcap;
+  float* slut8 = reinterpret_cast<float*>(stl + BTWL + 2 * G * args.phcap) + G * args.phcap;
   float2* __restrict__ scr = scr_base + (long long)g * L;
   if (load_tw) {
     __syncthreads();                              // earlier users of the twiddle tables are done
     for (int m = tid; m < P::TWN; m += NTB) stw[m] = tw2048[m];
+    for (int m = tid; m < L / 64; m += NTB) stl[m] = twL[64 * m];
+    for (int m = tid; m < 64; m += NTB) stl[L / 64 + m] = twL[m];
     for (int m = tid; m < 256; m += NTB) slut8[m] = lut8(m);    // 8-bit levels, once per workgroup
     for (int m = tid; m < 8; m += NTB) slut8[256 + m] = lut2(m < 4 ? FXG_FMT_VDIF : FXG_FMT_MARK5B, m & 3);
   }
@@ -807,6 +811,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
   const int C = k.C, S = k.S;
   const bool twobit = (k.nbits == 2);
   const float xsc = dsp->xsc;
+  const double inv_spf = 1.0 / (double)k.spf;
   const float* __restrict__ slev2 = slut8 + 256 + (k.fmt == FXG_FMT_MARK5B ? 4 : 0);   // 2-bit levels
   const int out_mode = args.out_mode;
   const int cap = args.chunkcap;
@@ -832,6 +837,29 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
       __syncthreads();
       for (int kb = 0; kb < rf.nbands; ++kb) {
         const int band = dsp->rf[f].band[kb];
+        // 2-bit real data of an FFT whose samples are all valid (weight 1: every frame of the
+        // window present and valid, mode.cpp:656-669 and the good-sample count of
+        // mk5mode.cpp:94-174): the words of sub-FFT r + 1 are loaded before the scratch stores
+        // of sub-FFT r, so their wait does not also wait for those stores.  Other FFTs decode
+        // with per-sample checks.
+        const bool fast = !CPLX && twobit && fok_all && fd.weight == 1.0f;
+        uint32_t wcur[8], wnxt[8];
+        unsigned qcur = 0u, qnxt = 0u;
+        auto fetch = [&](int r, uint32_t* w8, unsigned& qb) {
+          const unsigned g0 = (unsigned)fd.n0 + (unsigned)(R * t + r);
+          qb = 0u;
+#pragma unroll
+          for (int s = 0; s < 8; ++s) {
+            const unsigned gs = g0 + (unsigned)(s * R * BNT);
+            unsigned fr = (unsigned)((double)gs * inv_spf);     // exact: see fe_range
+            unsigned within = gs - fr * (unsigned)k.spf;
+            if (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
+            const unsigned q = within * (unsigned)k.nrec + (unsigned)band;
+            qb |= (q & 15u) << (4 * s);
+            w8[s] = *(GWords)(data + fr * (unsigned)k.framebytes + (unsigned)k.headerbytes + ((q >> 4) << 2));
+          }
+        };
+        if (fast) fetch(0, wcur, qcur);
         for (int r = 0; r < R; ++r) {
           // samples n = R*(t + 256 s) + r of the FFT window
           float2 v[1][8];
@@ -846,57 +874,73 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
             sincosf((float)(R * BNT / S) * th, &sn, &cs);
             qstep = make_float2(cs, sn);
           }
-          const unsigned g0 = (unsigned)fd.n0 + (unsigned)nb0;
-          unsigned fr = g0 / (unsigned)k.spf;
-          unsigned within = g0 - fr * (unsigned)k.spf;
-#pragma unroll
-          for (int s = 0; s < 8; ++s) {
-            if (s > 0) {
-              within += R * BNT;
-              while (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
-            }
-            const int n = nb0 + s * R * BNT;
-            const int st = n / S;
-            float2 rot;
-            if (ORDER == 1) {                    // step 0's phasor is (1, -0): no branch
-              rot = cmul(sub, srot[S + st]);
-            } else if (ORDER == 2) {
+          // the rotator of sample s (in increasing s: order 2 steps its quadratic term)
+          auto rot_next = [&](int s) {
+            const int st = (nb0 + s * R * BNT) / S;
+            if (ORDER == 1) return cmul(sub, srot[S + st]);   // step 0's phasor is (1, -0)
+            if (ORDER == 2) {
               const float2 pq = cmul(sub, qpow);
-              rot = cmul(pq, srot[S + st]);
               qpow = cmul(qpow, qstep);
-            } else {
-              rot = sub;
+              return cmul(pq, srot[S + st]);
             }
-            const bool ok = fok_all && ((int)fr < nframes) && fv[fr];
-            const unsigned fbase = fr * (unsigned)k.framebytes + (unsigned)k.headerbytes;
-            if (!CPLX) {
-              float x = 0.0f;
-              if (ok) {
-                if (twobit) {
-                  const unsigned q = within * (unsigned)k.nrec + (unsigned)band;
-                  const uint32_t w = *(GWords)(data + fbase + ((q >> 4) << 2));
-                  x = slev2[(w >> (2 * (q & 15u))) & 3u];
-                } else {
-                  x = slut8[data[fbase + within * (unsigned)k.nrec + band]];
-                }
-              }
+            return sub;
+          };
+          if (fast) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+              const float2 rot = rot_next(s);
+              const float x = slev2[(wcur[s] >> (2 * ((qcur >> (4 * s)) & 15u))) & 3u];
               v[0][s] = ORDER == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
-            } else {
-              float2 x = make_float2(0.f, 0.f);
-              if (ok) {
-                const GBytes pp = data + fbase + (within * (unsigned)k.nrec + band) * 2u;
-                x = make_float2(slut8[pp[0]], slut8[pp[1]]);
+            }
+          } else {
+            const unsigned g0 = (unsigned)fd.n0 + (unsigned)nb0;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+              const unsigned gs = g0 + (unsigned)(s * R * BNT);
+              unsigned fr = (unsigned)((double)gs * inv_spf);
+              unsigned within = gs - fr * (unsigned)k.spf;
+              if (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
+              const bool ok = fok_all && ((int)fr < nframes) && fv[fr];
+              const unsigned fbase = fr * (unsigned)k.framebytes + (unsigned)k.headerbytes;
+              const float2 rot = rot_next(s);
+              if (!CPLX) {
+                float x = 0.0f;
+                if (ok) {
+                  if (twobit) {
+                    const unsigned q = within * (unsigned)k.nrec + (unsigned)band;
+                    const uint32_t w = *(GWords)(data + fbase + ((q >> 4) << 2));
+                    x = slev2[(w >> (2 * (q & 15u))) & 3u];
+                  } else {
+                    x = slut8[data[fbase + within * (unsigned)k.nrec + band]];
+                  }
+                }
+                v[0][s] = ORDER == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
+              } else {
+                float2 x = make_float2(0.f, 0.f);
+                if (ok) {
+                  const GBytes pp = data + fbase + (within * (unsigned)k.nrec + band) * 2u;
+                  x = make_float2(slut8[pp[0]], slut8[pp[1]]);
+                }
+                v[0][s] = cmul(rot, x);
+                if (rf.lsb) v[0][s].y = -v[0][s].y;   // conj input (see fe_range)
               }
-              v[0][s] = cmul(rot, x);
-              if (rf.lsb) v[0][s].y = -v[0][s].y;   // conj input (see fe_range)
             }
           }
           __syncthreads();                        // sbuf free
           fft_stockham<BM, 1, BNT>(v, sbuf, stw, t);
+          // Z_r[k1] = Y_r[k1] W_L^(r k1), the twiddle as W_L^(64 jh) W_L^jl with r k1 = 64 jh + jl
 #pragma unroll
-          for (int s = 0; s < 8; ++s) {           // Z_r[k1] = Y_r[k1] W_L^(r k1)
-            const int k1 = t + s * BNT;
-            scr[r * BM + k1] = cmul(v[0][s], twL[r * k1]);
+          for (int s = 0; s < 8; ++s) {
+            const int j = r * (t + s * BNT);
+            v[0][s] = cmul(v[0][s], cmul(stl[j >> 6], stl[L / 64 + (j & 63)]));
+          }
+          if (fast && r + 1 < R) fetch(r + 1, wnxt, qnxt);
+#pragma unroll
+          for (int s = 0; s < 8; ++s) scr[r * BM + t + s * BNT] = v[0][s];
+          if (fast) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) wcur[s] = wnxt[s];
+            qcur = qnxt;
           }
         }
         __syncthreads();                          // every Z_r of this band is in the scratch row
@@ -911,12 +955,12 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
             const CDevXGroup xgr = ((CDevXGroup)args.xgroups) + dsp->band_xfreq[band];
             o16 = args.spec16 + xgr->spec_off + (long long)dsp->band_xrow[band] * xgr->rowstride + lt * 8;
           }
-#pragma unroll 1
-          for (int s = 0; s < 8; ++s) {
-            const int k1 = t + s * BNT;
-            float2 z[R];
+          auto load_z = [&](float2* z, int s) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) z[r] = scr[r * BM + k1];
+            for (int r = 0; r < R; ++r) z[r] = scr[r * BM + t + s * BNT];
+          };
+          auto emit = [&](float2* z, int s) {
+            const int k1 = t + s * BNT;
             dftR<R>(z);                           // z[k2] = X[k1 + 2048 k2]
 #pragma unroll
             for (int j = 0; j < KEEP; ++j) {
@@ -945,6 +989,14 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
                 }
               }
             }
+          };
+          // (a second register buffer that loads block s + 1 before the stores of block s
+          // measured no faster: it costs the third wave per SIMD)
+#pragma unroll 1
+          for (int s = 0; s < 8; ++s) {
+            float2 z[R];
+            load_z(z, s);
+            emit(z, s);
           }
         };
         if (out_mode == 0) finish(IntC<0>());
@@ -960,7 +1012,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
 template <int NTB>
 size_t fb_lds_bytes(int phcap) {
   constexpr int G = NTB / BNT;
-  return sizeof(float2) * ((size_t)G * BM + Plan<BM>::TWN + 2 * (size_t)G * phcap) +
+  return sizeof(float2) * ((size_t)G * BM + Plan<BM>::TWN + BTWL + 2 * (size_t)G * phcap) +
          sizeof(float) * ((size_t)G * phcap + 264) + stage_lds_bytes(phcap);
 }
 
