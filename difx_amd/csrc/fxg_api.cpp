@@ -267,6 +267,8 @@ int validate_and_derive(fxg_ctx* ctx) {
       return fail(ctx, FXG_E_INVALID, "frame size inconsistent with the sample format");
     if ((long long)pay * 8 / bits > (1LL << 20))      // the F-engine's frame index arithmetic
       return fail(ctx, FXG_E_INVALID, "more than 2^20 samples per frame");
+    if (L > FXG_MAX_L && pay * 8 / bits < L / 2048)   // four-step: R consecutive samples cross
+      return fail(ctx, FXG_E_INVALID, "frames too short for the FFT length");   // one frame edge at most
     int nb_per_freq[FXG_MAX_FREQS] = {0};
     for (int b = 0; b < ds.nrecbands; ++b) {
       int lf = ds.recband_localfreq[b];

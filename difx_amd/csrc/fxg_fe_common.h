@@ -839,27 +839,37 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
         const int band = dsp->rf[f].band[kb];
         // 2-bit real data of an FFT whose samples are all valid (weight 1: every frame of the
         // window present and valid, mode.cpp:656-669 and the good-sample count of
-        // mk5mode.cpp:94-174): the words of sub-FFT r + 1 are loaded before the scratch stores
-        // of sub-FFT r, so their wait does not also wait for those stores.  Other FFTs decode
-        // with per-sample checks.
+        // mk5mode.cpp:94-174): each thread first gathers this band's fields of its R
+        // consecutive samples R*m .. R*m + R - 1 of each of its eight m = t + 256 s into one
+        // 2R-bit word.  The R loads of a word are issued back to back and neighbouring lanes
+        // read neighbouring bytes, so the loads hit in L1; the sub-FFT loop then decodes from
+        // registers.  Other FFTs decode with per-sample checks.
         const bool fast = !CPLX && twobit && fok_all && fd.weight == 1.0f;
-        uint32_t wcur[8], wnxt[8];
-        unsigned qcur = 0u, qnxt = 0u;
-        auto fetch = [&](int r, uint32_t* w8, unsigned& qb) {
-          const unsigned g0 = (unsigned)fd.n0 + (unsigned)(R * t + r);
-          qb = 0u;
+        uint32_t pk[8];
+        if (fast) {
+          // the window start, opaque to the compiler in each band: otherwise it keeps the 8 R
+          // band-independent address terms live across the band loop (and spills them)
+          unsigned n0b = (unsigned)fd.n0;
+          asm volatile("" : "+v"(n0b));
 #pragma unroll
           for (int s = 0; s < 8; ++s) {
-            const unsigned gs = g0 + (unsigned)(s * R * BNT);
-            unsigned fr = (unsigned)((double)gs * inv_spf);     // exact: see fe_range
-            unsigned within = gs - fr * (unsigned)k.spf;
-            if (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
-            const unsigned q = within * (unsigned)k.nrec + (unsigned)band;
-            qb |= (q & 15u) << (4 * s);
-            w8[s] = *(GWords)(data + fr * (unsigned)k.framebytes + (unsigned)k.headerbytes + ((q >> 4) << 2));
+            const unsigned gs0 = n0b + (unsigned)(R * (t + s * BNT));
+            unsigned fr0 = (unsigned)((double)gs0 * inv_spf);     // exact: see fe_range
+            unsigned w0 = gs0 - fr0 * (unsigned)k.spf;
+            if (w0 >= (unsigned)k.spf) { w0 -= k.spf; ++fr0; }
+            uint32_t acc = 0u;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+              unsigned wi = w0 + i, fi = fr0;
+              if (wi >= (unsigned)k.spf) { wi -= k.spf; ++fi; }   // at most one frame edge (spf >= R)
+              const unsigned q = wi * (unsigned)k.nrec + (unsigned)band;
+              const uint32_t w = *(GWords)(data + fi * (unsigned)k.framebytes + (unsigned)k.headerbytes + ((q >> 4) << 2));
+              acc |= ((w >> (2 * (q & 15u))) & 3u) << (2 * i);
+            }
+            pk[s] = acc;
+            __builtin_amdgcn_sched_barrier(0);    // R loads in flight at a time, not 8 R
           }
-        };
-        if (fast) fetch(0, wcur, qcur);
+        }
         for (int r = 0; r < R; ++r) {
           // samples n = R*(t + 256 s) + r of the FFT window
           float2 v[1][8];
@@ -889,7 +899,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
               const float2 rot = rot_next(s);
-              const float x = slev2[(wcur[s] >> (2 * ((qcur >> (4 * s)) & 15u))) & 3u];
+              const float x = slev2[(pk[s] >> (2 * r)) & 3u];
               v[0][s] = ORDER == 0 ? make_float2(x, 0.0f) : make_float2(x * rot.x, x * rot.y);
             }
           } else {
@@ -934,14 +944,8 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
             const int j = r * (t + s * BNT);
             v[0][s] = cmul(v[0][s], cmul(stl[j >> 6], stl[L / 64 + (j & 63)]));
           }
-          if (fast && r + 1 < R) fetch(r + 1, wnxt, qnxt);
 #pragma unroll
           for (int s = 0; s < 8; ++s) scr[r * BM + t + s * BNT] = v[0][s];
-          if (fast) {
-#pragma unroll
-            for (int s = 0; s < 8; ++s) wcur[s] = wnxt[s];
-            qcur = qnxt;
-          }
         }
         __syncthreads();                          // every Z_r of this band is in the scratch row
         // the second step and the stores; the output form is tested once per band
