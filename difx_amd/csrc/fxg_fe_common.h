@@ -81,7 +81,7 @@ __device__ __forceinline__ int padx(int e) { return e ^ ((e >> 3) & 31); }
 // Which per-freq quantities fe_range keeps in registers across the bands of a freq (bit 0:
 // the rotator of the thread's samples, bit 1: the fractional-sample phasors of its channels).
 #ifndef FXG_F_HOIST
-#define FXG_F_HOIST 0
+#define FXG_F_HOIST 3
 #endif
 
 template <int L>

@@ -122,27 +122,30 @@ __global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
     const int c = c0 + wave * CPW + cw;
     const float* accp = a.acc + g.acc_off + (long long)c * NPAIR * 2 * 1024;
 #pragma unroll
-    for (int p = 0; p < NPAIR; ++p)
+    for (int p = 0; p < NPAIR; ++p) {
+      const float* pr = accp + (p * 2 + 0) * 1024 + lane;    // immediate offsets r * 256 bytes
+      const float* pi = accp + (p * 2 + 1) * 1024 + lane;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        accr[cw][p][r] = accp[(p * 2 + 0) * 1024 + r * 64 + lane];
-        acci[cw][p][r] = accp[(p * 2 + 1) * 1024 + r * 64 + lane];
+        accr[cw][p][r] = pr[r * 64];
+        acci[cw][p][r] = pi[r * 64];
       }
+    }
   }
 
   // chunk q of a k-step: row r = q / 16, FFT t = (q % 16) / 2, channel quad h = q % 2
-  uint4 buf[QMAX];
-  auto gload = [&](int k) {
+  // The loads are unconditional (a chunk past the group's inputs re-reads row 0 and is not
+  // stored), so the compiler can count them exactly in its waits.
+  auto gload = [&](int k, uint4* b) {
 #pragma unroll
     for (int i = 0; i < QMAX; ++i) {
-      const int q = tid + NTH * i;
-      if (q < nq) {
-        const int r = q >> 4, rem = q & 15;
-        buf[i] = *reinterpret_cast<const uint4*>(gbase + (long long)r * g.rowstride + (long long)(8 * k) * 8 + rem * 4);
-      }
+      const int q0 = tid + NTH * i;
+      const int q = q0 < nq ? q0 : (q0 & 15);
+      const int r = q >> 4, rem = q & 15;
+      b[i] = *reinterpret_cast<const uint4*>(gbase + (long long)r * g.rowstride + (long long)(8 * k) * 8 + rem * 4);
     }
   };
-  auto lstore = [&](uint32_t* L) {
+  auto lstore = [&](uint32_t* L, const uint4* b) {
 #pragma unroll
     for (int i = 0; i < QMAX; ++i) {
       const int q = tid + NTH * i;
@@ -150,42 +153,48 @@ __global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
         const int r = q >> 4, rem = q & 15;
         const int t = rem >> 1, h = rem & 1;
         uint32_t* d = L + (4 * h) * CSTR + r * 8 + t;
-        d[0] = buf[i].x; d[CSTR] = buf[i].y; d[2 * CSTR] = buf[i].z; d[3 * CSTR] = buf[i].w;
+        d[0] = b[i].x; d[CSTR] = b[i].y; d[2 * CSTR] = b[i].z; d[3 * CSTR] = b[i].w;
       }
     }
   };
+  auto mfma_step = [&](const uint32_t* L) {
+#pragma unroll
+      for (int cw = 0; cw < CPW; ++cw) {
+        const uint32_t* cb = L + (wave * CPW + cw) * CSTR + 4 * half;
+        uint4 x[NTILE];
+  #pragma unroll
+        for (int ti = 0; ti < NTILE; ++ti) x[ti] = *reinterpret_cast<const uint4*>(cb + (row + 32 * ti) * 8);
+        int pi = 0;
+  #pragma unroll
+        for (int ti = 0; ti < NTILE; ++ti) {
+          const uint4 xr = rot_negi(x[ti]);
+  #pragma unroll
+          for (int tj = 0; tj <= ti; ++tj) {
+            const V8 av = __builtin_bit_cast(V8, x[ti]);
+            const V8 bv = __builtin_bit_cast(V8, x[tj]);
+            const V8 arv = __builtin_bit_cast(V8, xr);
+            accr[cw][pi] = mfma32<V8>(av, bv, accr[cw][pi]);
+            acci[cw][pi] = mfma32<V8>(arv, bv, acci[cw][pi]);
+            ++pi;
+          }
+        }
+      }
+  };
 
   const int nk = a.nffts_pad / 8;
+  // one k-step of global loads in flight over a double-buffered LDS stage (two k-steps ahead
+  // in registers measured slower)
+  uint4 buf[QMAX];
   __syncthreads();
-  gload(0);
-  lstore(lds0);
+  gload(0, buf);
+  lstore(lds0, buf);
   __syncthreads();
   for (int k = 0; k < nk; ++k) {
     uint32_t* cur = (k & 1) ? lds1 : lds0;
     uint32_t* nxt = (k & 1) ? lds0 : lds1;
-    if (k + 1 < nk) gload(k + 1);
-#pragma unroll
-    for (int cw = 0; cw < CPW; ++cw) {
-      const uint32_t* cb = cur + (wave * CPW + cw) * CSTR + 4 * half;
-      uint4 x[NTILE];
-#pragma unroll
-      for (int ti = 0; ti < NTILE; ++ti) x[ti] = *reinterpret_cast<const uint4*>(cb + (row + 32 * ti) * 8);
-      int pi = 0;
-#pragma unroll
-      for (int ti = 0; ti < NTILE; ++ti) {
-        const uint4 xr = rot_negi(x[ti]);
-#pragma unroll
-        for (int tj = 0; tj <= ti; ++tj) {
-          const V8 av = __builtin_bit_cast(V8, x[ti]);
-          const V8 bv = __builtin_bit_cast(V8, x[tj]);
-          const V8 arv = __builtin_bit_cast(V8, xr);
-          accr[cw][pi] = mfma32<V8>(av, bv, accr[cw][pi]);
-          acci[cw][pi] = mfma32<V8>(arv, bv, acci[cw][pi]);
-          ++pi;
-        }
-      }
-    }
-    if (k + 1 < nk) lstore(nxt);
+    gload(min(k + 1, nk - 1), buf);               // past the end: re-read, unused
+    mfma_step(cur);
+    if (k + 1 < nk) lstore(nxt, buf);
     __syncthreads();
   }
 
@@ -194,12 +203,15 @@ __global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
     const int c = c0 + wave * CPW + cw;
     float* accp = a.acc + g.acc_off + (long long)c * NPAIR * 2 * 1024;
 #pragma unroll
-    for (int p = 0; p < NPAIR; ++p)
+    for (int p = 0; p < NPAIR; ++p) {
+      float* pr = accp + (p * 2 + 0) * 1024 + lane;
+      float* pi = accp + (p * 2 + 1) * 1024 + lane;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        accp[(p * 2 + 0) * 1024 + r * 64 + lane] = accr[cw][p][r];
-        accp[(p * 2 + 1) * 1024 + r * 64 + lane] = acci[cw][p][r];
+        pr[r * 64] = accr[cw][p][r];
+        pi[r * 64] = acci[cw][p][r];
       }
+    }
   }
 }
 
