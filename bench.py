@@ -466,7 +466,7 @@ def main():
                     "other_ms_per_step": tm["other_ms"] / args.steps,
                     "f_launches_per_step": nchunks_per_step,
                     "x_engine_useful_tflops": x_engine_flops(cfg, B) / (tm["x_ms"] / args.steps / 1e3) / 1e12},
-        "roofline": {"kernel": "k_fengine<2048,false> (fused unpack+rotate+FFT+post-F)",
+        "roofline": {"kernel": "k_fengine<2048,false,1,256> (fused unpack+rotate+FFT+post-F)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(ROOT / "profiles" / "k_fengine_2048_traffic.txt", ffts_per_launch),
