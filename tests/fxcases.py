@@ -108,12 +108,15 @@ def damaged_slot(cfg, offsets=(0, 1, 0), seed: int = 12):
     ds = cfg.datastreams[1]
     buf = data[1].copy()
     fb, hb = ds.framebytes, ds.headerbytes
+    nfr = len(buf) // fb
+    # frames 2 and 5, or a third and two thirds into a shorter buffer
+    f_inv, f_fill = (2, 5) if nfr >= 6 else (nfr // 3, (2 * nfr) // 3)
     if ds.format.upper() == "VDIF":
-        buf[2 * fb + 3] |= 0x80                          # invalid bit of frame 2
+        buf[f_inv * fb + 3] |= 0x80                      # invalid bit
     else:
-        buf[2 * fb + 5] |= 0x80
+        buf[f_inv * fb + 5] |= 0x80
     fill = np.frombuffer(np.uint64(0x1122334411223344).tobytes(), np.uint8)
-    buf[5 * fb + hb: 5 * fb + hb + 8] = fill             # fill pattern opens frame 5
+    buf[f_fill * fb + hb: f_fill * fb + hb + 8] = fill    # fill pattern opens the frame
     data[1] = buf
     return data, ctrl
 

@@ -29,3 +29,16 @@ def test_slots_fit_and_cover(name):
         assert len(data[d]) <= cap, (name, d, len(data[d]), cap)
         st = fo.process_station(cfg, d, offsets, data[d], ctrl[d], B)
         assert np.all(st.weight > 0), (name, d, np.flatnonzero(st.weight == 0)[:8])
+
+
+def test_damaged_fourstep_slot_takes_both_decode_paths():
+    """The damaged slot of the four-step GPU case holds FFTs with every sample valid (weight 1,
+    the gathered decode) and FFTs with some invalid samples (0 < weight < 1, the per-sample
+    decode), beside flagged ones (weight 0)."""
+    cfg = fc.case_fourstep_ffts(B=4)
+    offsets = [0, 1, 0]
+    data, ctrl = fc.damaged_slot(cfg, offsets)
+    B = cfg.blockspersend()
+    w = np.concatenate([fo.process_station(cfg, d, offsets, data[d], ctrl[d], B).weight
+                        for d in range(len(cfg.datastreams))])
+    assert (w == 1).sum() > 0 and ((w > 0) & (w < 1)).sum() > 0 and (w == 0).sum() > 0

@@ -272,12 +272,14 @@ def test_autotune_picks_a_form(gpu):
 
 
 @pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
-@pytest.mark.parametrize("kind", ["real", "complex"])
+@pytest.mark.parametrize("kind", ["real", "complex", "real_damaged"])
 def test_fourstep_ffts(gpu, kind, xe, bound):
     """FFT lengths 16384 and 32768 (four-step F-engine: 2048-point LDS sub-FFTs, twiddles,
-    R-point DFTs) with zoom windows; a 16384-point complex FFT."""
-    cfg = case_fourstep_ffts(B=4, xengine=xe) if kind == "real" else case_complex_fourstep(B=4, xengine=xe)
-    got, ref, e, lay = _run_and_compare(cfg)
+    R-point DFTs) with zoom windows; a 16384-point complex FFT; flagged FFTs, an invalid frame
+    and a fill-pattern frame (FFTs with invalid samples take the per-sample decode, the others
+    the gathered one)."""
+    cfg = case_complex_fourstep(B=4, xengine=xe) if kind == "complex" else case_fourstep_ffts(B=4, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg, damaged=(kind == "real_damaged"))
     _say(f"four-step {kind} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
     assert max(e["cross"]) < bound and max(e["auto"]) < bound
 
