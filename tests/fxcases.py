@@ -145,3 +145,23 @@ def record_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
         if den > 0:
             out["auto"].append(float(np.mean(np.abs(g - r)) / den))
     return out
+
+
+def channel_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
+    """Per-channel errors of every cross record of a configuration whose datastreams hold one
+    recorded band each (cfg2): |V_got - V_ref| / sqrt(A_1 A_2) with A the reference
+    autocorrelations of the two inputs, and the phase difference arg(V_got conj(V_ref)) in
+    degrees.  Returns the two flattened arrays."""
+    norm, dphi = [], []
+    for (fi, j), off in lay.baseline_off.items():
+        bl = cfg.baselines[j]
+        fr = cfg.freqs[fi]
+        assert fr.chanavg == 1
+        n = fr.nchan
+        a1 = ref[lay.autocorr_off[bl.ds1]: lay.autocorr_off[bl.ds1] + n].real
+        a2 = ref[lay.autocorr_off[bl.ds2]: lay.autocorr_off[bl.ds2] + n].real
+        assert len(cfg.datastreams[bl.ds1].recbands) == 1 and len(cfg.datastreams[bl.ds2].recbands) == 1
+        g, r = got[off: off + n], ref[off: off + n]
+        norm.append(np.abs(g - r) / np.sqrt(a1 * a2))
+        dphi.append(np.degrees(np.abs(np.angle(g * np.conj(r)))))
+    return np.concatenate(norm), np.concatenate(dphi)

@@ -13,7 +13,7 @@ import fxoracle as fo
 from difx_amd import synth
 from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_complex_fourstep,
                      case_fourstep_ffts, case_long_ffts, case_mark5b, case_offsets, case_real8,
-                     damaged_slot, record_errors, slot)
+                     damaged_slot, record_errors, slot, channel_errors)
 
 pytestmark = pytest.mark.gpu
 
@@ -159,10 +159,15 @@ def test_mfma_cfg1(gpu, xe, bound):
 
 @pytest.mark.parametrize("xe,bound", [("fp16", 1e-3), ("bf16", 1e-2)])
 def test_mfma_cfg2(gpu, xe, bound):
+    """64 stations, complex 8-bit: record errors, and per-channel amplitude and phase errors
+    |dV| / sqrt(A_i A_j) and arg(V conj(V_ref)) (median and max over every channel of every
+    baseline)."""
     cfg = case_cfg2(B=16, xengine=xe)
     got, ref, e, lay = _run_and_compare(cfg)
+    norm, dphi = channel_errors(cfg, lay, got.astype(np.complex128), ref)
     _say(f"cfg2 {xe} cross max", max(e["cross"]), "median", float(np.median(e["cross"])),
-         "auto max", max(e["auto"]))
+         "auto max", max(e["auto"]), "|dV|/sqrt(AiAj) median", float(np.median(norm)), "max",
+         float(np.max(norm)), "dphase deg median", float(np.median(dphi)), "max", float(np.max(dphi)))
     assert max(e["cross"]) < bound and max(e["auto"]) < bound
 
 

@@ -4,7 +4,7 @@ import pytest
 
 import fxoracle as fo
 from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_fourstep_ffts, case_long_ffts, case_mark5b,
-                     case_offsets, case_real8, damaged_slot, record_errors, slot)
+                     case_offsets, case_real8, damaged_slot, record_errors, slot, channel_errors)
 
 
 def _port():
@@ -28,6 +28,26 @@ def test_port_matches_fp64_oracle(name):
     e = record_errors(cfg, fo.result_layout(cfg), got.astype(np.complex128), ref)
     assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
     np.testing.assert_array_equal(w, np.array([s.weight for s in st], dtype=np.float32))
+
+
+def test_channel_errors_of_the_port():
+    """The per-channel error measure of the cfg2 GPU test, on the FP32 C restatement: zero
+    for the oracle itself, FP32-sized for the port, and it sees a planted 1e-3 error."""
+    fxport = _port()
+    cfg = case_cfg2(B=8, xengine="fp32", nstations=4)
+    lay = fo.result_layout(cfg)
+    data, ctrl = slot(cfg)
+    got, _ = fxport.process_subint(cfg, [0, 1, 0], data, ctrl, nthreads=2)
+    ref, _ = fo.process_subint(cfg, [0, 1, 0], data, ctrl)
+    n0, p0 = channel_errors(cfg, lay, ref, ref)
+    assert np.max(n0) == 0 and np.max(p0) < 1e-12
+    n1, p1 = channel_errors(cfg, lay, got.astype(np.complex128), ref)
+    assert np.max(n1) < 1e-6 and np.median(p1) < 1e-2
+    bad = ref.copy()
+    off = next(iter(lay.baseline_off.values()))
+    bad[off: off + 8] *= 1.001
+    n2, _ = channel_errors(cfg, lay, bad, ref)
+    assert np.max(n2) > 0
 
 
 def test_port_thread_count_invariance():
