@@ -101,6 +101,8 @@ struct fxg_ctx {
   std::vector<std::pair<std::string, double>> tune_log;   // candidate, ms per subint (< 0: failed)
   std::vector<FItem> items[2];   // persistent work items: [0] real, [1] complex sampling
   FItem* d_items[2] = {nullptr, nullptr};
+  std::vector<IClass> iclass[2];  // their FFT lengths, longest first, with item ranges
+  IClass* d_iclass[2] = {nullptr, nullptr};
   int* d_counter = nullptr;
   int ncu = 256;
   std::vector<long long> databytes;
@@ -956,7 +958,16 @@ int build_items(fxg_ctx* ctx) {
     std::vector<const fxg_ctx::FClass*> cls;
     for (const auto& fc : ctx->fclasses) if (fc.cplx == kind) cls.push_back(&fc);
     std::sort(cls.begin(), cls.end(), [](const fxg_ctx::FClass* a, const fxg_ctx::FClass* b) { return a->L > b->L; });
-    for (const auto* fc : cls)
+    ctx->iclass[kind].clear();
+    for (const auto* fc : cls) {
+      if (ctx->iclass[kind].empty() || ctx->iclass[kind].back().L != fc->L) {
+        IClass ic;
+        ic.L = fc->L;
+        ic.log2L = log2i(fc->L);
+        ic.begin = (int)ctx->items[kind].size();
+        ic.count = 0;
+        ctx->iclass[kind].push_back(ic);
+      }
       for (int d : fc->dsl)
         for (int pt = 0; pt < parts; ++pt) {
           FItem it;
@@ -966,8 +977,12 @@ int build_items(fxg_ctx* ctx) {
           it.part = pt;
           it.nparts = parts;
           ctx->items[kind].push_back(it);
+          ctx->iclass[kind].back().count++;
         }
+    }
+    if ((int)ctx->iclass[kind].size() > FXG_MAX_ICLASS) return fail(ctx, FXG_E_INVALID, "too many FFT lengths");
     if (int rc = dev_upload(ctx, &ctx->d_items[kind], ctx->items[kind])) return rc;
+    if (int rc = dev_upload(ctx, &ctx->d_iclass[kind], ctx->iclass[kind])) return rc;
   }
   // four-step classes: scratch rows of L cf32 per workgroup, shared by the classes (launched
   // one after another on the context's stream)
@@ -985,7 +1000,7 @@ int build_items(fxg_ctx* ctx) {
   ctx->bigL = bigL;
   if (rows > 0)
     if (int rc = dev_malloc(ctx, (void**)&ctx->d_bigscratch, (long long)sizeof(float2) * rows)) return rc;
-  return dev_malloc(ctx, (void**)&ctx->d_counter, sizeof(int));
+  return dev_malloc(ctx, (void**)&ctx->d_counter, sizeof(int) * FXG_MAX_ICLASS);
 }
 
 // The F-engine of one chunk in the selected launch form.
@@ -1017,7 +1032,7 @@ int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
       }
       fa.bigscratch = ctx->d_bigscratch;
       fa.bigL = ctx->bigL;
-      const int ntb = ctx->fsel.ntb ? ctx->fsel.ntb : 256;
+      const int ntb = ctx->fsel.ntb ? ctx->fsel.ntb : 512;      // 512 measured faster on uneven bands
       size_t lds = 0;
       for (const auto& fc : ctx->fclasses)
         if (fc.cplx == kind && !fc.big) lds = std::max(lds, fxg_fengine_lds(fc.L, ntb, phmax, ctx->rotorder));
@@ -1025,7 +1040,8 @@ int launch_fengines(fxg_ctx* ctx, FengineArgs& fa, hipStream_t st) {
       fa.phcap = phmax;
       const int nitems = (int)ctx->items[kind].size();
       const int nwg = std::max(1, std::min(nitems, ctx->ncu * ctx->fsel.grid_mult));
-      if (fxg_launch_fpersist(fa, ctx->d_items[kind], nitems, ctx->d_counter, nwg, ntb, kind, lds, st))
+      if (fxg_launch_fpersist(fa, ctx->d_items[kind], ctx->d_iclass[kind], (int)ctx->iclass[kind].size(),
+                              ctx->d_counter, nwg, ntb, kind, lds, st))
         return fail(ctx, FXG_E_HIP, "persistent F-engine launch");
     }
     return FXG_OK;
@@ -1272,6 +1288,7 @@ void free_ctx(fxg_ctx* ctx) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
   f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid);
   f(ctx->d_twiddle); f(ctx->d_dslist); f(ctx->d_params); f(ctx->d_items[0]); f(ctx->d_items[1]);
+  f(ctx->d_iclass[0]); f(ctx->d_iclass[1]);
   f(ctx->d_counter); f(ctx->d_bigscratch); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);

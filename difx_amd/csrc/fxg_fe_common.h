@@ -83,6 +83,10 @@ __device__ __forceinline__ int padx(int e) { return e ^ ((e >> 3) & 31); }
 #ifndef FXG_F_HOIST
 #define FXG_F_HOIST 3
 #endif
+// Waves per SIMD the persistent F-engine's register allocation is held to.
+#ifndef FXG_P_MINWAVES
+#define FXG_P_MINWAVES 1
+#endif
 
 template <int L>
 struct Plan {

@@ -149,6 +149,12 @@ struct FItem {
   int L, log2L, d, part, nparts;
 };
 
+// Persistent launch: the FFT lengths, longest first, with their ranges of work items.
+#define FXG_MAX_ICLASS 8
+struct IClass {
+  int L, log2L, begin, count;
+};
+
 // Setup for one subint (delay interpolators, frame validity).
 struct SetupArgs {
   const DevDs* ds;
@@ -219,8 +225,8 @@ int fxg_launch_setup(const SetupArgs& a, int maxframes, hipStream_t st);
 int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st);
 int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, int ntb, hipStream_t st);
 int fxg_launch_fftparams(const FengineArgs& a, int nds, hipStream_t st);
-int fxg_launch_fpersist(const FengineArgs& a, const FItem* items, int nitems, int* counter, int nwg, int ntb,
-                        int iscomplex, size_t lds, hipStream_t st);
+int fxg_launch_fpersist(const FengineArgs& a, const FItem* items, const IClass* cls, int ncls, int* counters,
+                        int nwg, int ntb, int iscomplex, size_t lds, hipStream_t st);
 size_t fxg_fengine_lds(int L, int ntb, int phcap, int order);
 int fxg_launch_fbig(const FengineArgs& a, int L, int iscomplex, int nds, int wgx, float2* scratch,
                     const float2* twL, hipStream_t st);

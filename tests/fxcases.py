@@ -151,7 +151,9 @@ def channel_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
     """Per-channel errors of every cross record of a configuration whose datastreams hold one
     recorded band each (cfg2): |V_got - V_ref| / sqrt(A_1 A_2) with A the reference
     autocorrelations of the two inputs, and the phase difference arg(V_got conj(V_ref)) in
-    degrees.  Returns the two flattened arrays."""
+    degrees over the channels whose |V_ref| / sqrt(A_1 A_2) is at least the median (where the
+    phase is defined; near |V| = 0 any error turns the phase freely).  Returns the two
+    flattened arrays."""
     norm, dphi = [], []
     for (fi, j), off in lay.baseline_off.items():
         bl = cfg.baselines[j]
@@ -163,5 +165,7 @@ def channel_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
         assert len(cfg.datastreams[bl.ds1].recbands) == 1 and len(cfg.datastreams[bl.ds2].recbands) == 1
         g, r = got[off: off + n], ref[off: off + n]
         norm.append(np.abs(g - r) / np.sqrt(a1 * a2))
-        dphi.append(np.degrees(np.abs(np.angle(g * np.conj(r)))))
+        coh = np.abs(r) / np.sqrt(a1 * a2)
+        keep = coh >= np.median(coh)
+        dphi.append(np.degrees(np.abs(np.angle(g[keep] * np.conj(r[keep])))))
     return np.concatenate(norm), np.concatenate(dphi)
