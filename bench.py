@@ -253,6 +253,107 @@ def time_path(cfg, steps, warmup, torch, dev, offsets):
     return info
 
 
+MUX_STATIONS, MUX_THREADS, MUX_SLOTS, MUX_TFB = 10, 16, 1000, 8032
+
+
+def mux_inputs(torch, dev, seed=20261206):
+    """Per station: 0.5 s of 8 x 32 MHz dual-polarisation 2-bit data recorded as 16
+    single-channel VDIF threads (8000-byte payloads, 2000 frames/s per thread): 1000 frame
+    times x 16 threads, in a random order, generated on the device."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    n, ns, tfb, fps = MUX_THREADS, MUX_SLOTS, MUX_TFB, 2000
+    s = np.repeat(np.arange(ns), n)
+    l = np.tile(np.arange(n), ns)
+    hdr = np.zeros((ns * n, 8), dtype=np.uint32)           # vdif_header fields, epoch 40
+    hdr[:, 0] = 5000 + s // fps
+    hdr[:, 1] = (s % fps) | (40 << 24)
+    hdr[:, 2] = tfb // 8
+    hdr[:, 3] = 0x4142 | (l << 16) | (1 << 26)
+    hdr_d = torch.from_numpy(hdr.view(np.uint8).reshape(-1, 32)).to(dev)
+    bufs = []
+    for st in range(MUX_STATIONS):
+        fr = torch.randint(0, 256, (ns * n, tfb), dtype=torch.uint8, device=dev, generator=g)
+        fr[:, :32] = hdr_d
+        perm = torch.randperm(ns * n, device=dev, generator=g)
+        bufs.append(fr[perm].reshape(-1).contiguous())
+        del fr
+    ref_sec = 58849 * 86400 + 5000                         # epoch 40 = MJD 58849
+    return bufs, ref_sec, fps
+
+
+def mux_bench(torch, dev, steps=5, warmup=1):
+    """Device corner turn of 10 stations' segments per step (one enqueue per station on one
+    stream, then the results).  Bytes per step: every input frame read once, every output
+    frame written once."""
+    from difx_amd.vdifmux import VDIFMuxer
+    bufs, ref_sec, fps = mux_inputs(torch, dev)
+    muxers = []
+    for st in range(MUX_STATIONS):
+        m = VDIFMuxer(MUX_THREADS, MUX_TFB, fps, 2, list(range(MUX_THREADS)))
+        m.set_reference(ref_sec, 0)
+        muxers.append(m)
+    ofb = muxers[0].outputframebytes
+    outs = [torch.empty(MUX_SLOTS * ofb, dtype=torch.uint8, device=dev) for _ in range(MUX_STATIONS)]
+    stream = torch.cuda.Stream(device=dev)
+    def step():
+        for m, b, o in zip(muxers, bufs, outs):
+            m.enqueue(b, MUX_SLOTS, out=o, start=0, stream=stream)
+        return [m.finish() for m in muxers]
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        evs[k][0].record(stream)
+        nb = step()
+        evs[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = [a.elapsed_time(b) for a, b in evs]
+    assert all(x == MUX_SLOTS * ofb for x in nb), nb
+    in_bytes = sum(b.numel() for b in bufs)
+    out_bytes = MUX_STATIONS * MUX_SLOTS * ofb
+    samples = MUX_STATIONS * MUX_SLOTS * MUX_THREADS * (MUX_TFB - 32) * 4
+    dev_ms = float(np.median(ms))
+    res = {"workload": "VDIF thread corner turn: 10 stations x 1000 frame times x 16 threads of 8032-byte "
+                       "2-bit frames (0.5 s of 8 x 32 MHz dual-pol each), shuffled",
+           "value": samples / (dev_ms / 1e3) / 1e9, "unit": "Gsamples/s",
+           "ms_per_step_device": dev_ms, "ms_per_step_wall": wall * 1e3 / steps,
+           "bytes_per_step": in_bytes + out_bytes,
+           "achieved_GBps": (in_bytes + out_bytes) / (dev_ms / 1e3) / 1e9, "peak_GBps": HBM_PEAK_GBS,
+           "frac": (in_bytes + out_bytes) / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    del bufs, outs
+    torch.cuda.empty_cache()
+    return res
+
+
+def mux_cpu_baseline(budget_s: float = 6.0):
+    """The corner-turn restatement (oracle/vdifmux.py, numpy) on segments of 20 frame times of
+    the same 16-thread workload, repeated for about budget_s."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import vdifmux as vm
+    rng = np.random.default_rng(5)
+    n, ns, tfb, fps = MUX_THREADS, 20, MUX_TFB, 2000
+    fr = rng.integers(0, 256, size=(ns * n, tfb), dtype=np.uint8)
+    for i in range(ns * n):
+        fr[i, :32] = vm.make_header(5000, i // n, 40, tfb, i % n, 2).view(np.uint8)
+    buf = fr[rng.permutation(ns * n)].reshape(-1)
+    ref_sec = vm.epoch_mjd(40) * 86400 + 5000
+    runs, dt = 0, 0.0
+    while dt < budget_s and runs < 100:
+        t0 = time.perf_counter()
+        vm.mux_segment(buf, tfb, list(range(n)), fps, 2, ref_sec, 0, 0, ns)
+        dt += time.perf_counter() - t0
+        runs += 1
+    samples = runs * ns * n * (tfb - 32) * 4
+    return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": 1, "kind": "port",
+            "sample": f"numpy restatement of VDIFMuxer, {runs} segments of {ns} frame times x {n} threads, "
+                      f"{dt:.2f} s"}
+
+
 def cpu_baseline(budget_s: float = 12.0):
     """Time the CPU restatement on a bounded sample of the workload: all 10 stations x 16
     bands x all 45 baselines, for as many FFT blocks as fit the time budget."""
@@ -344,6 +445,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-target", action="store_true")
+    ap.add_argument("--no-mux", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copy for PMC scaling")
     args = ap.parse_args()
     if args.copy_probe:
@@ -436,10 +538,16 @@ def main():
     if not args.no_serving:
         print("[bench] serving: Poisson job stream", file=sys.stderr, flush=True)
         serving = serving_bench(torch, dev)
+    mux = None
+    if not args.no_mux:
+        print("[bench] VDIF thread corner turn, 1 warmup + 5 steps", file=sys.stderr, flush=True)
+        mux = mux_bench(torch, dev)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline()
+        if mux is not None:
+            mux["cpu_baseline"] = mux_cpu_baseline()
     realtime = nsamp / (cfg.subint_ns * 1e-9) / 1e9
     line = {
         "metric": "input Gsamples/s (all stations) + p99 subint latency; MFMA vs plain FP32 path",
@@ -475,6 +583,7 @@ def main():
         "target_16_stations": target,
         "uneven_bands": uneven,
         "serving": serving,
+        "vdif_corner_turn": mux,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

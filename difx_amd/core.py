@@ -74,6 +74,11 @@ def load_library() -> C.CDLL:
     lib.fxg_server_wait_job.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, P(C.c_double), P(C.c_double)]
     lib.fxg_server_get_stats.argtypes = [C.c_void_p, P(abi.fxg_server_stats)]
     lib.fxg_server_job_error.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]
+    lib.fxg_vdif_mux_workspace_bytes.argtypes = [C.c_int64, C.c_int32]
+    lib.fxg_vdif_mux_workspace_bytes.restype = C.c_int64
+    lib.fxg_vdif_mux_device.argtypes = [P(abi.fxg_vdif_mux_params), C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                        C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
+    lib.fxg_vdif_mux_result.argtypes = [C.c_void_p, C.c_void_p, P(C.c_int64), P(C.c_int64)]
     _LIB = lib
     return lib
 

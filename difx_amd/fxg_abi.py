@@ -144,6 +144,23 @@ class fxg_server_stats(C.Structure):
     ]
 
 
+FXG_MUX_MAX_THREADS = 32
+
+
+class fxg_vdif_mux_params(C.Structure):
+    _fields_ = [
+        ("inputframebytes", C.c_int32),
+        ("nthreads", C.c_int32),
+        ("bitspersample", C.c_int32),
+        ("framespersecond", C.c_int32),
+        ("threadmap", C.c_int32 * FXG_MUX_MAX_THREADS),
+        ("ref_seconds", C.c_int64),
+        ("ref_frame", C.c_int32),
+        ("reserved", C.c_int32),
+        ("start", C.c_int64),
+    ]
+
+
 # exported symbols of libfxg.so (every function declared in include/fxg.h)
 EXPORTED_SYMBOLS = [
     "fxg_create", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
@@ -154,4 +171,5 @@ EXPORTED_SYMBOLS = [
     "fxg_footprint", "fxg_input_samples", "fxg_device_bytes",
     "fxg_server_create", "fxg_server_destroy", "fxg_server_add_job", "fxg_server_push_subint",
     "fxg_server_wait_job", "fxg_server_get_stats", "fxg_server_job_error",
+    "fxg_vdif_mux_workspace_bytes", "fxg_vdif_mux_device", "fxg_vdif_mux_result",
 ]

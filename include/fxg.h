@@ -228,6 +228,53 @@ int fxg_input_samples(const fxg_ctx* ctx, int64_t* samples);
 /* Device bytes held by a context. */
 int fxg_device_bytes(const fxg_ctx* ctx, int64_t* bytes);
 
+/* ---- VDIF thread corner turn ---------------------------------------------------------
+ * Replaces VDIFMuxer (datamuxer.h:148-240): deinterlace (datamuxer.cpp:764-833) and
+ * multiplex (datamuxer.cpp:670-757) with cornerturn_generic (datamuxer.cpp:243-266) for one
+ * segment, on the device.  Input: single-channel VDIF frames of N threads in any order;
+ * output: N-channel frames, the stream an fxg datastream of nrecbands = N decodes.  Slot s of
+ * the segment is the frame whose count since the reference frame,
+ *   ((mjd - ref_mjd) * 86400 + second - ref_second) * fps + frame - ref_frame,
+ * equals start + s.  Per slot, in slot order, starting at the first slot with any thread
+ * present (earlier ones are not output):
+ *   all threads present   thread 0's header with invalid 0, log2(N) channels, the output
+ *                         frame length and thread id 0, then the corner-turned payload
+ *   some present          the first present thread's header with invalid 1, the output frame
+ *                         length and thread id 0; the payload bytes are not written
+ *   none present          the previous output header with invalid 1, frame number + 1
+ *                         (wrapping to frame 0 of the next second at fps)
+ * A later copy of a (slot, thread) in the input replaces an earlier one.  Frames of an
+ * unmapped thread, before the segment or after its last slot are dropped and counted (the
+ * reference keeps frames after the segment for the next one: pass them again). */
+#define FXG_MUX_MAX_THREADS 32
+typedef struct fxg_vdif_mux_params {
+  int32_t inputframebytes;     /* one single-thread frame incl. its 32-byte header; multiple of 8 */
+  int32_t nthreads;            /* N: a power of two <= FXG_MUX_MAX_THREADS */
+  int32_t bitspersample;       /* 1, 2, 4, 8, 16 or 32, with N * bits <= 32 */
+  int32_t framespersecond;     /* frames per second of one thread */
+  int32_t threadmap[FXG_MUX_MAX_THREADS];  /* VDIF thread id of output channel j (the
+                                              reference's threadindexmap) */
+  int64_t ref_seconds;         /* reference frame: MJD * 86400 + seconds of the epoch */
+  int32_t ref_frame;           /* reference frame: frame number within its second */
+  int32_t reserved;
+  int64_t start;               /* count of slot 0 since the reference frame */
+} fxg_vdif_mux_params;
+/* Device workspace for nslots slots of N threads (bytes). */
+int64_t fxg_vdif_mux_workspace_bytes(int64_t nslots, int32_t nthreads);
+/* Enqueue the corner turn of one segment of nslots slots on `stream` (0 = default).
+ * d_in: in_bytes of input frames (8-byte aligned); d_out: out_bytes >= nslots * ((ifb - 32)
+ * * N + 32) (16-byte aligned), written from its start; d_work: workspace_bytes.  Returns
+ * at once; fxg_vdif_mux_result gives the outcome. */
+int fxg_vdif_mux_device(const fxg_vdif_mux_params* p, const uint8_t* d_in, int64_t in_bytes,
+                        uint8_t* d_out, int64_t out_bytes, int64_t nslots, void* d_work,
+                        int64_t work_bytes, void* stream);
+/* Synchronize `stream` and read the outcome of the last corner turn with this workspace: the
+ * number of output frames (multiplex returns that times the output frame length) and the
+ * dropped input frames {before the segment, after it, unmapped thread}.  FXG_E_INVALID if
+ * an input frame's length field differed from inputframebytes (the reference aborts
+ * there; the output is then undefined). */
+int fxg_vdif_mux_result(const void* d_work, void* stream, int64_t* nout, int64_t* dropped /* [3] */);
+
 /* ---- Serving: concurrent correlation jobs on one device ------------------------------
  * No counterpart in the reference, which runs one job per mpifxcorr process; this replaces
  * the FxManager -> Core subint dispatch (fxmanager.cpp:331-336, core.cpp:205-330) and the

@@ -21,7 +21,7 @@ def _lib():
 def test_exports_match_header():
     lib = _lib()
     hdr = (ROOT / "include" / "fxg.h").read_text()
-    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(fxg_\w+)\s*\(", hdr, re.M))
+    declared = set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(fxg_\w+)\s*\(", hdr, re.M))
     assert declared == set(abi.EXPORTED_SYMBOLS), declared ^ set(abi.EXPORTED_SYMBOLS)
     for name in declared:
         assert hasattr(lib, name), name
@@ -39,6 +39,8 @@ def test_struct_sizes():
     assert ctypes.sizeof(abi.fxg_freq) == 32
     assert ctypes.sizeof(abi.fxg_baseline) == 4 * (3 + 2 * 32 + 2 * 32 * 4)
     assert ctypes.sizeof(abi.fxg_layout) == 24
+    # 4 int32, threadmap[32], int64, 2 int32, int64
+    assert ctypes.sizeof(abi.fxg_vdif_mux_params) == 16 + 4 * abi.FXG_MUX_MAX_THREADS + 24
 
 
 def test_create_rejects_null_config():
