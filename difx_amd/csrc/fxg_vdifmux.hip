@@ -6,12 +6,14 @@
 // Kernels, all streaming through HBM once:
 //   k_vmux_init    slot table (slots x threads: input frame index + 1, 0 = missing), counters
 //   k_vmux_index   one lane per input frame: header fields -> (slot, thread); a later copy of a
-//                  (slot, thread) replaces an earlier one (atomic max of the frame index)
-//   k_vmux_scan    per slot, the last slot at or before it with any thread present
-//   k_vmux_frames  workgroups of one slot: the output header by the reference's rules and,
-//                  for complete slots, the corner-turned payload: a lane reads 8 bytes of each
-//                  thread's frame at one payload position and writes the 2N output words they
-//                  become (8N contiguous bytes)
+//                  (slot, thread) replaces an earlier one (atomic max of the frame index); the
+//                  slot's presence mask gets the thread's bit
+//   k_vmux_frames  workgroups of one slot: the output header by the reference's rules (a slot
+//                  with no thread looks back for the last one with any) and, for complete
+//                  slots, the corner-turned payload: a lane reads 8 bytes of each thread's
+//                  frame at one payload position and turns them into the 2N output words
+//                  (8N bytes); for N >= 4 those pass through LDS so that the workgroup's
+//                  stores leave as contiguous 16-byte pieces
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <algorithm>
@@ -32,19 +34,19 @@ struct MuxArgs {
 
 // Workspace: 8 int64 counters -- [0] first slot with any thread present (nslots if none),
 // [1] frames before the segment, [2] after it, [3] of unmapped threads, [4] of another frame
-// length, [5] nslots -- then the slot table (int32 nslots x n) and the last-present slots
-// (int32 nslots).
+// length, [5] nslots -- then the slot table (int32 nslots x n) and the per-slot presence
+// masks (uint32 nslots, bit l = thread l present).
 struct MuxWork {
   unsigned long long* ctr;
   int* table;
-  int* last;
+  unsigned* mask;
 };
 
 MuxWork work_of(void* d_work, long long nslots, int n) {
   MuxWork w;
   w.ctr = reinterpret_cast<unsigned long long*>(d_work);
   w.table = reinterpret_cast<int*>(reinterpret_cast<uint8_t*>(d_work) + 64);
-  w.last = w.table + nslots * n;
+  w.mask = reinterpret_cast<unsigned*>(w.table + nslots * n);
   return w;
 }
 
@@ -75,83 +77,53 @@ __device__ __forceinline__ uint4 load_16(const uint8_t* p) {             // 8-by
 __global__ void k_vmux_init(MuxWork w, long long ntable, long long nslots) {
   const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long step = (long long)gridDim.x * blockDim.x;
-  for (long long i = i0; i < ntable; i += step) w.table[i] = 0;
+  for (long long i = i0; i < ntable; i += step) w.table[i] = 0;   // table and masks
   if (i0 < 8) w.ctr[i0] = (i0 == 0 || i0 == 5) ? (unsigned long long)nslots : 0ull;
 }
 
-__global__ void k_vmux_index(MuxArgs a, const uint8_t* __restrict__ in, MuxWork w) {
-  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= a.nin) return;
-  const uint4 h = load_16(in + f * a.ifb);
-  const int seconds = (int)(h.x & 0x3FFFFFFFu);
-  const int frame = (int)(h.y & 0xFFFFFFu), epoch = (int)((h.y >> 24) & 0x3Fu);
-  const int fbytes = (int)(h.z & 0xFFFFFFu) * 8;
-  const int tid = (int)((h.w >> 16) & 0x3FFu);
-  if (fbytes != a.ifb) {                           // the reference aborts (datamuxer.cpp:784-787)
-    atomicAdd(&w.ctr[4], 1ull);
-    return;
-  }
-  int ti = -1;
-#pragma unroll
-  for (int j = 0; j < FXG_MUX_MAX_THREADS; ++j)
-    if (j < a.n && ti < 0 && a.map[j] == tid) ti = j;
-  if (ti < 0) {
-    atomicAdd(&w.ctr[3], 1ull);
-    return;
-  }
-  const long long fullsec = epoch_mjd_d(epoch) * 86400LL + seconds;
-  const long long cur = (fullsec - a.ref_seconds) * a.fps + frame - a.ref_frame - a.start;
-  if (cur < 0) {
-    atomicAdd(&w.ctr[1], 1ull);
-    return;
-  }
-  if (cur >= a.nslots) {
-    atomicAdd(&w.ctr[2], 1ull);
-    return;
-  }
-  atomicMax(&w.table[cur * a.n + ti], (int)(f + 1));
-  atomicMin(&w.ctr[0], (unsigned long long)cur);
-}
-
-// One workgroup of 1024 lanes: last[s] = the largest s' <= s with any thread present, or -1.
-__global__ void __launch_bounds__(1024) k_vmux_scan(MuxArgs a, MuxWork w) {
-  __shared__ int carry;
-  __shared__ int part[1024];
-  if (threadIdx.x == 0) carry = -1;
+// 256 lanes per workgroup; the first present slot is reduced in LDS first (one global atomic
+// per workgroup instead of one per frame on a single address)
+__global__ void __launch_bounds__(256) k_vmux_index(MuxArgs a, const uint8_t* __restrict__ in, MuxWork w) {
+  __shared__ unsigned s_first;
+  if (threadIdx.x == 0) s_first = 0xFFFFFFFFu;
   __syncthreads();
-  for (long long base = 0; base < a.nslots; base += 1024) {
-    const long long s = base + threadIdx.x;
-    int v = -1;
-    if (s < a.nslots) {
-      bool any = false;
-      for (int l = 0; l < a.n; ++l) any |= w.table[s * a.n + l] != 0;
-      v = any ? (int)s : -1;
+  const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < a.nin) {
+    const uint4 h = load_16(in + f * a.ifb);
+    const int seconds = (int)(h.x & 0x3FFFFFFFu);
+    const int frame = (int)(h.y & 0xFFFFFFu), epoch = (int)((h.y >> 24) & 0x3Fu);
+    const int fbytes = (int)(h.z & 0xFFFFFFu) * 8;
+    const int tid = (int)((h.w >> 16) & 0x3FFu);
+    int ti = -1;
+#pragma unroll
+    for (int j = 0; j < FXG_MUX_MAX_THREADS; ++j)
+      if (j < a.n && ti < 0 && a.map[j] == tid) ti = j;
+    const long long fullsec = epoch_mjd_d(epoch) * 86400LL + seconds;
+    const long long cur = (fullsec - a.ref_seconds) * a.fps + frame - a.ref_frame - a.start;
+    if (fbytes != a.ifb) {                         // the reference aborts (datamuxer.cpp:784-787)
+      atomicAdd(&w.ctr[4], 1ull);
+    } else if (ti < 0) {
+      atomicAdd(&w.ctr[3], 1ull);
+    } else if (cur < 0) {
+      atomicAdd(&w.ctr[1], 1ull);
+    } else if (cur >= a.nslots) {
+      atomicAdd(&w.ctr[2], 1ull);
+    } else {
+      atomicMax(&w.table[cur * a.n + ti], (int)(f + 1));
+      atomicOr(&w.mask[cur], 1u << ti);
+      atomicMin(&s_first, (unsigned)cur);
     }
-    part[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {    // inclusive max scan
-      const int o = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : -1;
-      __syncthreads();
-      part[threadIdx.x] = max(part[threadIdx.x], o);
-      __syncthreads();
-    }
-    const int r = max(part[threadIdx.x], carry);
-    if (s < a.nslots) w.last[s] = r;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = r;
-    __syncthreads();
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_first != 0xFFFFFFFFu) atomicMin(&w.ctr[0], (unsigned long long)s_first);
 }
 
 // The output header of a slot with at least one thread present (8 words), and whether all are.
 __device__ void present_header(const MuxArgs& a, const uint8_t* in, const MuxWork& w, long long s, int ofb,
                                uint32_t* hw, bool* complete) {
-  int first = -1;
-  bool all = true;
-  for (int l = 0; l < a.n; ++l) {
-    if (w.table[s * a.n + l] == 0) all = false;
-    else if (first < 0) first = l;
-  }
+  const unsigned m = w.mask[s];
+  const bool all = m == (a.n == 32 ? 0xFFFFFFFFu : (1u << a.n) - 1u);
+  const int first = __builtin_ctz(m);              // m != 0
   *complete = all;
   const long long fi = (long long)(w.table[s * a.n + (all ? 0 : first)] - 1);
   const uint8_t* src = in + fi * a.ifb;
@@ -227,6 +199,11 @@ __device__ __forceinline__ void turn_word(uint32_t (&w)[N]) {
   }
 }
 
+// LDS bytes of one lane's 2N turned words, padded by 16 bytes so that the rows of consecutive
+// lanes start 4 banks apart
+template <int N>
+constexpr int row_bytes() { return 8 * N + 16; }
+
 // grid (slots, chunks of MUXNT 8-byte payload positions)
 template <int N, int BITS>
 __global__ void __launch_bounds__(MUXNT) k_vmux_frames(MuxArgs a, const uint8_t* __restrict__ in,
@@ -237,14 +214,30 @@ __global__ void __launch_bounds__(MUXNT) k_vmux_frames(MuxArgs a, const uint8_t*
   const int ofb = (a.ifb - MUXHDR) * N + MUXHDR;
   uint8_t* o = out + (s - first) * ofb;
   __shared__ uint32_t hw[8];
-  __shared__ int complete;
+  __shared__ int complete, s_p;
+  long long p = s;
+  if (w.mask[s] == 0) {
+    // no thread present (datamuxer.cpp:716-726): the last slot before s with any thread
+    // present, which exists because slot `first` has one
+    if (threadIdx.x == 0) s_p = -1;
+    __syncthreads();
+    for (long long base = s - 1; base >= first; base -= MUXNT) {
+      const long long c = base - threadIdx.x;
+      if (c >= first && w.mask[c] != 0) atomicMax(&s_p, (int)c);
+      __syncthreads();
+      const bool found = s_p >= 0;
+      __syncthreads();                             // every lane has read s_p
+      if (found) break;
+    }
+    if (s_p < 0) return;                           // (unreachable: slot `first` has a thread)
+    p = s_p;
+  }
   if (threadIdx.x == 0) {
-    const int p = w.last[s];
     bool all = false;
     present_header(a, in, w, p, ofb, hw, &all);
     if (p == s) {
       complete = all ? 1 : 0;
-    } else {                                       // no thread present (datamuxer.cpp:716-726)
+    } else {
       complete = 0;
       hw[0] |= 0x80000000u;
       advance_frames(hw, s - p, a.fps);
@@ -260,28 +253,45 @@ __global__ void __launch_bounds__(MUXNT) k_vmux_frames(MuxArgs a, const uint8_t*
     src[l] = in + (long long)(e - 1) * a.ifb + MUXHDR;
   }
   const int npos = (a.ifb - MUXHDR) / 8;          // 8-byte payload positions
-  for (int q = blockIdx.y * MUXNT + threadIdx.x; q < npos; q += gridDim.y * MUXNT) {
+  constexpr int RB = row_bytes<N>();
+  __shared__ __attribute__((aligned(16))) uint8_t stage[N >= 4 ? MUXNT * RB : 16];
+  for (int qb = blockIdx.y * MUXNT; qb < npos; qb += gridDim.y * MUXNT) {
+    const int q = qb + threadIdx.x;
     uint32_t w0[N], w1[N];
+    if (q < npos) {
 #pragma unroll
-    for (int l = 0; l < N; ++l) {
-      const uint2 v = *reinterpret_cast<const uint2*>(src[l] + 8 * q);
-      w0[l] = v.x;
-      w1[l] = v.y;
+      for (int l = 0; l < N; ++l) {
+        const uint2 v = *reinterpret_cast<const uint2*>(src[l] + 8 * q);
+        w0[l] = v.x;
+        w1[l] = v.y;
+      }
+      turn_word<N, BITS>(w0);
+      turn_word<N, BITS>(w1);
     }
-    turn_word<N, BITS>(w0);
-    turn_word<N, BITS>(w1);
     // output words 2qN .. 2qN + 2N - 1: w0 then w1
     if constexpr (N == 1) {
-      reinterpret_cast<uint2*>(o + MUXHDR)[q] = make_uint2(w0[0], w1[0]);
+      if (q < npos) reinterpret_cast<uint2*>(o + MUXHDR)[q] = make_uint2(w0[0], w1[0]);
     } else if constexpr (N == 2) {
-      reinterpret_cast<uint4*>(o + MUXHDR)[q] = make_uint4(w0[0], w0[1], w1[0], w1[1]);
+      if (q < npos) reinterpret_cast<uint4*>(o + MUXHDR)[q] = make_uint4(w0[0], w0[1], w1[0], w1[1]);
     } else {
-      uint4* dst = reinterpret_cast<uint4*>(o + MUXHDR) + (long long)q * (N / 2);
+      if (q < npos) {
+        uint4* row = reinterpret_cast<uint4*>(stage + threadIdx.x * RB);
 #pragma unroll
-      for (int j = 0; j < N; j += 4) {
-        dst[j / 4] = make_uint4(w0[j], w0[j + 1], w0[j + 2], w0[j + 3]);
-        dst[N / 4 + j / 4] = make_uint4(w1[j], w1[j + 1], w1[j + 2], w1[j + 3]);
+        for (int j = 0; j < N; j += 4) {
+          row[j / 4] = make_uint4(w0[j], w0[j + 1], w0[j + 2], w0[j + 3]);
+          row[N / 4 + j / 4] = make_uint4(w1[j], w1[j + 1], w1[j + 2], w1[j + 3]);
+        }
       }
+      __syncthreads();
+      // 16-byte piece c of this chunk's output: lane c / (N/2)'s row, piece c % (N/2)
+      uint4* dst = reinterpret_cast<uint4*>(o + MUXHDR) + (long long)qb * (N / 2);
+#pragma unroll
+      for (int it = 0; it < N / 2; ++it) {
+        const int c = it * MUXNT + threadIdx.x;
+        const int r = c / (N / 2), k = c % (N / 2);
+        if (qb + r < npos) dst[c] = *reinterpret_cast<const uint4*>(stage + r * RB + 16 * k);
+      }
+      __syncthreads();                             // the rows are rewritten next pass
     }
   }
 }
@@ -350,12 +360,11 @@ extern "C" int fxg_vdif_mux_device(const fxg_vdif_mux_params* p, const uint8_t* 
   a.nslots = nslots; a.nin = in_bytes / p->inputframebytes;
   const MuxWork w = work_of(d_work, nslots, n);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  const long long ntable = nslots * n;
+  const long long ntable = nslots * (n + 1);       // slot table and presence masks
   const unsigned ib = (unsigned)std::min<long long>(4096, (ntable + 255) / 256 + 1);
   hipLaunchKernelGGL(k_vmux_init, dim3(ib), dim3(256), 0, st, w, ntable, nslots);
   if (a.nin > 0)
     hipLaunchKernelGGL(k_vmux_index, dim3((unsigned)((a.nin + 255) / 256)), dim3(256), 0, st, a, d_in, w);
-  hipLaunchKernelGGL(k_vmux_scan, dim3(1), dim3(1024), 0, st, a, w);
   if (hipGetLastError() != hipSuccess) return FXG_E_HIP;
   const int r = dispatch_frames(a, d_in, d_out, w, st);
   return r == 0 ? FXG_OK : (r == -2 ? FXG_E_INVALID : FXG_E_HIP);

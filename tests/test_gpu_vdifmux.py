@@ -223,3 +223,19 @@ def test_muxed_stream_correlates_like_the_recording(gpu):
     torch.cuda.synchronize()
     got = d_res.cpu().numpy().view(np.complex64)
     np.testing.assert_array_equal(got, want)
+
+
+def test_long_gap(gpu):
+    """A gap of 600 frame times with no thread (longer than one workgroup's look-back pass)
+    between two stretches of data, fps 7 so that the advanced headers cross many seconds."""
+    _say("vdifmux long gap")
+    rng = np.random.default_rng(91)
+    n, bits, fps, tfb, nslots = 4, 2, 7, 32 + 64, 700
+    tids = [3, 1, 2, 0]
+    fr = _thread_frames(nslots, n, tfb, fps, tids, rng, sec0=100)
+    frames = [f for s, row in enumerate(fr) for f in row if not (50 <= s < 650)]
+    inbuf = np.concatenate([frames[i] for i in rng.permutation(len(frames))])
+    ref_sec = vm.epoch_mjd(40) * 86400 + 100
+    got, nout, st, ref, ref_n, ref_st = _run_both(gpu, inbuf, tfb, tids, fps, bits, ref_sec, 0, 0, nslots)
+    assert nout == ref_n == nslots and st == ref_st
+    np.testing.assert_array_equal(got, ref)
