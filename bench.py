@@ -297,23 +297,37 @@ def mux_bench(torch, dev, steps=5, warmup=1):
     ofb = muxers[0].outputframebytes
     outs = [torch.empty(MUX_SLOTS * ofb, dtype=torch.uint8, device=dev) for _ in range(MUX_STATIONS)]
     stream = torch.cuda.Stream(device=dev)
-    def step():
+    def enqueue_all():
         for m, b, o in zip(muxers, bufs, outs):
             m.enqueue(b, MUX_SLOTS, out=o, start=0, stream=stream)
-        return [m.finish() for m in muxers]
     for _ in range(warmup):
-        step()
+        enqueue_all()
+        [m.finish() for m in muxers]
     torch.cuda.synchronize()
+    # device time: the enqueued corner turns of all stations; each segment's 64-byte counter
+    # readback (finish) follows the end event, as it would overlap the next segment's work
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
     for k in range(steps):
         evs[k][0].record(stream)
-        nb = step()
+        enqueue_all()
         evs[k][1].record(stream)
+        nb = [m.finish() for m in muxers]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = [a.elapsed_time(b) for a, b in evs]
     assert all(x == MUX_SLOTS * ofb for x in nb), nb
+    # yardstick: a plain device copy of the same bytes (each station's input into its output)
+    cev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    with torch.cuda.stream(stream):
+        for k in range(steps):
+            cev[k][0].record(stream)
+            for b, o in zip(bufs, outs):
+                n = min(b.numel(), o.numel())
+                o[:n].copy_(b[:n])
+            cev[k][1].record(stream)
+    torch.cuda.synchronize()
+    copy_ms = float(np.median([a.elapsed_time(b) for a, b in cev]))
     in_bytes = sum(b.numel() for b in bufs)
     out_bytes = MUX_STATIONS * MUX_SLOTS * ofb
     samples = MUX_STATIONS * MUX_SLOTS * MUX_THREADS * (MUX_TFB - 32) * 4
@@ -324,7 +338,8 @@ def mux_bench(torch, dev, steps=5, warmup=1):
            "ms_per_step_device": dev_ms, "ms_per_step_wall": wall * 1e3 / steps,
            "bytes_per_step": in_bytes + out_bytes,
            "achieved_GBps": (in_bytes + out_bytes) / (dev_ms / 1e3) / 1e9, "peak_GBps": HBM_PEAK_GBS,
-           "frac": (in_bytes + out_bytes) / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+           "frac": (in_bytes + out_bytes) / (dev_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+           "device_copy_GBps": 2 * sum(min(b.numel(), o.numel()) for b, o in zip(bufs, outs)) / (copy_ms / 1e3) / 1e9}
     del bufs, outs
     torch.cuda.empty_cache()
     return res

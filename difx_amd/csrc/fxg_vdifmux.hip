@@ -210,42 +210,47 @@ __global__ void __launch_bounds__(MUXNT) k_vmux_frames(MuxArgs a, const uint8_t*
                                                       uint8_t* __restrict__ out, MuxWork w) {
   const long long s = blockIdx.x;
   const long long first = (long long)w.ctr[0];
+  const unsigned m = w.mask[s];
   if (s < first) return;                           // leading slots with no frame are not output
   const int ofb = (a.ifb - MUXHDR) * N + MUXHDR;
   uint8_t* o = out + (s - first) * ofb;
-  __shared__ uint32_t hw[8];
-  __shared__ int complete, s_p;
-  long long p = s;
-  if (w.mask[s] == 0) {
-    // no thread present (datamuxer.cpp:716-726): the last slot before s with any thread
-    // present, which exists because slot `first` has one
-    if (threadIdx.x == 0) s_p = -1;
-    __syncthreads();
-    for (long long base = s - 1; base >= first; base -= MUXNT) {
-      const long long c = base - threadIdx.x;
-      if (c >= first && w.mask[c] != 0) atomicMax(&s_p, (int)c);
+  constexpr unsigned FULL = N == 32 ? 0xFFFFFFFFu : (1u << N) - 1u;
+  if (m != FULL) {
+    // header only: the first present thread's header, or with no thread present
+    // (datamuxer.cpp:716-726) that of the last slot before s with any thread present, which
+    // exists because slot `first` has one; chunk 0's workgroup writes it
+    if (blockIdx.y != 0) return;
+    __shared__ uint32_t hw[8];
+    __shared__ int s_p;
+    long long p = s;
+    if (m == 0) {
+      if (threadIdx.x == 0) s_p = -1;
       __syncthreads();
-      const bool found = s_p >= 0;
-      __syncthreads();                             // every lane has read s_p
-      if (found) break;
+      for (long long base = s - 1; base >= first; base -= MUXNT) {
+        const long long c = base - threadIdx.x;
+        if (c >= first && w.mask[c] != 0) atomicMax(&s_p, (int)c);
+        __syncthreads();
+        const bool found = s_p >= 0;
+        __syncthreads();                           // every lane has read s_p
+        if (found) break;
+      }
+      if (s_p < 0) return;                         // (unreachable: slot `first` has a thread)
+      p = s_p;
     }
-    if (s_p < 0) return;                           // (unreachable: slot `first` has a thread)
-    p = s_p;
-  }
-  if (threadIdx.x == 0) {
-    bool all = false;
-    present_header(a, in, w, p, ofb, hw, &all);
-    if (p == s) {
-      complete = all ? 1 : 0;
-    } else {
-      complete = 0;
-      hw[0] |= 0x80000000u;
-      advance_frames(hw, s - p, a.fps);
+    if (threadIdx.x == 0) {
+      bool all = false;
+      present_header(a, in, w, p, ofb, hw, &all);
+      if (p != s) {
+        hw[0] |= 0x80000000u;
+        advance_frames(hw, s - p, a.fps);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) reinterpret_cast<uint32_t*>(o)[k] = hw[k];
     }
+    return;
   }
-  __syncthreads();
-  if (blockIdx.y == 0 && threadIdx.x < 8) reinterpret_cast<uint32_t*>(o)[threadIdx.x] = hw[threadIdx.x];
-  if (!complete) return;
+  // every thread present: the payload is turned by every chunk's workgroup, the header
+  // (thread 0's, made valid) is written by chunk 0's first lane once its loads are issued
   const uint8_t* src[N];
 #pragma unroll
   for (int l = 0; l < N; ++l) {
@@ -293,6 +298,13 @@ __global__ void __launch_bounds__(MUXNT) k_vmux_frames(MuxArgs a, const uint8_t*
       }
       __syncthreads();                             // the rows are rewritten next pass
     }
+  }
+  if (blockIdx.y == 0 && threadIdx.x == 0) {
+    uint32_t hw[8];
+    bool all = true;
+    present_header(a, in, w, s, ofb, hw, &all);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) reinterpret_cast<uint32_t*>(o)[k] = hw[k];
   }
 }
 
