@@ -19,7 +19,7 @@ CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libfxg.so"
 SOURCES = ["fxg_fengine.hip", "fxg_fpersist.hip", "fxg_fbig.hip", "fxg_xengine.hip", "fxg_api.cpp",
-           "fxg_server.cpp", "fxg_vdifmux.hip"]
+           "fxg_server.cpp", "fxg_vdifmux.hip", "fxg_control.cpp"]
 ARCH = "gfx950"
 
 

@@ -1485,6 +1485,100 @@ int fxg_data_bytes(const fxg_ctx* ctx, int32_t ds, int64_t* bytes) {
   return FXG_OK;
 }
 
+// Configuration::getFramesPerSecond (configuration.cpp:654-668) as the framed datastreams
+// hold it, in an int (vdiffile.cpp:350, mark5bfile.cpp:123), decimation 1
+static int frames_per_second(const fxg_ctx* ctx, int ds) {
+  const fxg_datastream& d = ctx->dss[ds];
+  const double samplerate = 2.0e6 * ctx->freqs[d.recfreq_index[0]].bw_mhz;
+  const int payload = d.framebytes - d.headerbytes;
+  if (payload < 1) return 0;
+  return (int)(samplerate * d.nrecbands * d.nbits / (8.0 * payload));
+}
+
+static long long gcd_ll(long long a, long long b) {
+  while (b) { const long long t = a % b; a = b; b = t; }
+  return a;
+}
+
+// DataStream::updateConfig's per-segment constants (datastream.cpp:745-763), with the
+// bytes-per-sample fraction of Configuration (configuration.cpp:1656-1664, decimation 1).
+static int segment_params(const fxg_ctx* ctx, int32_t ds, int32_t numdatasegments, int32_t bufferbytes,
+                          fxg_segment* seg) {
+  if (ds < 0 || ds >= ctx->nds || numdatasegments < 1 || bufferbytes < numdatasegments) return FXG_E_INVALID;
+  const fxg_datastream& d = ctx->dss[ds];
+  const fxg_freq& f = ctx->freqs[d.recfreq_index[0]];
+  const bool cplx = d.complex_sampling != 0;
+  seg->sendbytes = int((ctx->databytes[ds] * (ctx->subint_ns + ctx->guard_ns)) / ctx->subint_ns);
+  seg->blockspersend = ctx->B;
+  const int samplebits = d.nbits * (cplx ? 2 : 1);
+  seg->bytespersamplenum = (d.nrecbands * samplebits) / 8;
+  if (seg->bytespersamplenum == 0) {
+    seg->bytespersamplenum = 1;
+    seg->bytespersampledenom = 8 / (d.nrecbands * samplebits);
+  } else {
+    seg->bytespersampledenom = 1;
+  }
+  seg->numchannels = f.nchan;
+  seg->complex_sampling = cplx ? 1 : 0;
+  seg->sampletimens = 500.0 / f.bw_mhz;
+  if (cplx) seg->sampletimens *= 2;
+  const double tiny50 = 50 * 0.000000001f;          // 50 * Mode::TINY (a float, mode.cpp:27)
+  seg->bytesbetweenintegerns = 0;
+  double nsaccumulate = 0.0;
+  int guard = 0;
+  do {
+    nsaccumulate += seg->bytespersampledenom * seg->sampletimens;
+    seg->bytesbetweenintegerns += seg->bytespersamplenum;
+    if (++guard > 1000000) return FXG_E_INVALID;    // no whole-nanosecond sample boundary
+  } while (!(std::fabs(nsaccumulate - int(nsaccumulate + 0.5)) < tiny50));
+  seg->nsinc = int((seg->sampletimens * (bufferbytes / numdatasegments) * seg->bytespersampledenom) /
+                   seg->bytespersamplenum + 0.5);
+  // framed streams count whole frames (vdiffile.cpp:484-491, mark5bfile.cpp:258-266)
+  const int fb = d.framebytes, fps = frames_per_second(ctx, ds);
+  if (fps < 1) return FXG_E_INVALID;
+  seg->nsinc = int(((bufferbytes / numdatasegments) / fb) * (1000000000.0 / double(fps)) + 0.5);
+  seg->sendbytes = int((((double)seg->sendbytes) * ((double)ctx->subint_ns)) / (ctx->subint_ns + ctx->guard_ns) + 0.5);
+  return FXG_OK;
+}
+
+// (validates the configuration as fxg_create does, without a device)
+int fxg_buffer_params(const fxg_config* cfg, int32_t ds, int32_t numdatasegments, int32_t bufferbytes,
+                      fxg_databuffer* buf) {
+  if (!cfg || !buf || numdatasegments < 1 || bufferbytes < numdatasegments) return FXG_E_INVALID;
+  fxg_ctx* ctx = nullptr;
+  int rc = create_impl(cfg, &ctx, true);
+  if (rc != FXG_OK) return rc;
+  if (ds < 0 || ds >= ctx->nds) {
+    delete ctx;
+    return FXG_E_INVALID;
+  }
+  const fxg_datastream& d = ctx->dss[ds];
+  buf->numdatasegments = numdatasegments;
+  buf->bufferbytes = bufferbytes;
+  buf->readbytes = bufferbytes / numdatasegments;
+  buf->intclockseconds = (int)std::floor(d.clock_coeff0_us / 1000000.0 + 0.5);
+  buf->framing = d.format == FXG_FMT_MARK5B ? FXG_FRAMING_MARK5B : FXG_FRAMING_VDIF;
+  buf->framebytes = d.framebytes;
+  buf->payloadbytes = d.framebytes - d.headerbytes;
+  buf->framespersecond = frames_per_second(ctx, ds);
+  // the fewest frames that span whole nanoseconds (vdifmux.c:140)
+  buf->framegranularity = buf->framespersecond > 0
+      ? (int)(buf->framespersecond / gcd_ll(buf->framespersecond, 1000000000LL)) : 0;
+  delete ctx;
+  return buf->framespersecond > 0 ? FXG_OK : FXG_E_INVALID;
+}
+
+int fxg_segment_params(const fxg_config* cfg, int32_t ds, int32_t numdatasegments, int32_t bufferbytes,
+                       fxg_segment* seg) {
+  if (!cfg || !seg) return FXG_E_INVALID;
+  fxg_ctx* ctx = nullptr;
+  int rc = create_impl(cfg, &ctx, true);
+  if (rc != FXG_OK) return rc;
+  rc = segment_params(ctx, ds, numdatasegments, bufferbytes, seg);
+  delete ctx;
+  return rc;
+}
+
 int fxg_result_offsets(const fxg_ctx* ctx, int32_t* baseline, int32_t* bweight, int32_t* autocorr,
                        int32_t* acweight) {
   if (!ctx) return FXG_E_INVALID;

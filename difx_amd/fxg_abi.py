@@ -161,6 +161,38 @@ class fxg_vdif_mux_params(C.Structure):
     ]
 
 
+FXG_INVALID_SUBINT = -99999999
+FXG_FLAGS_PER_INT = 30
+
+
+class fxg_segment(C.Structure):
+    _fields_ = [
+        ("scan", C.c_int32), ("scanseconds", C.c_int32), ("scanns", C.c_int32),
+        ("validbytes", C.c_int32), ("readto", C.c_int32), ("sendbytes", C.c_int32),
+        ("blockspersend", C.c_int32), ("bytespersamplenum", C.c_int32), ("bytespersampledenom", C.c_int32),
+        ("bytesbetweenintegerns", C.c_int32), ("nsinc", C.c_int32), ("numchannels", C.c_int32),
+        ("complex_sampling", C.c_int32), ("reserved", C.c_int32), ("sampletimens", C.c_double),
+    ]
+
+
+FXG_FRAMING_NONE = 0
+FXG_FRAMING_VDIF = 1
+FXG_FRAMING_MARK5B = 2
+
+
+class fxg_databuffer(C.Structure):
+    _fields_ = [
+        ("numdatasegments", C.c_int32), ("bufferbytes", C.c_int32), ("readbytes", C.c_int32),
+        ("keepreading", C.c_int32), ("lastvalidsegment", C.c_int32), ("intclockseconds", C.c_int32),
+        ("framing", C.c_int32), ("framebytes", C.c_int32), ("payloadbytes", C.c_int32),
+        ("framespersecond", C.c_int32), ("framegranularity", C.c_int32), ("reserved", C.c_int32),
+        ("seg", C.POINTER(fxg_segment)),
+    ]
+
+
+FXG_ADVANCE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32)
+
+
 # exported symbols of libfxg.so (every function declared in include/fxg.h)
 EXPORTED_SYMBOLS = [
     "fxg_create", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
@@ -172,4 +204,5 @@ EXPORTED_SYMBOLS = [
     "fxg_server_create", "fxg_server_destroy", "fxg_server_add_job", "fxg_server_push_subint",
     "fxg_server_wait_job", "fxg_server_get_stats", "fxg_server_job_error",
     "fxg_vdif_mux_workspace_bytes", "fxg_vdif_mux_device", "fxg_vdif_mux_result",
+    "fxg_segment_params", "fxg_buffer_params", "fxg_control_params",
 ]

@@ -275,6 +275,78 @@ int fxg_vdif_mux_device(const fxg_vdif_mux_params* p, const uint8_t* d_in, int64
  * there; the output is then undefined). */
 int fxg_vdif_mux_result(const void* d_work, void* stream, int64_t* nout, int64_t* dropped /* [3] */);
 
+/* ---- Slot start and valid flags: DataStream::calculateControlParams ------------------
+ * Replaces DataStream::calculateControlParams (datastream.cpp:368-627), the coarse-delay
+ * choice of where in the datastream's segmented read buffer a subint's data starts and which
+ * of its FFT blocks have data, which fills the control words {scan, sec, ns, validflags...}
+ * that fxg_process_subint takes.  Host code; the buffer ring, its readers and its locks stay
+ * the caller's: when the slot lies beyond the current segment, `advance` is called before
+ * each move to the next segment (the reference's waitForSendComplete, lock of segment
+ * at + 2 and unlock of at, datastream.cpp:438-457). */
+#define FXG_INVALID_SUBINT (-99999999)  /* Mode::INVALID_SUBINT (mode.h:222) */
+#define FXG_FLAGS_PER_INT  30           /* FLAGS_PER_INT (mpifxcorr.h:29) */
+/* One segment of the read buffer (DataStream::readinfo, datastream.h:87-113: the fields the
+ * choice reads).  The reader sets scan .. readto as it fills the segment;
+ * fxg_segment_params sets the rest from the configuration (DataStream::updateConfig,
+ * datastream.cpp:745-763). */
+typedef struct fxg_segment {
+  int32_t scan, scanseconds, scanns;  /* time of the segment's first byte */
+  int32_t validbytes;                 /* bytes read into the segment */
+  int32_t readto;                     /* 1 once the segment has been read into */
+  int32_t sendbytes;                  /* data bytes of one subint plus its guard */
+  int32_t blockspersend;              /* FFT blocks per subint */
+  int32_t bytespersamplenum, bytespersampledenom;
+  int32_t bytesbetweenintegerns;      /* bytes between samples at whole nanoseconds */
+  int32_t nsinc;                      /* nanoseconds one segment spans */
+  int32_t numchannels;                /* channels of the first recorded freq */
+  int32_t complex_sampling;           /* 1 if the sampling is complex */
+  int32_t reserved;
+  double  sampletimens;               /* nanoseconds per sample */
+} fxg_segment;
+/* Framed streams: the choice above counts payload bytes only; VDIFDataStream
+ * (vdiffile.cpp:348-472) and Mark5BDataStream (mark5bfile.cpp:121-247) then step back to
+ * the start of a frame (a multiple of framegranularity frames into the segment) and take
+ * its time. */
+#define FXG_FRAMING_NONE   0
+#define FXG_FRAMING_VDIF   1
+#define FXG_FRAMING_MARK5B 2
+typedef struct fxg_databuffer {
+  int32_t numdatasegments;
+  int32_t bufferbytes;                /* whole ring */
+  int32_t readbytes;                  /* bufferbytes / numdatasegments */
+  int32_t keepreading;                /* the reader is still active */
+  int32_t lastvalidsegment;           /* last segment read when the reader stopped */
+  int32_t intclockseconds;            /* whole seconds of the clock offset (mode.cpp:69) */
+  int32_t framing;                    /* FXG_FRAMING_* */
+  int32_t framebytes, payloadbytes;   /* framed: bytes of a frame and of its payload */
+  int32_t framespersecond;            /* framed: frames per second */
+  int32_t framegranularity;           /* framed: frames per whole-nanosecond step */
+  int32_t reserved;
+  const fxg_segment* seg;             /* [numdatasegments] */
+} fxg_databuffer;
+typedef int (*fxg_advance_fn)(void* user, int32_t from_segment);
+/* The config-derived fields of a segment for datastream ds (no device needed), with the
+ * frame-based nsinc and sendbytes of VDIFDataStream::updateConfig (vdiffile.cpp:475-492) and
+ * Mark5BDataStream::updateConfig (mark5bfile.cpp:249-267). */
+int fxg_segment_params(const fxg_config* cfg, int32_t ds, int32_t numdatasegments, int32_t bufferbytes,
+                       fxg_segment* seg);
+/* The config-derived fields of a ring (numdatasegments .. intclockseconds and the framing of
+ * the datastream's format); keepreading, lastvalidsegment and seg stay the caller's. */
+int fxg_buffer_params(const fxg_config* cfg, int32_t ds, int32_t numdatasegments, int32_t bufferbytes,
+                      fxg_databuffer* buf);
+/* The choice for the subint at (scan, offsetsec, offsetns).  delayus1 / delayus2: the
+ * datastream's delay in microseconds at the subint start and at the end of its data span
+ * (Model::calculateDelayInterpolator with order 0; delay_ok = 0 if the model has no
+ * interpolator there).  *atsegment is the caller's current segment, moved on as the
+ * reference moves it.  control receives 3 + ceil(blockspersend / 30) words (an invalid
+ * subint has control[1] = FXG_INVALID_SUBINT and zero flags); *bufferindex the byte offset of
+ * the subint's data in the ring (0 when invalid).  `advance` returns 0 to move on (after the
+ * segments have been refilled as needed) and nonzero to stop with FXG_E_STATE; with a null
+ * `advance` the choice stops with FXG_E_STATE after one pass round the ring. */
+int fxg_control_params(const fxg_databuffer* buf, int32_t* atsegment, int32_t scan, int32_t offsetsec,
+                       int32_t offsetns, int32_t delay_ok, double delayus1, double delayus2,
+                       fxg_advance_fn advance, void* user, int32_t* control, int32_t* bufferindex);
+
 /* ---- Serving: concurrent correlation jobs on one device ------------------------------
  * No counterpart in the reference, which runs one job per mpifxcorr process; this replaces
  * the FxManager -> Core subint dispatch (fxmanager.cpp:331-336, core.cpp:205-330) and the
