@@ -39,6 +39,7 @@ def load_library() -> C.CDLL:
     lib = C.CDLL(str(LIB_PATH))
     P = C.POINTER
     lib.fxg_create.argtypes = [P(abi.fxg_config), P(C.c_void_p)]
+    lib.fxg_create_layout.argtypes = [P(abi.fxg_config), P(C.c_void_p)]
     lib.fxg_destroy.argtypes = [C.c_void_p]
     lib.fxg_strerror.argtypes = [C.c_int]
     lib.fxg_strerror.restype = C.c_char_p
@@ -81,6 +82,9 @@ def load_library() -> C.CDLL:
     lib.fxg_vdif_mux_result.argtypes = [C.c_void_p, C.c_void_p, P(C.c_int64), P(C.c_int64)]
     lib.fxg_segment_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_segment)]
     lib.fxg_buffer_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_databuffer)]
+    lib.fxg_normalise_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+    lib.fxg_difx_records.argtypes = [C.c_void_p, C.c_void_p, P(abi.fxg_difx_meta), C.c_void_p, C.c_int64,
+                                     P(C.c_int64), P(C.c_int32)]
     lib.fxg_control_params.argtypes = [P(abi.fxg_databuffer), P(C.c_int32), C.c_int32, C.c_int32, C.c_int32,
                                        C.c_int32, C.c_double, C.c_double, abi.FXG_ADVANCE_FN, C.c_void_p,
                                        P(C.c_int32), P(C.c_int32)]
@@ -190,6 +194,30 @@ class FxCore:
         self._check(self.lib.fxg_process_subint_device(
             self.h, offs, _ptr_array(list(d_data)), nb, _ptr_array([c.ctypes.data for c in ctrl]),
             C.c_void_p(d_results), C.c_void_p(stream) if stream else None))
+
+    def normalise_device(self, d_results: int, meansubints: float, tsys=None, stream: int = 0):
+        """Visibility::writedata's amplitude scaling of a device results array, in place."""
+        t = None if tsys is None else (C.c_double * self.ndatastreams)(*[float(x) for x in tsys])
+        self._check(self.lib.fxg_normalise_device(self.h, C.c_void_p(d_results), float(meansubints), t,
+                                                  C.c_void_p(stream) if stream else None))
+
+    def difx_records(self, results: np.ndarray, mjd: int, seconds: float, configindex: int, sourceindex: int,
+                     fftsperintegration: float, baselinenumber, uvw, telescopeindex) -> bytes:
+        """Visibility::writeSWIN's records of a normalised host results array (complex64)."""
+        res = np.ascontiguousarray(results, dtype=np.complex64)
+        bl = np.ascontiguousarray(baselinenumber, dtype=np.int32)
+        uv = np.ascontiguousarray(uvw, dtype=np.float64)
+        tel = np.ascontiguousarray(telescopeindex, dtype=np.int32)
+        m = abi.fxg_difx_meta(int(mjd), int(configindex), int(sourceindex), 0, float(seconds),
+                              float(fftsperintegration), bl.ctypes.data_as(C.POINTER(C.c_int32)),
+                              uv.ctypes.data_as(C.POINTER(C.c_double)), tel.ctypes.data_as(C.POINTER(C.c_int32)))
+        n, nrec = C.c_int64(), C.c_int32()
+        self._check(self.lib.fxg_difx_records(self.h, res.ctypes.data, C.byref(m), None, 0, C.byref(n),
+                                              C.byref(nrec)))
+        out = np.zeros(n.value, dtype=np.uint8)
+        self._check(self.lib.fxg_difx_records(self.h, res.ctypes.data, C.byref(m), out.ctypes.data, out.size,
+                                              C.byref(n), C.byref(nrec)))
+        return out.tobytes()
 
     def submit(self, offsets, data, control) -> int:
         offs = (C.c_int32 * 3)(*[int(x) for x in offsets])

@@ -151,6 +151,17 @@ struct fxg_ctx {
   FinMfma* d_finm = nullptr;
   FinWeight* d_finw = nullptr;
   int maxoutp = 0, maxoutm = 0;
+  // amplitude scaling (Visibility::writedata)
+  std::vector<NormSeg> normseg;
+  std::vector<NormCal> normcal;     // + one zero entry at the end (bands without a mean)
+  std::vector<int> normcal_of;      // [ds][total band] -> normcal index (zero entry if none)
+  std::vector<int> acw_of;          // [ds][total band] -> float index of its parallel autocorr weight
+  int maxbands = 0;
+  NormSeg* d_normseg = nullptr;
+  NormCal* d_normcal = nullptr;
+  float* d_calmean = nullptr;
+  double* d_tsys = nullptr;
+  float* d_dpct = nullptr;
   // streams, staging, ring
   hipStream_t stream = nullptr;
   Staging stage[2];
@@ -445,6 +456,125 @@ void build_layout(fxg_ctx* ctx) {
 
 // For recorded band b of datastream ds: the partner band for cross-polarisation autos
 // (first two matching bands of its freq, mode.cpp:1320-1352), or -1.
+int crosspol_partner(const fxg_datastream& ds, int b);
+
+// zoom bandwidth over its parent's, in the reference's float steps (visibility.cpp:525-530)
+static float zoom_ratio(const fxg_ctx* ctx, const fxg_datastream& s, int band) {
+  const int lz = s.zoomband_localfreq[band - s.nrecbands];
+  float m = 1.0f;
+  m = (float)(m * ctx->freqs[s.zoomfreq_index[lz]].bw_mhz);
+  m = (float)(m / ctx->freqs[s.recfreq_index[s.zoomfreq_parent_localfreq[lz]]].bw_mhz);
+  return m;
+}
+
+// Configuration::getDMatchingBand (configuration.cpp:885-904) for recorded bands: the first
+// other band of the same local freq; -1 for zoom bands here
+static int matching_band(const fxg_datastream& s, int band) {
+  if (band >= s.nrecbands) return -1;
+  for (int i = 0; i < s.nrecbands; ++i)
+    if (i != band && s.recband_localfreq[i] == s.recband_localfreq[band]) return i;
+  return -1;
+}
+
+// The segments of Visibility::writedata's scaling in the results layout: cross-correlations
+// per (freq, baseline, product), autocorrelations per (datastream, product, band) when they
+// are written, and the parallel-autocorrelation band means the Tsys calibration uses.
+int build_norm(fxg_ctx* ctx) {
+  const int nds = ctx->nds, nb = ctx->nbl, B = ctx->B;
+  ctx->normseg.clear();
+  ctx->normcal.clear();
+  ctx->maxbands = 1;
+  for (int d = 0; d < nds; ++d)
+    ctx->maxbands = std::max(ctx->maxbands, ctx->dss[d].nrecbands + ctx->dss[d].nzoombands);
+  const int MB = ctx->maxbands;
+  ctx->normcal_of.assign((size_t)nds * MB, -1);
+  ctx->acw_of.assign((size_t)nds * MB, -1);
+  for (int d = 0; d < nds; ++d) {                  // j = 0 autocorrelation bands
+    const fxg_datastream& s = ctx->dss[d];
+    long long idx = ctx->acoff[d];
+    int widx = ctx->acwoff[d] * 2;
+    for (int k = 0; k < s.nrecbands + s.nzoombands; ++k) {
+      const int fi = freq_of_band(s, k);
+      if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
+      const int fc = ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
+      ctx->normcal_of[(size_t)d * MB + k] = (int)ctx->normcal.size();
+      ctx->normcal.push_back(NormCal{(int)idx, fc});
+      ctx->acw_of[(size_t)d * MB + k] = widx++;
+      idx += fc;
+    }
+  }
+  const int zero = (int)ctx->normcal.size();
+  ctx->normcal.push_back(NormCal{0, 0});           // never computed: stays 0
+  for (auto& c : ctx->normcal_of) if (c < 0) c = zero;
+  for (int f = 0; f < ctx->nfreqs; ++f) {          // cross-correlations (visibility.cpp:486-586)
+    if (!ctx->used[f]) continue;
+    const fxg_freq& fr = ctx->freqs[f];
+    const int fc = (fr.nchan + fr.chanavg - 1) / fr.chanavg;
+    for (int j = 0; j < nb; ++j) {
+      const fxg_baseline& b = ctx->bls[j];
+      for (int k = 0; k < b.nfreqs; ++k) {
+        if (b.freq_index[k] != f) continue;
+        for (int p = 0; p < b.npolprod[k]; ++p) {
+          NormSeg g{};
+          g.kind = 0;
+          g.off = ctx->boff[(size_t)f * nb + j] + p * fc;
+          g.nout = fc;
+          g.wf = ctx->bwoff[(size_t)f * nb + j] * 2 + p;
+          g.ds1 = b.ds1; g.ds2 = b.ds2;
+          const int b1 = b.ds1band[k][p], b2 = b.ds2band[k][p];
+          const fxg_datastream& s1 = ctx->dss[b.ds1];
+          const fxg_datastream& s2 = ctx->dss[b.ds2];
+          g.cal1 = ctx->normcal_of[(size_t)b.ds1 * MB + b1];
+          g.cal2 = ctx->normcal_of[(size_t)b.ds2 * MB + b2];
+          g.aw1 = ctx->acw_of[(size_t)b.ds1 * MB + b1];   // -1: weight 0
+          g.aw2 = ctx->acw_of[(size_t)b.ds2 * MB + b2];
+          g.zs1 = b1 >= s1.nrecbands ? std::sqrt(zoom_ratio(ctx, s1, b1)) : 1.0f;
+          g.zs2 = b2 >= s2.nrecbands ? std::sqrt(zoom_ratio(ctx, s2, b2)) : 1.0f;
+          g.chanfactor = (float)(B * 2 * fc * fr.chanavg);
+          ctx->normseg.push_back(g);
+        }
+      }
+    }
+  }
+  if (ctx->crossautos) {                           // autocorrelations (visibility.cpp:588-646)
+    for (int d = 0; d < nds; ++d) {
+      const fxg_datastream& s = ctx->dss[d];
+      long long idx = ctx->acoff[d];
+      int widx = ctx->acwoff[d] * 2;
+      for (int w = 0; w < ctx->bpa; ++w) {
+        for (int k = 0; k < s.nrecbands + s.nzoombands; ++k) {
+          const int fi = freq_of_band(s, k);
+          if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
+          const fxg_freq& fr = ctx->freqs[fi];
+          const int fc = fr.nchan / fr.chanavg;
+          NormSeg g{};
+          g.kind = 1;
+          g.off = (int)idx;
+          g.nout = fc;
+          g.wf = widx;
+          g.ds1 = g.ds2 = d;
+          g.cal1 = ctx->normcal_of[(size_t)d * MB + k];
+          const int mk = w == 0 ? k : matching_band(s, k);
+          g.cal2 = mk >= 0 ? ctx->normcal_of[(size_t)d * MB + mk] : zero;
+          g.zs1 = k >= s.nrecbands ? zoom_ratio(ctx, s, k) : 1.0f;
+          g.zs2 = 1.0f;
+          g.chanfactor = (float)(B * 2 * fc * fr.chanavg);
+          ctx->normseg.push_back(g);
+          idx += fc;
+          widx++;
+        }
+      }
+    }
+  }
+  if (int rc = dev_upload(ctx, &ctx->d_normseg, ctx->normseg)) return rc;
+  if (int rc = dev_upload(ctx, &ctx->d_normcal, ctx->normcal)) return rc;
+  if (int rc = dev_malloc(ctx, (void**)&ctx->d_calmean, sizeof(float) * ctx->normcal.size())) return rc;
+  if (int rc = dev_zero(ctx, ctx->d_calmean, sizeof(float) * ctx->normcal.size())) return rc;
+  if (int rc = dev_malloc(ctx, (void**)&ctx->d_tsys, sizeof(double) * nds)) return rc;
+  if (int rc = dev_malloc(ctx, (void**)&ctx->d_dpct, sizeof(float) * nds)) return rc;
+  return FXG_OK;
+}
+
 int crosspol_partner(const fxg_datastream& ds, int b) {
   int lf = ds.recband_localfreq[b];
   int idx[2] = {-1, -1}, n = 0;
@@ -901,6 +1031,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
     }
   }
   if (int rc = dev_upload(ctx, &ctx->d_finw, ctx->finw)) return rc;
+  if (int rc = build_norm(ctx)) return rc;
   if (int rc = dev_upload(ctx, &ctx->d_ds, ctx->h_ds)) return rc;
   return FXG_OK;
 }
@@ -1292,6 +1423,7 @@ void free_ctx(fxg_ctx* ctx) {
   f(ctx->d_counter); f(ctx->d_bigscratch); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
+  f(ctx->d_normseg); f(ctx->d_normcal); f(ctx->d_calmean); f(ctx->d_tsys); f(ctx->d_dpct);
   for (auto& sg : ctx->stage) {
     if (sg.h_run) (void)hipHostFree(sg.h_run);
     if (sg.h_flags) (void)hipHostFree(sg.h_flags);
@@ -1431,6 +1563,8 @@ extern "C" {
 
 int fxg_create(const fxg_config* cfg, fxg_ctx** out) { return create_impl(cfg, out, false); }
 
+int fxg_create_layout(const fxg_config* cfg, fxg_ctx** out) { return create_impl(cfg, out, true); }
+
 int fxg_footprint(const fxg_config* cfg, int64_t* device_bytes, int64_t* coreresultlength, int64_t* slot_bytes) {
   fxg_ctx* ctx = nullptr;
   int rc = create_impl(cfg, &ctx, true);
@@ -1462,8 +1596,10 @@ int fxg_device_bytes(const fxg_ctx* ctx, int64_t* bytes) {
 
 int fxg_destroy(fxg_ctx* ctx) {
   if (!ctx) return FXG_E_INVALID;
-  (void)hipDeviceSynchronize();
-  free_ctx(ctx);
+  if (!ctx->dry) {
+    (void)hipDeviceSynchronize();
+    free_ctx(ctx);
+  }
   delete ctx;
   return FXG_OK;
 }
@@ -1579,6 +1715,138 @@ int fxg_segment_params(const fxg_config* cfg, int32_t ds, int32_t numdatasegment
   return rc;
 }
 
+// ---- Visibility::writedata scaling and the DIFX record writer -----------------------------
+
+int fxg_normalise_device(fxg_ctx* ctx, float* d_results, double meansubints, const double* tsys, void* stream) {
+  if (!ctx || !d_results || !(meansubints > 0)) return FXG_E_INVALID;
+  if (ctx->dry) return FXG_E_STATE;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  static const float dpct_table[8] = {0.63662f, 0.88f, 0.94f, 0.96f, 0.98f, 0.99f, 0.996f, 0.998f};  // mode.cpp:1571
+  const int nds = ctx->nds;
+  std::vector<double> t(nds, 0.0);
+  std::vector<float> dp(nds, 0.0f);
+  bool anyt = false;
+  for (int d = 0; d < nds; ++d) {
+    if (tsys) t[d] = tsys[d];
+    anyt = anyt || t[d] > 0.0;
+    const int nb = ctx->dss[d].nbits;
+    dp[d] = (nb >= 1 && nb <= 8) ? dpct_table[nb - 1] : 0.0f;
+  }
+  hipStream_t st = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->stream;
+  HIPCHK(hipMemcpyAsync(ctx->d_tsys, t.data(), sizeof(double) * nds, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(ctx->d_dpct, dp.data(), sizeof(float) * nds, hipMemcpyHostToDevice, st));
+  NormArgs a;
+  a.res = d_results;
+  a.seg = ctx->d_normseg;
+  a.cal = ctx->d_normcal;
+  a.calmean = ctx->d_calmean;
+  a.tsys = ctx->d_tsys;
+  a.dpct = ctx->d_dpct;
+  a.msi = meansubints;
+  a.fftsperint = meansubints * ctx->B;
+  const int ncal = anyt ? (int)ctx->normcal.size() - 1 : 0;   // (not the final zero entry)
+  if (fxg_launch_normalise(a, (int)ctx->normseg.size(), ncal, st))
+    return fail(ctx, FXG_E_HIP, "normalisation launch failed");
+  HIPCHK(hipStreamSynchronize(st));                // the host arrays above are copied from
+  return FXG_OK;
+}
+
+static char opposite_pol(char p) {                   // Configuration::getOppositePol
+  return p == 'R' ? 'L' : p == 'L' ? 'R' : p == 'X' ? 'Y' : 'X';
+}
+
+static char band_pol(const fxg_datastream& s, int b) {
+  return (char)(b < s.nrecbands ? s.recband_pol[b] : s.zoomband_pol[b - s.nrecbands]);
+}
+
+// Visibility::appendSWINHeaderBuffered (visibility.cpp:1205-1234): 74 bytes, little-endian
+static void put_header(uint8_t* o, int32_t bl, int32_t mjd, double sec, int32_t cfgi, int32_t src, int32_t freq,
+                       char p0, char p1, int32_t bin, double weight, const double* uvw) {
+  const uint32_t sync = 0xFF00FF00u;              // SYNC_WORD (visibility.h:125)
+  const int32_t version = 1;                       // BINARY_HEADER_VERSION
+  std::memcpy(o + 0, &sync, 4);
+  std::memcpy(o + 4, &version, 4);
+  std::memcpy(o + 8, &bl, 4);
+  std::memcpy(o + 12, &mjd, 4);
+  std::memcpy(o + 16, &sec, 8);
+  std::memcpy(o + 24, &cfgi, 4);
+  std::memcpy(o + 28, &src, 4);
+  std::memcpy(o + 32, &freq, 4);
+  o[36] = (uint8_t)p0;
+  o[37] = (uint8_t)p1;
+  std::memcpy(o + 38, &bin, 4);
+  std::memcpy(o + 42, &weight, 8);
+  std::memcpy(o + 50, uvw, 24);
+}
+
+int fxg_difx_records(const fxg_ctx* ctx, const float* results, const fxg_difx_meta* m, uint8_t* out,
+                     int64_t capacity, int64_t* nbytes, int32_t* nrecords) {
+  if (!ctx || !results || !m || !nbytes || !(m->fftsperintegration > 0)) return FXG_E_INVALID;
+  const int nb = ctx->nbl;
+  if (nb > 0 && (!m->baselinenumber || !m->uvw)) return FXG_E_INVALID;
+  if (ctx->crossautos && !m->telescopeindex) return FXG_E_INVALID;
+  int64_t pos = 0;
+  int32_t nrec = 0;
+  auto emit = [&](int32_t bl, int32_t freq, char p0, char p1, double w, const double* uvw, const float* data, int fc) {
+    const int64_t len = 74 + 8LL * fc;
+    if (out) {
+      if (pos + len > capacity) return false;
+      put_header(out + pos, bl, m->mjd, m->seconds, m->configindex, m->sourceindex, freq, p0, p1, 0, w, uvw);
+      std::memcpy(out + pos + 74, data, 8 * (size_t)fc);
+    }
+    pos += len;
+    nrec++;
+    return true;
+  };
+  // cross-correlations (visibility.cpp:815-877): baseline, then output freq, then product
+  for (int i = 0; i < nb; ++i) {
+    const fxg_baseline& b = ctx->bls[i];
+    for (int f = 0; f < ctx->nfreqs; ++f) {
+      int k = -1;
+      for (int q = 0; q < b.nfreqs; ++q) if (b.freq_index[q] == f) { k = q; break; }
+      if (k < 0) continue;
+      const fxg_freq& fr = ctx->freqs[f];
+      const int fc = fr.nchan / fr.chanavg;
+      const int core = ctx->boff[(size_t)f * nb + i];
+      for (int p = 0; p < b.npolprod[k]; ++p) {
+        const float w = (float)(results[ctx->bwoff[(size_t)f * nb + i] * 2 + p] / m->fftsperintegration);
+        if (!(w > 0.0f)) continue;
+        if (!emit(m->baselinenumber[i], f, band_pol(ctx->dss[b.ds1], b.ds1band[k][p]),
+                  band_pol(ctx->dss[b.ds2], b.ds2band[k][p]), (double)w, m->uvw + 3 * i,
+                  results + 2LL * (core + (long long)p * fc), fc))
+          return FXG_E_INVALID;
+      }
+    }
+  }
+  // autocorrelations (visibility.cpp:889-931)
+  if (ctx->crossautos) {
+    const double zero[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < ctx->nds; ++d) {
+      const fxg_datastream& s = ctx->dss[d];
+      const int32_t bl = 257 * (m->telescopeindex[d] + 1);
+      long long idx = ctx->acoff[d];
+      int widx = ctx->acwoff[d] * 2;
+      for (int j = 0; j < ctx->bpa; ++j) {
+        for (int k = 0; k < s.nrecbands + s.nzoombands; ++k) {
+          const int fi = freq_of_band(s, k);
+          if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
+          const int fc = ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
+          const float w = (float)(results[widx++] / m->fftsperintegration);
+          if (w > 0.0f) {
+            const char p0 = band_pol(s, k);
+            if (!emit(bl, fi, p0, j == 0 ? p0 : opposite_pol(p0), (double)w, zero, results + 2 * idx, fc))
+              return FXG_E_INVALID;
+          }
+          idx += fc;
+        }
+      }
+    }
+  }
+  *nbytes = pos;
+  if (nrecords) *nrecords = nrec;
+  return FXG_OK;
+}
+
 int fxg_result_offsets(const fxg_ctx* ctx, int32_t* baseline, int32_t* bweight, int32_t* autocorr,
                        int32_t* acweight) {
   if (!ctx) return FXG_E_INVALID;
@@ -1592,6 +1860,7 @@ int fxg_result_offsets(const fxg_ctx* ctx, int32_t* baseline, int32_t* bweight, 
 int fxg_process_subint_device(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_data,
                               const int32_t* nbytes, const int32_t* const* control, float* d_results,
                               void* stream) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || !offsets || !d_data || !nbytes || !control || !d_results) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
@@ -1602,6 +1871,7 @@ int fxg_ring_length(const fxg_ctx* ctx) { return ctx ? RING : 0; }
 
 int fxg_submit_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* data,
                       const int32_t* nbytes, const int32_t* const* control, fxg_handle* handle) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || !offsets || !data || !nbytes || !control || !handle) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
   int rc = ensure_slots(ctx);
@@ -1630,6 +1900,7 @@ int fxg_submit_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* con
 }
 
 int fxg_wait(fxg_ctx* ctx, fxg_handle handle, float* results) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || handle < 0 || handle >= RING) return FXG_E_INVALID;
   Slot& s = ctx->slots[handle];
   if (!s.busy) return fail(ctx, FXG_E_STATE, "wait on a slot that was not submitted");
@@ -1642,6 +1913,7 @@ int fxg_wait(fxg_ctx* ctx, fxg_handle handle, float* results) {
 
 int fxg_process_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* data,
                        const int32_t* nbytes, const int32_t* const* control, float* results) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   fxg_handle h;
   int rc = fxg_submit_subint(ctx, offsets, data, nbytes, control, &h);
   if (rc) return rc;
@@ -1649,6 +1921,7 @@ int fxg_process_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* co
 }
 
 int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || !out || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
   fxg_ctx* c = const_cast<fxg_ctx*>(ctx);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return FXG_E_HIP;
@@ -1658,6 +1931,7 @@ int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out) {
 }
 
 int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, float* out) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || !out || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
   const DevDs& x = ctx->h_ds[ds];
   if (band < 0 || band >= x.nrecbands) return FXG_E_INVALID;
@@ -1705,6 +1979,7 @@ int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, 
 
 int fxg_unpack_device(fxg_ctx* ctx, int32_t ds, const uint8_t* d_data, int32_t nbytes, int64_t startsample,
                       int32_t nsamples, float* d_out, int32_t* d_goodsamples, void* stream) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || ds < 0 || ds >= ctx->nds || !d_data || !d_out || !d_goodsamples || nsamples <= 0 || startsample < 0)
     return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -1754,6 +2029,7 @@ int fxg_unpack_device(fxg_ctx* ctx, int32_t ds, const uint8_t* d_data, int32_t n
 }
 
 int fxg_delay_interpolator(fxg_ctx* ctx, int32_t ds, int32_t sec, int32_t ns, double* out) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || !out || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
   hipStream_t st = ctx->stream;
@@ -1798,6 +2074,7 @@ int fxg_autotune_report(const fxg_ctx* ctx, char* buf, int32_t len) {
 }
 
 int fxg_set_timing(fxg_ctx* ctx, int32_t enable) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
   HIPCHK(hipDeviceSynchronize());
@@ -1811,6 +2088,7 @@ int fxg_set_timing(fxg_ctx* ctx, int32_t enable) {
 }
 
 int fxg_get_timing(fxg_ctx* ctx, fxg_timing* out) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || !out) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
   std::memset(out, 0, sizeof(*out));

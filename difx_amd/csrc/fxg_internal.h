@@ -240,3 +240,35 @@ int fxg_launch_fin_weights(const FinWeight* e, int n, const float* w, int B, flo
 int fxg_launch_zero(void* p, long long bytes, hipStream_t st);
 int fxg_launch_fill(float* p, long long n, float v, hipStream_t st);
 int fxg_launch_add(float* dst, const float* src, long long n, hipStream_t st);
+
+// Visibility::writedata's amplitude scaling (visibility.cpp:353-646), one segment per
+// (baseline, freq, product) of cross-correlations and per (datastream, product, band) of
+// autocorrelations.  Scales follow the reference's float / double order; zoom factors are
+// precomputed on the host (cross: sqrt of the bandwidth ratio, autos: the ratio).
+struct NormSeg {
+  int off;            // cf32 offset of the first channel in the results
+  int nout;           // channels
+  int wf;             // float index of the segment's weight in the results
+  int kind;           // 0 cross, 1 autocorrelation
+  int ds1, ds2;       // datastreams (autos: ds2 = ds1)
+  int cal1, cal2;     // Tsys: band-mean indices (cross: parallel autos of each band; autos: k and
+                      // its matching band); -1 when that datastream has no Tsys
+  int aw1, aw2;       // Tsys, cross: float indices of the two bands' parallel autocorrelation weights
+  float zs1, zs2;     // zoom factors (1 for recorded bands)
+  float chanfactor;   // (float)(blockspersend * 2 * freqchannels * chanavg)
+  int pad;
+};
+// one band mean of a parallel autocorrelation (the Tsys calibration's autocorrcalibs)
+struct NormCal {
+  int off, nout;
+};
+struct NormArgs {
+  float* res;                 // results, cf32 as floats
+  const NormSeg* seg;
+  const NormCal* cal;
+  float* calmean;             // [ncal] real parts of the band means
+  const double* tsys;         // [nds]
+  const float* dpct;          // [nds] decorrelation fraction of each datastream's bits
+  double fftsperint, msi;
+};
+int fxg_launch_normalise(const NormArgs& a, int nseg, int ncal, hipStream_t st);

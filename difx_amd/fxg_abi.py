@@ -193,9 +193,18 @@ class fxg_databuffer(C.Structure):
 FXG_ADVANCE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32)
 
 
+class fxg_difx_meta(C.Structure):
+    _fields_ = [
+        ("mjd", C.c_int32), ("configindex", C.c_int32), ("sourceindex", C.c_int32), ("reserved", C.c_int32),
+        ("seconds", C.c_double), ("fftsperintegration", C.c_double),
+        ("baselinenumber", C.POINTER(C.c_int32)), ("uvw", C.POINTER(C.c_double)),
+        ("telescopeindex", C.POINTER(C.c_int32)),
+    ]
+
+
 # exported symbols of libfxg.so (every function declared in include/fxg.h)
 EXPORTED_SYMBOLS = [
-    "fxg_create", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
+    "fxg_create", "fxg_create_layout", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
     "fxg_data_bytes", "fxg_result_offsets", "fxg_process_subint",
     "fxg_process_subint_device", "fxg_ring_length", "fxg_submit_subint", "fxg_wait",
     "fxg_get_spectrum", "fxg_get_data_weights", "fxg_unpack_device",
@@ -205,4 +214,5 @@ EXPORTED_SYMBOLS = [
     "fxg_server_wait_job", "fxg_server_get_stats", "fxg_server_job_error",
     "fxg_vdif_mux_workspace_bytes", "fxg_vdif_mux_device", "fxg_vdif_mux_result",
     "fxg_segment_params", "fxg_buffer_params", "fxg_control_params",
+    "fxg_normalise_device", "fxg_difx_records",
 ]

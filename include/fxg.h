@@ -145,6 +145,9 @@ typedef int32_t fxg_handle;
  * device-resident state.  Replaces Configuration::getMode + the Core constructor
  * (configuration.cpp:932-998, core.cpp:23-154). */
 int fxg_create(const fxg_config* cfg, fxg_ctx** out);
+/* A context without device state (no device needed): layout queries, fxg_difx_records;
+ * the processing calls return FXG_E_STATE on it.  Released with fxg_destroy. */
+int fxg_create_layout(const fxg_config* cfg, fxg_ctx** out);
 int fxg_destroy(fxg_ctx* ctx);
 const char* fxg_strerror(int code);
 const char* fxg_last_error(const fxg_ctx* ctx);
@@ -346,6 +349,35 @@ int fxg_buffer_params(const fxg_config* cfg, int32_t ds, int32_t numdatasegments
 int fxg_control_params(const fxg_databuffer* buf, int32_t* atsegment, int32_t scan, int32_t offsetsec,
                        int32_t offsetns, int32_t delay_ok, double delayus1, double delayus2,
                        fxg_advance_fn advance, void* user, int32_t* control, int32_t* bufferindex);
+
+/* ---- Normalisation and the DIFX record writer ------------------------------------------
+ * Visibility::writedata's amplitude scaling (visibility.cpp:353-646) and writeSWIN's binary
+ * records (visibility.cpp:781-931, header appendSWINHeaderBuffered :1205-1234, SYNC_WORD
+ * visibility.h:125), for one phase centre and no pulsar bins. */
+/* Scale a results array (device memory; one slot, or the sum of meansubints slots) in place:
+ * cross-correlations by 1/(weight * meansubints * blockspersend * 2 * nchan) (with the zoom
+ * bandwidth factor), or, where both datastreams have tsys[ds] > 0, by
+ * sqrt(Tsys1 Tsys2 / divisor) / weight from the band means of the parallel autocorrelations;
+ * autocorrelations likewise when the configuration writes them.  tsys may be null (none).
+ * Returns when the scaling is done. */
+int fxg_normalise_device(fxg_ctx* ctx, float* d_results, double meansubints, const double* tsys,
+                         void* stream);
+typedef struct fxg_difx_meta {
+  int32_t mjd;                     /* integration centre: MJD */
+  int32_t configindex;
+  int32_t sourceindex;             /* phase centre source index */
+  int32_t reserved;
+  double  seconds;                 /* integration centre: seconds of the day */
+  double  fftsperintegration;      /* meansubints * blockspersend (record weights = weight / it) */
+  const int32_t* baselinenumber;   /* [nbaselines] Configuration::getBNumber */
+  const double* uvw;               /* [nbaselines][3] metres (Model::interpolateUVW) */
+  const int32_t* telescopeindex;   /* [ndatastreams] for the autocorrelation baseline numbers */
+} fxg_difx_meta;
+/* The DIFX records of a normalised results array (host memory) into out[capacity]: cross-
+ * correlations by baseline, freq, product; then autocorrelations when the configuration
+ * writes them; a product of weight 0 is skipped.  out = null: size only. */
+int fxg_difx_records(const fxg_ctx* ctx, const float* results, const fxg_difx_meta* meta, uint8_t* out,
+                     int64_t capacity, int64_t* nbytes, int32_t* nrecords);
 
 /* ---- Serving: concurrent correlation jobs on one device ------------------------------
  * No counterpart in the reference, which runs one job per mpifxcorr process; this replaces
