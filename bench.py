@@ -345,6 +345,36 @@ def mux_bench(torch, dev, steps=5, warmup=1):
     return res
 
 
+def difx_bench(cfg, results: np.ndarray, torch, dev, reps: int = 20):
+    """Output side of one headline subint: Visibility::writedata's scaling on the device
+    (fxg_normalise_device, one call = cal + apply kernels + its synchronisation) and the DIFX
+    records written on the host (fxg_difx_records)."""
+    from difx_amd.core import FxCore
+    core = FxCore(cfg)
+    d = torch.from_numpy(results.view(np.float32).copy()).to(dev)
+    core.normalise_device(d.data_ptr(), 1.0)            # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        core.normalise_device(d.data_ptr(), 1.0)
+    norm_ms = (time.perf_counter() - t0) * 1e3 / reps
+    host = d.cpu().numpy().view(np.complex64)
+    nb = len(cfg.baselines)
+    bl = [256 * (b.ds1 + 1) + b.ds2 + 1 for b in cfg.baselines]
+    uvw = np.zeros((nb, 3))
+    tel = list(range(len(cfg.datastreams)))
+    t0 = time.perf_counter()
+    for _ in range(3):
+        recs = core.difx_records(host, 60000, 0.25, 0, 0, float(cfg.blockspersend()), bl, uvw, tel)
+    rec_ms = (time.perf_counter() - t0) * 1e3 / 3
+    core.close()
+    vis_bytes = results.nbytes
+    return {"workload": "scaling and DIFX records of one subint's results of the headline configuration "
+                        f"({results.size} cf32, {vis_bytes / 1e6:.1f} MB)",
+            "normalise_ms_per_call": norm_ms, "normalise_GBps_incl_sync": 2 * vis_bytes / (norm_ms / 1e3) / 1e9,
+            "records_bytes": len(recs), "records_ms_host": rec_ms,
+            "records_GBps_host": len(recs) / (rec_ms / 1e3) / 1e9}
+
+
 def mux_cpu_baseline(budget_s: float = 6.0):
     """The corner-turn restatement (oracle/vdifmux.py, numpy) on segments of 20 frame times of
     the same 16-thread workload, repeated for about budget_s."""
@@ -461,6 +491,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--no-mux", action="store_true")
+    ap.add_argument("--no-output", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copy for PMC scaling")
     args = ap.parse_args()
     if args.copy_probe:
@@ -553,6 +584,10 @@ def main():
     if not args.no_serving:
         print("[bench] serving: Poisson job stream", file=sys.stderr, flush=True)
         serving = serving_bench(torch, dev)
+    output = None
+    if not args.no_output:
+        print("[bench] output side: scaling and DIFX records of one subint", file=sys.stderr, flush=True)
+        output = difx_bench(cfg, head["results"], torch, dev)
     mux = None
     if not args.no_mux:
         print("[bench] VDIF thread corner turn, 1 warmup + 5 steps", file=sys.stderr, flush=True)
@@ -599,6 +634,7 @@ def main():
         "uneven_bands": uneven,
         "serving": serving,
         "vdif_corner_turn": mux,
+        "output_side": output,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
