@@ -64,7 +64,9 @@ def test_ring_start_choice_correlates(gpu):
               flush=True)
         assert c[1] != abi.FXG_INVALID_SUBINT and bi.value % fb == 0
         assert all(int(w) == (1 << 30) - 1 for w in c[3:3 + B // 30])
-        data.append(ring[bi.value:])
+        # DataStream sends sendbytes from the chosen byte (datastream.cpp:294-314)
+        data.append(ring[bi.value:bi.value + segs[at.value].sendbytes])
+        assert data[-1].size == segs[at.value].sendbytes
         ctrl.append(c)
     core = FxCore(cfg)
     got = core.processdata(offsets, data, ctrl)
