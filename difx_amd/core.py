@@ -82,6 +82,8 @@ def load_library() -> C.CDLL:
     lib.fxg_vdif_mux_result.argtypes = [C.c_void_p, C.c_void_p, P(C.c_int64), P(C.c_int64)]
     lib.fxg_segment_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_segment)]
     lib.fxg_buffer_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_databuffer)]
+    lib.fxg_set_kurtosis.argtypes = [C.c_void_p, C.c_int32]
+    lib.fxg_get_kurtosis.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, P(C.c_int32)]
     lib.fxg_normalise_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
     lib.fxg_difx_records.argtypes = [C.c_void_p, C.c_void_p, P(abi.fxg_difx_meta), C.c_void_p, C.c_int64,
                                      P(C.c_int64), P(C.c_int32)]
@@ -194,6 +196,18 @@ class FxCore:
         self._check(self.lib.fxg_process_subint_device(
             self.h, offs, _ptr_array(list(d_data)), nb, _ptr_array([c.ctypes.data for c in ctrl]),
             C.c_void_p(d_results), C.c_void_p(stream) if stream else None))
+
+    def set_kurtosis(self, enable: bool):
+        """Mode::setDumpKurtosis: sum |X|^2 and |X|^4 per channel in every following subint."""
+        self._check(self.lib.fxg_set_kurtosis(self.h, int(bool(enable))))
+
+    def getKurtosis(self, ds: int, band: int, maxchannels: int) -> np.ndarray:
+        """Mode::calculateAndAverageKurtosis + getKurtosis for the last subint (empty when the
+        band had no data)."""
+        out = np.zeros(max(maxchannels, 1) + 65536, dtype=np.float32)
+        n = C.c_int32()
+        self._check(self.lib.fxg_get_kurtosis(self.h, ds, band, maxchannels, out.ctypes.data, C.byref(n)))
+        return out[:n.value].copy()
 
     def normalise_device(self, d_results: int, meansubints: float, tsys=None, stream: int = 0):
         """Visibility::writedata's amplitude scaling of a device results array, in place."""

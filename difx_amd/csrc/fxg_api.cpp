@@ -162,6 +162,14 @@ struct fxg_ctx {
   float* d_calmean = nullptr;
   double* d_tsys = nullptr;
   float* d_dpct = nullptr;
+  // spectral kurtosis sums (off until fxg_set_kurtosis)
+  bool kurt_on = false;
+  std::vector<KurtBand> kurt_tab;
+  std::vector<int> kurt_index;     // [ds] first kurt_tab entry of the datastream
+  KurtBand* d_kurt_tab = nullptr;
+  float2* d_kurt_acc = nullptr;
+  long long kurt_len = 0;          // float2 sums
+  int kurt_maxC = 0;
   // streams, staging, ring
   hipStream_t stream = nullptr;
   Staging stage[2];
@@ -1276,6 +1284,8 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   fa.fvalid = ctx->d_fvalid;
   fa.chunkcap = ctx->chunkcap;
   fa.rotorder = ctx->rotorder;
+  if (ctx->kurt_on && fxg_launch_zero(ctx->d_kurt_acc, sizeof(float2) * ctx->kurt_len, st))
+    return fail(ctx, FXG_E_HIP, "kurtosis zero launch");
   for (int c0 = 0; c0 < ctx->B; c0 += ctx->chunkcap) {
     const int n = std::min(ctx->chunkcap, ctx->B - c0);
     const int npad = (int)round_up(n, FXG_XPAD);
@@ -1317,6 +1327,20 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
       TimedLaunch tl(ctx, 1, st);
       if (fxg_launch_xplain(xa, (int)ctx->prods.size(), ctx->maxchan, st))
         return fail(ctx, FXG_E_HIP, "X-engine launch");
+    }
+    if (ctx->kurt_on) {                            // before the next chunk overwrites the spectra
+      KurtArgs ka;
+      ka.tab = ctx->d_kurt_tab;
+      ka.spec32 = ctx->d_spec32;
+      ka.spec16 = ctx->d_spec16;
+      ka.xscale = ctx->d_xscale;
+      ka.acc = ctx->d_kurt_acc;
+      ka.nffts = n;
+      ka.chunkcap = ctx->chunkcap;
+      ka.mode = ctx->xengine;
+      TimedLaunch tk(ctx, 2, st);
+      if (fxg_launch_kurt(ka, (int)ctx->kurt_tab.size(), ctx->kurt_maxC, st))
+        return fail(ctx, FXG_E_HIP, "kurtosis launch");
     }
     ctx->last_chunk0 = c0;
     ctx->last_nffts = n;
@@ -1424,6 +1448,7 @@ void free_ctx(fxg_ctx* ctx) {
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   f(ctx->d_normseg); f(ctx->d_normcal); f(ctx->d_calmean); f(ctx->d_tsys); f(ctx->d_dpct);
+  f(ctx->d_kurt_tab); f(ctx->d_kurt_acc);
   for (auto& sg : ctx->stage) {
     if (sg.h_run) (void)hipHostFree(sg.h_run);
     if (sg.h_flags) (void)hipHostFree(sg.h_flags);
@@ -1927,6 +1952,91 @@ int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out) {
   if (hipStreamSynchronize(c->stream) != hipSuccess) return FXG_E_HIP;
   if (hipMemcpy(out, ctx->d_weights + (size_t)ds * ctx->B, sizeof(float) * ctx->B, hipMemcpyDeviceToHost) != hipSuccess)
     return FXG_E_HIP;
+  return FXG_OK;
+}
+
+int fxg_set_kurtosis(fxg_ctx* ctx, int32_t enable) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
+  if (!ctx) return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (enable && !ctx->d_kurt_tab) {
+    ctx->kurt_tab.clear();
+    ctx->kurt_index.assign(ctx->nds, 0);
+    long long acc = 0;
+    for (int d = 0; d < ctx->nds; ++d) {
+      const DevDs& x = ctx->h_ds[d];
+      ctx->kurt_index[d] = (int)ctx->kurt_tab.size();
+      for (int b = 0; b < x.nrecbands; ++b) {
+        KurtBand k{};
+        k.C = x.C;
+        if (ctx->xengine == FXG_X_FP32) {
+          k.off = x.spec_off + (long long)b * ctx->chunkcap * x.C;
+          k.scale_idx = 0;
+        } else {
+          const DevXGroup& g = ctx->groups[x.band_xfreq[b]];
+          k.off = g.spec_off + (long long)x.band_xrow[b] * g.rowstride;
+          k.scale_idx = g.scale_off + x.band_xrow[b];
+        }
+        k.acc_off = (int)acc;
+        acc += x.C;
+        ctx->kurt_maxC = std::max(ctx->kurt_maxC, x.C);
+        ctx->kurt_tab.push_back(k);
+      }
+    }
+    ctx->kurt_len = std::max<long long>(acc, 1);
+    if (int rc = dev_upload(ctx, &ctx->d_kurt_tab, ctx->kurt_tab)) return rc;
+    if (int rc = dev_malloc(ctx, (void**)&ctx->d_kurt_acc, sizeof(float2) * ctx->kurt_len)) return rc;
+    if (int rc = dev_zero(ctx, ctx->d_kurt_acc, sizeof(float2) * ctx->kurt_len)) return rc;
+  }
+  ctx->kurt_on = enable != 0;
+  return FXG_OK;
+}
+
+// Mode::calculateAndAverageKurtosis (mode.cpp:1405-1450) on the last subint's sums, with the
+// subint's FFT blocks as numblocks, in float as the reference computes it
+int fxg_get_kurtosis(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t maxchannels, float* out,
+                     int32_t* nout) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
+  if (!ctx || !out || !nout || ds < 0 || ds >= ctx->nds || maxchannels < 1) return FXG_E_INVALID;
+  if (!ctx->kurt_on || !ctx->d_kurt_acc) return FXG_E_STATE;
+  if (band < 0 || band >= ctx->h_ds[ds].nrecbands) return FXG_E_INVALID;
+  const KurtBand& k = ctx->kurt_tab[ctx->kurt_index[ds] + band];
+  const int C = k.C;
+  std::vector<float2> v((size_t)C);
+  if (hipDeviceSynchronize() != hipSuccess) return FXG_E_HIP;
+  if (hipMemcpy(v.data(), ctx->d_kurt_acc + k.acc_off, sizeof(float2) * C, hipMemcpyDeviceToHost) != hipSuccess)
+    return FXG_E_HIP;
+  bool nonzero = false;
+  for (int j = 0; j < C; ++j)
+    if (v[j].x > 0.0f) { nonzero = true; break; }
+  if (!nonzero) { *nout = 0; return FXG_OK; }     // the reference sends no record
+  const int numblocks = ctx->B;
+  std::vector<float> sk((size_t)C);
+  const float nb = (float)numblocks;
+  const float nbfac = (float)((float)numblocks / (numblocks - 1.0));
+  for (int j = 0; j < C; ++j) {
+    const float s1sq = v[j].x * v[j].x;
+    const float s2 = v[j].y * nb;
+    float x = s2 / s1sq;
+    x = x + -1.0f;
+    x = x * nbfac;
+    x = x + -1.0f;
+    sk[j] = x;
+  }
+  if (maxchannels < C) {
+    const int kchanavg = C / maxchannels;
+    const float inv = (float)(1.0 / ((float)kchanavg));
+    for (int j = 0; j < C; ++j) sk[j] *= inv;
+    for (int j = 0; j < maxchannels; ++j) {
+      float a = sk[(size_t)j * kchanavg];
+      for (int q = 1; q < kchanavg; ++q) a += sk[(size_t)j * kchanavg + q];
+      out[j] = a;
+    }
+    *nout = maxchannels;
+  } else {
+    for (int j = 0; j < C; ++j) out[j] = sk[j];
+    *nout = C;
+  }
   return FXG_OK;
 }
 

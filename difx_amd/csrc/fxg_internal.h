@@ -272,3 +272,23 @@ struct NormArgs {
   double fftsperint, msi;
 };
 int fxg_launch_normalise(const NormArgs& a, int nseg, int ncal, hipStream_t st);
+
+// Spectral kurtosis sums (Mode::process, mode.cpp:1237-1253): per recorded band and channel,
+// the sums over a subint's FFTs of |X|^2 and |X|^4, read from the spectra chunk the F-engine
+// wrote.
+struct KurtBand {
+  long long off;      // FP32 spectra: float2 offset of (band, FFT 0); FP16/BF16: uint32 offset of the row
+  int C;              // channels
+  int scale_idx;      // FP16/BF16: xscale index of the row
+  int acc_off;        // float2 offset of the band's (s1, s2) sums
+  int pad;
+};
+struct KurtArgs {
+  const KurtBand* tab;
+  const float2* spec32;
+  const uint32_t* spec16;
+  const float* xscale;
+  float2* acc;
+  int nffts, chunkcap, mode;   // mode: FXG_X_FP32, FXG_X_FP16 or FXG_X_BF16
+};
+int fxg_launch_kurt(const KurtArgs& a, int nband, int maxC, hipStream_t st);

@@ -214,5 +214,5 @@ EXPORTED_SYMBOLS = [
     "fxg_server_wait_job", "fxg_server_get_stats", "fxg_server_job_error",
     "fxg_vdif_mux_workspace_bytes", "fxg_vdif_mux_device", "fxg_vdif_mux_result",
     "fxg_segment_params", "fxg_buffer_params", "fxg_control_params",
-    "fxg_normalise_device", "fxg_difx_records",
+    "fxg_normalise_device", "fxg_difx_records", "fxg_set_kurtosis", "fxg_get_kurtosis",
 ]

@@ -189,6 +189,18 @@ int fxg_wait(fxg_ctx* ctx, fxg_handle handle, float* results);
 int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, float* out);
 int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out /* [blockspersend] */);
 
+/* Spectral kurtosis (Mode::setDumpKurtosis / calculateAndAverageKurtosis, mode.cpp:1237-1253,
+ * 1405-1450; sent by Core::averageAndSendKurtosis, core.cpp:1378-1429).  When enabled, every
+ * subint also sums |X|^2 and |X|^4 per recorded band and channel over its FFTs (one more pass
+ * over each spectra chunk).  fxg_get_kurtosis gives the last subint's estimator
+ * (numblocks s2 / s1^2 - 1) numblocks/(numblocks-1) - 1, numblocks = blockspersend, summed
+ * over groups of C/maxchannels channels (and divided by the group size) when maxchannels < C;
+ * *nout = the values written (0 when the band had no data, where the reference sends no
+ * record). */
+int fxg_set_kurtosis(fxg_ctx* ctx, int32_t enable);
+int fxg_get_kurtosis(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t maxchannels, float* out,
+                     int32_t* nout);
+
 /* Mk5Mode::unpack (mk5mode.cpp:94-174) as a device call: decode nsamples time samples starting
  * at sample `startsample` of datastream ds from d_data (nbytes) into d_out[nrecbands][nsamples]
  * (f32, or interleaved cf32 for complex sampling), and the count of valid samples into
