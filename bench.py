@@ -219,9 +219,11 @@ def serving_bench(torch, dev, window_s=6.0, rate_per_s=4.0, subints_per_job=8, s
             "job_shapes_n_m_nchan": [list(j[3]) for j in jobs]}
 
 
-def time_path(cfg, steps, warmup, torch, dev, offsets):
+def time_path(cfg, steps, warmup, torch, dev, offsets, kurtosis=False):
     from difx_amd.core import FxCore
     core = FxCore(cfg)
+    if kurtosis:
+        core.set_kurtosis(True)
     bufs, nbytes, ctrl = device_slot(cfg, offsets, torch, dev)
     res = torch.zeros(2 * core.coreresultlength, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
@@ -492,6 +494,7 @@ def main():
     ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--no-mux", action="store_true")
     ap.add_argument("--no-output", action="store_true")
+    ap.add_argument("--no-kurtosis", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copy for PMC scaling")
     args = ap.parse_args()
     if args.copy_probe:
@@ -584,6 +587,17 @@ def main():
     if not args.no_serving:
         print("[bench] serving: Poisson job stream", file=sys.stderr, flush=True)
         serving = serving_bench(torch, dev)
+    kurt = None
+    if not args.no_kurtosis:
+        print("[bench] headline path with spectral kurtosis, 1 warmup + 3 steps", file=sys.stderr, flush=True)
+        kr = time_path(cfg, 3, 1, torch, dev, offsets, kurtosis=True)
+        k_ms = kr["wall_s"] * 1e3 / 3
+        spec_bytes = float(samples_per_subint(cfg)) / 2 * 4           # FP16 complex spectra: C = L/2 per FFT
+        kurt = {"workload": "headline subint with Mode's kurtosis sums (|X|^2, |X|^4 per band and channel)",
+                "ms_per_step": k_ms, "extra_ms_vs_headline": k_ms - head["wall_s"] * 1e3 / args.steps,
+                "other_ms_per_step": kr["timing"]["other_ms"] / 3,
+                "spectra_bytes_read_per_step": spec_bytes,
+                "kurtosis_pass_GBps": spec_bytes / max((kr["timing"]["other_ms"] - tm["other_ms"] / args.steps * 3) / 3 / 1e3, 1e-9) / 1e9}
     output = None
     if not args.no_output:
         print("[bench] output side: scaling and DIFX records of one subint", file=sys.stderr, flush=True)
@@ -635,6 +649,7 @@ def main():
         "serving": serving,
         "vdif_corner_turn": mux,
         "output_side": output,
+        "kurtosis": kurt,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -1,53 +1,76 @@
 // Spectral kurtosis sums on the device: Mode::process accumulates, for every recorded band and
 // channel, s1 += |X|^2 and s2 += |X|^4 over the FFTs of its block range (mpifxcorr/src/
-// mode.cpp:1237-1253, magnitude then two squares).  Here one lane per (band, channel) walks the
-// FFTs of the spectra chunk in order and adds its partial sums to the band's accumulators; the
-// chunks of a subint are launched in order on one stream, so no two lanes share a sum.
+// mode.cpp:1237-1253, magnitude then two squares).  Here the sums of each spectra chunk are
+// added to the band's accumulators by one lane per channel; the chunks of a subint are launched
+// in order on one stream, so no two lanes share a sum.
 #include "fxg_internal.h"
 #include <hip/hip_fp16.h>
 
 namespace {
 
-__device__ __forceinline__ float2 spec_value(const KurtArgs& a, const KurtBand& b, int c, int lt) {
+// the stored value of channel c of FFT lt: FP16/BF16 spectra carry the row's scale, which the
+// sums take out once at the end (|X/sc|^2 = |X|^2 / sc^2)
+__device__ __forceinline__ float2 spec_raw(const KurtArgs& a, const KurtBand& b, int c, int lt) {
   if (a.mode == FXG_X_FP32) return a.spec32[b.off + (long long)lt * b.C + c];
   const uint32_t u = a.spec16[b.off + ((long long)(c >> 3) * a.chunkcap + lt) * 8 + (c & 7)];
-  float re, im;
   if (a.mode == FXG_X_FP16) {
     const __half2 h = *reinterpret_cast<const __half2*>(&u);
-    const float2 f = __half22float2(h);
-    re = f.x;
-    im = f.y;
-  } else {
-    re = __uint_as_float((u & 0xFFFFu) << 16);
-    im = __uint_as_float(u & 0xFFFF0000u);
+    return __half22float2(h);
   }
-  const float sc = a.xscale[b.scale_idx];
-  return make_float2(re / sc, im / sc);
+  return make_float2(__uint_as_float((u & 0xFFFFu) << 16), __uint_as_float(u & 0xFFFF0000u));
 }
 
+// One workgroup per (band, tile of channels): its lanes cover FFTS_PER_PASS FFTs x CH channels
+// of the spectra layout (FP16/BF16: the 8 channels of one 32-byte group, so a pass reads
+// 32 FFTs x 32 bytes contiguous; FP32: 64 channels of 8 bytes, 4 FFTs per pass), each lane
+// summing every FFTS_PER_PASS-th FFT, and the partial sums are added over the FFTs in LDS.
+template <int CH>
 __global__ void __launch_bounds__(256) k_kurt(KurtArgs a) {
+  constexpr int FPP = 256 / CH;                          // FFTs per pass
   const KurtBand b = a.tab[blockIdx.x];
-  const int c = blockIdx.y * 256 + threadIdx.x;
-  if (c >= b.C) return;
+  const int cl = threadIdx.x % CH, fl = threadIdx.x / CH;
+  const int c = blockIdx.y * CH + cl;
+  __shared__ float2 part[256];
   float s1 = 0.f, s2 = 0.f;
-  for (int lt = 0; lt < a.nffts; ++lt) {
-    const float2 x = spec_value(a, b, c, lt);
-    const float m = sqrtf(x.x * x.x + x.y * x.y);     // vectorMagnitude_cf32
-    const float p = m * m;                             // first square
-    s1 += p;
-    s2 += p * p;                                       // second square
+  if (c < b.C) {
+#pragma unroll 8
+    for (int lt = fl; lt < a.nffts; lt += FPP) {
+      const float2 x = spec_raw(a, b, c, lt);
+      const float p = x.x * x.x + x.y * x.y;           // |X|^2 (the reference squares |X|)
+      s1 += p;
+      s2 += p * p;                                     // |X|^4
+    }
   }
-  float2* acc = a.acc + b.acc_off + c;
-  float2 v = *acc;
-  v.x += s1;
-  v.y += s2;
-  *acc = v;
+  part[threadIdx.x] = make_float2(s1, s2);
+  __syncthreads();
+  if (threadIdx.x < CH && c < b.C) {
+    float2 t = part[threadIdx.x];
+    for (int f = 1; f < FPP; ++f) {
+      const float2 q = part[f * CH + threadIdx.x];
+      t.x += q.x;
+      t.y += q.y;
+    }
+    if (a.mode != FXG_X_FP32) {
+      const float sc = a.xscale[b.scale_idx];
+      const float i2 = 1.0f / (sc * sc);
+      t.x *= i2;
+      t.y *= i2 * i2;
+    }
+    float2* acc = a.acc + b.acc_off + c;
+    float2 v = *acc;
+    v.x += t.x;
+    v.y += t.y;
+    *acc = v;
+  }
 }
 
 }  // namespace
 
 int fxg_launch_kurt(const KurtArgs& a, int nband, int maxC, hipStream_t st) {
   if (nband <= 0) return 0;
-  hipLaunchKernelGGL(k_kurt, dim3(nband, (maxC + 255) / 256), dim3(256), 0, st, a);
+  if (a.mode == FXG_X_FP32)
+    hipLaunchKernelGGL(k_kurt<64>, dim3(nband, (maxC + 63) / 64), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_kurt<8>, dim3(nband, (maxC + 7) / 8), dim3(256), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
