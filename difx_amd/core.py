@@ -83,6 +83,8 @@ def load_library() -> C.CDLL:
     lib.fxg_segment_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_segment)]
     lib.fxg_buffer_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_databuffer)]
     lib.fxg_set_kurtosis.argtypes = [C.c_void_p, C.c_int32]
+    lib.fxg_sta_autocorr.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                     C.c_void_p, P(C.c_int32)]
     lib.fxg_get_kurtosis.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, P(C.c_int32)]
     lib.fxg_normalise_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
     lib.fxg_difx_records.argtypes = [C.c_void_p, C.c_void_p, P(abi.fxg_difx_meta), C.c_void_p, C.c_int64,

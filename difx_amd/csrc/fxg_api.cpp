@@ -1872,6 +1872,52 @@ int fxg_difx_records(const fxg_ctx* ctx, const float* results, const fxg_difx_me
   return FXG_OK;
 }
 
+// Core::averageAndSendAutocorrs' STA record data (core.cpp:1166-1258) for a recorded band,
+// from the parallel autocorrelation and its weight in a results array
+int fxg_sta_autocorr(const fxg_ctx* ctx, const float* results, int32_t ds, int32_t band, int32_t stadumpchannels,
+                     double nswidth, float* out, int32_t* nout) {
+  if (!ctx || !results || !out || !nout || ds < 0 || ds >= ctx->nds || stadumpchannels < 1) return FXG_E_INVALID;
+  const fxg_datastream& s = ctx->dss[ds];
+  if (band < 0 || band >= s.nrecbands) return FXG_E_INVALID;
+  // datastreamsaveraged: every freq keeps at least stadumpchannels after averaging (core.cpp:1180-1186)
+  int minpostav = 1 << 30;
+  for (int f = 0; f < ctx->nfreqs; ++f)
+    if (ctx->used[f]) minpostav = std::min(minpostav, ctx->freqs[f].nchan / ctx->freqs[f].chanavg);
+  const bool averaged = minpostav >= stadumpchannels;
+  // the parallel autocorrelation of the band and its weight in the results (recorded bands lead)
+  long long idx = ctx->acoff[ds];
+  int widx = ctx->acwoff[ds] * 2;
+  for (int k = 0; k < band; ++k) {
+    const int fi = freq_of_band(s, k);
+    if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
+    idx += ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
+    widx++;
+  }
+  const int fi = freq_of_band(s, band);
+  if (!(ctx->used[fi] || ctx->equiv[fi])) return FXG_E_INVALID;
+  const fxg_freq& fr = ctx->freqs[fi];
+  if (fr.chanavg > 1 && !averaged) return FXG_E_INVALID;   // needs the unaveraged spectrum
+  int freqchannels = fr.nchan;
+  const double stasamples = 0.001 * nswidth * 2 * fr.bw_mhz;
+  const double minimumweight = 0.333 * stasamples / (2 * freqchannels);   // MINIMUM_FILTERBANK_WEIGHT
+  const float weight = results[widx];
+  if (weight < minimumweight) { *nout = 0; return FXG_OK; }              // no record for this band
+  float renormvalue = (float)(1.0 / (2 * freqchannels * weight));
+  if (averaged) {
+    renormvalue /= fr.chanavg;
+    freqchannels /= fr.chanavg;
+  }
+  const int nchan = std::min(stadumpchannels, freqchannels);
+  const int cta = freqchannels / nchan;
+  for (int k = 0; k < nchan; ++k) {
+    float a = results[2 * (idx + (long long)k * cta)];
+    for (int l = 1; l < cta; ++l) a += results[2 * (idx + (long long)k * cta + l)];
+    out[k] = a * renormvalue;
+  }
+  *nout = nchan;
+  return FXG_OK;
+}
+
 int fxg_result_offsets(const fxg_ctx* ctx, int32_t* baseline, int32_t* bweight, int32_t* autocorr,
                        int32_t* acweight) {
   if (!ctx) return FXG_E_INVALID;

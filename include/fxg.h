@@ -198,6 +198,13 @@ int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out /* [blockspe
  * *nout = the values written (0 when the band had no data, where the reference sends no
  * record). */
 int fxg_set_kurtosis(fxg_ctx* ctx, int32_t enable);
+/* The STA autocorrelation record data of a recorded band (Core::averageAndSendAutocorrs,
+ * core.cpp:1166-1258) from a slot's results (host memory): the parallel autocorrelation's real
+ * parts summed over groups of channels down to min(stadumpchannels, channels) and scaled by
+ * 1/(2 nchan weight); nswidth is the record's span in ns (its minimum weight is 0.333 of a full
+ * span's).  *nout = 0 when the weight is below it (the reference sends no record). */
+int fxg_sta_autocorr(const fxg_ctx* ctx, const float* results, int32_t ds, int32_t band, int32_t stadumpchannels,
+                     double nswidth, float* out, int32_t* nout);
 int fxg_get_kurtosis(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t maxchannels, float* out,
                      int32_t* nout);
 

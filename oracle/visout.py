@@ -21,11 +21,15 @@ This module is a CHECKER, not the product: only tests/ may import it.
             skipped.
   parse     the header layout of the reference's reader (libraries/python/parseDiFX/
             Common.py:28-124), for the tests.
+  STA       Core::averageAndSendAutocorrs (mpifxcorr/src/core.cpp:1166-1258): a recorded
+            band's parallel autocorrelation summed to the dump channels and scaled by
+            1/(2 nchan weight); no record below 0.333 of a full span's weight.
 
 Pins (tests/test_difx_cpu.py): the scale's closed forms (scale x weight x meansubints x
 2 x blockspersend x nchan = 1; the zoom factors sqrt(ratio) and ratio; with Tsys1 = Tsys2 and
-acal = acw the Tsys scale reduces to Tsys / (decorr^2 ... ) by hand), and records parsed back
-field by field with the reference reader's layout.
+acal = acw the Tsys scale reduces to Tsys / (decorr^2 ... ) by hand), records parsed back
+field by field with the reference reader's layout, and the STA data of a flat
+autocorrelation in closed form.
 """
 from __future__ import annotations
 
@@ -226,3 +230,41 @@ def parse(buf: bytes, nchan_of_freq: Dict[int, int]) -> List[dict]:
         recs.append(dict(baseline=bl, mjd=mjd, seconds=sec, config=cfgi, source=src, freq=freq,
                          pol=(p0 + p1).decode(), bin=pbin, weight=w, uvw=(u, v, ww), vis=vis))
     return recs
+
+
+def sta_autocorr(cfg, res: np.ndarray, lay, d: int, band: int, stadumpchannels: int, nswidth: float):
+    """Core::averageAndSendAutocorrs' STA record data (core.cpp:1166-1258) of recorded band
+    `band` of datastream d from a results array: None when its weight is below 0.333 of a
+    full span's (no record)."""
+    src = np.asarray(res, dtype=np.complex64)
+    f32 = src.view(np.float32)
+    used, equiv = fo.freq_used(cfg)
+    minpostav = min(cfg.freqs[f].nchan // cfg.freqs[f].chanavg for f in range(len(cfg.freqs)) if used[f])
+    averaged = minpostav >= stadumpchannels
+    ds = cfg.datastreams[d]
+    bands = _autocorr_bands(cfg, d, used, equiv)
+    off, widx = lay.autocorr_off[d], lay.acweight_off[d] * 2
+    for k in bands:
+        if k == band:
+            break
+        f = cfg.freqs[band_freq(ds, k)]
+        off += f.nchan // f.chanavg
+        widx += 1
+    fr = cfg.freqs[band_freq(ds, band)]
+    fc = fr.nchan
+    stasamples = 0.001 * nswidth * 2 * fr.bw_mhz
+    if F64(f32[widx]) < 0.333 * stasamples / (2 * fc):
+        return None
+    renorm = F32(1.0 / F64(F32(2 * fc) * f32[widx]))
+    if averaged:
+        renorm = F32(renorm / F32(fr.chanavg))
+        fc //= fr.chanavg
+    n = min(stadumpchannels, fc)
+    cta = fc // n
+    out = np.zeros(n, dtype=np.float32)
+    for k in range(n):
+        a = f32[2 * (off + k * cta)]
+        for l in range(1, cta):
+            a = F32(a + f32[2 * (off + k * cta + l)])
+        out[k] = F32(a * renorm)
+    return out

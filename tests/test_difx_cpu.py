@@ -189,3 +189,50 @@ def test_layout_context_refuses_processing():
         assert lib.fxg_delay_interpolator(h, 0, 0, 0, out) == abi.FXG_E_STATE
     finally:
         lib.fxg_destroy(h)
+
+
+@pytest.mark.parametrize("dumpchannels", [16, 64, 4096])
+def test_sta_autocorr_records(dumpchannels):
+    """fxg_sta_autocorr (host-only context) equals the restatement of averageAndSendAutocorrs;
+    for a flat autocorrelation of value c and weight w the record is cta c / (2 nchan w) in every
+    channel (cta = channels per record channel); a weight below 0.333 of a full span gives no
+    record."""
+    lib = _lib()
+    cfg = case_cfg3(B=32)
+    lay = fo.result_layout(cfg)
+    rng = np.random.default_rng(11)
+    res = _results(cfg, lay, rng, zero_some=False)
+    h, keep = _layout_ctx(lib, cfg)
+    nswidth = float(cfg.subint_ns)
+    try:
+        for d, ds in enumerate(cfg.datastreams):
+            for band in range(len(ds.recbands)):
+                out = np.zeros(16384, dtype=np.float32)
+                n = C.c_int32()
+                f = res.view(np.float32)
+                assert lib.fxg_sta_autocorr(h, f.ctypes.data, d, band, dumpchannels, nswidth,
+                                            out.ctypes.data, C.byref(n)) == 0
+                want = vo.sta_autocorr(cfg, res, lay, d, band, dumpchannels, nswidth)
+                assert want is not None and n.value == want.size
+                np.testing.assert_array_equal(out[:n.value], want)
+        # closed form on band 0 of datastream 0: flat autocorrelation c, weight w
+        ds = cfg.datastreams[0]
+        fr = cfg.freqs[vo.band_freq(ds, 0)]
+        off, widx = lay.autocorr_off[0], lay.acweight_off[0] * 2
+        c, w = 3.0, 0.9 * fr.nchan * 2 * cfg.blockspersend() / (2 * fr.nchan)   # a full span's weight
+        r2 = res.copy()
+        r2[off:off + fr.nchan // fr.chanavg] = c
+        r2.view(np.float32)[widx] = w
+        out = np.zeros(16384, dtype=np.float32)
+        n = C.c_int32()
+        assert lib.fxg_sta_autocorr(h, r2.view(np.float32).ctypes.data, 0, 0, dumpchannels, nswidth,
+                                    out.ctypes.data, C.byref(n)) == 0
+        fc = fr.nchan // fr.chanavg
+        nout = min(dumpchannels, fc)
+        np.testing.assert_allclose(out[:n.value], (fc // nout) * c / (2 * fr.nchan * w), rtol=2e-6)
+        r2.view(np.float32)[widx] = 0.3 * 0.001 * nswidth * 2 * fr.bw_mhz / (2 * fr.nchan)
+        assert lib.fxg_sta_autocorr(h, r2.view(np.float32).ctypes.data, 0, 0, dumpchannels, nswidth,
+                                    out.ctypes.data, C.byref(n)) == 0
+        assert n.value == 0
+    finally:
+        lib.fxg_destroy(h)
