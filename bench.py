@@ -17,7 +17,18 @@ on its own stream inside the library (fxg_set_timing).  The CPU baseline runs th
 restatement in oracle/ (when built) or the numpy oracle on a bounded sample of the same
 workload on the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Further sections of the same JSON line (each can be skipped with its --no-* flag):
+  plain_fp32          the plain path on the same inputs
+  target_16_stations  16 stations, MFMA path vs plain path
+  uneven_bands        mixed bandwidths and zooms, both F-engine launch forms, autotuner report
+  serving             Poisson job stream through the server (p50 / p99 subint latency)
+  kurtosis            the headline subint with the spectral-kurtosis sums enabled
+  output_side         device scaling (writedata) and DIFX records of one subint's results
+  vdif_corner_turn    10 stations' 16-thread VDIF segments corner-turned on the device, with
+                      a device-copy yardstick and the numpy restatement's rate
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain] [--no-uneven]
+                    [--no-serving] [--no-target] [--no-kurtosis] [--no-output] [--no-mux]
 """
 from __future__ import annotations
 
