@@ -206,7 +206,9 @@ class FxCore:
     def getKurtosis(self, ds: int, band: int, maxchannels: int) -> np.ndarray:
         """Mode::calculateAndAverageKurtosis + getKurtosis for the last subint (empty when the
         band had no data)."""
-        out = np.zeros(max(maxchannels, 1) + 65536, dtype=np.float32)
+        d = self.cfg.datastreams[ds]
+        nchan = self.cfg.freqs[d.recfreqs[d.recbands[band][0]]].nchan
+        out = np.zeros(max(min(maxchannels, nchan), 1), dtype=np.float32)
         n = C.c_int32()
         self._check(self.lib.fxg_get_kurtosis(self.h, ds, band, maxchannels, out.ctypes.data, C.byref(n)))
         return out[:n.value].copy()
