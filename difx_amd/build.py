@@ -20,7 +20,7 @@ INCLUDE = HERE.parent / "include"
 LIB = HERE / "libfxg.so"
 SOURCES = ["fxg_fengine.hip", "fxg_fpersist.hip", "fxg_fbig.hip", "fxg_xengine.hip", "fxg_api.cpp",
            "fxg_server.cpp", "fxg_vdifmux.hip", "fxg_control.cpp",
-           "fxg_visnorm.hip", "fxg_kurt.hip"]
+           "fxg_visnorm.hip", "fxg_kurt.hip", "fxg_pcal.hip"]
 ARCH = "gfx950"
 
 

@@ -83,6 +83,8 @@ def load_library() -> C.CDLL:
     lib.fxg_segment_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_segment)]
     lib.fxg_buffer_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_databuffer)]
     lib.fxg_set_kurtosis.argtypes = [C.c_void_p, C.c_int32]
+    lib.fxg_set_pcal.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int64]
+    lib.fxg_get_pcal.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, P(C.c_int32)]
     lib.fxg_sta_autocorr.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                      C.c_void_p, P(C.c_int32)]
     lib.fxg_get_kurtosis.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, P(C.c_int32)]
@@ -212,6 +214,19 @@ class FxCore:
         n = C.c_int32()
         self._check(self.lib.fxg_get_kurtosis(self.h, ds, band, maxchannels, out.ctypes.data, C.byref(n)))
         return out[:n.value].copy()
+
+    def set_pcal(self, ds: int, interval_hz: int, base_hz: int = 0):
+        """Phase-cal extraction of datastream ds at a whole-Hz tone interval (0 turns it off)."""
+        self._check(self.lib.fxg_set_pcal(self.h, ds, int(interval_hz), int(base_hz)))
+
+    def getPcal(self, ds: int, band: int) -> np.ndarray:
+        """Mode::finalisepcal's tones of a recorded band for the last subint (complex64)."""
+        d = self.cfg.datastreams[ds]
+        bw = self.cfg.freqs[d.recfreqs[d.recbands[band][0]]].bw_mhz
+        out = np.zeros(2 * (int(bw * 1e6) + 2), dtype=np.float32)   # at most one tone per Hz
+        n = C.c_int32()
+        self._check(self.lib.fxg_get_pcal(self.h, ds, band, out.ctypes.data, C.byref(n)))
+        return out[: 2 * n.value].view(np.complex64).copy()
 
     def normalise_device(self, d_results: int, meansubints: float, tsys=None, stream: int = 0):
         """Visibility::writedata's amplitude scaling of a device results array, in place."""

@@ -170,6 +170,20 @@ struct fxg_ctx {
   float2* d_kurt_acc = nullptr;
   long long kurt_len = 0;          // float2 sums
   int kurt_maxC = 0;
+  // phase-cal extraction (off until fxg_set_pcal)
+  struct PcalTones {
+    int ds, band;
+    long long offset_hz;           // baseband frequency of tone 0 (-1: no tone in the band)
+    long long interval_hz, fs_hz;
+    int ntones;
+  };
+  std::vector<long long> pcal_interval, pcal_base;   // [ds]; interval 0 = off
+  std::vector<PcalTones> pcal_tones;
+  std::vector<PcalBand> pcal_tab;  // entries with tones (bins on the device)
+  std::vector<int> pcal_entry;     // pcal_tones index -> pcal_tab index or -1
+  PcalBand* d_pcal_tab = nullptr;
+  float* d_pcal_bins = nullptr;
+  int pcal_maxM = 0;
   // streams, staging, ring
   hipStream_t stream = nullptr;
   Staging stage[2];
@@ -1286,6 +1300,11 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   fa.rotorder = ctx->rotorder;
   if (ctx->kurt_on && fxg_launch_zero(ctx->d_kurt_acc, sizeof(float2) * ctx->kurt_len, st))
     return fail(ctx, FXG_E_HIP, "kurtosis zero launch");
+  if (!ctx->pcal_tab.empty()) {
+    const PcalBand& last = ctx->pcal_tab.back();
+    if (fxg_launch_zero(ctx->d_pcal_bins, sizeof(float) * ((long long)last.bins_off + last.M), st))
+      return fail(ctx, FXG_E_HIP, "phase-cal zero launch");
+  }
   for (int c0 = 0; c0 < ctx->B; c0 += ctx->chunkcap) {
     const int n = std::min(ctx->chunkcap, ctx->B - c0);
     const int npad = (int)round_up(n, FXG_XPAD);
@@ -1295,6 +1314,20 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     {
       TimedLaunch tl(ctx, 0, st);
       if (int rc = launch_fengines(ctx, fa, st)) return rc;
+    }
+    if (!ctx->pcal_tab.empty()) {                  // while the chunk's per-FFT delays are current
+      PcalArgs pa;
+      pa.ds = ctx->d_ds;
+      pa.run = ctx->d_run;
+      pa.params = ctx->d_params;
+      pa.fvalid = ctx->d_fvalid;
+      pa.tab = ctx->d_pcal_tab;
+      pa.bins = ctx->d_pcal_bins;
+      pa.nffts = n;
+      pa.chunkcap = ctx->chunkcap;
+      TimedLaunch tp(ctx, 2, st);
+      if (fxg_launch_pcal(pa, (int)ctx->pcal_tab.size(), ctx->pcal_maxM, st))
+        return fail(ctx, FXG_E_HIP, "phase-cal launch");
     }
     if (mfma) {
       XMfmaArgs xa;
@@ -1448,7 +1481,7 @@ void free_ctx(fxg_ctx* ctx) {
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   f(ctx->d_normseg); f(ctx->d_normcal); f(ctx->d_calmean); f(ctx->d_tsys); f(ctx->d_dpct);
-  f(ctx->d_kurt_tab); f(ctx->d_kurt_acc);
+  f(ctx->d_kurt_tab); f(ctx->d_kurt_acc); f(ctx->d_pcal_tab); f(ctx->d_pcal_bins);
   for (auto& sg : ctx->stage) {
     if (sg.h_run) (void)hipHostFree(sg.h_run);
     if (sg.h_flags) (void)hipHostFree(sg.h_flags);
@@ -1998,6 +2031,132 @@ int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out) {
   if (hipStreamSynchronize(c->stream) != hipSuccess) return FXG_E_HIP;
   if (hipMemcpy(out, ctx->d_weights + (size_t)ds * ctx->B, sizeof(float) * ctx->B, hipMemcpyDeviceToHost) != hipSuccess)
     return FXG_E_HIP;
+  return FXG_OK;
+}
+
+// The tones of one recorded freq (Configuration's phase-cal table, configuration.cpp:1768-1835,
+// for a whole-Hz interval): the first tone inside the band away from the LO edge, their count,
+// and the baseband offset of that tone.
+static void pcal_tones_of(double lofreq, double bw, bool lsb, long long interval, long long base, int* ntones,
+                          long long* offset) {
+  const double stepMHz = interval * 1e-6;
+  long harmonic = (long)((lofreq - base * 1e-6) / stepMHz);
+  double tonefreq = harmonic * stepMHz + base * 1e-6;
+  int n = 0;
+  if (lsb) {
+    if (tonefreq >= lofreq) { tonefreq -= stepMHz; harmonic -= 1; }
+    if (tonefreq >= lofreq - bw)
+      while (tonefreq - n * stepMHz >= lofreq - bw) n++;
+    *offset = n > 0 ? (long long)(1e6 * lofreq) - (long long)harmonic * interval : -1;
+  } else {
+    if (tonefreq <= lofreq) { tonefreq += stepMHz; harmonic += 1; }
+    if (tonefreq <= lofreq + bw)
+      while (tonefreq + n * stepMHz <= lofreq + bw) n++;
+    *offset = n > 0 ? (long long)harmonic * interval - (long long)(1e6 * lofreq) : -1;
+  }
+  *ntones = n > 0 ? n : 1;                         // one placeholder tone when none fits
+}
+
+int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
+  if (!ctx || ds < 0 || ds >= ctx->nds || interval_hz < 0) return FXG_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  if (ctx->dss[ds].complex_sampling && interval_hz > 0)
+    return fail(ctx, FXG_E_INVALID, "phase-cal extraction of complex-sampled data is not supported");
+  ctx->pcal_interval.resize(ctx->nds, 0);
+  ctx->pcal_base.resize(ctx->nds, 0);
+  ctx->pcal_interval[ds] = interval_hz;
+  ctx->pcal_base[ds] = base_hz;
+  // rebuild the tone table and the device bins for every datastream with phase-cal on
+  std::vector<fxg_ctx::PcalTones> tones;
+  std::vector<PcalBand> tab;
+  std::vector<int> entry;
+  long long off = 0;
+  int maxM = 0;
+  for (int d = 0; d < ctx->nds; ++d) {
+    if (ctx->pcal_interval[d] <= 0) continue;
+    const fxg_datastream& x = ctx->dss[d];
+    for (int b = 0; b < x.nrecbands; ++b) {
+      const fxg_freq& f = ctx->freqs[x.recfreq_index[x.recband_localfreq[b]]];
+      fxg_ctx::PcalTones t{};
+      t.ds = d;
+      t.band = b;
+      t.interval_hz = ctx->pcal_interval[d];
+      t.fs_hz = (long long)std::llround(2e6 * f.bw_mhz);
+      pcal_tones_of(f.freq_mhz, f.bw_mhz, f.lsb != 0, t.interval_hz, ctx->pcal_base[d], &t.ntones, &t.offset_hz);
+      int e = -1;
+      if (t.offset_hz >= 0) {
+        const long long g = gcd_ll(t.fs_hz, gcd_ll(t.offset_hz, t.interval_hz));   // offset 0: gcd = interval
+        const long long M = t.fs_hz / std::max<long long>(g, 1);
+        if (M > 16384)
+          return fail(ctx, FXG_E_INVALID, "phase-cal tone period longer than 16384 samples");
+        PcalBand pb{};
+        pb.ds = d;
+        pb.band = b;
+        pb.M = (int)M;
+        pb.bins_off = (int)off;
+        off += M;
+        maxM = std::max(maxM, (int)M);
+        e = (int)tab.size();
+        tab.push_back(pb);
+      }
+      entry.push_back(e);
+      tones.push_back(t);
+    }
+  }
+  if (ctx->d_pcal_tab) { (void)hipFree(ctx->d_pcal_tab); ctx->d_pcal_tab = nullptr; }
+  if (ctx->d_pcal_bins) { (void)hipFree(ctx->d_pcal_bins); ctx->d_pcal_bins = nullptr; }
+  ctx->pcal_tones = tones;
+  ctx->pcal_tab = tab;
+  ctx->pcal_entry = entry;
+  ctx->pcal_maxM = maxM;
+  if (!tab.empty()) {
+    if (int rc = dev_upload(ctx, &ctx->d_pcal_tab, ctx->pcal_tab)) return rc;
+    if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_bins, sizeof(float) * off)) return rc;
+    if (int rc = dev_zero(ctx, ctx->d_pcal_bins, sizeof(float) * off)) return rc;
+  }
+  return FXG_OK;
+}
+
+// The extractors' getFinalPCal tones for the last subint (pcal.cpp:550-580, 746-790,
+// 1171-1215): sum over the folded bins j of bins[j] exp(-2 pi i (offset + k interval) j / fs),
+// formed in double
+int fxg_get_pcal(const fxg_ctx* ctx, int32_t ds, int32_t band, float* out, int32_t* ntones) {
+  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
+  if (!ctx || !out || !ntones || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
+  int t = -1;
+  for (size_t i = 0; i < ctx->pcal_tones.size(); ++i)
+    if (ctx->pcal_tones[i].ds == ds && ctx->pcal_tones[i].band == band) t = (int)i;
+  if (t < 0) return FXG_E_STATE;                   // phase-cal not on for this band
+  const fxg_ctx::PcalTones& pt = ctx->pcal_tones[t];
+  const int e = ctx->pcal_entry[t];
+  if (e < 0) {                                     // offset flagged (-1 for no tone in the band,
+    for (int k = 0; k < 2 * pt.ntones; ++k)        // or negative through base_hz): the reference's
+      out[k] = 0.0f;                               // dummy extractor leaves the tones at zero
+    *ntones = pt.ntones;
+    return FXG_OK;
+  }
+  const PcalBand& pb = ctx->pcal_tab[e];
+  std::vector<float> bins((size_t)pb.M);
+  if (hipDeviceSynchronize() != hipSuccess) return FXG_E_HIP;
+  if (hipMemcpy(bins.data(), ctx->d_pcal_bins + pb.bins_off, sizeof(float) * pb.M, hipMemcpyDeviceToHost) != hipSuccess)
+    return FXG_E_HIP;
+  for (int k = 0; k < pt.ntones; ++k) {
+    // a zero offset leaves the DC bin out: the tones are then interval, 2 interval, ...
+    // (PCalExtractorTrivial::getFinalPCal, pcal.cpp:573-580)
+    const long long f = pt.offset_hz + (long long)(pt.offset_hz == 0 ? k + 1 : k) * pt.interval_hz;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < pb.M; ++j) {
+      // phase (f j / fs) mod 1, exact in integers: f * j mod fs
+      const long long r = f * j % pt.fs_hz;
+      const double ph = -2.0 * M_PI * (double)r / (double)pt.fs_hz;
+      re += bins[j] * std::cos(ph);
+      im += bins[j] * std::sin(ph);
+    }
+    out[2 * k] = (float)re;
+    out[2 * k + 1] = (float)im;
+  }
+  *ntones = pt.ntones;
   return FXG_OK;
 }
 

@@ -292,3 +292,22 @@ struct KurtArgs {
   int nffts, chunkcap, mode;   // mode: FXG_X_FP32, FXG_X_FP16 or FXG_X_BF16
 };
 int fxg_launch_kurt(const KurtArgs& a, int nband, int maxC, hipStream_t st);
+
+// Phase-cal folding (the PCal extractors fed by Mode::process, mode.cpp:758-772): the unpacked
+// samples of each valid FFT, indexed by their sample number since the scan start, summed into
+// the M bins of the band's tone period (M = fs / gcd(fs, tone offset, tone interval)).
+struct PcalBand {
+  int ds, band;         // datastream, recorded band
+  int M;                // bins of the period
+  int bins_off;         // float offset of the band's bins
+};
+struct PcalArgs {
+  const DevDs* ds;
+  const DevDsRun* run;
+  const FftDelay* params;    // [nds][chunkcap] of the current chunk
+  const uint8_t* fvalid;
+  const PcalBand* tab;
+  float* bins;
+  int nffts, chunkcap;
+};
+int fxg_launch_pcal(const PcalArgs& a, int nband, int maxM, hipStream_t st);

@@ -215,5 +215,5 @@ EXPORTED_SYMBOLS = [
     "fxg_vdif_mux_workspace_bytes", "fxg_vdif_mux_device", "fxg_vdif_mux_result",
     "fxg_segment_params", "fxg_buffer_params", "fxg_control_params",
     "fxg_normalise_device", "fxg_difx_records", "fxg_set_kurtosis", "fxg_get_kurtosis",
-    "fxg_sta_autocorr",
+    "fxg_sta_autocorr", "fxg_set_pcal", "fxg_get_pcal",
 ]

@@ -208,6 +208,22 @@ int fxg_sta_autocorr(const fxg_ctx* ctx, const float* results, int32_t ds, int32
 int fxg_get_kurtosis(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t maxchannels, float* out,
                      int32_t* nout);
 
+/* Phase-cal extraction of datastream ds (Mode's PCal extractors, mode.cpp:401-420, 758-772,
+ * 1552-1568; tone table of Configuration, configuration.cpp:1765-1835), real-sampled data with
+ * a whole-Hz tone interval.  interval_hz = 0 turns it off.  The tones of each recorded band
+ * follow Configuration: the first tone above (USB) or below (LSB) the band edge, counted up to
+ * the far edge; tone k sits at baseband frequency offset + k interval with
+ * offset = harmonic interval - long(1e6 lofreq) (USB; mirrored for LSB; base_hz moves the comb
+ * but, as in the reference, not the offset).  Every subint folds the samples of each valid FFT
+ * of the band by their sample number since the scan start modulo the tone period
+ * M = fs / gcd(fs, offset, interval) (at most 16384).  fxg_get_pcal gives the last subint's
+ * tones, sum over samples x[m] exp(-2 pi i f_k m / fs), as interleaved (re, im) floats;
+ * *ntones = the count.  A flagged offset (a band with no tone: one placeholder tone; or an offset
+ * made negative by base_hz) gives zero tones, as the reference's dummy extractor.  FXG_E_STATE
+ * when phase-cal is off for the band. */
+int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz);
+int fxg_get_pcal(const fxg_ctx* ctx, int32_t ds, int32_t band, float* out, int32_t* ntones);
+
 /* Mk5Mode::unpack (mk5mode.cpp:94-174) as a device call: decode nsamples time samples starting
  * at sample `startsample` of datastream ds from d_data (nbytes) into d_out[nrecbands][nsamples]
  * (f32, or interleaved cf32 for complex sampling), and the count of valid samples into

@@ -169,3 +169,43 @@ def channel_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
         keep = coh >= np.median(coh)
         dphi.append(np.degrees(np.abs(np.angle(g[keep] * np.conj(r[keep])))))
     return np.concatenate(norm), np.concatenate(dphi)
+
+
+def pcal_tone_slot(cfg, tones, offsets=(0, 1, 0), noise: float = 0.0, seed: int = 13):
+    """8-bit real slot whose every band carries sum_j A_j cos(2 pi f_j t + phi_j) (+ Gaussian
+    noise of rms `noise`), t the sample's time since the scan start, levels round(128 + 3.3 x)
+    (the 8-bit decode is (level - 128) / 3.3).  tones: [(f_hz, A, phi)].  Returns (data,
+    control, decoded) with decoded[d] the function m -> decoded value of datastream d's sample
+    number m since the scan start (without the noise)."""
+    rng = np.random.default_rng(seed)
+    data, ctrl, funcs = [], [], []
+    for d, ds in enumerate(cfg.datastreams):
+        assert ds.nbits == 8 and not ds.complex_sampling
+        start_ns = synth.buffer_start_ns(cfg, d, list(offsets))
+        nbytes = synth.slot_bytes(cfg, d, list(offsets), start_ns)
+        nframes = nbytes // ds.framebytes
+        nb = len(ds.recbands)
+        spf = (ds.framebytes - ds.headerbytes) // nb
+        dt_ns = cfg.sampletime_us(d) * 1e3
+        s_start = start_ns / dt_ns
+        assert s_start == int(s_start), "buffer start on a sample boundary"
+        fs = 1e9 / dt_ns
+
+        def levels(m, fs=fs):
+            m = np.asarray(m, dtype=np.int64)
+            x = np.zeros(m.shape)
+            for f, a, ph in tones:
+                x += a * np.cos(2 * np.pi * ((int(f) * (m % int(fs))) % int(fs)) / fs + ph)
+            return x
+
+        m = int(s_start) + np.arange(nframes * spf, dtype=np.int64)
+        x = levels(m)
+        if noise:
+            x = x + noise * rng.standard_normal(x.shape)
+        lev = np.clip(np.round(128.0 + 3.3 * x), 0, 255).astype(np.uint8)
+        payload = np.repeat(lev, nb)                          # the same signal in every band
+        frame_ns = int(round(spf * dt_ns))
+        data.append(synth.frame_payloads(payload, ds, int(start_ns // frame_ns)))
+        ctrl.append(synth.control_words(cfg, d, list(offsets), start_ns))
+        funcs.append(lambda mm, lv=levels: (np.clip(np.round(128.0 + 3.3 * lv(mm)), 0, 255) - 128.0) / 3.3)
+    return data, ctrl, funcs
