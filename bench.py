@@ -23,12 +23,14 @@ Further sections of the same JSON line (each can be skipped with its --no-* flag
   uneven_bands        mixed bandwidths and zooms, both F-engine launch forms, autotuner report
   serving             Poisson job stream through the server (p50 / p99 subint latency)
   kurtosis            the headline subint with the spectral-kurtosis sums enabled
+  phase_cal           the headline subint with phase-cal extraction on every band (1 MHz comb)
   output_side         device scaling (writedata) and DIFX records of one subint's results
   vdif_corner_turn    10 stations' 16-thread VDIF segments corner-turned on the device, with
                       a device-copy yardstick and the numpy restatement's rate
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain] [--no-uneven]
-                    [--no-serving] [--no-target] [--no-kurtosis] [--no-output] [--no-mux]
+                    [--no-serving] [--no-target] [--no-kurtosis] [--no-pcal] [--no-output]
+                    [--no-mux]
 """
 from __future__ import annotations
 
@@ -230,11 +232,13 @@ def serving_bench(torch, dev, window_s=6.0, rate_per_s=4.0, subints_per_job=8, s
             "job_shapes_n_m_nchan": [list(j[3]) for j in jobs]}
 
 
-def time_path(cfg, steps, warmup, torch, dev, offsets, kurtosis=False):
+def time_path(cfg, steps, warmup, torch, dev, offsets, kurtosis=False, pcal_hz=0):
     from difx_amd.core import FxCore
     core = FxCore(cfg)
     if kurtosis:
         core.set_kurtosis(True)
+    for d in range(len(cfg.datastreams) if pcal_hz else 0):
+        core.set_pcal(d, pcal_hz)
     bufs, nbytes, ctrl = device_slot(cfg, offsets, torch, dev)
     res = torch.zeros(2 * core.coreresultlength, dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
@@ -506,6 +510,7 @@ def main():
     ap.add_argument("--no-mux", action="store_true")
     ap.add_argument("--no-output", action="store_true")
     ap.add_argument("--no-kurtosis", action="store_true")
+    ap.add_argument("--no-pcal", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copy for PMC scaling")
     args = ap.parse_args()
     if args.copy_probe:
@@ -609,6 +614,18 @@ def main():
                 "other_ms_per_step": kr["timing"]["other_ms"] / 3,
                 "spectra_bytes_read_per_step": spec_bytes,
                 "kurtosis_pass_GBps": spec_bytes / max((kr["timing"]["other_ms"] - tm["other_ms"] / args.steps * 3) / 3 / 1e3, 1e-9) / 1e9}
+    pcal = None
+    if not args.no_pcal:
+        print("[bench] headline path with phase-cal extraction, 1 warmup + 3 steps", file=sys.stderr, flush=True)
+        pr = time_path(cfg, 3, 1, torch, dev, offsets, pcal_hz=1_000_000)
+        p_ms = pr["wall_s"] * 1e3 / 3
+        extra_other = (pr["timing"]["other_ms"] - tm["other_ms"] / args.steps * 3) / 3
+        raw_bytes = float(samples_per_subint(cfg)) * cfg.datastreams[0].nbits / 8   # sample bytes read once
+        pcal = {"workload": "headline subint with phase-cal extraction, 1 MHz comb on every band",
+                "ms_per_step": p_ms, "extra_ms_vs_headline": p_ms - head["wall_s"] * 1e3 / args.steps,
+                "pcal_ms_per_step": extra_other,
+                "sample_bytes_per_step": raw_bytes,
+                "pcal_pass_GBps": raw_bytes / max(extra_other / 1e3, 1e-9) / 1e9}
     output = None
     if not args.no_output:
         print("[bench] output side: scaling and DIFX records of one subint", file=sys.stderr, flush=True)
@@ -661,6 +678,7 @@ def main():
         "vdif_corner_turn": mux,
         "output_side": output,
         "kurtosis": kurt,
+        "phase_cal": pcal,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

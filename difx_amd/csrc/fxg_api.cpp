@@ -13,6 +13,7 @@
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -179,11 +180,16 @@ struct fxg_ctx {
   };
   std::vector<long long> pcal_interval, pcal_base;   // [ds]; interval 0 = off
   std::vector<PcalTones> pcal_tones;
-  std::vector<PcalBand> pcal_tab;  // entries with tones (bins on the device)
+  struct PcalBins { int M; int bins_off; };
+  std::vector<PcalBins> pcal_tab;  // bands with tones (bins on the device)
   std::vector<int> pcal_entry;     // pcal_tones index -> pcal_tab index or -1
-  PcalBand* d_pcal_tab = nullptr;
+  std::vector<PcalGroup> pcal_groups;
+  PcalGroup* d_pcal_groups = nullptr;
+  float* d_pcal_part = nullptr;    // per-workgroup partial bins
   float* d_pcal_bins = nullptr;
-  int pcal_maxM = 0;
+  std::vector<std::array<int, 3>> pcal_runs;   // {form, first group, groups}
+  long long pcal_part_len = 0;
+  int pcal_max_gsize = 0, pcal_nsplit = 1;
   // streams, staging, ring
   hipStream_t stream = nullptr;
   Staging stage[2];
@@ -1300,11 +1306,19 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   fa.rotorder = ctx->rotorder;
   if (ctx->kurt_on && fxg_launch_zero(ctx->d_kurt_acc, sizeof(float2) * ctx->kurt_len, st))
     return fail(ctx, FXG_E_HIP, "kurtosis zero launch");
-  if (!ctx->pcal_tab.empty()) {
-    const PcalBand& last = ctx->pcal_tab.back();
-    if (fxg_launch_zero(ctx->d_pcal_bins, sizeof(float) * ((long long)last.bins_off + last.M), st))
-      return fail(ctx, FXG_E_HIP, "phase-cal zero launch");
-  }
+  PcalArgs pa{};
+  pa.ds = ctx->d_ds;
+  pa.run = ctx->d_run;
+  pa.params = ctx->d_params;
+  pa.fvalid = ctx->d_fvalid;
+  pa.groups = ctx->d_pcal_groups;
+  pa.part = ctx->d_pcal_part;
+  pa.bins = ctx->d_pcal_bins;
+  pa.chunkcap = ctx->chunkcap;
+  pa.nsplit = ctx->pcal_nsplit;
+  const int npg = (int)ctx->pcal_groups.size();
+  if (npg && fxg_launch_zero(ctx->d_pcal_part, sizeof(float) * ctx->pcal_part_len, st))
+    return fail(ctx, FXG_E_HIP, "phase-cal zero launch");
   for (int c0 = 0; c0 < ctx->B; c0 += ctx->chunkcap) {
     const int n = std::min(ctx->chunkcap, ctx->B - c0);
     const int npad = (int)round_up(n, FXG_XPAD);
@@ -1315,19 +1329,15 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
       TimedLaunch tl(ctx, 0, st);
       if (int rc = launch_fengines(ctx, fa, st)) return rc;
     }
-    if (!ctx->pcal_tab.empty()) {                  // while the chunk's per-FFT delays are current
-      PcalArgs pa;
-      pa.ds = ctx->d_ds;
-      pa.run = ctx->d_run;
-      pa.params = ctx->d_params;
-      pa.fvalid = ctx->d_fvalid;
-      pa.tab = ctx->d_pcal_tab;
-      pa.bins = ctx->d_pcal_bins;
+    if (npg) {                                     // while the chunk's per-FFT delays are current
       pa.nffts = n;
-      pa.chunkcap = ctx->chunkcap;
       TimedLaunch tp(ctx, 2, st);
-      if (fxg_launch_pcal(pa, (int)ctx->pcal_tab.size(), ctx->pcal_maxM, st))
-        return fail(ctx, FXG_E_HIP, "phase-cal launch");
+      for (const auto& r : ctx->pcal_runs) {
+        PcalArgs pr = pa;
+        pr.groups = ctx->d_pcal_groups + r[1];
+        if (fxg_launch_pcal(pr, r[0], r[2], ctx->pcal_max_gsize, st))
+          return fail(ctx, FXG_E_HIP, "phase-cal launch");
+      }
     }
     if (mfma) {
       XMfmaArgs xa;
@@ -1377,6 +1387,11 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     }
     ctx->last_chunk0 = c0;
     ctx->last_nffts = n;
+  }
+  if (npg) {
+    TimedLaunch tp(ctx, 2, st);
+    if (fxg_launch_pcal_reduce(pa, npg, ctx->pcal_max_gsize, st))
+      return fail(ctx, FXG_E_HIP, "phase-cal reduce launch");
   }
   TimedLaunch tl(ctx, 2, st);
   if (mfma) {
@@ -1481,7 +1496,7 @@ void free_ctx(fxg_ctx* ctx) {
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   f(ctx->d_normseg); f(ctx->d_normcal); f(ctx->d_calmean); f(ctx->d_tsys); f(ctx->d_dpct);
-  f(ctx->d_kurt_tab); f(ctx->d_kurt_acc); f(ctx->d_pcal_tab); f(ctx->d_pcal_bins);
+  f(ctx->d_kurt_tab); f(ctx->d_kurt_acc); f(ctx->d_pcal_groups); f(ctx->d_pcal_part); f(ctx->d_pcal_bins);
   for (auto& sg : ctx->stage) {
     if (sg.h_run) (void)hipHostFree(sg.h_run);
     if (sg.h_flags) (void)hipHostFree(sg.h_flags);
@@ -2061,18 +2076,26 @@ int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz)
   if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || ds < 0 || ds >= ctx->nds || interval_hz < 0) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  if (ctx->dss[ds].complex_sampling && interval_hz > 0)
-    return fail(ctx, FXG_E_INVALID, "phase-cal extraction of complex-sampled data is not supported");
+  const fxg_datastream& xs = ctx->dss[ds];
+  if (interval_hz > 0) {
+    if (xs.complex_sampling)
+      return fail(ctx, FXG_E_INVALID, "phase-cal extraction of complex-sampled data is not supported");
+    if (xs.nbits != 2 && xs.nbits != 8)
+      return fail(ctx, FXG_E_INVALID, "phase-cal extraction needs 2-bit or 8-bit samples");
+    if (xs.framebytes % 4 || xs.headerbytes % 4)
+      return fail(ctx, FXG_E_INVALID, "phase-cal extraction needs 4-byte aligned frame payloads");
+  }
   ctx->pcal_interval.resize(ctx->nds, 0);
   ctx->pcal_base.resize(ctx->nds, 0);
   ctx->pcal_interval[ds] = interval_hz;
   ctx->pcal_base[ds] = base_hz;
-  // rebuild the tone table and the device bins for every datastream with phase-cal on
+  // rebuild the tone table, the band groups and the device bins for every datastream with
+  // phase-cal on
   std::vector<fxg_ctx::PcalTones> tones;
-  std::vector<PcalBand> tab;
+  std::vector<fxg_ctx::PcalBins> tab;
   std::vector<int> entry;
+  std::vector<PcalGroup> groups;
   long long off = 0;
-  int maxM = 0;
   for (int d = 0; d < ctx->nds; ++d) {
     if (ctx->pcal_interval[d] <= 0) continue;
     const fxg_datastream& x = ctx->dss[d];
@@ -2088,30 +2111,69 @@ int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz)
       if (t.offset_hz >= 0) {
         const long long g = gcd_ll(t.fs_hz, gcd_ll(t.offset_hz, t.interval_hz));   // offset 0: gcd = interval
         const long long M = t.fs_hz / std::max<long long>(g, 1);
-        if (M > 16384)
+        if (M > 16384) {
+          ctx->pcal_interval[ds] = 0;              // leave the datastream off
           return fail(ctx, FXG_E_INVALID, "phase-cal tone period longer than 16384 samples");
-        PcalBand pb{};
-        pb.ds = d;
-        pb.band = b;
-        pb.M = (int)M;
-        pb.bins_off = (int)off;
-        off += M;
-        maxM = std::max(maxM, (int)M);
+        }
+        // extend the open group (same datastream, the previous band, same M, room in LDS)
+        PcalGroup* gr = groups.empty() ? nullptr : &groups.back();
+        if (!(gr && gr->ds == d && gr->band0 + gr->nb == b && gr->M == M && gr->nb < PCAL_MAXNB &&
+              (long long)(gr->nb + 1) * M <= PCAL_MAX_GSIZE)) {
+          PcalGroup ng{};
+          ng.ds = d;
+          ng.band0 = b;
+          ng.nb = 0;
+          ng.M = (int)M;
+          ng.bins_off = (int)off;
+          groups.push_back(ng);
+          gr = &groups.back();
+        }
+        gr->nb += 1;
         e = (int)tab.size();
-        tab.push_back(pb);
+        tab.push_back({(int)M, (int)off});
+        off += M;
       }
       entry.push_back(e);
       tones.push_back(t);
     }
   }
-  if (ctx->d_pcal_tab) { (void)hipFree(ctx->d_pcal_tab); ctx->d_pcal_tab = nullptr; }
+  // kernel form of each group, and the groups ordered by form (one launch per form)
+  for (auto& g : groups) {
+    const fxg_datastream& x = ctx->dss[g.ds];
+    g.form = (256 % g.M == 0 ? PCAL_FORM_REG : 0) | (x.nbits == 8 ? PCAL_FORM_8BIT : 0) |
+             (x.nrecbands * x.nbits == 32 && g.band0 == 0 && g.nb == x.nrecbands ? PCAL_FORM_WHOLE : 0);
+  }
+  std::stable_sort(groups.begin(), groups.end(),
+                   [](const PcalGroup& p, const PcalGroup& q) { return p.form < q.form; });
+  std::vector<std::array<int, 3>> runs;
+  for (int i = 0; i < (int)groups.size(); ++i) {
+    if (runs.empty() || runs.back()[0] != groups[i].form) runs.push_back({groups[i].form, i, 0});
+    runs.back()[2] += 1;
+  }
+  // workgroups per group: about eight per CU over all groups, at most one per FFT of a chunk
+  const int ng = (int)groups.size();
+  const int nsplit = ng ? std::max(1, std::min(ctx->chunkcap, (8 * ctx->ncu + ng - 1) / ng)) : 1;
+  long long part = 0;
+  int max_gsize = 0;
+  for (auto& g : groups) {
+    g.part_off = part;
+    part += (long long)nsplit * g.nb * g.M;
+    max_gsize = std::max(max_gsize, g.nb * g.M);
+  }
+  if (ctx->d_pcal_groups) { (void)hipFree(ctx->d_pcal_groups); ctx->d_pcal_groups = nullptr; }
+  if (ctx->d_pcal_part) { (void)hipFree(ctx->d_pcal_part); ctx->d_pcal_part = nullptr; }
   if (ctx->d_pcal_bins) { (void)hipFree(ctx->d_pcal_bins); ctx->d_pcal_bins = nullptr; }
   ctx->pcal_tones = tones;
   ctx->pcal_tab = tab;
   ctx->pcal_entry = entry;
-  ctx->pcal_maxM = maxM;
-  if (!tab.empty()) {
-    if (int rc = dev_upload(ctx, &ctx->d_pcal_tab, ctx->pcal_tab)) return rc;
+  ctx->pcal_groups = groups;
+  ctx->pcal_nsplit = nsplit;
+  ctx->pcal_runs = runs;
+  ctx->pcal_part_len = part;
+  ctx->pcal_max_gsize = max_gsize;
+  if (!groups.empty()) {
+    if (int rc = dev_upload(ctx, &ctx->d_pcal_groups, ctx->pcal_groups)) return rc;
+    if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_part, sizeof(float) * part)) return rc;
     if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_bins, sizeof(float) * off)) return rc;
     if (int rc = dev_zero(ctx, ctx->d_pcal_bins, sizeof(float) * off)) return rc;
   }
@@ -2136,7 +2198,7 @@ int fxg_get_pcal(const fxg_ctx* ctx, int32_t ds, int32_t band, float* out, int32
     *ntones = pt.ntones;
     return FXG_OK;
   }
-  const PcalBand& pb = ctx->pcal_tab[e];
+  const fxg_ctx::PcalBins& pb = ctx->pcal_tab[e];
   std::vector<float> bins((size_t)pb.M);
   if (hipDeviceSynchronize() != hipSuccess) return FXG_E_HIP;
   if (hipMemcpy(bins.data(), ctx->d_pcal_bins + pb.bins_off, sizeof(float) * pb.M, hipMemcpyDeviceToHost) != hipSuccess)

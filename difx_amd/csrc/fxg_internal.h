@@ -296,18 +296,30 @@ int fxg_launch_kurt(const KurtArgs& a, int nband, int maxC, hipStream_t st);
 // Phase-cal folding (the PCal extractors fed by Mode::process, mode.cpp:758-772): the unpacked
 // samples of each valid FFT, indexed by their sample number since the scan start, summed into
 // the M bins of the band's tone period (M = fs / gcd(fs, tone offset, tone interval)).
-struct PcalBand {
-  int ds, band;         // datastream, recorded band
-  int M;                // bins of the period
-  int bins_off;         // float offset of the band's bins
+// A group is a run of consecutive recorded bands of one datastream with the same M; band
+// band0 + k has bins [bins_off + k M, bins_off + (k + 1) M).  Workgroup split s of group g adds
+// into part[part_off + s nb M ...]; a reduce sums the nsplit slices into the bins.
+constexpr int PCAL_MAXNB = 16;           // bands per group
+constexpr int PCAL_MAX_GSIZE = 32768;    // nb M floats per group (LDS)
+constexpr int PCAL_FORM_REG = 1;         // 256 is a multiple of M: register sums
+constexpr int PCAL_FORM_WHOLE = 2;       // the group is every band of a one-time-sample word
+constexpr int PCAL_FORM_8BIT = 4;        // 8-bit samples (else 2-bit)
+struct PcalGroup {
+  int ds, band0, nb, M;
+  int bins_off;
+  int form;                              // PCAL_FORM_* bits
+  long long part_off;
 };
 struct PcalArgs {
   const DevDs* ds;
   const DevDsRun* run;
   const FftDelay* params;    // [nds][chunkcap] of the current chunk
   const uint8_t* fvalid;
-  const PcalBand* tab;
+  const PcalGroup* groups;
+  float* part;
   float* bins;
-  int nffts, chunkcap;
+  int nffts, chunkcap, nsplit;
 };
-int fxg_launch_pcal(const PcalArgs& a, int nband, int maxM, hipStream_t st);
+size_t fxg_pcal_lds(int max_gsize);
+int fxg_launch_pcal(const PcalArgs& a, int form, int ngroups, int max_gsize, hipStream_t st);
+int fxg_launch_pcal_reduce(const PcalArgs& a, int ngroups, int max_gsize, hipStream_t st);

@@ -3,64 +3,186 @@
 // (mpifxcorr/src/mode.cpp:758-772); every extractor PCal::getNew picks (pcal.cpp:191-274)
 // computes, for tone k at baseband frequency offset + k interval, sum x[m] exp(-2 pi i f m / fs)
 // over those samples.  Here the samples are folded by their sample number modulo the tone
-// period M (M = fs / gcd(fs, offset, interval)), in LDS per workgroup and then into the band's
-// bins; the tones follow from the bins by an M-point transform when they are read.
+// period M (M = fs / gcd(fs, offset, interval)); the tones follow from the M bins by an M-point
+// transform when they are read.
+//
+// Work split: a workgroup takes a group of consecutive recorded bands of one datastream that
+// share M, and a contiguous share of the chunk's FFTs.  Thread t takes time samples t, t + 256,
+// ... of each FFT, issues the loads of eight of them before decoding any, and decodes all the
+// group's bands of a time sample from one 32-bit load (the bands of a sample are adjacent
+// fields of the payload).  When 256 is a multiple of M a thread's
+// bin stays fixed through an FFT, so it sums in registers and adds to the LDS bins only when its
+// bin changes; otherwise every sample is an LDS atomic add.  Each workgroup adds its LDS bins to
+// its own slice of a partial array (no global atomics, a fixed summation order), and a reduce
+// kernel sums the slices into the bands' bins after the subint's last chunk.
 #include "fxg_fe_common.h"
 
 namespace {
 
-// grid (band entries, groups of FFTS_PER_WG FFTs of the chunk)
-constexpr int FFTS_PER_WG = 16;
+constexpr int PCAL_NT = 256;
+constexpr int PCAL_UNR = 8;            // time samples per thread in flight (loads issued together)
 
-__global__ void __launch_bounds__(256) k_pcal(PcalArgs a) {
-  extern __shared__ float bins[];
-  const PcalBand pb = a.tab[blockIdx.x];
-  const DevDs& ds = a.ds[pb.ds];
-  const DevDsRun& run = a.run[pb.ds];
-  const int M = pb.M;
-  for (int j = threadIdx.x; j < M; j += 256) bins[j] = 0.f;
+// NBITS: sample width (2 or 8).  WHOLE: the group is every band of a sample word that holds
+// exactly one time sample (nrec NBITS = 32), so field k of the word is band k.  REG: 256 is a
+// multiple of M, so a thread's bin is fixed through an FFT and it sums in registers.
+template <int NBITS, bool WHOLE, bool REG>
+__global__ void __launch_bounds__(PCAL_NT) k_pcal(PcalArgs a) {
+  constexpr int LGF = NBITS == 2 ? 4 : 2;          // log2 fields per 32-bit word
+  constexpr unsigned FMASK = (1u << NBITS) - 1u;
+  constexpr unsigned FSUB = (1u << LGF) - 1u;
+  extern __shared__ float sm[];
+  const PcalGroup g = a.groups[blockIdx.x];
+  const DevDs& ds = a.ds[g.ds];
+  const DevDsRun& run = a.run[g.ds];
+  const int M = g.M;
+  const int gsize = g.nb * M;
+  float* sb = sm;                                  // [nb][M]
+  float* lut = sm + gsize;                         // sample levels: 256 (8-bit) or 4 (2-bit)
+  const int tid = threadIdx.x;
+  for (int j = tid; j < gsize; j += PCAL_NT) sb[j] = 0.f;
+  if (NBITS == 2) {
+    if (tid < 4) lut[tid] = lut2(ds.format, tid);
+  } else {
+    lut[tid] = lut8(tid);
+  }
   __syncthreads();
-  // sample number of the buffer start since the scan start (mode.cpp:760)
-  const long long s0 = (long long)(((double)run.datasec * 1e9 + (double)run.datans) / (ds.sampletime_us * 1e3) + 0.5);
-  const uint8_t* fv = a.fvalid + ds.fvalid_off;
-  const int lt0 = blockIdx.y * FFTS_PER_WG;
-  const int lt1 = min(lt0 + FFTS_PER_WG, a.nffts);
-  for (int lt = lt0; lt < lt1; ++lt) {
-    const FftDelay fd = a.params[(long long)pb.ds * a.chunkcap + lt];
-    if (!fd.ok || !run.valid) continue;               // uniform over the workgroup
-    const int base = (int)((s0 + fd.n0) % M);       // bin of the FFT's first sample
-    for (int n = threadIdx.x; n < ds.L; n += 256) {
-      const int g = fd.n0 + n;                        // sample of the buffer (n0 >= 0)
-      const int fr = g / ds.spf;
-      const int within = g - fr * ds.spf;
-      float x = 0.f;
-      if (fr < run.nframes && fv[fr]) {
-        const long long fbase = (long long)fr * ds.framebytes + ds.headerbytes;
-        if (ds.nbits == 2) {
-          const long long q = (long long)within * ds.nrecbands + pb.band;
-          const uint8_t byte = run.data[fbase + (q >> 2)];
-          x = lut2(ds.format, (byte >> (2 * (q & 3))) & 3u);
-        } else {
-          x = lut8(run.data[fbase + (long long)within * ds.nrecbands + pb.band]);
+  const int per = (a.nffts + a.nsplit - 1) / a.nsplit;
+  const int lt0 = blockIdx.y * per;
+  const int lt1 = min(lt0 + per, a.nffts);
+  if (run.valid) {
+    // sample number of the buffer start since the scan start (mode.cpp:760)
+    const long long s0 = (long long)(((double)run.datasec * 1e9 + (double)run.datans) / (ds.sampletime_us * 1e3) + 0.5);
+    const uint8_t* __restrict__ fv = a.fvalid + ds.fvalid_off;
+    const uint8_t* __restrict__ data = run.data;
+    const int L = ds.L, spf = ds.spf, nrec = ds.nrecbands, nb = g.nb;
+    const int nsamp = run.nframes * spf;
+    const int step = PCAL_NT % M;                  // bin advance per time step (0 when REG)
+    float acc[PCAL_MAXNB];
+#pragma unroll
+    for (int k = 0; k < PCAL_MAXNB; ++k) acc[k] = 0.f;
+    int accbin = -1;
+    for (int lt = lt0; lt < lt1; ++lt) {
+      const FftDelay fd = a.params[(long long)g.ds * a.chunkcap + lt];
+      if (!fd.ok) continue;                        // uniform over the workgroup
+      int t = fd.n0 + tid;                         // time sample of the buffer (n0 >= 0)
+      int fr = t / spf;
+      int within = t - fr * spf;
+      int bin = (int)((s0 + t) % M);
+      if (REG && bin != accbin) {
+        if (accbin >= 0) {
+#pragma unroll
+          for (int k = 0; k < PCAL_MAXNB; ++k)
+            if (k < nb) { atomicAdd(&sb[k * M + accbin], acc[k]); acc[k] = 0.f; }
+        }
+        accbin = bin;
+      }
+      for (int nbase = 0; nbase < L; nbase += PCAL_NT * PCAL_UNR) {
+        uint32_t w[PCAL_UNR];
+        unsigned q0[PCAL_UNR];
+        const uint8_t* pay[PCAL_UNR];
+        bool ok[PCAL_UNR];
+        int binu[PCAL_UNR];
+        // addresses and loads of all the thread's time samples of this pass first
+#pragma unroll
+        for (int u = 0; u < PCAL_UNR; ++u) {
+          const bool in = nbase + u * PCAL_NT + tid < L && t < nsamp;
+          pay[u] = data + (long long)fr * ds.framebytes + ds.headerbytes;
+          q0[u] = (unsigned)(within * nrec + g.band0);
+          w[u] = in ? *reinterpret_cast<const uint32_t*>(pay[u] + 4u * (q0[u] >> LGF)) : 0u;
+          ok[u] = in ? fv[fr] != 0 : false;
+          binu[u] = bin;
+          t += PCAL_NT;
+          within += PCAL_NT;
+          while (within >= spf) { within -= spf; ++fr; }
+          if (!REG) { bin += step; if (bin >= M) bin -= M; }
+        }
+#pragma unroll
+        for (int u = 0; u < PCAL_UNR; ++u) {
+          if (!ok[u]) continue;
+          if (WHOLE) {
+#pragma unroll
+            for (int k = 0; k < (32 / NBITS); ++k) {
+              const float x = lut[(w[u] >> (NBITS * k)) & FMASK];
+              if (REG) acc[k] += x;
+              else atomicAdd(&sb[k * M + binu[u]], x);
+            }
+          } else {
+            unsigned wi = q0[u] >> LGF;
+            uint32_t ww = w[u];
+#pragma unroll
+            for (int k = 0; k < PCAL_MAXNB; ++k) {
+              if (k < nb) {
+                const unsigned q = q0[u] + k;
+                if ((q >> LGF) != wi) {            // the group's fields run into the next word
+                  wi = q >> LGF;
+                  ww = *reinterpret_cast<const uint32_t*>(pay[u] + 4u * wi);
+                }
+                const float x = lut[(ww >> (NBITS * (q & FSUB))) & FMASK];
+                if (REG) acc[k] += x;
+                else atomicAdd(&sb[k * M + binu[u]], x);
+              }
+            }
+          }
         }
       }
-      if (x != 0.f) atomicAdd(&bins[(base + n) % M], x);
+    }
+    if (REG && accbin >= 0) {
+#pragma unroll
+      for (int k = 0; k < PCAL_MAXNB; ++k)
+        if (k < nb) atomicAdd(&sb[k * M + accbin], acc[k]);
     }
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < M; j += 256) {
-    const float v = bins[j];
-    if (v != 0.f) atomicAdd(&a.bins[pb.bins_off + j], v);
+  float* part = a.part + g.part_off + (long long)blockIdx.y * gsize;
+  for (int j = tid; j < gsize; j += PCAL_NT) part[j] += sb[j];
+}
+
+// bins of every band of group blockIdx.x = the sum of the group's nsplit partial slices; one
+// 64-lane workgroup per 64 bins, so that many workgroups share the slices' reads
+__global__ void __launch_bounds__(64) k_pcal_reduce(PcalArgs a) {
+  const PcalGroup g = a.groups[blockIdx.x];
+  const int gsize = g.nb * g.M;
+  for (int j = blockIdx.y * 64 + threadIdx.x; j < gsize; j += gridDim.y * 64) {
+    const float* src = a.part + g.part_off + j;
+    double s = 0.0;
+#pragma unroll 8
+    for (int p = 0; p < a.nsplit; ++p) s += src[(long long)p * gsize];
+    a.bins[g.bins_off + j] = (float)s;
   }
 }
 
 }  // namespace
 
-int fxg_launch_pcal(const PcalArgs& a, int nband, int maxM, hipStream_t st) {
-  if (nband <= 0 || a.nffts <= 0) return 0;
-  const size_t lds = sizeof(float) * (size_t)maxM;
+size_t fxg_pcal_lds(int max_gsize) { return sizeof(float) * ((size_t)max_gsize + 256); }
+
+template <int NBITS, bool WHOLE, bool REG>
+int launch_pcal_form(const PcalArgs& a, int ngroups, size_t lds, hipStream_t st) {
   static size_t raised = 0;
-  if (raise_lds(k_pcal, lds, &raised)) return -1;
-  hipLaunchKernelGGL(k_pcal, dim3(nband, (a.nffts + FFTS_PER_WG - 1) / FFTS_PER_WG), dim3(256), lds, st, a);
+  if (raise_lds(k_pcal<NBITS, WHOLE, REG>, lds, &raised)) return -1;
+  hipLaunchKernelGGL((k_pcal<NBITS, WHOLE, REG>), dim3(ngroups, a.nsplit), dim3(PCAL_NT), lds, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// groups a.groups[0, ngroups) all of kernel form `form` (PCAL_FORM_* bits)
+int fxg_launch_pcal(const PcalArgs& a, int form, int ngroups, int max_gsize, hipStream_t st) {
+  if (ngroups <= 0 || a.nffts <= 0) return 0;
+  const size_t lds = fxg_pcal_lds(max_gsize);
+  switch (form) {
+    case 0: return launch_pcal_form<2, false, false>(a, ngroups, lds, st);
+    case PCAL_FORM_REG: return launch_pcal_form<2, false, true>(a, ngroups, lds, st);
+    case PCAL_FORM_WHOLE: return launch_pcal_form<2, true, false>(a, ngroups, lds, st);
+    case PCAL_FORM_WHOLE | PCAL_FORM_REG: return launch_pcal_form<2, true, true>(a, ngroups, lds, st);
+    case PCAL_FORM_8BIT: return launch_pcal_form<8, false, false>(a, ngroups, lds, st);
+    case PCAL_FORM_8BIT | PCAL_FORM_REG: return launch_pcal_form<8, false, true>(a, ngroups, lds, st);
+    case PCAL_FORM_8BIT | PCAL_FORM_WHOLE: return launch_pcal_form<8, true, false>(a, ngroups, lds, st);
+    case PCAL_FORM_8BIT | PCAL_FORM_WHOLE | PCAL_FORM_REG: return launch_pcal_form<8, true, true>(a, ngroups, lds, st);
+    default: return -1;
+  }
+}
+
+int fxg_launch_pcal_reduce(const PcalArgs& a, int ngroups, int max_gsize, hipStream_t st) {
+  if (ngroups <= 0) return 0;
+  const int gy = std::max(1, std::min(512, (max_gsize + 63) / 64));
+  hipLaunchKernelGGL(k_pcal_reduce, dim3(ngroups, gy), dim3(64), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -9,7 +9,8 @@ import pytest
 
 import pcal as pc
 from difx_amd import config as cf
-from fxcases import case_cfg0, case_cfg1, case_real8, damaged_slot, pcal_tone_slot, slot
+from fxcases import (case_cfg0, case_cfg1, case_mark5b, case_offsets, case_real8, damaged_slot,
+                     pcal_tone_slot, slot)
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +38,10 @@ def check_against_oracle(core, cfg, offsets, data, ctrl, interval, base=0):
     ("2-bit VDIF USB, 5 MHz comb", lambda: case_cfg0(B=32), 5_000_000),
     ("2-bit VDIF LSB", lambda: case_cfg0(B=32, lsb=True), 1_000_000),
     ("2-bit VDIF 16 bands", lambda: case_cfg1(B=16, nstations=2, nfreq=4), 1_000_000),
+    ("2-bit Mark5B 16 bands", lambda: case_mark5b(B=16), 1_000_000),
+    # 4800.37 MHz: offset 630 kHz, a 6400-sample period (per-sample LDS adds) beside a
+    # 64-sample one (register sums) in the same datastream
+    ("fractional-MHz band edge", lambda: case_offsets(B=32), 1_000_000),
 ])
 def test_pcal_matches_the_oracle(gpu, name, make, interval):
     from difx_amd.core import FxCore
