@@ -36,12 +36,14 @@ __global__ void __launch_bounds__(PCAL_NT) k_pcal(PcalArgs a) {
   const DevDsRun& run = a.run[g.ds];
   const int M = g.M;
   const int gsize = g.nb * M;
-  float* sb = sm;                                  // [nb][M]
-  float* lut = sm + gsize;                         // sample levels: 256 (8-bit) or 4 (2-bit)
+  float2* lutp = reinterpret_cast<float2*>(sm);    // 2-bit: levels of two fields (4-bit index)
+  float* lut = sm + 32;                            // sample levels: 256 (8-bit) or 4 (2-bit)
+  float* sb = sm + 288;                            // [nb][M]
   const int tid = threadIdx.x;
   for (int j = tid; j < gsize; j += PCAL_NT) sb[j] = 0.f;
   if (NBITS == 2) {
     if (tid < 4) lut[tid] = lut2(ds.format, tid);
+    if (tid < 16) lutp[tid] = make_float2(lut2(ds.format, tid & 3), lut2(ds.format, tid >> 2));
   } else {
     lut[tid] = lut8(tid);
   }
@@ -99,7 +101,19 @@ __global__ void __launch_bounds__(PCAL_NT) k_pcal(PcalArgs a) {
 #pragma unroll
         for (int u = 0; u < PCAL_UNR; ++u) {
           if (!ok[u]) continue;
-          if (WHOLE) {
+          if (WHOLE && NBITS == 2) {                 // two bands per table read
+#pragma unroll
+            for (int k = 0; k < 16; k += 2) {
+              const float2 x = lutp[(w[u] >> (2 * k)) & 15u];
+              if (REG) {
+                acc[k] += x.x;
+                acc[k + 1] += x.y;
+              } else {
+                atomicAdd(&sb[k * M + binu[u]], x.x);
+                atomicAdd(&sb[(k + 1) * M + binu[u]], x.y);
+              }
+            }
+          } else if (WHOLE) {
 #pragma unroll
             for (int k = 0; k < (32 / NBITS); ++k) {
               const float x = lut[(w[u] >> (NBITS * k)) & FMASK];
@@ -153,7 +167,7 @@ __global__ void __launch_bounds__(64) k_pcal_reduce(PcalArgs a) {
 
 }  // namespace
 
-size_t fxg_pcal_lds(int max_gsize) { return sizeof(float) * ((size_t)max_gsize + 256); }
+size_t fxg_pcal_lds(int max_gsize) { return sizeof(float) * ((size_t)max_gsize + 288); }
 
 template <int NBITS, bool WHOLE, bool REG>
 int launch_pcal_form(const PcalArgs& a, int ngroups, size_t lds, hipStream_t st) {
