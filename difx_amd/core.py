@@ -85,6 +85,7 @@ def load_library() -> C.CDLL:
     lib.fxg_set_kurtosis.argtypes = [C.c_void_p, C.c_int32]
     lib.fxg_set_pcal.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int64]
     lib.fxg_get_pcal.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, P(C.c_int32)]
+    lib.fxg_pcal_info.argtypes = [C.c_void_p, C.c_int32, C.c_int32, P(C.c_int32), P(C.c_int64), P(C.c_int32)]
     lib.fxg_sta_autocorr.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                      C.c_void_p, P(C.c_int32)]
     lib.fxg_get_kurtosis.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, P(C.c_int32)]

@@ -2073,7 +2073,6 @@ static void pcal_tones_of(double lofreq, double bw, bool lsb, long long interval
 }
 
 int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz) {
-  if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || ds < 0 || ds >= ctx->nds || interval_hz < 0) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
   const fxg_datastream& xs = ctx->dss[ds];
@@ -2087,6 +2086,7 @@ int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz)
   }
   ctx->pcal_interval.resize(ctx->nds, 0);
   ctx->pcal_base.resize(ctx->nds, 0);
+  const long long prev_interval = ctx->pcal_interval[ds], prev_base = ctx->pcal_base[ds];
   ctx->pcal_interval[ds] = interval_hz;
   ctx->pcal_base[ds] = base_hz;
   // rebuild the tone table, the band groups and the device bins for every datastream with
@@ -2111,8 +2111,9 @@ int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz)
       if (t.offset_hz >= 0) {
         const long long g = gcd_ll(t.fs_hz, gcd_ll(t.offset_hz, t.interval_hz));   // offset 0: gcd = interval
         const long long M = t.fs_hz / std::max<long long>(g, 1);
-        if (M > 16384) {
-          ctx->pcal_interval[ds] = 0;              // leave the datastream off
+        if (M > 16384) {                         // keep the previous setting and tables
+          ctx->pcal_interval[ds] = prev_interval;
+          ctx->pcal_base[ds] = prev_base;
           return fail(ctx, FXG_E_INVALID, "phase-cal tone period longer than 16384 samples");
         }
         // extend the open group (same datastream, the previous band, same M, room in LDS)
@@ -2171,13 +2172,27 @@ int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz)
   ctx->pcal_runs = runs;
   ctx->pcal_part_len = part;
   ctx->pcal_max_gsize = max_gsize;
-  if (!groups.empty()) {
+  if (!groups.empty() && !ctx->dry) {              // a host-only context keeps the tables only
     if (int rc = dev_upload(ctx, &ctx->d_pcal_groups, ctx->pcal_groups)) return rc;
     if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_part, sizeof(float) * part)) return rc;
     if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_bins, sizeof(float) * off)) return rc;
     if (int rc = dev_zero(ctx, ctx->d_pcal_bins, sizeof(float) * off)) return rc;
   }
   return FXG_OK;
+}
+
+int fxg_pcal_info(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t* ntones, int64_t* offset_hz,
+                  int32_t* period) {
+  if (!ctx || !ntones || !offset_hz || !period || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
+  for (size_t i = 0; i < ctx->pcal_tones.size(); ++i) {
+    const fxg_ctx::PcalTones& t = ctx->pcal_tones[i];
+    if (t.ds != ds || t.band != band) continue;
+    *ntones = t.ntones;
+    *offset_hz = t.offset_hz;
+    *period = ctx->pcal_entry[i] < 0 ? 0 : ctx->pcal_tab[ctx->pcal_entry[i]].M;
+    return FXG_OK;
+  }
+  return FXG_E_STATE;                              // phase-cal not on for this band
 }
 
 // The extractors' getFinalPCal tones for the last subint (pcal.cpp:550-580, 746-790,

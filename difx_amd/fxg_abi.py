@@ -216,4 +216,5 @@ EXPORTED_SYMBOLS = [
     "fxg_segment_params", "fxg_buffer_params", "fxg_control_params",
     "fxg_normalise_device", "fxg_difx_records", "fxg_set_kurtosis", "fxg_get_kurtosis",
     "fxg_sta_autocorr", "fxg_set_pcal", "fxg_get_pcal",
+    "fxg_pcal_info",
 ]

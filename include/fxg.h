@@ -223,6 +223,12 @@ int fxg_get_kurtosis(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t maxch
  * when phase-cal is off for the band. */
 int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz);
 int fxg_get_pcal(const fxg_ctx* ctx, int32_t ds, int32_t band, float* out, int32_t* ntones);
+/* The tone table entry of a recorded band (Configuration's getDRecordedFreqNumPCalTones and
+ * getDRecordedFreqPCalOffsetsHz) and its tone period in samples (0 for a flagged offset).
+ * fxg_set_pcal and this also work on a host-only context (fxg_create_layout), which builds
+ * the tables without device memory. */
+int fxg_pcal_info(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t* ntones, int64_t* offset_hz,
+                  int32_t* period);
 
 /* Mk5Mode::unpack (mk5mode.cpp:94-174) as a device call: decode nsamples time samples starting
  * at sample `startsample` of datastream ds from d_data (nbytes) into d_out[nrecbands][nsamples]

@@ -137,3 +137,68 @@ def test_flagged_offset_gives_zero_tones():
     n, off, _ = pc.tone_table(1400.0, 16.0, False, 5_000_000, 6_000_000)
     assert (n, off) == (4, -5_000_000)
     assert out.shape == (n,) and not np.any(out)
+
+
+def _lib():
+    from pathlib import Path
+    if not (Path(__file__).resolve().parent.parent / "difx_amd" / "libfxg.so").exists():
+        pytest.skip("libfxg.so not built")
+    from difx_amd.core import load_library
+    return load_library()
+
+
+@pytest.mark.parametrize("lsb", [False, True])
+@pytest.mark.parametrize("interval,base", [(1_000_000, 0), (5_000_000, 0), (500_000, 0),
+                                           (5_000_000, 1_000_000), (5_000_000, 6_000_000),
+                                           (10_000_000, 0)])
+def test_library_tone_table_equals_the_restatement(lsb, interval, base):
+    """fxg_set_pcal / fxg_pcal_info on a host-only context: the library's tone count, offset
+    and tone period of every recorded band equal the restatement's, over band edges on whole,
+    quarter, hundredth and fractional MHz, and 8, 16 and 32 MHz bands."""
+    import ctypes as C
+    lib = _lib()
+    for freqs, bw in [([8400.25, 8400.01, 4800.37, 1400.0], 16.0), ([4800.0, 4832.5], 32.0),
+                      ([1400.0, 1408.0], 8.0)]:
+        cfg = cf.standard_config(2, 1, freqs, bw, 256, subint_ns=8 * (256 * 1000 // int(bw)),
+                                 nbits=2, lsb=lsb, seed=5)
+        fa, keep = cfg.to_abi()
+        h = C.c_void_p()
+        assert lib.fxg_create_layout(C.byref(fa), C.byref(h)) == 0
+        try:
+            fs = int(2e6 * bw)
+            want = [pc.tone_table(f, bw, lsb, interval, base)[:2] for f in freqs]
+            periods = [pc.tone_period(fs, off, interval) if off >= 0 else 0 for _, off in want]
+            rc = lib.fxg_set_pcal(h, 0, interval, base)
+            if max(periods) > 16384:                   # rejected: the device folds in LDS
+                assert rc != 0
+                continue
+            assert rc == 0
+            for b, ((n, off), M) in enumerate(zip(want, periods)):
+                nt, of, per = C.c_int32(), C.c_int64(), C.c_int32()
+                assert lib.fxg_pcal_info(h, 0, b, C.byref(nt), C.byref(of), C.byref(per)) == 0
+                assert (nt.value, of.value, per.value) == (n, off, M), (freqs[b], bw)
+            # phase-cal stays off on the other datastream
+            nt, of, per = C.c_int32(), C.c_int64(), C.c_int32()
+            assert lib.fxg_pcal_info(h, 1, 0, C.byref(nt), C.byref(of), C.byref(per)) != 0
+        finally:
+            lib.fxg_destroy(h)
+
+
+def test_library_rejects_long_tone_periods():
+    """An 8400.0123 MHz band edge with a 1 MHz comb puts the first tone 987.7 kHz in: the tone
+    period is 32 MHz / gcd(32 MHz, 987.7 kHz, 1 MHz) = 320000 samples, beyond the 16384 the
+    device folds in LDS, so fxg_set_pcal refuses it and keeps the previous setting (off)."""
+    import ctypes as C
+    lib = _lib()
+    n, off, _ = pc.tone_table(8400.0123, 16.0, False, 1_000_000)
+    assert pc.tone_period(32_000_000, off, 1_000_000) == 320000
+    cfg = cf.standard_config(2, 1, [8400.0123], 16.0, 256, subint_ns=8 * 16000, nbits=2, seed=5)
+    fa, keep = cfg.to_abi()
+    h = C.c_void_p()
+    assert lib.fxg_create_layout(C.byref(fa), C.byref(h)) == 0
+    try:
+        assert lib.fxg_set_pcal(h, 0, 1_000_000, 0) == -1               # FXG_E_INVALID
+        nt, of, per = C.c_int32(), C.c_int64(), C.c_int32()
+        assert lib.fxg_pcal_info(h, 0, 0, C.byref(nt), C.byref(of), C.byref(per)) == -4   # off
+    finally:
+        lib.fxg_destroy(h)
