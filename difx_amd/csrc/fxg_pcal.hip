@@ -69,7 +69,10 @@ __global__ void __launch_bounds__(PCAL_NT) k_pcal(PcalArgs a) {
       int t = fd.n0 + tid;                         // time sample of the buffer (n0 >= 0)
       int fr = t / spf;
       int within = t - fr * spf;
+      // the sample number modulo M, kept in [0, M) also for samples before the scan start
+      // (a buffer that starts before it has a negative datasec)
       int bin = (int)((s0 + t) % M);
+      if (bin < 0) bin += M;
       if (REG && bin != accbin) {
         if (accbin >= 0) {
 #pragma unroll

@@ -23,6 +23,11 @@ with f_k = offset + k interval, or (k + 1) interval when the offset is 0 (the tr
 drops the DC bin, pcal.cpp:573-580).  Here the sum is taken directly in FP64 with the phase
 reduced exactly in integers ((f_k (m mod fs)) mod fs).
 
+A buffer that starts before the scan start (negative datasec) gives negative m; this
+restatement (and the device) use m mod M in [0, M).  The reference converts the negative
+double to size_t, which C leaves undefined; x86-64 wraps it to 2^64 + m, and that agrees with
+m mod M whenever M divides 2^64 (M a power of two).
+
 Pins (tests/test_pcal_cpu.py): tone tables worked by hand for USB and LSB bands, a band with no
 tone and a zero offset; a planted cosine at a tone frequency returns its amplitude N A / 2 and
 its phase; the FFTs that feed the sums are exactly those to which the FP64 F-engine restatement

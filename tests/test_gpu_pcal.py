@@ -112,3 +112,20 @@ def test_planted_tones(gpu, lsb):
             assert abs(abs(got[k]) / (B * L * a / 2) - 1) < 0.01
             assert abs(np.angle(got[k] * np.exp(-1j * ph))) < 0.01
     core.close()
+
+
+def test_buffer_before_the_scan_start(gpu):
+    """offsets (0, 0, 0): station 0's buffer starts 5.6 ms before the scan start (datasec -1),
+    so every sample number of its subint is negative; the bins take m mod M in [0, M)."""
+    from difx_amd.core import FxCore
+    print("pcal buffer before the scan start", flush=True)
+    cfg = case_real8(B=32)
+    offsets = [0, 0, 0]
+    data, ctrl = slot(cfg, offsets)
+    assert int(ctrl[0][1]) == -1
+    core = FxCore(cfg)
+    for d in range(len(cfg.datastreams)):
+        core.set_pcal(d, 1_000_000)
+    core.processdata(offsets, data, ctrl)
+    check_against_oracle(core, cfg, offsets, data, ctrl, 1_000_000)
+    core.close()
