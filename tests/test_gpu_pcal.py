@@ -9,7 +9,7 @@ import pytest
 
 import pcal as pc
 from difx_amd import config as cf
-from fxcases import (case_cfg0, case_cfg1, case_mark5b, case_offsets, case_real8, damaged_slot,
+from fxcases import (case_cfg0, case_cfg1, case_cfg3, case_mark5b, case_offsets, case_real8, damaged_slot,
                      pcal_tone_slot, slot)
 
 pytestmark = pytest.mark.gpu
@@ -123,6 +123,24 @@ def test_buffer_before_the_scan_start(gpu):
     offsets = [0, 0, 0]
     data, ctrl = slot(cfg, offsets)
     assert int(ctrl[0][1]) == -1
+    core = FxCore(cfg)
+    for d in range(len(cfg.datastreams)):
+        core.set_pcal(d, 1_000_000)
+    core.processdata(offsets, data, ctrl)
+    check_against_oracle(core, cfg, offsets, data, ctrl, 1_000_000)
+    core.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_uneven_bands_both_fengine_forms(gpu, mode):
+    """Uneven bands (8, 32 and 64 MHz datastreams with zooms; tone periods 16, 64 and 128
+    samples) with the per-length F-engine launches (1) and the persistent one (2): the
+    phase-cal pass reads each chunk's per-FFT delays from either."""
+    from difx_amd.core import FxCore
+    print("pcal uneven bands, F-engine form", mode, flush=True)
+    cfg = case_cfg3(B=32, fengine_mode=mode)
+    offsets = [0, 1, 0]
+    data, ctrl = damaged_slot(cfg, offsets)
     core = FxCore(cfg)
     for d in range(len(cfg.datastreams)):
         core.set_pcal(d, 1_000_000)
