@@ -244,6 +244,32 @@ int validate_and_derive(fxg_ctx* ctx) {
   if (ctx->rotorder < 0 || ctx->rotorder > 2)
     return fail(ctx, FXG_E_INVALID, "fringe rotation order must be 0, 1 or 2");
   if (ctx->nds < 1) return fail(ctx, FXG_E_INVALID, "no datastreams");
+  if (ctx->nfreqs < 1) return fail(ctx, FXG_E_INVALID, "no freqs");
+  // every table index is checked before any lookup through it
+  for (int d = 0; d < ctx->nds; ++d) {
+    const fxg_datastream& ds = ctx->dss[d];
+    if (ds.nrecfreqs < 1 || ds.nrecfreqs > FXG_MAX_FREQS || ds.nrecbands < 1 || ds.nrecbands > FXG_MAX_BANDS)
+      return fail(ctx, FXG_E_INVALID, "bad recorded freq/band count");
+    if (ds.nzoomfreqs < 0 || ds.nzoomfreqs > FXG_MAX_FREQS || ds.nzoombands < 0 || ds.nzoombands > FXG_MAX_BANDS ||
+        ds.nrecbands + ds.nzoombands > FXG_MAX_BANDS)
+      return fail(ctx, FXG_E_INVALID, "bad zoom freq/band count");
+    for (int k = 0; k < ds.nrecfreqs; ++k)
+      if (ds.recfreq_index[k] < 0 || ds.recfreq_index[k] >= ctx->nfreqs)
+        return fail(ctx, FXG_E_INVALID, "recorded freq index out of range");
+    for (int k = 0; k < ds.nzoomfreqs; ++k)
+      if (ds.zoomfreq_index[k] < 0 || ds.zoomfreq_index[k] >= ctx->nfreqs)
+        return fail(ctx, FXG_E_INVALID, "zoom freq index out of range");
+    for (int b = 0; b < ds.nrecbands; ++b)
+      if (ds.recband_localfreq[b] < 0 || ds.recband_localfreq[b] >= ds.nrecfreqs)
+        return fail(ctx, FXG_E_INVALID, "band local freq out of range");
+    for (int z = 0; z < ds.nzoombands; ++z)
+      if (ds.zoomband_localfreq[z] < 0 || ds.zoomband_localfreq[z] >= ds.nzoomfreqs)
+        return fail(ctx, FXG_E_INVALID, "zoom band freq out of range");
+    for (int k = 0; k < ds.nrecfreqs; ++k)
+      if (!std::isfinite(ds.phase_offset_deg[k]) || !std::isfinite(ds.clock_offset_us[k]) ||
+          !std::isfinite(ds.clock_offset_delta_us[k]) || !std::isfinite(ds.lo_offset_hz[k]))
+        return fail(ctx, FXG_E_INVALID, "non-finite per-freq offset");
+  }
   // blockspersend from the first datastream's first recorded freq (configuration.cpp:1843-1852)
   {
     const fxg_freq& f = ctx->freqs[ctx->dss[0].recfreq_index[0]];
@@ -655,10 +681,13 @@ int build_tables(fxg_ctx* ctx) {
     x.model_nsamples = s.model_nsamples;
     x.model_off = (int)(model_all.size() / FXG_MAX_POLY);
     for (int k = 0; k < s.model_nsamples * FXG_MAX_POLY; ++k) model_all.push_back(s.model_delay_us[k]);
-    // fractional LO frequency (mode.cpp:83-99)
+    // fractional LO frequency (mode.cpp:83-101); for double-sideband data the LO is the band
+    // centre, the edge moved by BW/2 towards the band
     x.fractionallo = 0;
     for (int k = 0; k < s.nrecfreqs; ++k) {
-      double lo = ctx->freqs[s.recfreq_index[k]].freq_mhz;
+      const fxg_freq& f = ctx->freqs[s.recfreq_index[k]];
+      double lo = f.freq_mhz;
+      if (s.complex_sampling == 2) lo += f.lsb ? -f.bw_mhz / 2.0 : f.bw_mhz / 2.0;
       if (std::fabs(lo - (int)(lo + 0.5)) > TINY) x.fractionallo = 1;
     }
     for (int k = 0; k < s.nrecfreqs; ++k) {
@@ -679,6 +708,24 @@ int build_tables(fxg_ctx* ctx) {
       r.nbands = 0;
       for (int b = 0; b < s.nrecbands; ++b)
         if (s.recband_localfreq[b] == k) r.band[r.nbands++] = b;
+      // the per-pol fractional-sample phasors: R takes fracsamprotatorA (+delta/2), the other
+      // pols fracsamprotatorB (-delta/2) (mode.cpp:1076-1118, 1257-1261); with two or more
+      // bands the second pol (the first non-R band of the first two) turns by phasecorrA =
+      // exp(+i pi phi / 360), phi the pol phase offset in degrees, computed in double and
+      // rounded to float (mode.cpp:421-447, 1318-1330)
+      int lcp = -1;
+      if (r.nbands > 1) lcp = s.recband_pol[r.band[0]] == 'R' ? r.band[1] : r.band[0];
+      const double degphase = -s.phase_offset_deg[k] / 2;
+      const float2 pca = make_float2((float)std::cos(degphase * M_PI / 180.0),
+                                     (float)-std::sin(degphase * M_PI / 180.0));
+      for (int kb = 0; kb < r.nbands; ++kb) {
+        const int b = r.band[kb];
+        r.bhd[kb] = s.recband_pol[b] == 'R' ? r.halfdelta : -r.halfdelta;
+        r.pcre[kb] = b == lcp ? pca.x : 1.0f;
+        r.pcim[kb] = b == lcp ? pca.y : 0.0f;
+        r.newfrac[kb] = kb > 0 && (r.bhd[kb] != r.bhd[kb - 1] || r.pcre[kb] != r.pcre[kb - 1] ||
+                                   r.pcim[kb] != r.pcim[kb - 1]);
+      }
     }
     // channel-frequency slopes (mode.cpp:303-324), computed as the reference does
     x.chanfreq_off = (int)chanfreq.size();

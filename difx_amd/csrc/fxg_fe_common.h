@@ -326,13 +326,18 @@ __device__ void build_rotator(float2* srot, float* squad, const DsConst& k, cons
   }
 }
 
-// Fractional-sample phasors: sub (j < S) and step (S <= j < S+NFS) (mode.cpp:1030-1073).
-// Order 0 adds the post-F fringe phase to the sub arguments (mode.cpp:1040-1049).
+// Fractional-sample phasors of band kb of a freq: sub (j < S) and step (S <= j < S+NFS)
+// (mode.cpp:1030-1122), with the band's clock-delta term hd (+delta/2 for R, -delta/2 for
+// the other pols).  Order 0 adds the post-F fringe phase to the sub arguments
+// (mode.cpp:1040-1049).  A pol phase offset pc (mode.cpp:1318-1330) multiplies the sub
+// phasors, so every channel of the band turns by it.
 template <int ORDER>
 __device__ void build_frac(float2* sfrac, const DsConst& k, const FreqConst& rf, bool iscomplex,
-                           const float* __restrict__ cf, const FftDelay& fd, int t, int nt) {
+                           const float* __restrict__ cf, const FftDelay& fd, double hd, float2 pc,
+                           int t, int nt) {
   const int S = k.S, NFS = k.NFS;
-  const float e = (float)((double)fd.fse - rf.clockoff + rf.halfdelta);
+  const bool turn = !(pc.x == 1.0f && pc.y == 0.0f);
+  const float e = (float)((double)fd.fse - rf.clockoff + hd);
   const float* sub = cf;
   // double sideband: dsbstepchannelfreqs; any other LSB: lsbstepchannelfreqs (mode.cpp:838-857)
   const float* step = k.dsb ? cf + S + 2 * NFS : rf.lsb ? cf + S + NFS : cf + S;
@@ -348,7 +353,9 @@ __device__ void build_frac(float2* sfrac, const DsConst& k, const FreqConst& rf,
     const float arg = (j < S) ? (ORDER == 0 ? sub[j] * e + prf : sub[j] * e) : step[j - S] * e;
     float sn, cs;
     sincosf(arg, &sn, &cs);
-    sfrac[j] = make_float2(cs, sn);
+    float2 ph = make_float2(cs, sn);
+    if (turn && j < S) ph = cmul(ph, pc);
+    sfrac[j] = ph;
   }
 }
 
@@ -535,7 +542,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
       if (rf.nbands == 0) continue;
       __syncthreads();                            // previous users of srot/sfrac are done
       if (ORDER != 0) build_rotator<ORDER>(srot, squad, k, rf, fd, L, tl, TPF);
-      build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, tl, TPF);
+      build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, dsp->rf[f].bhd[0], make_float2(dsp->rf[f].pcre[0], dsp->rf[f].pcim[0]), tl, TPF);
       __syncthreads();
       // rotator (orders 1, 2) of this thread's samples and fractional-sample phasors of its
       // channels: shared by the bands of the freq, so they can be computed once per freq
@@ -595,6 +602,12 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
       if (HOIST) make_frq(0, frq_h);
       for (int kb = 0; kb < rf.nbands; ++kb) {
         const int band = dsp->rf[f].band[kb];
+        if (kb > 0 && dsp->rf[f].newfrac[kb]) {  // another pol's phasor table (uniform branch)
+          __syncthreads();                        // the previous band's phasor reads are done
+          build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, dsp->rf[f].bhd[kb], make_float2(dsp->rf[f].pcre[kb], dsp->rf[f].pcim[kb]), tl, TPF);
+          __syncthreads();
+          if (HOIST) make_frq(0, frq_h);
+        }
         float2 v[V][8];
 #pragma unroll
         for (int u = 0; u < V; ++u) {
@@ -837,10 +850,15 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
       if (rf.nbands == 0) continue;
       __syncthreads();                            // previous users of srot/sfrac are done
       if (ORDER != 0) build_rotator<ORDER>(srot, squad, k, rf, fd, L, t, BNT);
-      build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, t, BNT);
+      build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, dsp->rf[f].bhd[0], make_float2(dsp->rf[f].pcre[0], dsp->rf[f].pcim[0]), t, BNT);
       __syncthreads();
       for (int kb = 0; kb < rf.nbands; ++kb) {
         const int band = dsp->rf[f].band[kb];
+        if (kb > 0 && dsp->rf[f].newfrac[kb]) {  // another pol's phasor table (uniform branch;
+          // the previous band ended on a barrier after its last phasor read)
+          build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, dsp->rf[f].bhd[kb], make_float2(dsp->rf[f].pcre[kb], dsp->rf[f].pcim[kb]), t, BNT);
+          __syncthreads();
+        }
         // 2-bit real data of an FFT whose samples are all valid (weight 1: every frame of the
         // window present and valid, mode.cpp:656-669 and the good-sample count of
         // mk5mode.cpp:94-174): each thread first gathers this band's fields of its R

@@ -24,6 +24,14 @@ struct DevRecFreq {
   int    nbands;          // recorded bands of this freq
   int    band[4];         // their band indices (sample-word order)
   int    pad;
+  // per band of the freq: the clock-delta term of its fractional-sample phasor, +halfdelta
+  // for R (fracsamprotatorA) and -halfdelta for the other pols (fracsamprotatorB,
+  // mode.cpp:1076-1118, 1257-1261); the pol phase-offset phasor of the second pol
+  // (phasecorrA, mode.cpp:421-447, 1318-1330), (1, 0) for the others; and whether band k's
+  // phasor table differs from band k-1's
+  double bhd[4];
+  float  pcre[4], pcim[4];
+  int    newfrac[4];
 };
 
 // Per datastream (everything the F-engine and the unpacker read).

@@ -75,6 +75,7 @@ struct Slot {
   hipEvent_t done = nullptr;
   float* d_res = nullptr;                 // this subint's results (coreresultlength cf32)
   std::vector<uint8_t*> d_in;             // input copies (host-data jobs only)
+  std::vector<long long> in_cap;          // their capacities in bytes
 };
 
 enum JobState { JOB_PENDING = 0, JOB_ACTIVE = 1, JOB_FINISHED = 2, JOB_FAILED = 3 };
@@ -174,11 +175,13 @@ int admit_job(fxg_server* S, Job* j) {
     if (!j->on_device) {
       const int nds = j->cc.cfg.ndatastreams;
       s.d_in.assign(nds, nullptr);
+      s.in_cap.assign(nds, 0);
       for (int d = 0; d < nds; ++d) {
         int64_t db = 0;
         (void)fxg_data_bytes(j->ctx, d, &db);
         const long long cap = db + 4LL * j->cc.dss[d].framebytes;
         if (hipMalloc((void**)&s.d_in[d], (size_t)cap) != hipSuccess) { j->err = "input slot"; return FXG_E_NOMEM; }
+        s.in_cap[d] = cap;
       }
     }
   }
@@ -195,6 +198,10 @@ int launch(fxg_server* S, Job* j, Slot& s, Request& r) {
     if (j->on_device) {
       ptr[d] = r.data[d];
     } else {
+      if (r.nbytes[d] < 0 || r.nbytes[d] > s.in_cap[d]) {   // the copy must fit the input slot
+        j->err = "subint data larger than the job's input slot";
+        return FXG_E_INVALID;
+      }
       if (r.nbytes[d] > 0 &&
           hipMemcpyAsync(s.d_in[d], r.data[d], r.nbytes[d], hipMemcpyHostToDevice, j->stream) != hipSuccess)
         return FXG_E_HIP;

@@ -377,6 +377,34 @@ def postf_phase(avgdelay: float, intdelay: int, lofreq: float, fractional_lo: bo
     return ph
 
 
+def is_fractional_lo(cfg, ds) -> bool:
+    """Mode's fractionalLoFreq (mode.cpp:83-101): some recorded freq's LO is not a whole MHz.
+    For double-sideband data the LO is the band centre: the edge moved by BW/2 towards the
+    band (down for LSB, up for USB) before the test."""
+    for fi in ds.recfreqs:
+        fr = cfg.freqs[fi]
+        lo = fr.freq_mhz
+        if ds.complex_sampling == 2:
+            lo += -fr.bw_mhz / 2.0 if fr.lsb else fr.bw_mhz / 2.0
+        if abs(lo - int(lo + 0.5)) > TINY:
+            return True
+    return False
+
+
+def pol_halfdelta(pol: str, halfdelta: float) -> float:
+    """The clock-delta term of a band's fractional-sample phasor (mode.cpp:1030-1122,
+    1257-1261): +delta/2 for R (fracsamprotatorA), -delta/2 for every other pol
+    (fracsamprotatorB).  With delta 0 both phasors are the same."""
+    return halfdelta if pol == "R" else -halfdelta
+
+
+def phase_offset_phasors(ds) -> np.ndarray:
+    """phasecorrA per recorded freq (mode.cpp:421-447): exp(+i pi phi / 360) for a pol phase
+    offset of phi degrees, i.e. half the offset, applied to the second pol (mode.cpp:1318-1330)."""
+    deg = np.asarray(ds.per_freq("phase_offset_deg"), dtype=np.float64)
+    return np.exp(1j * np.radians(deg / 2.0))
+
+
 def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
                     B: Optional[int] = None) -> StationResult:
     """Mode::setValidFlags/setData/setOffsets + Mode::process for every FFT of the subint
@@ -429,8 +457,8 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
     if bpb_num == 0:
         bpb_num, bpb_den = 1, 8 // (nrec * nsb)
     S = ds.arraystridelen or calc_stride(C)
-    fractional_lo = any(abs(cfg.freqs[fi].freq_mhz - int(cfg.freqs[fi].freq_mhz + 0.5)) > TINY
-                        for fi in ds.recfreqs)
+    fractional_lo = is_fractional_lo(cfg, ds)
+    pcorr = phase_offset_phasors(ds)
 
     for i in range(B):
         if not ((int(valid_bits[i // FLAGS_PER_INT]) >> (i % FLAGS_PER_INT)) & 1):
@@ -470,8 +498,9 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
             phase = rotator_phase(order, L, S, a[i], b[i], cq[i], lofreq, int(intdelay[i]), fractional_lo,
                                   loff, dt, intwall, fracwall)
             rot = np.exp(-1j * TWO_PI * phase)
-            # fractional-sample correction (mode.cpp:1030-1073), f64
-            e = fse - ds.per_freq("clock_offset_us")[lf] + ds.per_freq("clock_offset_delta_us")[lf] / 2
+            # fractional-sample correction (mode.cpp:1030-1122), f64: the R band takes
+            # fracsamprotatorA (+delta/2), every other pol fracsamprotatorB (-delta/2)
+            halfdelta = ds.per_freq("clock_offset_delta_us")[lf] / 2
             k = np.arange(C)
             if usedouble:                                                # dsbstepchannelfreqs
                 chanfreq = TWO_PI * (k - C / 2) * bw / C
@@ -479,12 +508,14 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
                 chanfreq = TWO_PI * (k - C) * bw / C
             else:
                 chanfreq = TWO_PI * k * bw / C
-            frac = np.exp(1j * chanfreq * e)
+            postf = 1.0
             if order == 0:                                               # mode.cpp:1040-1049
                 pf = postf_phase(avgdelay[i], int(intdelay[i]), lofreq, fractional_lo, loff, walltime)
-                frac = frac * np.exp(-1j * TWO_PI * pf)
+                postf = np.exp(-1j * TWO_PI * pf)
             match = [j for j, (blf, _) in enumerate(ds.recbands) if blf == lf]
             for j in match:
+                e = fse - ds.per_freq("clock_offset_us")[lf] + pol_halfdelta(ds.recbands[j][1], halfdelta)
+                frac = np.exp(1j * chanfreq * e) * postf
                 x = seg[j].astype(np.complex128)
                 X = np.fft.fft(x * rot)                                  # mode.cpp:1174-1183
                 if order == 0:
@@ -510,6 +541,12 @@ def process_station(cfg, d: int, offsets, data: np.ndarray, control: np.ndarray,
                 spectra[j, i] = F
                 autos[0, j] += F * np.conj(F)                           # mode.cpp:1272
                 acw[0, j] += w
+            if len(match) > 1:
+                # the second pol of a dual-pol freq turns by the pol phase offset, after the
+                # parallel-hand autocorrelation and before the cross-pol ones
+                # (mode.cpp:1318-1330)
+                lcp = match[1] if ds.recbands[match[0]][1] == "R" else match[0]
+                spectra[lcp, i] *= pcorr[lf]
             if len(match) > 1 and cfg.write_cross_autocorrs:            # mode.cpp:1333-1352
                 j0, j1 = match[0], match[1]
                 autos[1, j0] += spectra[j0, i] * np.conj(spectra[j1, i])

@@ -41,6 +41,8 @@ typedef struct { float re, im; } cf32;
 typedef struct port_freq {
   double lofreq, lo_offset_hz, fraclooffset, clockoff, halfdelta;
   int lsb, looff, isfraclooffset, nbands, band[4];
+  double bhd[4];                  /* per band: +halfdelta (R, fracsamprotatorA) or -halfdelta */
+  float pcre[4], pcim[4];         /* per band: pol phase-offset phasor (1, 0) unless rotated */
 } port_freq;
 
 typedef struct port_ds {
@@ -339,8 +341,8 @@ static void* worker(void* arg) {
               for (int j = 0; j < S; ++j) tab[j] = cmulf(quad[j], tab[j]);
             }
           }
-          /* fractional-sample phasors */
-          const float e = (float)((double)fse - R->clockoff + R->halfdelta);
+          /* fractional-sample phasors: built for the first band, and again for a band whose
+             clock-delta term differs (fracsamprotatorB of the non-R pol, mode.cpp:1076-1118) */
           const float* sub = D->chanfreq;
           const float* step = D->dsb ? D->chanfreq + S + 2 * NFS
                                      : R->lsb ? D->chanfreq + S + NFS : D->chanfreq + S;   /* mode.cpp:838-857 */
@@ -351,18 +353,24 @@ static void* worker(void* arg) {
             pr -= walltime * R->lo_offset_hz;
             prf = (float)(-TWO_PI * (pr - (int)pr));
           }
-          for (int j = 0; j < S; ++j) {
-            float ar = sub[j] * e;
-            if (order == 0) ar += prf;
-            tab[j].re = cosf(ar); tab[j].im = sinf(ar);
-          }
-          for (int s = 0; s < NFS; ++s) { const float ar = step[s] * e; tab[S + s].re = cosf(ar); tab[S + s].im = sinf(ar); }
-          for (int j = 0; j < S; ++j) frac[j] = tab[j];
-          for (int s = 1; s < NFS; ++s)
-            for (int j = 0; j < S; ++j) frac[s * S + j] = cmulf(tab[j], tab[S + s]);
-          for (int j = 0; j < S; ++j) frac[j] = cmulf(frac[j], tab[S]);
           for (int kb = 0; kb < R->nbands; ++kb) {
             const int b = R->band[kb];
+            if (kb == 0 || R->bhd[kb] != R->bhd[kb - 1]) {
+              const float e = (float)((double)fse - R->clockoff + R->bhd[kb]);
+              for (int j = 0; j < S; ++j) {
+                float ar = sub[j] * e;
+                if (order == 0) ar += prf;
+                tab[j].re = cosf(ar); tab[j].im = sinf(ar);
+              }
+              for (int s = 0; s < NFS; ++s) { const float ar = step[s] * e; tab[S + s].re = cosf(ar); tab[S + s].im = sinf(ar); }
+              for (int j = 0; j < S; ++j) frac[j] = tab[j];
+              for (int s = 1; s < NFS; ++s)
+                for (int j = 0; j < S; ++j) frac[s * S + j] = cmulf(tab[j], tab[S + s]);
+              for (int j = 0; j < S; ++j) frac[j] = cmulf(frac[j], tab[S]);
+            }
+            /* the pol phase offset of the second pol (phasecorrA, mode.cpp:1318-1330) */
+            const cf32 pc = {R->pcre[kb], R->pcim[kb]};
+            const int rot_pc = !(pc.re == 1.0f && pc.im == 0.0f);
             cf32* in = fc[d].in;
             const cf32* u = unp + (size_t)b * L;
             cf32* F = spec[d][b * NB + sl];
@@ -376,6 +384,7 @@ static void* worker(void* arg) {
                 for (int c = 0; c < C; ++c) F[c] = X[c];
               }
               for (int c = 0; c < C; ++c) F[c] = cmulf(F[c], frac[c]);
+              if (rot_pc) for (int c = 0; c < C; ++c) F[c] = cmulf(F[c], pc);
               continue;
             }
             if (D->cplx) {
@@ -401,6 +410,7 @@ static void* worker(void* arg) {
               for (int c = 0; c < C; ++c) F[c] = X[c];
             }
             for (int c = 0; c < C; ++c) F[c] = cmulf(F[c], frac[c]);
+            if (rot_pc) for (int c = 0; c < C; ++c) F[c] = cmulf(F[c], pc);
           }
         }
       }

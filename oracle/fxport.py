@@ -23,7 +23,8 @@ class port_freq(C.Structure):
     _fields_ = [("lofreq", C.c_double), ("lo_offset_hz", C.c_double), ("fraclooffset", C.c_double),
                 ("clockoff", C.c_double), ("halfdelta", C.c_double),
                 ("lsb", C.c_int), ("looff", C.c_int), ("isfraclooffset", C.c_int), ("nbands", C.c_int),
-                ("band", C.c_int * 4)]
+                ("band", C.c_int * 4), ("bhd", C.c_double * 4), ("pcre", C.c_float * 4),
+                ("pcim", C.c_float * 4)]
 
 
 class port_ds(C.Structure):
@@ -111,8 +112,7 @@ def process_subint(cfg, offsets, data, control, nthreads: Optional[int] = None, 
         x.L, x.C, x.S, x.NS, x.NFS = L, Cn, S, L // S, Cn // S
         num = (nrec * nsb) // 8
         x.bpb_num, x.bpb_den = (num, 1) if num else (1, 8 // (nrec * nsb))
-        x.fractionallo = int(any(abs(cfg.freqs[fi].freq_mhz - int(cfg.freqs[fi].freq_mhz + 0.5)) > fo.TINY
-                                 for fi in ds.recfreqs))
+        x.fractionallo = int(fo.is_fractional_lo(cfg, ds))
         x.sampletime_us = cfg.sampletime_us(d)
         ctrl = np.ascontiguousarray(control[d], dtype=np.int32)
         buf = np.ascontiguousarray(data[d], dtype=np.uint8)
@@ -153,8 +153,17 @@ def process_subint(cfg, offsets, data, control, nthreads: Optional[int] = None, 
             r.lsb = int(fr.lsb)
             bands = [j for j, (blf, _) in enumerate(ds.recbands) if blf == lf]
             r.nbands = len(bands)
+            lcp = -1
+            if len(bands) > 1:                  # mode.cpp:1318-1330: the second pol turns
+                lcp = bands[1] if ds.recbands[bands[0]][1] == "R" else bands[0]
+            deg = ds.per_freq("phase_offset_deg")[lf]
+            # phasecorrA (mode.cpp:421-447), double rounded to float
+            pcre = float(np.float32(math.cos(-deg / 2 * math.pi / 180.0)))
+            pcim = float(np.float32(-math.sin(-deg / 2 * math.pi / 180.0)))
             for k, b in enumerate(bands[:4]):
                 r.band[k] = b
+                r.bhd[k] = fo.pol_halfdelta(ds.recbands[b][1], r.halfdelta)
+                r.pcre[k], r.pcim[k] = (pcre, pcim) if b == lcp else (1.0, 0.0)
     # products: baseline products then autocorrelation products (parallel, cross-pol)
     lay = fo.result_layout(cfg)
     used, equiv = fo.freq_used(cfg)

@@ -67,7 +67,9 @@ def case_complex_fourstep(B: int = 4, xengine: str = "fp32", nstations: int = 4,
 
 def case_offsets(B: int = 32, xengine: str = "fp32", seed: int = 9, **kw):
     """4 stations dual pol, 2 x 32 MHz with a fractional-MHz band edge, per-freq LO offsets
-    (integer and fractional Hz), clock offsets and pol deltas, channel averaging 4."""
+    (integer and fractional Hz), clock offsets, pol clock deltas and pol phase offsets,
+    channel averaging 4.  Station 1 records L before R, so the per-pol terms follow the pol
+    and not the band order."""
     cfg = cf.standard_config(4, 2, [4800.37, 4840.0], 32.0, 512, subint_ns=B * 16000, xengine=xengine,
                              seed=seed, **kw)
     rng = np.random.default_rng(seed)
@@ -75,8 +77,36 @@ def case_offsets(B: int = 32, xengine: str = "fp32", seed: int = 9, **kw):
         d.lo_offset_hz = [float(rng.choice([0.0, 10.0, -2.5, 37.25])) for _ in d.recfreqs]
         d.clock_offset_us = [float(rng.uniform(-0.02, 0.02)) for _ in d.recfreqs]
         d.clock_offset_delta_us = [float(rng.uniform(-0.005, 0.005)) for _ in d.recfreqs]
+        d.phase_offset_deg = [float(rng.uniform(-90.0, 90.0)) for _ in d.recfreqs]
+    d1 = cfg.datastreams[1]
+    d1.recbands = [(lf, "L" if p == "R" else "R") for lf, p in d1.recbands]
     for f in cfg.freqs:
         f.chanavg = 4
+    return cfg
+
+
+def case_offsets_fourstep(B: int = 4, xengine: str = "fp32", seed: int = 19, **kw):
+    """The per-freq terms of case_offsets on the four-step F-engine: 3 stations dual pol,
+    32 MHz with nchan 8192 (L = 16384), a fractional-MHz edge."""
+    cfg = cf.standard_config(3, 2, [6000.37], 32.0, 8192, subint_ns=B * 256000, xengine=xengine,
+                             seed=seed, max_delay_us=200.0, **kw)
+    rng = np.random.default_rng(seed)
+    for d in cfg.datastreams:
+        d.lo_offset_hz = [37.25]
+        d.clock_offset_us = [float(rng.uniform(-0.02, 0.02))]
+        d.clock_offset_delta_us = [float(rng.uniform(-0.005, 0.005))]
+        d.phase_offset_deg = [float(rng.uniform(-90.0, 90.0))]
+    d1 = cfg.datastreams[1]
+    d1.recbands = [(lf, "L" if p == "R" else "R") for lf, p in d1.recbands]
+    return cfg
+
+
+def case_dsb_fraclo(B: int = 16, xengine: str = "fp32", seed: int = 10, **kw):
+    """Complex 8-bit double-sideband data, 4 stations, a 5 MHz band with a whole-MHz edge and a
+    half-MHz centre (fractional LO, mode.cpp:83-101), USB and LSB freqs, nchan 256."""
+    cfg = cf.standard_config(4, 1, [4800.0, 4900.0], 5.0, 256, subint_ns=B * 51200, nbits=8,
+                             complex_sampling=2, xengine=xengine, seed=seed, max_delay_us=300.0, **kw)
+    cfg.freqs[1].lsb = True
     return cfg
 
 

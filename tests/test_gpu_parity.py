@@ -13,6 +13,7 @@ import fxoracle as fo
 from difx_amd import synth
 from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_complex_fourstep,
                      case_fourstep_ffts, case_long_ffts, case_mark5b, case_offsets, case_real8,
+                     case_offsets_fourstep, case_dsb_fraclo,
                      damaged_slot, record_errors, slot, channel_errors)
 
 pytestmark = pytest.mark.gpu
@@ -304,20 +305,27 @@ def test_fourstep_in_persistent_launch_bit_identical(gpu):
         np.testing.assert_array_equal(v.view(np.uint64), ref.view(np.uint64))
 
 
-@pytest.mark.parametrize("name", ["offsets", "mark5b", "real8", "damaged"])
+@pytest.mark.parametrize("name", ["offsets", "offsets_fourstep", "dsb_fraclo", "mark5b", "real8", "damaged"])
 @pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
 def test_more_cases(gpu, name, xe, bound):
     """Per-freq LO offsets (integer and fractional Hz), a fractional-MHz band edge, clock
-    offsets and pol deltas with channel averaging 4; Mark5B frames; 8-bit real samples; a slot
-    with flagged FFTs, an invalid frame and a fill-pattern frame."""
+    offsets, per-pol clock deltas and pol phase offsets with channel averaging 4 (also with L
+    recorded before R, and on the four-step F-engine); double-sideband data whose band centre
+    is a fractional MHz; Mark5B frames; 8-bit real samples; a slot with flagged FFTs, an
+    invalid frame and a fill-pattern frame."""
     if name == "offsets":
         cfg = case_offsets(B=32, xengine=xe)
+    elif name == "offsets_fourstep":
+        cfg = case_offsets_fourstep(B=4, xengine=xe)
+    elif name == "dsb_fraclo":
+        cfg = case_dsb_fraclo(B=16, xengine=xe)
     elif name == "mark5b":
         cfg = case_mark5b(B=32, xengine=xe)
     elif name == "real8":
         cfg = case_real8(B=32, xengine=xe)
     else:
         cfg = case_cfg1(B=32, nstations=3, nfreq=2, xengine=xe)
-    got, ref, e, lay = _run_and_compare(cfg, damaged=(name == "damaged"), spectra_check=(name != "offsets"))
+    got, ref, e, lay = _run_and_compare(cfg, damaged=(name == "damaged"),
+                                        spectra_check=not name.startswith("offsets"))
     _say(f"{name} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
     assert max(e["cross"]) < bound and max(e["auto"]) < bound

@@ -75,6 +75,17 @@ def test_footprint_without_device():
     (lambda c: (setattr(c.freqs[0], "nchan", 16384), setattr(c, "subint_ns", 2 * 512000)), True),   # L 32768
     (lambda c: (setattr(c.freqs[0], "nchan", 32768), setattr(c, "subint_ns", 2 * 1024000)), False),  # L > 32768
     (lambda c: setattr(c.freqs[0], "chanavg", 3), False),                # does not divide nchan
+    # table indices out of range are refused before any lookup through them
+    (lambda c: setattr(c.datastreams[0], "recfreqs", [99]), False),
+    (lambda c: setattr(c.datastreams[1], "recfreqs", [-1]), False),
+    (lambda c: setattr(c.datastreams[0], "recbands", [(5, "R")]), False),
+    (lambda c: (setattr(c.datastreams[0], "zoomfreqs", [(99, 0, 0)]),
+                setattr(c.datastreams[0], "zoombands", [(0, "R")])), False),
+    (lambda c: (setattr(c.datastreams[0], "zoomfreqs", [(0, 0, 0)]),
+                setattr(c.datastreams[0], "zoombands", [(3, "R")])), False),
+    (lambda c: setattr(c.datastreams[0], "phase_offset_deg", [float("nan")]), False),
+    (lambda c: setattr(c.datastreams[0], "clock_offset_delta_us", [float("inf")]), False),
+    (lambda c: setattr(c.datastreams[0], "phase_offset_deg", [30.0]), True),
 ])
 def test_validation(mutate, ok):
     _lib()
