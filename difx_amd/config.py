@@ -50,6 +50,9 @@ class Datastream:
     zoombands: List[Tuple[int, str]] = field(default_factory=list)       # (local zoom freq, pol)
     clock_coeff0_us: float = 0.0
     model_order: int = 5
+    # delay polynomials of phase centres 1..P, each [nsamples, 6] like `model` (which is then
+    # the pointing centre's); only read when the config has more than one phase centre
+    phasecentre_models: Optional[List[np.ndarray]] = None
 
     @property
     def framebytes(self) -> int:
@@ -85,6 +88,9 @@ class Config:
     fengine_mode: int = 0       # 0 automatic, 1 one launch per FFT length, 2 one persistent launch
     fengine_threads: int = 0    # F-engine workgroup size 256 or 512 (0: by FFT length)
     autotune: int = 0           # 1: time the launch candidates at create and keep the fastest
+    nphasecentres: int = 1      # phase centres of the scan (Model::getNumPhaseCentres)
+    process_threads: int = 1    # Core process threads, whose block ranges set the uv-shift intervals
+    max_ns_between_xc_avg: int = 2_000_000_000   # Model::getMaxNSBetweenXCAvg
 
     # ---- derived quantities (configuration.cpp) ----
     def blockspersend(self) -> int:
@@ -165,6 +171,11 @@ class Config:
             keep.append(m)
             x.model_nsamples = m.shape[0]
             x.model_delay_us = m.ctypes.data_as(C.POINTER(C.c_double))
+            if self.nphasecentres > 1:
+                pcm = np.ascontiguousarray(np.stack(d.phasecentre_models[: self.nphasecentres]), dtype=np.float64)
+                assert pcm.shape == (self.nphasecentres,) + m.shape
+                keep.append(pcm)
+                x.phasecentre_delay_us = pcm.ctypes.data_as(C.POINTER(C.c_double))
         keep.append(dss)
         nb = len(self.baselines)
         bls = (abi.fxg_baseline * max(nb, 1))()
@@ -200,6 +211,9 @@ class Config:
         cfg.fengine_mode = self.fengine_mode
         cfg.fengine_threads = self.fengine_threads
         cfg.autotune = self.autotune
+        cfg.nphasecentres = self.nphasecentres
+        cfg.process_threads = self.process_threads
+        cfg.max_ns_between_xc_avg = int(self.max_ns_between_xc_avg)
         return cfg, keep
 
 

@@ -70,6 +70,7 @@ class fxg_datastream(C.Structure):
         ("model_order", C.c_int32),
         ("model_nsamples", C.c_int32),
         ("model_delay_us", C.POINTER(C.c_double)),
+        ("phasecentre_delay_us", C.POINTER(C.c_double)),
     ]
 
 
@@ -106,6 +107,10 @@ class fxg_config(C.Structure):
         ("fengine_mode", C.c_int32),
         ("fengine_threads", C.c_int32),
         ("autotune", C.c_int32),
+        ("nphasecentres", C.c_int32),
+        ("process_threads", C.c_int32),
+        ("reserved", C.c_int32),
+        ("max_ns_between_xc_avg", C.c_int64),
     ]
 
 

@@ -86,7 +86,12 @@ typedef struct fxg_datastream {
   double  clock_coeff0_us;    /* clock offset term 0; intclockseconds = floor(c0/1e6 + 0.5) (mode.cpp:69) */
   int32_t model_order;        /* delay polynomial order (<= 5) */
   int32_t model_nsamples;     /* number of model intervals of length model_inc_secs */
-  const double* model_delay_us; /* [model_nsamples][FXG_MAX_POLY]; t in s from interval start */
+  const double* model_delay_us; /* [model_nsamples][FXG_MAX_POLY]; t in s from interval start:
+                                   the phase centre with one phase centre, the pointing centre
+                                   with several (Mode::setOffsets source index, mode.cpp:1518-1521) */
+  const double* phasecentre_delay_us; /* [nphasecentres][model_nsamples][FXG_MAX_POLY]: phase
+                                   centres 1..P (Model source indices 1..P); read only when the
+                                   config has nphasecentres > 1, else may be NULL */
 } fxg_datastream;
 
 /* One baseline (Configuration::baselinedata, configuration.h): datastream pair and, per
@@ -122,7 +127,16 @@ typedef struct fxg_config {
   int32_t fengine_mode;       /* FXG_F_AUTO, FXG_F_PER_LENGTH or FXG_F_PERSISTENT */
   int32_t fengine_threads;    /* workgroup size 128, 256 or 512 (persistent: 256, 512); 0 = by FFT length (persistent: 512) */
   int32_t autotune;           /* 1: time the F-engine launch candidates at create and keep the fastest */
+  int32_t nphasecentres;      /* Model::getNumPhaseCentres; 0 or 1 = one phase centre, up to
+                                 FXG_MAX_PHASECENTRES: results per phase centre, uv-shifted
+                                 (core.cpp:1605-1938) */
+  int32_t process_threads;    /* Core process threads (their FFT-block ranges start the uv-shift
+                                 intervals, core.cpp:2158-2166); 0 = 1 */
+  int32_t reserved;
+  int64_t max_ns_between_xc_avg; /* Model::getMaxNSBetweenXCAvg: uv-shift interval (core.cpp:771-776) */
 } fxg_config;
+
+#define FXG_MAX_PHASECENTRES 16
 
 /* F-engine launch forms: one launch per FFT length (datastreams grouped by length), or every
  * length in one persistent launch whose workgroups take (length, datastream, part) work items. */

@@ -148,12 +148,10 @@ def samples_per_frame(ds) -> int:
 # --------------------------------------------------------------------------------------
 # Delay model
 # --------------------------------------------------------------------------------------
-def delay_interpolator(model: np.ndarray, order_poly: int, model_inc_secs: float,
-                       offsettime: float, timespan: float, numincrements: int) -> Optional[np.ndarray]:
-    """Model::calculateDelayInterpolator(order 2) (model.cpp:219-289) for one scan whose
-    polynomials start at the scan start (polyoffset = 0): three evaluations of the station
-    delay polynomial at the start, middle and end of the span, fitted with a quadratic in
-    increment units."""
+def _delay_samples(model: np.ndarray, order_poly: int, model_inc_secs: float, offsettime: float,
+                   timespan: float):
+    """The three polynomial evaluations of Model::calculateDelayInterpolator (model.cpp:
+    228-273): at the start, middle and end of the span, or None outside the model."""
     scansample = int(offsettime / float(model_inc_secs))
     nsamp = model.shape[0]
     if scansample == nsamp and (offsettime - scansample * model_inc_secs) < 1e-6:
@@ -170,10 +168,35 @@ def delay_interpolator(model: np.ndarray, order_poly: int, model_inc_secs: float
     d1 = ev(offsettime + timespan / 2.0 - base)
     d0 = ev(offsettime - base)
     d2 = ev(offsettime + timespan - base)
+    return d0, d1, d2
+
+
+def delay_interpolator(model: np.ndarray, order_poly: int, model_inc_secs: float,
+                       offsettime: float, timespan: float, numincrements: int) -> Optional[np.ndarray]:
+    """Model::calculateDelayInterpolator(order 2) (model.cpp:219-289) for one scan whose
+    polynomials start at the scan start (polyoffset = 0): three evaluations of the station
+    delay polynomial at the start, middle and end of the span, fitted with a quadratic in
+    increment units."""
+    ds = _delay_samples(model, order_poly, model_inc_secs, offsettime, timespan)
+    if ds is None:
+        return None
+    d0, d1, d2 = ds
     n = numincrements
     return np.array([(2.0 * d0 - 4.0 * d1 + 2.0 * d2) / (n * n),
                      (-3.0 * d0 + 4.0 * d1 - d2) / n,
                      d0])
+
+
+def delay_interpolator_linear(model: np.ndarray, order_poly: int, model_inc_secs: float,
+                              offsettime: float, timespan: float, numincrements: int):
+    """Model::calculateDelayInterpolator(order 1) (model.cpp:279-283): [slope per increment,
+    offset] of the straight line through the three evaluations."""
+    ds = _delay_samples(model, order_poly, model_inc_secs, offsettime, timespan)
+    if ds is None:
+        return None
+    d0, d1, d2 = ds
+    c0 = (d2 - d0) / numincrements
+    return np.array([c0, d0 + (d1 - (c0 * numincrements / 2.0 + d0)) / 3.0])
 
 
 def intclockseconds(ds) -> int:
@@ -192,6 +215,8 @@ class Layout:
     autocorr_off: List[int]
     acweight_off: List[int]
     bandsperautocorr: int
+    shiftdecorr_off: Dict[Tuple[int, int], int] = None   # (freq, baseline) -> cf32 offset (P > 1)
+    nphasecentres: int = 1
 
 
 def freq_used(cfg) -> Tuple[List[bool], List[bool]]:
@@ -220,13 +245,22 @@ def freq_used(cfg) -> Tuple[List[bool], List[bool]]:
     return used, equiv
 
 
+def nphasecentres(cfg) -> int:
+    return max(1, int(getattr(cfg, "nphasecentres", 1) or 1))
+
+
 def result_layout(cfg) -> Layout:
+    """populateResultLengths (configuration.cpp:2348-2630) for one config, no pulsar bins:
+    per used freq and baseline a cross block of P phase centres x products x output channels,
+    then the baseline weights, then (P > 1) P shift-decorrelation floats per (freq, baseline),
+    then the autocorrelations and their weights."""
     nf = len(cfg.freqs)
+    P = nphasecentres(cfg)
     used, equiv = freq_used(cfg)
     maxprod = max((len(p) for b in cfg.baselines for (_, p) in b.freqs), default=1)
     bpa = 2 if maxprod > 2 else 1
     idx = 0
-    boff, bw = {}, {}
+    boff, bw, sdo = {}, {}, {}
     for f in range(nf):                      # cross-correlations (configuration.cpp:2464-2521)
         if not used[f]:
             continue
@@ -235,7 +269,7 @@ def result_layout(cfg) -> Layout:
             for fi, prods in b.freqs:
                 if fi == f:
                     boff[(f, j)] = idx
-                    idx += len(prods) * ((fr.nchan + fr.chanavg - 1) // fr.chanavg)
+                    idx += P * len(prods) * ((fr.nchan + fr.chanavg - 1) // fr.chanavg)
     for f in range(nf):                      # baseline weights (:2523-2548)
         if not used[f]:
             continue
@@ -244,6 +278,14 @@ def result_layout(cfg) -> Layout:
                 if fi == f:
                     bw[(f, j)] = idx
                     idx += len(prods) // 2 + len(prods) % 2
+    for f in range(nf if P > 1 else 0):      # shift decorrelation factors (:2549-2566)
+        if not used[f]:
+            continue
+        for j, b in enumerate(cfg.baselines):
+            for fi, prods in b.freqs:
+                if fi == f:
+                    sdo[(f, j)] = idx
+                    idx += P // 2 + P % 2
     ac = []
     for d in cfg.datastreams:                # autocorrelations (:2567-2586)
         ac.append(idx)
@@ -271,7 +313,7 @@ def result_layout(cfg) -> Layout:
                 toadd += bpa
         toadd = (toadd + 1) // 2
         idx += max(toadd, 1)
-    return Layout(idx, boff, bw, ac, acw, bpa)
+    return Layout(idx, boff, bw, ac, acw, bpa, sdo, P)
 
 
 # --------------------------------------------------------------------------------------
@@ -585,6 +627,88 @@ def band_record_index(ds, band: int) -> int:
     return [j for j, (blf, bp) in enumerate(ds.recbands) if blf == parent_lf and bp == pol][0]
 
 
+# --------------------------------------------------------------------------------------
+# Multi-phase-centre uv shift (Core::uvshiftAndAverageBaselineFreq, core.cpp:1605-1938)
+# --------------------------------------------------------------------------------------
+def xc_intervals(cfg, B: int):
+    """The uv-shift intervals of one subint: each Core process thread owns a contiguous range
+    of FFT blocks (core.cpp:2158-2166) and shifts/averages every maxxcblocks FFTs of it, the
+    last interval shorter (core.cpp:761-776, 984-991, 1055-1057).  maxxcblocks is the
+    model's maximum ns between averages over the FFT duration, rounded down to a multiple of
+    numBufferedFFTs (at least numBufferedFFTs).  Returns (first FFT, FFTs, nsoffset, nswidth)
+    per interval, nsoffset the interval centre in ns from the subint start."""
+    T = max(1, int(getattr(cfg, "process_threads", 1) or 1))
+    nbuf = cfg.num_buffered_ffts
+    blockns = float(cfg.subint_ns) / float(B)
+    maxxc = int(cfg.max_ns_between_xc_avg / blockns)
+    maxxc -= maxxc % nbuf
+    if maxxc == 0:
+        maxxc = nbuf
+    out = []
+    start, nblk = 0, 0
+    for t in range(T):
+        start += nblk
+        nblk = B // T + (1 if t < B % T else 0)
+        k, shift = 0, 0
+        while k < nblk:
+            n = min(maxxc, nblk - k)
+            out.append((start + k, n, (start + shift * maxxc + n / 2.0) * blockns, n * blockns))
+            k += n
+            shift += 1
+    return out
+
+
+def differential_delays(cfg, bl, offsets, nsoffset: float) -> np.ndarray:
+    """differentialdelay[s] = (rate, delay) of every phase centre s of a baseline at the
+    interval centre (core.cpp:1636-1670): order-1 interpolators over 1 us at
+    offsets[1] + (offsets[2] + nsoffset)/1e9 of the pointing centre (model_delay_us) and of
+    each phase centre, per station; the applied delay is the phase-centre minus pointing delay,
+    stretched by the pointing rate."""
+    d1, d2 = cfg.datastreams[bl.ds1], cfg.datastreams[bl.ds2]
+    t = offsets[1] + (offsets[2] + nsoffset) / 1e9
+    lin = lambda m, ds: delay_interpolator_linear(m, ds.model_order, cfg.model_inc_secs, t, 0.000001, 1)
+    pt1, pt2 = lin(d1.model, d1), lin(d2.model, d2)
+    P = nphasecentres(cfg)
+    out = np.zeros((P, 2))
+    for sc in range(P):
+        pc1, pc2 = lin(d1.phasecentre_models[sc], d1), lin(d2.phasecentre_models[sc], d2)
+        a1 = pc1[1] - pt1[1]
+        a2 = pc2[1] - pt2[1]
+        a1 += a1 * pt1[0]
+        a2 += a2 * pt2[0]
+        out[sc, 1] = a2 - a1
+        out[sc, 0] = pc2[0] + pt1[0] - (pc1[0] + pt2[0])
+    return out
+
+
+def uvshift_phasor(fr, delay_us: float) -> np.ndarray:
+    """exp(+2 pi i delay f) per channel of a freq, f the channel's sky frequency as the
+    rotator of core.cpp:1699-1766 lays it out: the band edge plus c x channel width (USB), or
+    the edge minus (nchan - 1 - c) channel widths (LSB)."""
+    c = np.arange(fr.nchan)
+    chbw = fr.bw_mhz / fr.nchan
+    f = fr.freq_mhz - (fr.nchan - 1 - c) * chbw if fr.lsb else fr.freq_mhz + c * chbw
+    if abs(delay_us) <= 1.0e-20:
+        return np.ones(fr.nchan, dtype=np.complex128)
+    return np.exp(2j * np.pi * delay_us * f)
+
+
+def shift_decorrelation(fr, dd: np.ndarray, nswidth: float) -> float:
+    """One interval's contribution nswidth x time-smearing x delay decorrelation
+    (core.cpp:1887-1922)."""
+    ts, dl = 1.0, 1.0
+    if abs(dd[0]) > 1e-18:
+        m = 2 * np.pi * dd[0] * (nswidth / 1000.0) * fr.freq_mhz
+        ts = math.sin(m / 2.0) / (m / 2.0)
+        if ts < 0.0:
+            ts = 0.0
+    if abs(dd[1]) > 1e-18:
+        dl = 1.0 - abs(dd[1] / (fr.nchan / fr.bw_mhz))
+        if dl < 0.0:
+            dl = 0.0
+    return nswidth * ts * dl
+
+
 def process_subint(cfg, offsets, data: Sequence[np.ndarray], control: Sequence[np.ndarray],
                    B: Optional[int] = None, stations: Optional[List[StationResult]] = None):
     """Core::processdata for one slot (core.cpp:657-1129) with the accumulation of
@@ -599,23 +723,40 @@ def process_subint(cfg, offsets, data: Sequence[np.ndarray], control: Sequence[n
                     for d in range(len(cfg.datastreams))]
     fres = res.view(np.float64)
     used, equiv = freq_used(cfg)
+    P = lay.nphasecentres
+    # one phase centre: the whole subint at once, unrotated; several: per uv-shift interval,
+    # rotated to each phase centre (core.cpp:1730-1879)
+    intervals = xc_intervals(cfg, B) if P > 1 else [(0, B, 0.0, 0.0)]
     for j, bl in enumerate(cfg.baselines):
         s1, s2 = stations[bl.ds1], stations[bl.ds2]
-        d1, d2 = cfg.datastreams[bl.ds1], cfg.datastreams[bl.ds2]
-        for fi, prods in bl.freqs:
-            fr = cfg.freqs[fi]
-            nch = fr.nchan
-            avg = fr.chanavg
-            off = lay.baseline_off[(fi, j)]
-            for p, (b1, b2) in enumerate(prods):
-                v = np.einsum("tc,tc->c", s1.spectra[b1, :, :nch], np.conj(s2.spectra[b2, :, :nch]))
+        for i0, n, nsoff, nsw in intervals:
+            dd = differential_delays(cfg, bl, offsets, nsoff) if P > 1 else np.zeros((1, 2))
+            for fi, prods in bl.freqs:
+                fr = cfg.freqs[fi]
+                nch = fr.nchan
+                avg = fr.chanavg
                 nout = (nch + avg - 1) // avg
-                if avg == 1:
-                    res[off + p * nch: off + (p + 1) * nch] += v
-                else:
-                    pad = np.zeros(nout * avg, dtype=np.complex128)
-                    pad[:nch] = v
-                    res[off + p * nout: off + (p + 1) * nout] += pad.reshape(nout, avg).sum(1) / avg
+                off = lay.baseline_off[(fi, j)]
+                rots = [uvshift_phasor(fr, dd[sc, 1]) if P > 1 else None for sc in range(P)]
+                for p, (b1, b2) in enumerate(prods):
+                    v = np.einsum("tc,tc->c", s1.spectra[b1, i0:i0 + n, :nch],
+                                  np.conj(s2.spectra[b2, i0:i0 + n, :nch]))
+                    for sc in range(P):
+                        vr = v * rots[sc] if P > 1 else v
+                        o = off + (sc * len(prods) + p) * nout
+                        if avg == 1:
+                            res[o: o + nch] += vr
+                        else:
+                            pad = np.zeros(nout * avg, dtype=np.complex128)
+                            pad[:nch] = vr
+                            res[o: o + nout] += pad.reshape(nout, avg).sum(1) / avg
+                if P > 1:
+                    so = lay.shiftdecorr_off[(fi, j)] * 2
+                    for sc in range(P):
+                        fres[so + sc] += shift_decorrelation(fr, dd[sc], nsw)
+    for j, bl in enumerate(cfg.baselines):
+        s1, s2 = stations[bl.ds1], stations[bl.ds2]
+        for fi, prods in bl.freqs:
             # baseline weights (core.cpp:1005-1052, 1066-1112): sum over FFTs of w1*w2
             bwo = lay.bweight_off[(fi, j)] * 2
             for p, (b1, b2) in enumerate(prods):
