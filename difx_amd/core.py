@@ -236,13 +236,14 @@ class FxCore:
                                                   C.c_void_p(stream) if stream else None))
 
     def difx_records(self, results: np.ndarray, mjd: int, seconds: float, configindex: int, sourceindex: int,
-                     fftsperintegration: float, baselinenumber, uvw, telescopeindex) -> bytes:
-        """Visibility::writeSWIN's records of a normalised host results array (complex64)."""
+                     fftsperintegration: float, baselinenumber, uvw, telescopeindex, phasecentre: int = 0) -> bytes:
+        """Visibility::writeSWIN's records of a normalised host results array (complex64); with
+        several phase centres those of one phase centre's file."""
         res = np.ascontiguousarray(results, dtype=np.complex64)
         bl = np.ascontiguousarray(baselinenumber, dtype=np.int32)
         uv = np.ascontiguousarray(uvw, dtype=np.float64)
         tel = np.ascontiguousarray(telescopeindex, dtype=np.int32)
-        m = abi.fxg_difx_meta(int(mjd), int(configindex), int(sourceindex), 0, float(seconds),
+        m = abi.fxg_difx_meta(int(mjd), int(configindex), int(sourceindex), int(phasecentre), float(seconds),
                               float(fftsperintegration), bl.ctypes.data_as(C.POINTER(C.c_int32)),
                               uv.ctypes.data_as(C.POINTER(C.c_double)), tel.ctypes.data_as(C.POINTER(C.c_int32)))
         n, nrec = C.c_int64(), C.c_int32()

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -54,6 +55,10 @@ struct Slot {
 struct Staging {
   DevDsRun* h_run = nullptr;      // pinned
   int32_t* h_flags = nullptr;     // pinned
+  double* h_uvdd = nullptr;       // pinned: uv-shift delays [interval][baseline][phase centre]
+  float* h_uvdec = nullptr;       // pinned: shift decorrelation sums of the subint
+  double* d_uvdd = nullptr;
+  float* d_uvdec = nullptr;
   hipEvent_t used = nullptr;
   bool pending = false;
 };
@@ -70,6 +75,7 @@ struct fxg_ctx {
   std::vector<fxg_datastream> dss;
   std::vector<fxg_baseline> bls;
   std::vector<std::vector<double>> models;
+  std::vector<std::vector<double>> pcmodels;   // [ds] phase centres 1..P: [P][nsamples][POLY]
   long long subint_ns = 0;
   int guard_ns = 0, rotorder = 1, nbuffered = 10, crossautos = 1, xengine = 0;
   double model_inc = 120.0, scan_start = 0.0;
@@ -110,6 +116,17 @@ struct fxg_ctx {
   int bpa = 1;
   long long coreresultlength = 0;
   std::vector<int> boff, bwoff, acoff, acwoff;
+  // multi-phase-centre uv shift (core.cpp:1605-1938): P > 1 phase centres
+  int nphase = 1, nthreads = 1;
+  long long maxns_xc = 0;
+  int xms = 0, rs = 0;             // xmac and rotate stride lengths (configuration.cpp:2653-2734)
+  std::vector<int> sdoff;          // [freq][baseline] cf32 offset of the shift-decorrelation words
+  struct Interval { int i0, n; double nsoff, nswidth; };
+  std::vector<Interval> ivals;     // uv-shift intervals of a subint
+  std::vector<UvFreq> uvfreq;      // per freq: rotation constants
+  UvFreq* d_uvfreq = nullptr;
+  std::vector<int> uvdec_idx;      // float index in results of each shift-decorrelation sum
+  int* d_uvdec_idx = nullptr;
   std::vector<int> used, equiv;
   // device tables
   std::vector<DevDs> h_ds;
@@ -405,6 +422,15 @@ int validate_and_derive(fxg_ctx* ctx) {
     return fail(ctx, FXG_E_INVALID, "F-engine workgroup size must be 128, 256 or 512");
   if (ctx->cfg_fthreads == 128 && ctx->cfg_fmode == FXG_F_PERSISTENT)
     return fail(ctx, FXG_E_INVALID, "the persistent F-engine launch runs 256 or 512 threads");
+  if (ctx->nphase > FXG_MAX_PHASECENTRES)
+    return fail(ctx, FXG_E_INVALID, "too many phase centres");
+  if (ctx->nphase > 1) {
+    for (int d = 0; d < ctx->nds; ++d)
+      if (!ctx->dss[d].phasecentre_delay_us)
+        return fail(ctx, FXG_E_INVALID, "phase-centre delay polynomials missing");
+    if (ctx->nbuffered < 1) return fail(ctx, FXG_E_INVALID, "numBufferedFFTs must be positive");
+    if (ctx->maxns_xc < 0) return fail(ctx, FXG_E_INVALID, "negative uv-shift interval");
+  }
   return FXG_OK;
 }
 
@@ -471,7 +497,7 @@ void build_layout(fxg_ctx* ctx) {
       for (int k = 0; k < ctx->bls[j].nfreqs; ++k)
         if (ctx->bls[j].freq_index[k] == f) {
           ctx->boff[(size_t)f * nb + j] = (int)idx;
-          idx += (long long)ctx->bls[j].npolprod[k] * ((fr.nchan + fr.chanavg - 1) / fr.chanavg);
+          idx += (long long)ctx->nphase * ctx->bls[j].npolprod[k] * ((fr.nchan + fr.chanavg - 1) / fr.chanavg);
         }
   }
   for (int f = 0; f < nf; ++f) {
@@ -481,6 +507,16 @@ void build_layout(fxg_ctx* ctx) {
         if (ctx->bls[j].freq_index[k] == f) {
           ctx->bwoff[(size_t)f * nb + j] = (int)idx;
           idx += ctx->bls[j].npolprod[k] / 2 + ctx->bls[j].npolprod[k] % 2;
+        }
+  }
+  ctx->sdoff.assign((size_t)nf * nb, -1);        // shift decorrelation factors (configuration.cpp:2549-2566)
+  for (int f = 0; f < nf && ctx->nphase > 1; ++f) {
+    if (!ctx->used[f]) continue;
+    for (int j = 0; j < nb; ++j)
+      for (int k = 0; k < ctx->bls[j].nfreqs; ++k)
+        if (ctx->bls[j].freq_index[k] == f) {
+          ctx->sdoff[(size_t)f * nb + j] = (int)idx;
+          idx += ctx->nphase / 2 + ctx->nphase % 2;
         }
   }
   ctx->acoff.assign(ctx->nds, 0);
@@ -568,10 +604,12 @@ int build_norm(fxg_ctx* ctx) {
       const fxg_baseline& b = ctx->bls[j];
       for (int k = 0; k < b.nfreqs; ++k) {
         if (b.freq_index[k] != f) continue;
-        for (int p = 0; p < b.npolprod[k]; ++p) {
+        for (int p = 0; p < b.npolprod[k]; ++p)
+        for (int pc = 0; pc < ctx->nphase; ++pc) {
           NormSeg g{};
           g.kind = 0;
-          g.off = ctx->boff[(size_t)f * nb + j] + p * fc;
+          g.off = ctx->boff[(size_t)f * nb + j] + (pc * b.npolprod[k] + p) * fc;
+          g.decf = ctx->nphase > 1 ? ctx->sdoff[(size_t)f * nb + j] * 2 + pc : -1;
           g.nout = fc;
           g.wf = ctx->bwoff[(size_t)f * nb + j] * 2 + p;
           g.ds1 = b.ds1; g.ds2 = b.ds2;
@@ -603,6 +641,7 @@ int build_norm(fxg_ctx* ctx) {
           const int fc = fr.nchan / fr.chanavg;
           NormSeg g{};
           g.kind = 1;
+          g.decf = -1;
           g.off = (int)idx;
           g.nout = fc;
           g.wf = widx;
@@ -967,6 +1006,9 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
           e.dst = ctx->boff[(size_t)f * nb + j] + p * nout;
           e.nout = nout;
           e.avg = fr.chanavg;
+          e.bl = j;
+          e.fq = f;
+          e.pcstride = b.npolprod[k] * nout;
           ctx->finp.push_back(e);
           ctx->maxoutp = std::max(ctx->maxoutp, nout);
         }
@@ -1000,6 +1042,9 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
             e.dst = (int)idx;
             e.nout = nout;
             e.avg = cavg;
+            e.bl = -1;
+            e.fq = 0;
+            e.pcstride = 0;
             ctx->finp.push_back(e);
             ctx->maxoutp = std::max(ctx->maxoutp, nout);
           }
@@ -1030,6 +1075,9 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
           e.nout = nout;
           e.avg = fr.chanavg;
           e.isauto = 0;
+          e.bl = j;
+          e.fq = f;
+          e.pcstride = b.npolprod[k] * nout;
           ctx->finm.push_back(e);
           ctx->maxoutm = std::max(ctx->maxoutm, nout);
         }
@@ -1062,6 +1110,9 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
             e.nout = nout;
             e.avg = cavg;
             e.isauto = (w == 0);
+            e.bl = -1;
+            e.fq = 0;
+            e.pcstride = 0;
             ctx->finm.push_back(e);
             ctx->maxoutm = std::max(ctx->maxoutm, nout);
           }
@@ -1154,6 +1205,168 @@ int log2i(int v) {
 
 // Work items of the persistent launch: every datastream's chunk in `parts` parts, longest FFT
 // first (the items of one length run back to back, so twiddles reload rarely).
+// Configuration::calcgoodxmacstridelength (configuration.cpp:2653-2698): the divisor of the gcd
+// of the baseline freqs' channel counts closest to 150 (the larger one on a tie).
+int good_xmac_stride(const fxg_ctx* ctx) {
+  long long g = 0;
+  for (const auto& b : ctx->bls)
+    for (int k = 0; k < b.nfreqs; ++k) g = std::gcd(g, (long long)ctx->freqs[b.freq_index[k]].nchan);
+  int best = 0;
+  double err = 1.0e12;
+  for (long long guess = g; guess > 0; --guess) {
+    if (g % guess) continue;
+    const double d = std::fabs((double)guess - 150.0);
+    if (d < err) { err = d; best = (int)guess; }
+  }
+  return best;
+}
+
+// Multi-phase-centre uv shift: stride lengths, per-freq rotation constants, the intervals of a
+// subint (core.cpp:761-776, 984-991, 1055-1057 per process thread, core.cpp:2158-2166) and the
+// places of the shift-decorrelation sums.
+int build_uvshift(fxg_ctx* ctx) {
+  if (ctx->nphase <= 1) return FXG_OK;
+  if (ctx->xms <= 0) ctx->xms = good_xmac_stride(ctx);
+  if (ctx->xms <= 0) return fail(ctx, FXG_E_INVALID, "no xmac stride length");
+  ctx->rs = calcstridelength(ctx->xms);
+  for (const auto& b : ctx->bls)
+    for (int k = 0; k < b.nfreqs; ++k)
+      if (ctx->freqs[b.freq_index[k]].nchan % ctx->xms)
+        return fail(ctx, FXG_E_INVALID, "xmac stride length must divide every baseline freq's nchan");
+  ctx->uvfreq.assign(ctx->nfreqs, UvFreq{});
+  for (int f = 0; f < ctx->nfreqs; ++f) {
+    const fxg_freq& fr = ctx->freqs[f];
+    UvFreq& q = ctx->uvfreq[f];
+    q.lofreq = fr.freq_mhz;
+    q.chbw = fr.bw_mhz / (double)fr.nchan;
+    q.stepbw = ctx->rs * q.chbw;
+    q.lsb = fr.lsb;
+    q.rs = ctx->rs;
+    q.nsteps = (fr.nchan / ctx->xms) * (ctx->xms / ctx->rs);
+  }
+  const int B = ctx->B;
+  const double blockns = (double)ctx->subint_ns / (double)B;
+  int maxxc = (int)((double)ctx->maxns_xc / blockns);
+  maxxc -= maxxc % ctx->nbuffered;
+  if (maxxc == 0) maxxc = ctx->nbuffered;
+  ctx->ivals.clear();
+  int start = 0, nblk = 0;
+  for (int t = 0; t < ctx->nthreads; ++t) {
+    start += nblk;
+    nblk = B / ctx->nthreads + (t < B % ctx->nthreads ? 1 : 0);
+    for (int k = 0, sh = 0; k < nblk; k += maxxc, ++sh) {
+      const int n = std::min(maxxc, nblk - k);
+      ctx->ivals.push_back({start + k, n, (start + (double)sh * maxxc + n / 2.0) * blockns, n * blockns});
+    }
+  }
+  const int nb = ctx->nbl;
+  ctx->uvdec_idx.clear();
+  for (int f = 0; f < ctx->nfreqs; ++f) {
+    if (!ctx->used[f]) continue;
+    for (int j = 0; j < nb; ++j)
+      for (int k = 0; k < ctx->bls[j].nfreqs; ++k)
+        if (ctx->bls[j].freq_index[k] == f)
+          for (int p = 0; p < ctx->nphase; ++p) ctx->uvdec_idx.push_back(ctx->sdoff[(size_t)f * nb + j] * 2 + p);
+  }
+  int rc;
+  if ((rc = dev_upload(ctx, &ctx->d_uvfreq, ctx->uvfreq))) return rc;
+  if ((rc = dev_upload(ctx, &ctx->d_uvdec_idx, ctx->uvdec_idx))) return rc;
+  const long long ndd = (long long)ctx->ivals.size() * nb * ctx->nphase;
+  for (auto& sg : ctx->stage) {
+    if ((rc = dev_malloc(ctx, (void**)&sg.d_uvdd, sizeof(double) * std::max<long long>(ndd, 1)))) return rc;
+    if ((rc = dev_malloc(ctx, (void**)&sg.d_uvdec, sizeof(float) * std::max<size_t>(ctx->uvdec_idx.size(), 1)))) return rc;
+  }
+  return FXG_OK;
+}
+
+// Model::calculateDelayInterpolator of order 1 (model.cpp:219-283) on the host, for a model
+// table [nsamples][FXG_MAX_POLY] whose polynomials start at the scan start: (slope per
+// increment, offset); false outside the model.
+bool host_linear_interp(const double* m, int order, int nsamples, double inc, double t, double span, int ninc,
+                        double out[2]) {
+  int ss = (int)(t / inc);
+  if (ss == nsamples && (t - ss * inc) < 1e-6) --ss;
+  if (ss < 0 || ss >= nsamples) return false;
+  const double* c = m + (size_t)ss * FXG_MAX_POLY;
+  auto ev = [&](double dt) {
+    double tp = 1.0, v = 0.0;
+    for (int i = 0; i <= order; ++i) { v += tp * c[i]; tp *= dt; }
+    return v;
+  };
+  const double d1 = ev(t + span / 2.0 - ss * inc);
+  const double d0 = ev(t - ss * inc);
+  const double d2 = ev(t + span - ss * inc);
+  out[0] = (d2 - d0) / ninc;
+  out[1] = d0 + (d1 - (out[0] * ninc / 2.0 + d0)) / 3.0;
+  return true;
+}
+
+// Per subint: the applied delay of every (interval, baseline, phase centre) and the shift
+// decorrelation sums (core.cpp:1636-1670, 1887-1922), into the staging buffers.
+void uvshift_tables(fxg_ctx* ctx, const int32_t offsets[3], double* dd, float* dec) {
+  const int P = ctx->nphase, nb = ctx->nbl, nds = ctx->nds;
+  const int ni = (int)ctx->ivals.size();
+  std::vector<double> pt((size_t)nds * 2), pc((size_t)nds * P * 2);
+  std::vector<char> ok(nds);
+  std::vector<double> ddr((size_t)ni * nb * P);        // rate differences
+  for (int iv = 0; iv < ni; ++iv) {
+    const double t = offsets[1] + (offsets[2] + ctx->ivals[iv].nsoff) / 1000000000.0;
+    for (int d = 0; d < nds; ++d) {
+      const fxg_datastream& s = ctx->dss[d];
+      ok[d] = host_linear_interp(ctx->models[d].data(), s.model_order, s.model_nsamples, ctx->model_inc, t,
+                                 0.000001, 1, &pt[2 * d]);
+      for (int p = 0; p < P && ok[d]; ++p)
+        ok[d] = host_linear_interp(ctx->pcmodels[d].data() + (size_t)p * s.model_nsamples * FXG_MAX_POLY,
+                                   s.model_order, s.model_nsamples, ctx->model_inc, t, 0.000001, 1,
+                                   &pc[((size_t)d * P + p) * 2]);
+    }
+    for (int j = 0; j < nb; ++j) {
+      const int a = ctx->bls[j].ds1, b = ctx->bls[j].ds2;
+      for (int p = 0; p < P; ++p) {
+        double v = 0.0, r = 0.0;                      // outside the model: no shift
+        if (ok[a] && ok[b]) {
+          const double* p1 = &pc[((size_t)a * P + p) * 2];
+          const double* p2 = &pc[((size_t)b * P + p) * 2];
+          double a1 = p1[1] - pt[2 * a + 1], a2 = p2[1] - pt[2 * b + 1];
+          a1 += a1 * pt[2 * a];
+          a2 += a2 * pt[2 * b];
+          v = a2 - a1;
+          r = p2[0] + pt[2 * a] - (p1[0] + pt[2 * b]);
+        }
+        dd[((size_t)iv * nb + j) * P + p] = v;
+        ddr[((size_t)iv * nb + j) * P + p] = r;
+      }
+    }
+  }
+  size_t k = 0;
+  for (int f = 0; f < ctx->nfreqs; ++f) {
+    if (!ctx->used[f]) continue;
+    const fxg_freq& fr = ctx->freqs[f];
+    const double window = fr.nchan / fr.bw_mhz;     // delaywindow (core.cpp:1621)
+    for (int j = 0; j < nb; ++j)
+      for (int kk = 0; kk < ctx->bls[j].nfreqs; ++kk)
+        if (ctx->bls[j].freq_index[kk] == f)
+          for (int p = 0; p < P; ++p) {
+            float acc = 0.0f;
+            for (int iv = 0; iv < ni; ++iv) {
+              const double r = ddr[((size_t)iv * nb + j) * P + p], v = dd[((size_t)iv * nb + j) * P + p];
+              double ts = 1.0, dl = 1.0;
+              if (std::fabs(r) > 1e-18) {
+                const double m = TWO_PI * r * (ctx->ivals[iv].nswidth / 1000.0) * fr.freq_mhz;
+                ts = std::sin(m / 2.0) / (m / 2.0);
+                if (ts < 0.0) ts = 0.0;
+              }
+              if (std::fabs(v) > 1e-18) {
+                dl = 1.0 - std::fabs(v / window);
+                if (dl < 0.0) dl = 0.0;
+              }
+              acc += ctx->ivals[iv].nswidth * ts * dl;
+            }
+            dec[k++] = acc;
+          }
+  }
+}
+
 int build_items(fxg_ctx* ctx) {
   int nds_kind[2] = {0, 0};
   for (const auto& fc : ctx->fclasses) nds_kind[fc.cplx] += (int)fc.dsl.size();
@@ -1305,6 +1518,13 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   }
   HIPCHK(hipMemcpyAsync(ctx->d_run, sg.h_run, sizeof(DevDsRun) * nds, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(d_flags, sg.h_flags, sizeof(int32_t) * nds * ctx->flaglen, hipMemcpyHostToDevice, st));
+  const bool uvs = ctx->nphase > 1;
+  if (uvs) {                                       // uv-shift delays and decorrelation of this subint
+    uvshift_tables(ctx, offsets, sg.h_uvdd, sg.h_uvdec);
+    HIPCHK(hipMemcpyAsync(sg.d_uvdd, sg.h_uvdd, sizeof(double) * ctx->ivals.size() * ctx->nbl * ctx->nphase,
+                          hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(sg.d_uvdec, sg.h_uvdec, sizeof(float) * ctx->uvdec_idx.size(), hipMemcpyHostToDevice, st));
+  }
   HIPCHK(hipEventRecord(sg.used, st));
   sg.pending = true;
 
@@ -1366,8 +1586,36 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   const int npg = (int)ctx->pcal_groups.size();
   if (npg && fxg_launch_zero(ctx->d_pcal_part, sizeof(float) * ctx->pcal_part_len, st))
     return fail(ctx, FXG_E_HIP, "phase-cal zero launch");
-  for (int c0 = 0; c0 < ctx->B; c0 += ctx->chunkcap) {
-    const int n = std::min(ctx->chunkcap, ctx->B - c0);
+  // The subint in chunks of at most chunkcap FFTs; with several phase centres each uv-shift
+  // interval separately, its products shifted into every phase centre's block at its end
+  // (core.cpp:984-991, 1055-1057) and the accumulators cleared for the next interval.
+  std::vector<std::array<int, 3>> pieces;          // {first FFT, FFTs, interval ending here or -1}
+  if (!uvs) {
+    for (int c0 = 0; c0 < ctx->B; c0 += ctx->chunkcap) pieces.push_back({c0, std::min(ctx->chunkcap, ctx->B - c0), -1});
+  } else {
+    for (size_t iv = 0; iv < ctx->ivals.size(); ++iv) {
+      const int e = ctx->ivals[iv].i0 + ctx->ivals[iv].n;
+      for (int c0 = ctx->ivals[iv].i0; c0 < e; c0 += ctx->chunkcap) {
+        const int n = std::min(ctx->chunkcap, e - c0);
+        pieces.push_back({c0, n, c0 + n == e ? (int)iv : -1});
+      }
+    }
+  }
+  UvArgs uv{nullptr, ctx->d_uvfreq, uvs ? ctx->nphase : 1};
+  auto accumulate = [&](const UvArgs& u) -> int {
+    TimedLaunch tl(ctx, 2, st);
+    if (mfma) {
+      if (fxg_launch_fin_mfma(ctx->d_finm, (int)ctx->finm.size(), ctx->maxoutm, ctx->d_groups, ctx->d_vacc,
+                              ctx->d_xscale, d_results, u, st))
+        return fail(ctx, FXG_E_HIP, "accumulate launch");
+    } else {
+      if (fxg_launch_fin_plain(ctx->d_finp, (int)ctx->finp.size(), ctx->maxoutp, ctx->d_xacc, d_results, u, st))
+        return fail(ctx, FXG_E_HIP, "accumulate launch");
+    }
+    return FXG_OK;
+  };
+  for (const auto& piece : pieces) {
+    const int c0 = piece[0], n = piece[1];
     const int npad = (int)round_up(n, FXG_XPAD);
     fa.chunk0 = c0;
     fa.nffts = n;
@@ -1434,23 +1682,25 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     }
     ctx->last_chunk0 = c0;
     ctx->last_nffts = n;
+    if (piece[2] >= 0) {                           // end of a uv-shift interval
+      uv.dd = sg.d_uvdd + (size_t)piece[2] * ctx->nbl * ctx->nphase;
+      if (int rc = accumulate(uv)) return rc;
+      if (mfma ? fxg_launch_zero(ctx->d_vacc, ctx->vacc_len * 4, st) : fxg_launch_zero(ctx->d_xacc, ctx->xacc_len * 8, st))
+        return fail(ctx, FXG_E_HIP, "zero acc");
+    }
   }
   if (npg) {
     TimedLaunch tp(ctx, 2, st);
     if (fxg_launch_pcal_reduce(pa, npg, ctx->pcal_max_gsize, st))
       return fail(ctx, FXG_E_HIP, "phase-cal reduce launch");
   }
+  if (!uvs)
+    if (int rc = accumulate(uv)) return rc;
   TimedLaunch tl(ctx, 2, st);
-  if (mfma) {
-    if (fxg_launch_fin_mfma(ctx->d_finm, (int)ctx->finm.size(), ctx->maxoutm, ctx->d_groups, ctx->d_vacc,
-                            ctx->d_xscale, d_results, st))
-      return fail(ctx, FXG_E_HIP, "accumulate launch");
-  } else {
-    if (fxg_launch_fin_plain(ctx->d_finp, (int)ctx->finp.size(), ctx->maxoutp, ctx->d_xacc, d_results, st))
-      return fail(ctx, FXG_E_HIP, "accumulate launch");
-  }
   if (fxg_launch_fin_weights(ctx->d_finw, (int)ctx->finw.size(), ctx->d_weights, ctx->B, (float*)d_results, st))
     return fail(ctx, FXG_E_HIP, "weights launch");
+  if (uvs && fxg_launch_add_at(ctx->d_uvdec_idx, sg.d_uvdec, (int)ctx->uvdec_idx.size(), (float*)d_results, st))
+    return fail(ctx, FXG_E_HIP, "shift decorrelation launch");
   return FXG_OK;
 }
 
@@ -1544,9 +1794,13 @@ void free_ctx(fxg_ctx* ctx) {
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   f(ctx->d_normseg); f(ctx->d_normcal); f(ctx->d_calmean); f(ctx->d_tsys); f(ctx->d_dpct);
   f(ctx->d_kurt_tab); f(ctx->d_kurt_acc); f(ctx->d_pcal_groups); f(ctx->d_pcal_part); f(ctx->d_pcal_bins);
+  f(ctx->d_uvfreq); f(ctx->d_uvdec_idx);
   for (auto& sg : ctx->stage) {
+    f(sg.d_uvdd); f(sg.d_uvdec);
     if (sg.h_run) (void)hipHostFree(sg.h_run);
     if (sg.h_flags) (void)hipHostFree(sg.h_flags);
+    if (sg.h_uvdd) (void)hipHostFree(sg.h_uvdd);
+    if (sg.h_uvdec) (void)hipHostFree(sg.h_uvdec);
     if (sg.used) (void)hipEventDestroy(sg.used);
   }
   for (auto& s : ctx->slots) {
@@ -1633,6 +1887,19 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
       ctx->models[d].assign(s.model_delay_us, s.model_delay_us + (size_t)s.model_nsamples * FXG_MAX_POLY);
     ctx->dss[d].model_delay_us = ctx->models[d].empty() ? nullptr : ctx->models[d].data();
   }
+  ctx->xms = cfg->xmac_stride_len;
+  ctx->nphase = cfg->nphasecentres > 1 ? cfg->nphasecentres : 1;
+  ctx->nthreads = cfg->process_threads > 0 ? cfg->process_threads : 1;
+  ctx->maxns_xc = cfg->max_ns_between_xc_avg;
+  ctx->pcmodels.resize(ctx->nds);
+  for (int d = 0; d < ctx->nds; ++d) {
+    const fxg_datastream& s = ctx->dss[d];
+    if (ctx->nphase > 1 && cfg->nphasecentres <= FXG_MAX_PHASECENTRES && s.phasecentre_delay_us &&
+        s.model_nsamples > 0)
+      ctx->pcmodels[d].assign(s.phasecentre_delay_us,
+                              s.phasecentre_delay_us + (size_t)ctx->nphase * s.model_nsamples * FXG_MAX_POLY);
+    ctx->dss[d].phasecentre_delay_us = ctx->pcmodels[d].empty() ? nullptr : ctx->pcmodels[d].data();
+  }
   ctx->subint_ns = cfg->subint_ns;
   ctx->guard_ns = cfg->guard_ns;
   ctx->rotorder = cfg->fringe_rot_order;
@@ -1656,10 +1923,15 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
   if (rc == FXG_OK) rc = build_tables(ctx);
   if (rc == FXG_OK) rc = build_engines(ctx, cfg->chunk_ffts);
   if (rc == FXG_OK) rc = build_items(ctx);
+  if (rc == FXG_OK) rc = build_uvshift(ctx);
   if (rc == FXG_OK && !dry) {
+    const size_t ndd = std::max<size_t>(ctx->ivals.size() * ctx->nbl * ctx->nphase, 1);
+    const size_t ndec = std::max<size_t>(ctx->uvdec_idx.size(), 1);
     for (auto& sg : ctx->stage) {
       if (hipHostMalloc((void**)&sg.h_run, sizeof(DevDsRun) * ctx->nds, 0) != hipSuccess ||
           hipHostMalloc((void**)&sg.h_flags, sizeof(int32_t) * ctx->nds * ctx->flaglen + 16, 0) != hipSuccess ||
+          hipHostMalloc((void**)&sg.h_uvdd, sizeof(double) * ndd, 0) != hipSuccess ||
+          hipHostMalloc((void**)&sg.h_uvdec, sizeof(float) * ndec, 0) != hipSuccess ||
           hipEventCreateWithFlags(&sg.used, hipEventDisableTiming) != hipSuccess) {
         rc = fail(ctx, FXG_E_HIP, "staging allocation");
         break;
@@ -1864,6 +2136,7 @@ int fxg_normalise_device(fxg_ctx* ctx, float* d_results, double meansubints, con
   a.dpct = ctx->d_dpct;
   a.msi = meansubints;
   a.fftsperint = meansubints * ctx->B;
+  a.inttime_s = meansubints * (double)ctx->subint_ns / 1e9;
   const int ncal = anyt ? (int)ctx->normcal.size() - 1 : 0;   // (not the final zero entry)
   if (fxg_launch_normalise(a, (int)ctx->normseg.size(), ncal, st))
     return fail(ctx, FXG_E_HIP, "normalisation launch failed");
@@ -1905,6 +2178,9 @@ int fxg_difx_records(const fxg_ctx* ctx, const float* results, const fxg_difx_me
   const int nb = ctx->nbl;
   if (nb > 0 && (!m->baselinenumber || !m->uvw)) return FXG_E_INVALID;
   if (ctx->crossautos && !m->telescopeindex) return FXG_E_INVALID;
+  const int P = ctx->nphase, pc = ctx->nphase > 1 ? m->phasecentre : 0;
+  if (pc < 0 || pc >= P) return FXG_E_INVALID;
+  const double inttime_s = m->fftsperintegration / ctx->B * (double)ctx->subint_ns / 1e9;
   int64_t pos = 0;
   int32_t nrec = 0;
   auto emit = [&](int32_t bl, int32_t freq, char p0, char p1, double w, const double* uvw, const float* data, int fc) {
@@ -1931,15 +2207,18 @@ int fxg_difx_records(const fxg_ctx* ctx, const float* results, const fxg_difx_me
       for (int p = 0; p < b.npolprod[k]; ++p) {
         const float w = (float)(results[ctx->bwoff[(size_t)f * nb + i] * 2 + p] / m->fftsperintegration);
         if (!(w > 0.0f)) continue;
+        float cw = w;                                // visibility.cpp:869-873
+        if (P > 1)
+          cw = w * (float)(results[ctx->sdoff[(size_t)f * nb + i] * 2 + pc] / (1000000000.0 * inttime_s));
         if (!emit(m->baselinenumber[i], f, band_pol(ctx->dss[b.ds1], b.ds1band[k][p]),
-                  band_pol(ctx->dss[b.ds2], b.ds2band[k][p]), (double)w, m->uvw + 3 * i,
-                  results + 2LL * (core + (long long)p * fc), fc))
+                  band_pol(ctx->dss[b.ds2], b.ds2band[k][p]), (double)cw, m->uvw + 3 * i,
+                  results + 2LL * (core + (long long)(pc * b.npolprod[k] + p) * fc), fc))
           return FXG_E_INVALID;
       }
     }
   }
-  // autocorrelations (visibility.cpp:889-931)
-  if (ctx->crossautos) {
+  // autocorrelations (visibility.cpp:889-931), in the first phase centre's file
+  if (ctx->crossautos && pc == 0) {
     const double zero[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < ctx->nds; ++d) {
       const fxg_datastream& s = ctx->dss[d];

@@ -206,12 +206,35 @@ struct XMfmaArgs {
   int mode;               // 1 = FP16, 2 = BF16
 };
 
+// Multi-phase-centre uv shift (core.cpp:1699-1808): the channel layout of a freq's rotator,
+// sub phasors over rs channels and step phasors over strides of rs channels, with channel
+// sky frequencies edge + c chbw (USB) or edge - (nchan - 1 - c) chbw (LSB).
+struct UvFreq {
+  double lofreq;          // band-edge sky frequency, MHz
+  double chbw;            // channel width, MHz
+  double stepbw;          // rs x chbw
+  int lsb;
+  int rs;                 // rotate stride length
+  int nsteps;             // step phasors: (nchan / xmacstridelen) x (xmacstridelen / rs)
+  int pad;
+};
+
+// One uv-shift interval: the applied delays of its baselines' phase centres.
+struct UvArgs {
+  const double* dd;       // [baseline][phase centre] differential delay (us); P <= 1: unused
+  const UvFreq* fq;
+  int P;
+};
+
 // Accumulation of a product accumulator into the results (core.cpp:1819-1853).
 struct FinPlain {
   long long src;          // cf32 offset in the product accumulators
-  int dst;                // cf32 offset in results
+  int dst;                // cf32 offset in results (phase centre 0)
   int nout;               // output channels
   int avg;                // channels averaged per output channel
+  int bl;                 // baseline of a cross product (-1: autocorrelation, not shifted)
+  int fq;                 // freq of the product (uv-shift rotation constants)
+  int pcstride;           // cf32 offset between the phase-centre blocks
 };
 
 struct FinMfma {
@@ -222,6 +245,7 @@ struct FinMfma {
   int nout;
   int avg;
   int isauto;             // parallel-hand autocorrelation: imaginary part forced to zero
+  int bl, fq, pcstride;   // as FinPlain
 };
 
 struct FinWeight {
@@ -241,9 +265,11 @@ int fxg_launch_fbig(const FengineArgs& a, int L, int iscomplex, int nds, int wgx
 size_t fxg_fbig_lds(int ntb, int phcap);
 int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st);
 int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st);
-int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, hipStream_t st);
+int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, const UvArgs& uv,
+                         hipStream_t st);
 int fxg_launch_fin_mfma(const FinMfma* e, int n, int maxout, const DevXGroup* g, const float* acc,
-                        const float* xscale, float2* res, hipStream_t st);
+                        const float* xscale, float2* res, const UvArgs& uv, hipStream_t st);
+int fxg_launch_add_at(const int* idx, const float* val, int n, float* resf, hipStream_t st);
 int fxg_launch_fin_weights(const FinWeight* e, int n, const float* w, int B, float* resf, hipStream_t st);
 int fxg_launch_zero(void* p, long long bytes, hipStream_t st);
 int fxg_launch_fill(float* p, long long n, float v, hipStream_t st);
@@ -264,7 +290,8 @@ struct NormSeg {
   int aw1, aw2;       // Tsys, cross: float indices of the two bands' parallel autocorrelation weights
   float zs1, zs2;     // zoom factors (1 for recorded bands)
   float chanfactor;   // (float)(blockspersend * 2 * freqchannels * chanavg)
-  int pad;
+  int decf;           // several phase centres: float index of this segment's shift decorrelation
+                      // sum, the scale divided by it over the integration time; -1 otherwise
 };
 // one band mean of a parallel autocorrelation (the Tsys calibration's autocorrcalibs)
 struct NormCal {
@@ -278,6 +305,7 @@ struct NormArgs {
   const double* tsys;         // [nds]
   const float* dpct;          // [nds] decorrelation fraction of each datastream's bits
   double fftsperint, msi;
+  double inttime_s;           // integration time: meansubints subints
 };
 int fxg_launch_normalise(const NormArgs& a, int nseg, int ncal, hipStream_t st);
 

@@ -54,7 +54,14 @@ __device__ float seg_scale(const NormArgs& a, const NormSeg& g) {
 __global__ void __launch_bounds__(256) k_norm_apply(NormArgs a) {
   const NormSeg g = a.seg[blockIdx.x];
   __shared__ float s_scale;
-  if (threadIdx.x == 0) s_scale = seg_scale(a, g);
+  if (threadIdx.x == 0) {
+    float sc = seg_scale(a, g);
+    if (g.decf >= 0) {          // uv-shifted phase centre: divided by its shift decorrelation
+      const float dec = (float)((double)a.res[g.decf] / (1000000000.0 * a.inttime_s));
+      sc = sc / dec;
+    }
+    s_scale = sc;
+  }
   __syncthreads();
   const float sc = s_scale;
   if (!(sc > 0.0f)) return;                                      // only a positive scale is applied

@@ -225,10 +225,51 @@ __device__ __forceinline__ int mfma_idx(int i, int j) {
 
 // ---- accumulation into the slot results (core.cpp:1819-1853, 1258-1376) ----------------
 
-__global__ void k_fin_plain(const FinPlain* e, const float2* acc, float2* res) {
+// The uv-shift rotation of channel ch of a freq to a phase centre at applied delay dd (us),
+// in the reference's steps (core.cpp:1699-1766, 1789-1808): FP64 turns of the sub phasor
+// (channel within its rotate stride, plus the band edge) and of the step phasor (the stride),
+// wrapped, as FP32 arguments of FP32 sin/cos; the value times the sub phasor, then the step.
+constexpr double UV_TWO_PI = 6.283185307179586476925286766559;   // architecture.h.in:430
+
+__device__ __forceinline__ float2 uv_rotate(float2 v, double dd, const UvFreq& q, int ch) {
+  const int c = ch % q.rs, j = ch / q.rs;
+  double edge = dd * q.lofreq;
+  edge -= floor(edge);
+  const double subf = q.lsb ? -(double)(q.rs - (c + 1)) * q.chbw : (double)c * q.chbw;
+  const double stepf = q.lsb ? -(double)(q.nsteps - (j + 1)) * q.stepbw : (double)j * q.stepbw;
+  const double t1 = dd * subf + edge;
+  const double t2 = dd * stepf;
+  const float a1 = (float)((t1 - floor(t1)) * UV_TWO_PI);
+  const float a2 = (float)((t2 - floor(t2)) * UV_TWO_PI);
+  float s1, c1, s2, c2;
+  sincosf(a1, &s1, &c1);
+  sincosf(a2, &s2, &c2);
+  const float2 r = make_float2(c1 * v.x - s1 * v.y, c1 * v.y + s1 * v.x);
+  return make_float2(r.x * c2 - r.y * s2, r.x * s2 + r.y * c2);
+}
+
+__global__ void k_fin_plain(const FinPlain* e, const float2* acc, float2* res, UvArgs uv) {
   const FinPlain f = e[blockIdx.y];
   const int co = blockIdx.x * blockDim.x + threadIdx.x;
   if (co >= f.nout) return;
+  if (uv.P > 1 && f.bl >= 0) {                    // rotated into every phase centre's block
+    const UvFreq q = uv.fq[f.fq];
+    for (int p = 0; p < uv.P; ++p) {
+      const double dd = uv.dd[f.bl * uv.P + p];
+      const bool turn = fabs(dd) > 1.0e-20;
+      float2 s = make_float2(0.f, 0.f);
+      for (int k = 0; k < f.avg; ++k) {
+        const int ch = co * f.avg + k;
+        float2 v = acc[f.src + ch];
+        if (turn) v = uv_rotate(v, dd, q, ch);
+        s.x += v.x; s.y += v.y;
+      }
+      if (f.avg > 1) { s.x /= (float)f.avg; s.y /= (float)f.avg; }
+      float2* o = res + f.dst + (long long)p * f.pcstride + co;
+      o->x += s.x; o->y += s.y;
+    }
+    return;
+  }
   float2 s = make_float2(0.f, 0.f);
   for (int k = 0; k < f.avg; ++k) {
     const float2 v = acc[f.src + (long long)co * f.avg + k];
@@ -242,7 +283,7 @@ __global__ void k_fin_plain(const FinPlain* e, const float2* acc, float2* res) {
 
 
 __global__ void k_fin_mfma(const FinMfma* e, const DevXGroup* groups, const float* acc,
-                           const float* xscale, float2* res) {
+                           const float* xscale, float2* res, UvArgs uv) {
   const FinMfma f = e[blockIdx.y];
   const int co = blockIdx.x * blockDim.x + threadIdx.x;
   if (co >= f.nout) return;
@@ -254,6 +295,26 @@ __global__ void k_fin_mfma(const FinMfma* e, const DevXGroup* groups, const floa
   if (ti < tj) { int tmp = r1; r1 = r2; r2 = tmp; ti = r1 >> 5; tj = r2 >> 5; sign = -1.0f; }
   const int pair = ti * (ti + 1) / 2 + tj;
   const int idx = mfma_idx(r1 & 31, r2 & 31);
+  if (uv.P > 1 && f.bl >= 0) {                    // rotated into every phase centre's block
+    const UvFreq q = uv.fq[f.fq];
+    const float inv = 1.0f / (xscale[g.scale_off + f.row1] * xscale[g.scale_off + f.row2]);
+    for (int p = 0; p < uv.P; ++p) {
+      const double dd = uv.dd[f.bl * uv.P + p];
+      const bool turn = fabs(dd) > 1.0e-20;
+      float2 s = make_float2(0.f, 0.f);
+      for (int k = 0; k < f.avg; ++k) {
+        const int ch = co * f.avg + k;
+        const float* base = acc + g.acc_off + ((long long)(f.chan0 + ch) * npair + pair) * 2 * 1024;
+        float2 v = make_float2(base[idx] * inv, sign * base[1024 + idx] * inv);
+        if (turn) v = uv_rotate(v, dd, q, ch);
+        s.x += v.x; s.y += v.y;
+      }
+      if (f.avg > 1) { s.x /= (float)f.avg; s.y /= (float)f.avg; }
+      float2* o = res + f.dst + (long long)p * f.pcstride + co;
+      o->x += s.x; o->y += s.y;
+    }
+    return;
+  }
   float2 s = make_float2(0.f, 0.f);
   for (int k = 0; k < f.avg; ++k) {
     const int c = f.chan0 + co * f.avg + k;
@@ -301,6 +362,12 @@ __global__ void k_add(float* dst, const float* src, long long n) {
     dst[i] += src[i];
 }
 
+// The shift decorrelation sums of a subint into their result words (core.cpp:1089-1104).
+__global__ void k_add_at(const int* idx, const float* val, int n, float* resf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) resf[idx[i]] += val[i];
+}
+
 __global__ void k_fill(float* p, long long n, float v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
@@ -330,16 +397,23 @@ int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntil
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, hipStream_t st) {
+int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, const UvArgs& uv,
+                         hipStream_t st) {
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_fin_plain, dim3((maxout + 255) / 256, n), dim3(256), 0, st, e, acc, res);
+  hipLaunchKernelGGL(k_fin_plain, dim3((maxout + 255) / 256, n), dim3(256), 0, st, e, acc, res, uv);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_add_at(const int* idx, const float* val, int n, float* resf, hipStream_t st) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_add_at, dim3((n + 255) / 256), dim3(256), 0, st, idx, val, n, resf);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int fxg_launch_fin_mfma(const FinMfma* e, int n, int maxout, const DevXGroup* g, const float* acc,
-                        const float* xscale, float2* res, hipStream_t st) {
+                        const float* xscale, float2* res, const UvArgs& uv, hipStream_t st) {
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_fin_mfma, dim3((maxout + 255) / 256, n), dim3(256), 0, st, e, g, acc, xscale, res);
+  hipLaunchKernelGGL(k_fin_mfma, dim3((maxout + 255) / 256, n), dim3(256), 0, st, e, g, acc, xscale, res, uv);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

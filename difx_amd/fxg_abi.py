@@ -200,7 +200,7 @@ FXG_ADVANCE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32)
 
 class fxg_difx_meta(C.Structure):
     _fields_ = [
-        ("mjd", C.c_int32), ("configindex", C.c_int32), ("sourceindex", C.c_int32), ("reserved", C.c_int32),
+        ("mjd", C.c_int32), ("configindex", C.c_int32), ("sourceindex", C.c_int32), ("phasecentre", C.c_int32),
         ("seconds", C.c_double), ("fftsperintegration", C.c_double),
         ("baselinenumber", C.POINTER(C.c_int32)), ("uvw", C.POINTER(C.c_double)),
         ("telescopeindex", C.POINTER(C.c_int32)),

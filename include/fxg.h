@@ -420,8 +420,11 @@ int fxg_normalise_device(fxg_ctx* ctx, float* d_results, double meansubints, con
 typedef struct fxg_difx_meta {
   int32_t mjd;                     /* integration centre: MJD */
   int32_t configindex;
-  int32_t sourceindex;             /* phase centre source index */
-  int32_t reserved;
+  int32_t sourceindex;             /* source index of the records (autocorrelations: the
+                                      pointing centre's when there are several phase centres) */
+  int32_t phasecentre;             /* several phase centres: the one whose file this is (0-based;
+                                      writeSWIN writes one file per phase centre, visibility.cpp:
+                                      848-885, the autocorrelations into the first); else 0 */
   double  seconds;                 /* integration centre: seconds of the day */
   double  fftsperintegration;      /* meansubints * blockspersend (record weights = weight / it) */
   const int32_t* baselinenumber;   /* [nbaselines] Configuration::getBNumber */
@@ -430,7 +433,9 @@ typedef struct fxg_difx_meta {
 } fxg_difx_meta;
 /* The DIFX records of a normalised results array (host memory) into out[capacity]: cross-
  * correlations by baseline, freq, product; then autocorrelations when the configuration
- * writes them; a product of weight 0 is skipped.  out = null: size only. */
+ * writes them; a product of weight 0 is skipped.  With several phase centres the records of
+ * meta->phasecentre's file, cross weights times that centre's shift decorrelation over the
+ * integration (fftsperintegration / blockspersend subints).  out = null: size only. */
 int fxg_difx_records(const fxg_ctx* ctx, const float* results, const fxg_difx_meta* meta, uint8_t* out,
                      int64_t capacity, int64_t* nbytes, int32_t* nrecords);
 

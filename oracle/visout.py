@@ -139,7 +139,16 @@ def normalise(cfg, res: np.ndarray, lay, meansubints: float, tsys: Optional[Sequ
                             sc = F32(sc * np.sqrt(zoom_ratio(cfg, ds2, b2)))
                     else:
                         sc = F32(0)
-                    apply(lay.baseline_off[(f, j)] + p * fc, fc, sc)
+                    P = lay.nphasecentres
+                    for pc in range(P):
+                        scp = sc
+                        if P > 1:
+                            # the shift decorrelation over the integration time (meansubints
+                            # subints), visibility.cpp:427-440, 555-557
+                            dec = F32(F64(src[lay.shiftdecorr_off[(f, j)] * 2 + pc])
+                                      / (1000000000.0 * (msi * F64(cfg.subint_ns) / 1e9)))
+                            scp = F32(sc / dec) if dec != 0 else F32(np.inf if sc > 0 else 0.0)
+                        apply(lay.baseline_off[(f, j)] + (pc * len(prods) + p) * fc, fc, scp)
     if cfg.write_cross_autocorrs:                            # autocorrelations
         for d in range(nds):
             ds = cfg.datastreams[d]
@@ -176,11 +185,15 @@ def header(bl, mjd, sec, cfgi, src, freq, pol, weight, uvw, pulsarbin=0) -> byte
 
 def records(cfg, res: np.ndarray, lay, mjd: int, seconds: float, configindex: int, sourceindex: int,
             fftsperintegration: float, baselinenumber: Sequence[int], uvw: np.ndarray,
-            telescopeindex: Sequence[int]) -> bytes:
-    """writeSWIN's records of a (scaled) results array."""
+            telescopeindex: Sequence[int], phasecentre: int = 0) -> bytes:
+    """writeSWIN's records of a (scaled) results array; with several phase centres those of
+    one phase centre's file (visibility.cpp:848-885), the autocorrelations in the first."""
     src = np.asarray(res, dtype=np.complex64)
     f32 = src.view(np.float32)
     used, equiv = fo.freq_used(cfg)
+    P = lay.nphasecentres
+    pc = phasecentre if P > 1 else 0
+    inttime_s = fftsperintegration / cfg.blockspersend() * F64(cfg.subint_ns) / 1e9
     out = []
     for i, b in enumerate(cfg.baselines):
         for f in range(len(cfg.freqs)):
@@ -194,11 +207,14 @@ def records(cfg, res: np.ndarray, lay, mjd: int, seconds: float, configindex: in
                 w = F32(F64(f32[lay.bweight_off[(f, i)] * 2 + p]) / fftsperintegration)
                 if not w > 0:
                     continue
+                if P > 1:                                   # visibility.cpp:869-873
+                    w = F32(w * F32(F64(f32[lay.shiftdecorr_off[(f, i)] * 2 + pc]) / (1000000000.0 * inttime_s)))
                 pol = band_pol(cfg.datastreams[b.ds1], b1) + band_pol(cfg.datastreams[b.ds2], b2)
                 out.append(header(baselinenumber[i], mjd, seconds, configindex, sourceindex, f, pol, float(w),
                                   [float(x) for x in uvw[i]]))
-                out.append(src[core + p * fc: core + (p + 1) * fc].tobytes())
-    if cfg.write_cross_autocorrs:
+                o = core + (pc * len(prods) + p) * fc
+                out.append(src[o: o + fc].tobytes())
+    if cfg.write_cross_autocorrs and pc == 0:
         for d, ds in enumerate(cfg.datastreams):
             off, widx = lay.autocorr_off[d], lay.acweight_off[d] * 2
             for j in range(lay.bandsperautocorr):

@@ -124,6 +124,38 @@ def case_real8(B: int = 32, xengine: str = "fp32", **kw):
                               xengine=xengine, seed=5, **kw)
 
 
+def with_phase_centres(cfg, P: int = 3, seed: int = 40, interval_ffts: int = 8, threads: int = 3,
+                       nbuf: int = 4):
+    """P phase centres around the pointing centre: each station's phase-centre delays are its
+    pointing delays plus a seeded offset of up to +-50 ns and a rate difference of up to
+    +-2e-5 us/s (datastreams with identical pointing models belong to one station); uv-shift
+    intervals of interval_ffts FFTs, rounded down to a multiple of nbuf buffered FFTs, over
+    `threads` Core process threads."""
+    rng = np.random.default_rng(seed)
+    stations = []
+    for ds in cfg.datastreams:
+        k = next((i for i, m in enumerate(stations) if np.array_equal(m, ds.model)), None)
+        if k is None:
+            stations.append(ds.model)
+    offs = {i: [(rng.uniform(-0.05, 0.05), rng.uniform(-2e-5, 2e-5)) for _ in range(P)]
+            for i in range(len(stations))}
+    for ds in cfg.datastreams:
+        k = next(i for i, m in enumerate(stations) if np.array_equal(m, ds.model))
+        ms = []
+        for d0, r0 in offs[k]:
+            m = ds.model.copy()
+            m[:, 0] += d0
+            m[:, 1] += r0
+            ms.append(m)
+        ds.phasecentre_models = ms
+    cfg.nphasecentres = P
+    cfg.process_threads = threads
+    cfg.num_buffered_ffts = nbuf
+    B = cfg.blockspersend()
+    cfg.max_ns_between_xc_avg = int(interval_ffts * cfg.subint_ns / B) + 1
+    return cfg
+
+
 def slot(cfg, offsets=(0, 1, 0), seed: int = 11):
     return synth.noise_slot(cfg, list(offsets), seed=seed)
 
@@ -157,11 +189,12 @@ def record_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
     and every datastream's autocorrelation block."""
     out = {"cross": [], "auto": []}
     nb = len(cfg.baselines)
+    P = getattr(lay, "nphasecentres", 1)
     for (fi, j), off in lay.baseline_off.items():
         fr = cfg.freqs[fi]
         prods = dict(cfg.baselines[j].freqs)[fi]
         nout = (fr.nchan + fr.chanavg - 1) // fr.chanavg
-        for p in range(len(prods)):
+        for p in range(len(prods) * P):             # phase-centre blocks follow each other
             g = got[off + p * nout: off + (p + 1) * nout]
             r = ref[off + p * nout: off + (p + 1) * nout]
             den = np.mean(np.abs(r))

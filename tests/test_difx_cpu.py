@@ -236,3 +236,65 @@ def test_sta_autocorr_records(dumpchannels):
         assert n.value == 0
     finally:
         lib.fxg_destroy(h)
+
+
+def test_phase_centre_layout_records_and_scale():
+    """Several phase centres: the library's layout equals the restatement's; each phase
+    centre's records (its own file) equal the restatement's byte for byte, with the weights
+    times the shift decorrelation and the autocorrelations only in the first file; the scale of
+    a phase centre's block is divided by its decorrelation over the integration."""
+    from fxcases import with_phase_centres
+    lib = _lib()
+    cfg = with_phase_centres(case_cfg3(B=32), P=3, interval_ffts=8, threads=2)
+    lay = fo.result_layout(cfg)
+    rng = np.random.default_rng(7)
+    res = _results(cfg, lay, rng)
+    f32 = res.view(np.float32)
+    msi = 2.0
+    inttime_ns = msi * cfg.subint_ns
+    for (fi, j), so in lay.shiftdecorr_off.items():
+        f32[2 * so: 2 * so + 3] = (rng.uniform(0.5, 1.0, 3) * inttime_ns).astype(np.float32)
+    ffts = msi * cfg.blockspersend()
+    h, keep = _layout_ctx(lib, cfg)
+    try:
+        from difx_amd import fxg_abi as abi
+        lo = abi.fxg_layout()
+        assert lib.fxg_get_layout(h, C.byref(lo)) == 0
+        assert lo.coreresultlength == lay.coreresultlength
+        fres = np.ascontiguousarray(res.view(np.float32))
+        outs = []
+        for pc in range(3):
+            m, arrs = _meta(cfg, rng, ffts)
+            m.phasecentre = pc
+            nbytes, nrec = C.c_int64(), C.c_int32()
+            assert lib.fxg_difx_records(h, fres.ctypes.data, C.byref(m), None, 0, C.byref(nbytes), C.byref(nrec)) == 0
+            out = np.zeros(nbytes.value, dtype=np.uint8)
+            assert lib.fxg_difx_records(h, fres.ctypes.data, C.byref(m), out.ctypes.data, out.size,
+                                        C.byref(nbytes), C.byref(nrec)) == 0
+            bl, uvw, tel = arrs
+            assert out.tobytes() == vo.records(cfg, res, lay, 59000, 43200.25, 0, 2, ffts, bl, uvw, tel, phasecentre=pc)
+            outs.append(vo.parse(out.tobytes(), {f: cfg.freqs[f].nchan // cfg.freqs[f].chanavg
+                                                 for f in range(len(cfg.freqs))}))
+        m.phasecentre = 3
+        assert lib.fxg_difx_records(h, fres.ctypes.data, C.byref(m), None, 0, C.byref(nbytes), C.byref(nrec)) != 0
+    finally:
+        lib.fxg_destroy(h)
+    autos = [sum(1 for r in recs if r["baseline"] % 257 == 0) for recs in outs]
+    assert autos[0] > 0 and autos[1] == 0 and autos[2] == 0
+    # record weights: baseline weight x decorrelation / (1e9 x integration time)
+    (fi, j), so = next(iter(lay.shiftdecorr_off.items()))
+    w = np.float32(f32[lay.bweight_off[(fi, j)] * 2 + 1] / ffts)
+    for pc in range(3):
+        dec = np.float32(f32[2 * so + pc] / inttime_ns)
+        assert any(r["weight"] == float(np.float32(w * dec)) for r in outs[pc])
+    # scale: the phase-centre block is the single-centre scale divided by its decorrelation
+    out = vo.normalise(cfg, res, lay, msi)
+    fr = cfg.freqs[fi]
+    fc = (fr.nchan + fr.chanavg - 1) // fr.chanavg
+    nprod = len(dict(cfg.baselines[j].freqs)[fi])
+    for pc in range(3):
+        o = lay.baseline_off[(fi, j)] + (pc * nprod + 1) * fc
+        wf = np.float32(f32[lay.bweight_off[(fi, j)] * 2 + 1] / ffts)
+        sc = np.float32(1.0 / (np.float64(wf) * msi * np.float64(np.float32(cfg.blockspersend() * 2 * fc * fr.chanavg))))
+        dec = np.float32(f32[2 * so + pc] / (1e9 * (msi * cfg.subint_ns / 1e9)))
+        np.testing.assert_array_equal(out[o:o + fc], (res[o:o + fc].view(np.float32) * np.float32(sc / dec)).view(np.complex64))
