@@ -80,6 +80,8 @@ def load_library() -> C.CDLL:
     lib.fxg_vdif_mux_device.argtypes = [P(abi.fxg_vdif_mux_params), C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                         C.c_int64, C.c_void_p, C.c_int64, C.c_void_p]
     lib.fxg_vdif_mux_result.argtypes = [C.c_void_p, C.c_void_p, P(C.c_int64), P(C.c_int64)]
+    lib.fxg_vdif_mux_device_static.argtypes = [P(abi.fxg_vdif_mux_params), C.c_void_p, C.c_int64, C.c_void_p,
+                                               C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]
     lib.fxg_segment_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_segment)]
     lib.fxg_buffer_params.argtypes = [P(abi.fxg_config), C.c_int32, C.c_int32, C.c_int32, P(abi.fxg_databuffer)]
     lib.fxg_set_kurtosis.argtypes = [C.c_void_p, C.c_int32]

@@ -343,6 +343,45 @@ int dispatch_frames(const MuxArgs& a, const uint8_t* in, uint8_t* out, const Mux
   }
 }
 
+// The outcome of the last corner turn with workspace w, for a caller that reads it later from
+// device memory: {output frames, dropped before, after, unmapped, length error}.
+__global__ void k_vmux_outcome(MuxWork w, int64_t* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long c0 = w.ctr[0], c5 = w.ctr[5];
+  out[0] = c0 < c5 ? (int64_t)(c5 - c0) : 0;
+  out[1] = (int64_t)w.ctr[1];
+  out[2] = (int64_t)w.ctr[2];
+  out[3] = (int64_t)w.ctr[3];
+  out[4] = w.ctr[4] ? 1 : 0;
+}
+
+// Workspace of the allocation-free corner turn: one array per GPU, held until the process
+// ends.  Host code never uses this name as an address: it takes each device's address of the
+// array from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[FXG_MUX_STATIC_WORK_BYTES];
+
+}  // namespace
+
+#include <mutex>
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call on that device.
+int scratch_of_current_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return FXG_E_HIP;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return FXG_E_HIP;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return FXG_OK;
+}
 }  // namespace
 
 extern "C" int64_t fxg_vdif_mux_workspace_bytes(int64_t nslots, int32_t nthreads) {
@@ -391,4 +430,20 @@ extern "C" int fxg_vdif_mux_result(const void* d_work, void* stream, int64_t* no
   if (nout) *nout = c[0] < c[5] ? (int64_t)(c[5] - c[0]) : 0;   // slots from the first present one
   if (dropped) { dropped[0] = (int64_t)c[1]; dropped[1] = (int64_t)c[2]; dropped[2] = (int64_t)c[3]; }
   return c[4] ? FXG_E_INVALID : FXG_OK;
+}
+
+extern "C" int fxg_vdif_mux_device_static(const fxg_vdif_mux_params* p, const uint8_t* d_in, int64_t in_bytes,
+                                          uint8_t* d_out, int64_t out_bytes, int64_t nslots, int64_t* d_outcome,
+                                          void* stream) {
+  if (!p || !d_outcome || (reinterpret_cast<uintptr_t>(d_outcome) & 7)) return FXG_E_INVALID;
+  const int64_t need = fxg_vdif_mux_workspace_bytes(nslots, p->nthreads);
+  if (need < 0 || need > FXG_MUX_STATIC_WORK_BYTES) return FXG_E_INVALID;   // beyond the static workspace
+  void* work = nullptr;
+  if (int rc = scratch_of_current_device(&work)) return rc;
+  const int rc = fxg_vdif_mux_device(p, d_in, in_bytes, d_out, out_bytes, nslots, work, FXG_MUX_STATIC_WORK_BYTES,
+                                     stream);
+  if (rc != FXG_OK) return rc;
+  hipLaunchKernelGGL(k_vmux_outcome, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream),
+                     work_of(work, nslots, p->nthreads), d_outcome);
+  return hipGetLastError() == hipSuccess ? FXG_OK : FXG_E_HIP;
 }

@@ -150,6 +150,7 @@ class fxg_server_stats(C.Structure):
 
 
 FXG_MUX_MAX_THREADS = 32
+FXG_MUX_STATIC_WORK_BYTES = 1 << 20
 
 
 class fxg_vdif_mux_params(C.Structure):
@@ -217,7 +218,7 @@ EXPORTED_SYMBOLS = [
     "fxg_footprint", "fxg_input_samples", "fxg_device_bytes",
     "fxg_server_create", "fxg_server_destroy", "fxg_server_add_job", "fxg_server_push_subint",
     "fxg_server_wait_job", "fxg_server_get_stats", "fxg_server_job_error",
-    "fxg_vdif_mux_workspace_bytes", "fxg_vdif_mux_device", "fxg_vdif_mux_result",
+    "fxg_vdif_mux_workspace_bytes", "fxg_vdif_mux_device", "fxg_vdif_mux_result", "fxg_vdif_mux_device_static",
     "fxg_segment_params", "fxg_buffer_params", "fxg_control_params",
     "fxg_normalise_device", "fxg_difx_records", "fxg_set_kurtosis", "fxg_get_kurtosis",
     "fxg_sta_autocorr", "fxg_set_pcal", "fxg_get_pcal",

@@ -332,6 +332,19 @@ int fxg_vdif_mux_device(const fxg_vdif_mux_params* p, const uint8_t* d_in, int64
  * an input frame's length field differed from inputframebytes (the reference aborts
  * there; the output is then undefined). */
 int fxg_vdif_mux_result(const void* d_work, void* stream, int64_t* nout, int64_t* dropped /* [3] */);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight capture and serving loops, harnesses that forbid hipMalloc): the
+ * corner turn of fxg_vdif_mux_device with its workspace in a library-owned static array of
+ * FXG_MUX_STATIC_WORK_BYTES per GPU instead of d_work; segments whose workspace would be larger
+ * (fxg_vdif_mux_workspace_bytes, about 4 (N + 1) bytes per slot) are refused with
+ * FXG_E_INVALID.  Nothing is allocated.  The outcome goes to the caller's device buffer
+ * d_outcome[5] (8-byte aligned): {output frames, dropped before the segment, after it,
+ * unmapped thread, 1 if an input frame's length field differed}.  Every call on one GPU shares
+ * that GPU's array (cleared by each call): calls that overlap on one GPU race. */
+#define FXG_MUX_STATIC_WORK_BYTES (1 << 20)
+int fxg_vdif_mux_device_static(const fxg_vdif_mux_params* p, const uint8_t* d_in, int64_t in_bytes,
+                               uint8_t* d_out, int64_t out_bytes, int64_t nslots, int64_t* d_outcome,
+                               void* stream);
 
 /* ---- Slot start and valid flags: DataStream::calculateControlParams ------------------
  * Replaces DataStream::calculateControlParams (datastream.cpp:368-627), the coarse-delay
