@@ -110,9 +110,18 @@ def test_static_workspace_limit_refused(gpu):
     nslots = (limit - 64) // (4 * (n + 1)) + 1
     assert lib.fxg_vdif_mux_workspace_bytes(nslots, n) > limit
     p = _params(n, tfb, 25, 1, list(range(n)), 0)
-    out = torch.empty(16, dtype=torch.uint8, device=gpu)
+    out = torch.empty(nslots * ((tfb - 32) * n + 32), dtype=torch.uint8, device=gpu)   # a valid output
     outcome = torch.zeros(5, dtype=torch.int64, device=gpu)
-    rc = lib.fxg_vdif_mux_device_static(C.byref(p), C.c_void_p(0), 0, C.c_void_p(out.data_ptr()),
-                                        nslots * ((tfb - 32) * n + 32), nslots, C.c_void_p(outcome.data_ptr()),
-                                        C.c_void_p(0))
+    rc = lib.fxg_vdif_mux_device_static(C.byref(p), C.c_void_p(0), 0, C.c_void_p(out.data_ptr()), out.numel(),
+                                        nslots, C.c_void_p(outcome.data_ptr()), C.c_void_p(0))
     assert rc == abi.FXG_E_INVALID
+    # one slot fewer fits, and runs: one frame of thread 0 at frame time 2
+    f = np.zeros(tfb, dtype=np.uint8)
+    f[:32] = vm.make_header(1000, 2, 40, tfb, 0, 1).view(np.uint8)
+    d_in = torch.from_numpy(f).to(gpu)
+    rc = lib.fxg_vdif_mux_device_static(C.byref(p), C.c_void_p(d_in.data_ptr()), d_in.numel(),
+                                        C.c_void_p(out.data_ptr()), out.numel(), nslots - 1,
+                                        C.c_void_p(outcome.data_ptr()), C.c_void_p(0))
+    torch.cuda.synchronize()
+    assert rc == abi.FXG_OK
+    assert outcome.cpu().tolist() == [nslots - 1 - 2, 0, 0, 0, 0]   # frames from the first present slot
