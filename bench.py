@@ -1,36 +1,41 @@
 #!/usr/bin/env python3
-"""Benchmark of the FX hot path on one MI355X: input Gsamples/s and subint latency.
+"""Benchmark of the FX hot path on one MI355X: input Gsamples/s, p99 subint latency, and the
+MFMA/batched path against the plain FP32 path.
 
-Workload (one "step" = one subintegration through the whole per-subint pipeline):
-10 stations, dual circular polarisation, 8 subbands x 32 MHz, 2-bit real VDIF with 16 bands
+Workload of `value` (one "step" = one subintegration through the whole per-subint pipeline):
+16 stations, dual circular polarisation, 8 subbands x 32 MHz, 2-bit real VDIF with 16 bands
 per 32-bit sample word (8000-byte payloads), nchan 1024 (2048-point FFTs), 0.5 s subints
 (15625 FFT blocks per station), fringe rotation order 1, seeded quintic delay models.
 Synthetic 2-bit data is generated on the device before the timed region (inputs resident
 in HBM), so `value` excludes PCIe transfers.
 
-Two paths are timed on the same inputs:
-  headline  fused F-engine + FP16 MFMA Hermitian X-engine   -> `value`
+Paths timed on the same inputs, with the same number of steps:
+  MFMA      fused F-engine + FP16 MFMA Hermitian X-engine   -> `value`
   plain     fused F-engine (FP32 spectra) + FP32 CMAC X-engine per baseline product
+            -> `plain_fp32_value`, and `speedup_mfma_vs_plain` = value / plain
+  BF16      the MFMA path with BF16 inputs (section mfma_bf16)
 
-The roofline object describes the headline path's dominant kernel, timed with HIP events
-on its own stream inside the library (fxg_set_timing).  The CPU baseline runs the C
-restatement in oracle/ (when built) or the numpy oracle on a bounded sample of the same
-workload on the host.
+The roofline object describes the dominant kernel (the F-engine) against HBM, timed with HIP
+events on its own stream inside the library (fxg_set_timing); `issue_roofline` states the
+same kernel against its VALU issue rate, with its per-sample instruction count from the
+committed counter profile.  The CPU baseline runs the C restatement in oracle/ (when built)
+on a bounded sample of the same workload on the host, with all host threads and with one.
 
 Further sections of the same JSON line (each can be skipped with its --no-* flag):
-  plain_fp32          the plain path on the same inputs
-  target_16_stations  16 stations, MFMA path vs plain path
+  cfg1_10_stations    the 10-station array-scale configuration, MFMA and plain
+  mfma_bf16           BF16 X-engine inputs on the 16-station workload
   uneven_bands        mixed bandwidths and zooms, both F-engine launch forms, autotuner report
   serving             Poisson job stream through the server (p50 / p99 subint latency)
-  kurtosis            the headline subint with the spectral-kurtosis sums enabled
-  phase_cal           the headline subint with phase-cal extraction on every band (1 MHz comb)
+  kurtosis            the 10-station subint with the spectral-kurtosis sums enabled
+  phase_cal           the 10-station subint with phase-cal extraction on every band (1 MHz comb)
   output_side         device scaling (writedata) and DIFX records of one subint's results
   vdif_corner_turn    10 stations' 16-thread VDIF segments corner-turned on the device, with
                       a device-copy yardstick and the numpy restatement's rate
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain] [--no-uneven]
-                    [--no-serving] [--no-target] [--no-kurtosis] [--no-pcal] [--no-output]
-                    [--no-mux]
+                    [--no-serving] [--no-cfg1] [--no-bf16] [--no-kurtosis] [--no-pcal]
+                    [--no-output] [--no-mux]
+    python bench.py --cpu-table      # the C restatement per configuration, 1 and all threads
 """
 from __future__ import annotations
 
@@ -50,11 +55,12 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_config(xengine: str):
+def workload_config(xengine: str, nstations: int = 16):
+    """The array workload: nstations dual-pol, 8 x 32 MHz 2-bit VDIF, nchan 1024, 0.5 s."""
     from difx_amd import config as cf
     freqs = [4800.0 + 40.0 * k for k in range(8)]
-    return cf.standard_config(10, 2, freqs, 32.0, 1024, subint_ns=500_000_000,
-                              xengine=xengine, seed=20261202)
+    return cf.standard_config(nstations, 2, freqs, 32.0, 1024, subint_ns=500_000_000,
+                              xengine=xengine, seed=20261202 if nstations == 10 else 20261205)
 
 
 def device_slot(cfg, offsets, torch, dev, seed=7):
@@ -77,7 +83,7 @@ def device_slot(cfg, offsets, torch, dev, seed=7):
     return bufs, nbytes, ctrl
 
 
-def uneven_config(xengine: str, nstations: int = 16, **kw):
+def uneven_config(xengine: str, nstations: int = 16, subint_ns: int = 512_000_000, **kw):
     """Uneven bands: per station a 4 x 64 MHz (nchan 16384, 32768-point FFTs), a 4 x 32 MHz
     (nchan 8192) and an 8 x 8 MHz (nchan 2048) datastream, dual pol, one 256 us FFT duration
     (3.9 kHz channels), plus six zoom windows of 0.5-2 MHz (128-512 channels); 0.512 s subints
@@ -88,7 +94,7 @@ def uneven_config(xengine: str, nstations: int = 16, **kw):
               (8.0, 2048, [5400.0 + 8.0 * k for k in range(8)])]
     zooms = [(0, 0, 10.0, 2.0), (0, 2, 30.0, 0.5), (1, 1, 4.0, 1.0), (1, 3, 20.0, 2.0),
              (2, 0, 1.0, 0.5), (2, 5, 3.0, 2.0)]
-    return cf.batched_config(nstations, groups, zooms, subint_ns=512_000_000, xengine=xengine,
+    return cf.batched_config(nstations, groups, zooms, subint_ns=subint_ns, xengine=xengine,
                              seed=20261203, **kw)
 
 
@@ -416,11 +422,37 @@ def mux_cpu_baseline(budget_s: float = 6.0):
                       f"{dt:.2f} s"}
 
 
-def cpu_baseline(budget_s: float = 12.0):
-    """Time the CPU restatement on a bounded sample of the workload: all 10 stations x 16
-    bands x all 45 baselines, for as many FFT blocks as fit the time budget."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _port_rate(cfg, nthreads: int, budget_s: float, offsets=(0, 1, 0)):
+    """Passes of the C restatement over one slot of cfg until about budget_s of wall time:
+    (Gsamples/s, per-pass seconds)."""
+    import fxport
+    from difx_amd import synth
+    data, ctrl = synth.noise_slot(cfg, list(offsets), seed=3)
+    fxport.process_subint(cfg, list(offsets), data, ctrl, nthreads=nthreads)     # warm-up, plans
+    times = []
+    while sum(times) < budget_s and len(times) < 200:
+        t0 = time.perf_counter()
+        fxport.process_subint(cfg, list(offsets), data, ctrl, nthreads=nthreads)
+        times.append(time.perf_counter() - t0)
+    return samples_per_subint(cfg) * len(times) / sum(times) / 1e9, times
+
+
+def cpu_baseline(budget_s: float = 12.0, nthreads: int = 0, nstations: int = 16):
+    """Time the CPU restatement on a bounded sample of the workload: all stations x 16 bands x
+    all baselines of the nstations dual-pol 8 x 32 MHz nchan-1024 workload, for a slot of
+    256 FFT blocks per thread (at most 4096), repeated for about budget_s."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    from difx_amd import config as cf, synth
+    from difx_amd import config as cf
     try:
         import fxport  # C restatement (oracle/fxport.c), multithreaded
         have_port = fxport.available()
@@ -429,42 +461,77 @@ def cpu_baseline(budget_s: float = 12.0):
     freqs = [4800.0 + 40.0 * k for k in range(8)]
     offsets = [0, 1, 0]
     if have_port:
-        # one 4096-FFT-block slot of the workload, processed repeatedly until about
-        # budget_s of wall time has been spent (the CPU work is the same every pass)
-        nthreads = fxport.default_threads()
-        nblk = 4096
-        cfg = cf.standard_config(10, 2, freqs, 32.0, 1024, subint_ns=nblk * 32000, seed=20261202)
-        data, ctrl = synth.noise_slot(cfg, offsets, seed=3)
-        fxport.process_subint(cfg, offsets, data, ctrl, nthreads=nthreads)     # warm-up, plans
-        runs, dt = 0, 0.0
-        while dt < budget_s and runs < 200:
-            t0 = time.perf_counter()
-            fxport.process_subint(cfg, offsets, data, ctrl, nthreads=nthreads)
-            dt += time.perf_counter() - t0
-            runs += 1
-        samples = samples_per_subint(cfg) * runs
-        return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": nthreads,
-                "kind": "port",
-                "sample": f"C restatement (oracle/fxport.c, FFT: {fxport.fft_backend()}), {runs} passes over "
-                          f"{nblk} FFT blocks of the 10-station dual-pol 8x32 MHz nchan-1024 workload "
-                          f"({samples/1e6:.0f} Msamples), {nthreads} threads, {dt:.2f} s wall"}
+        T = nthreads or fxport.default_threads()
+        nblk = min(4096, 256 * T)
+        cfg = cf.standard_config(nstations, 2, freqs, 32.0, 1024, subint_ns=nblk * 32000, seed=20261202)
+        rate, times = _port_rate(cfg, T, budget_s, offsets)
+        samples = samples_per_subint(cfg) * len(times)
+        return {"value": rate, "unit": "Gsamples/s", "cores": T, "kind": "port",
+                "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+                "sample": f"C restatement (oracle/fxport.c, FFT: {fxport.fft_backend()}), {len(times)} passes over "
+                          f"{nblk} FFT blocks of the {nstations}-station dual-pol 8x32 MHz nchan-1024 workload "
+                          f"({samples/1e6:.0f} Msamples), {T} threads, {sum(times):.2f} s wall"}
     import fxoracle as fo
+    from difx_amd import synth
     nblk = 4
-    cfg = cf.standard_config(10, 2, freqs, 32.0, 1024, subint_ns=nblk * 32000, seed=20261202)
+    cfg = cf.standard_config(nstations, 2, freqs, 32.0, 1024, subint_ns=nblk * 32000, seed=20261202)
     data, ctrl = synth.noise_slot(cfg, offsets, seed=3)
     t0 = time.perf_counter()
     fo.process_subint(cfg, offsets, data, ctrl)
     dt = time.perf_counter() - t0
     samples = samples_per_subint(cfg)
     return {"value": samples / dt / 1e9, "unit": "Gsamples/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"numpy FP64 oracle, {nblk} FFT blocks of the workload "
                       f"({samples/1e6:.1f} Msamples), 1 thread, {dt:.2f} s"}
 
 
-def measured_traffic(path: Path, ffts_per_launch: float):
+def cpu_table(budget_s: float = 8.0):
+    """The C restatement per benchmark configuration, with one thread and with all host
+    threads: Gsamples/s, and the wall time of one whole subint at that rate (p50 / p99 over
+    the timed passes of the sample, scaled to the subint)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import fxport
+    from difx_amd import config as cf
+    T = fxport.default_threads()
+    freqs8 = [4800.0 + 40.0 * k for k in range(8)]
+    rows = []
+
+    def row(name, cfg_of, full_cfg, T_):
+        sample = cfg_of(T_)
+        rate, times = _port_rate(sample, T_, budget_s)
+        per = np.array(times) * samples_per_subint(full_cfg) / samples_per_subint(sample)
+        rows.append({"config": name, "threads": T_, "gsamples_per_s": rate,
+                     "subint_s": full_cfg.subint_ns / 1e9,
+                     "p50_subint_s": float(np.percentile(per, 50)), "p99_subint_s": float(np.percentile(per, 99)),
+                     "passes": len(times), "sample_fft_blocks": sample.blockspersend()})
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+
+    for T_ in (1, T):
+        # cfg0: 4 stations, 1 x 32 MHz single pol, nchan 256, 1 s subints
+        row("cfg0", lambda t: cf.standard_config(4, 1, [4800.0], 32.0, 256, subint_ns=min(4096, 1024 * t) * 8000,
+                                                 seed=1),
+            cf.standard_config(4, 1, [4800.0], 32.0, 256, subint_ns=1_000_000_000), T_)
+        # cfg1: 10 stations dual pol 8 x 32 MHz nchan 1024, 0.5 s subints
+        row("cfg1", lambda t: cf.standard_config(10, 2, freqs8, 32.0, 1024, subint_ns=min(4096, 128 * t) * 32000,
+                                                 seed=2),
+            cf.standard_config(10, 2, freqs8, 32.0, 1024, subint_ns=500_000_000), T_)
+        # cfg2: 64 stations, 16 MHz complex 8-bit, nchan 4096, subint 999 936 000 ns
+        row("cfg2", lambda t: cf.standard_config(64, 1, [1400.0], 16.0, 4096, subint_ns=min(64, 4 * t) * 256000,
+                                                 nbits=8, complex_sampling=1, seed=3, max_delay_us=200.0),
+            cf.standard_config(64, 1, [1400.0], 16.0, 4096, subint_ns=999_936_000, nbits=8, complex_sampling=1),
+            T_)
+        # cfg3: the uneven 16-station workload of the uneven_bands section, 0.512 s subints
+        row("cfg3", lambda t: uneven_config("fp32", subint_ns=min(64, 4 * t) * 256000, max_delay_us=200.0),
+            uneven_config("fp32"), T_)
+    return {"cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "fft": fxport.fft_backend(), "rows": rows}
+
+
+def measured_traffic(path: Path, ffts_per_launch: float, stations: int = 16):
     """HBM bytes per F-engine launch from the committed rocprofv3 --pmc summary: FETCH_SIZE
     and WRITE_SIZE (KiB per dispatch) each scaled by what the same counter reads for a plain
-    device copy of known size (`bench.py --copy-probe`), rescaled to this launch's FFT count."""
+    device copy of known size (`bench.py --copy-probe`), rescaled to this launch's FFT count.
+    None when the summary was taken on another station count."""
     if not path.exists():
         return None
     kv = {}
@@ -476,8 +543,8 @@ def measured_traffic(path: Path, ffts_per_launch: float):
             except (ValueError, IndexError):
                 pass
     need = ("fetch_kib_per_dispatch", "write_kib_per_dispatch", "copy_fetch_kib", "copy_write_kib",
-            "copy_bytes", "ffts_per_dispatch")
-    if not all(k in kv for k in need):
+            "copy_bytes", "ffts_per_dispatch", "stations")
+    if not all(k in kv for k in need) or int(kv["stations"]) != stations:
         return None
     fscale = kv["copy_bytes"] / (kv["copy_fetch_kib"] * 1024.0)
     wscale = kv["copy_bytes"] / (kv["copy_write_kib"] * 1024.0)
@@ -497,24 +564,54 @@ def copy_probe():
     print("copy probe: 1073741824 bytes copied", flush=True)
 
 
+# VALU lane instructions per input sample of k_fengine<2048,false,1,256> (real 2-bit, 16 bands
+# per word), from the SQ_INSTS_VALU counter summary committed with the kernel's profile.
+FENGINE_VALU_PER_SAMPLE_FILE = ROOT / "profiles" / "k_fengine_2048_valu_counters.txt"
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9          # 256 CUs x 64 lanes x 2.4 GHz peak engine clock
+
+
+def valu_per_sample():
+    if not FENGINE_VALU_PER_SAMPLE_FILE.exists():
+        return None
+    for line in FENGINE_VALU_PER_SAMPLE_FILE.read_text().splitlines():
+        if line.startswith("SQ_INSTS_VALU ") and "lane instructions per input sample" in line:
+            return float(line.split("->")[1].split()[0])
+    return None
+
+
+def path_summary(r, nsamp, steps):
+    lat = np.array(r["lat_ms"])
+    tm = r["timing"]
+    return {"value": nsamp * steps / r["wall_s"] / 1e9, "unit": "Gsamples/s", "steps": steps,
+            "ms_per_step": r["wall_s"] * 1e3 / steps,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "f_ms_per_step": tm["f_ms"] / steps, "x_ms_per_step": tm["x_ms"] / steps,
+            "other_ms_per_step": tm["other_ms"] / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cfg1", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--no-target", action="store_true")
     ap.add_argument("--no-mux", action="store_true")
     ap.add_argument("--no-output", action="store_true")
     ap.add_argument("--no-kurtosis", action="store_true")
     ap.add_argument("--no-pcal", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="1 GiB device copy for PMC scaling")
+    ap.add_argument("--cpu-table", action="store_true", help="C restatement per configuration (no GPU)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.cpu_table:
+        print(json.dumps(cpu_table()))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the FX path on a single MI355X; "
@@ -529,35 +626,54 @@ def main():
     offsets = [0, 1, 0]
     cfg = workload_config("fp16")
     nsamp = samples_per_subint(cfg)
-    print(f"[bench] headline path (F + FP16 MFMA X), {args.warmup} warmup + {args.steps} steps", file=sys.stderr, flush=True)
+    print(f"[bench] 16 stations, MFMA path (F + FP16 MFMA X), {args.warmup} warmup + {args.steps} steps",
+          file=sys.stderr, flush=True)
     head = time_path(cfg, args.steps, args.warmup, torch, dev, offsets)
     B = head["B"]
-    value = nsamp * args.steps / head["wall_s"] / 1e9
-    lat = np.array(head["lat_ms"])
+    hs = path_summary(head, nsamp, args.steps)
+    value = hs["value"]
     tm = head["timing"]
     f_avg_ms = tm["f_ms"] / max(tm["f_launches"], 1)
     nchunks_per_step = tm["f_launches"] / args.steps
     ffts_per_launch = B / nchunks_per_step
-    fbytes = f_engine_bytes_per_launch(cfg, ffts_per_launch, 4)
+    fbytes = f_engine_bytes_per_launch(cfg, ffts_per_launch, 2)      # FP16 spectra out
     achieved = fbytes / (f_avg_ms / 1e3) / 1e9
-    x_avg_ms = tm["x_ms"] / max(tm["x_launches"], 1)
+    f_samples_per_launch = ffts_per_launch * sum(2 * cfg.freqs[d.recfreqs[0]].nchan * len(d.recbands)
+                                                 for d in cfg.datastreams)
+    vps = valu_per_sample()
+    issue = None
+    if vps is not None:
+        ach = vps * f_samples_per_launch / (f_avg_ms / 1e3)
+        issue = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_LANE_OPS, "unit": "lane-instructions/s",
+                 "frac": ach / VALU_PEAK_LANE_OPS, "valu_lane_instructions_per_sample": vps,
+                 "source": "profiles/k_fengine_2048_valu_counters.txt (SQ_INSTS_VALU x 64 / input samples)"}
     plain = None
     if not args.no_plain:
         pcfg = workload_config("fp32")
-        psteps = max(1, min(args.steps, 3))
-        print(f"[bench] plain FP32 path, 1 warmup + {psteps} steps", file=sys.stderr, flush=True)
-        p = time_path(pcfg, psteps, 1, torch, dev, offsets)
-        plat = np.array(p["lat_ms"])
-        pval = nsamp * psteps / p["wall_s"] / 1e9
-        ptm = p["timing"]
-        # agreement of the two paths on identical inputs (diffDiFX metric over the cross records)
+        print(f"[bench] 16 stations, plain FP32 path, {args.warmup} warmup + {args.steps} steps",
+              file=sys.stderr, flush=True)
+        p = time_path(pcfg, args.steps, args.warmup, torch, dev, offsets)
+        plain = path_summary(p, nsamp, args.steps)
+        # agreement of the two paths on identical inputs (diffDiFX metric over the whole array)
         a, b = head["results"], p["results"]
-        rel = float(np.mean(np.abs(a - b)) / max(np.mean(np.abs(b)), 1e-30))
-        plain = {"value": pval, "unit": "Gsamples/s", "p50_ms": float(np.percentile(plat, 50)),
-                 "p99_ms": float(np.percentile(plat, 99)),
-                 "f_ms_per_step": ptm["f_ms"] / psteps, "x_ms_per_step": ptm["x_ms"] / psteps,
-                 "speedup_headline_vs_plain": value / pval,
-                 "headline_vs_plain_rel_diff": rel}
+        plain["mfma_vs_plain_rel_diff"] = float(np.mean(np.abs(a - b)) / max(np.mean(np.abs(b)), 1e-30))
+    bf16 = None
+    if not args.no_bf16:
+        bsteps = max(1, min(args.steps, 10))
+        print(f"[bench] 16 stations, BF16 MFMA path, 1 warmup + {bsteps} steps", file=sys.stderr, flush=True)
+        r = time_path(workload_config("bf16"), bsteps, 1, torch, dev, offsets)
+        bf16 = path_summary(r, nsamp, bsteps)
+    cfg1 = None
+    if not args.no_cfg1:
+        c1 = workload_config("fp16", nstations=10)
+        n1 = samples_per_subint(c1)
+        csteps = max(1, min(args.steps, 10))
+        print(f"[bench] 10 stations, MFMA and plain paths, 1 warmup + {csteps} steps", file=sys.stderr, flush=True)
+        r1 = time_path(c1, csteps, 1, torch, dev, offsets)
+        r2 = time_path(workload_config("fp32", nstations=10), csteps, 1, torch, dev, offsets)
+        cfg1 = {"workload": "10 stations dual-pol, 8 x 32 MHz 2-bit VDIF, nchan 1024, 0.5 s subint",
+                "mfma_fp16": path_summary(r1, n1, csteps), "plain_fp32": path_summary(r2, n1, csteps)}
+        cfg1["speedup_mfma_vs_plain"] = cfg1["mfma_fp16"]["value"] / cfg1["plain_fp32"]["value"]
     uneven = None
     if not args.no_uneven:
         from difx_amd.core import FxCore
@@ -568,61 +684,47 @@ def main():
         core.close()
         usamp = samples_per_subint(tcfg)
         forms = {}
+        usteps = 5
         for name, mode in (("per_length", 1), ("persistent", 2)):
             ucfg = uneven_config("fp16", fengine_mode=mode)
-            print(f"[bench] uneven bands ({name} F-engine launch), 1 warmup + 3 steps", file=sys.stderr, flush=True)
-            u = time_path(ucfg, 3, 1, torch, dev, offsets)
-            ulat = np.array(u["lat_ms"])
-            forms[name] = {"value": usamp * 3 / u["wall_s"] / 1e9, "p50_ms": float(np.percentile(ulat, 50)),
-                           "p99_ms": float(np.percentile(ulat, 99)),
-                           "f_ms_per_step": u["timing"]["f_ms"] / 3, "x_ms_per_step": u["timing"]["x_ms"] / 3}
+            print(f"[bench] uneven bands ({name} F-engine launch), 1 warmup + {usteps} steps", file=sys.stderr, flush=True)
+            u = time_path(ucfg, usteps, 1, torch, dev, offsets)
+            forms[name] = path_summary(u, usamp, usteps)
         best = max(forms, key=lambda n: forms[n]["value"])
         uneven = {"workload": "16 stations dual-pol: 4 x 64 MHz (nchan 16384), 4 x 32 MHz (nchan 8192), "
                               "8 x 8 MHz (nchan 2048) datastreams + 6 zooms of 0.5-2 MHz (128-512 channels), "
                               "0.512 s subint",
                   "value": forms[best]["value"], "unit": "Gsamples/s", "p99_ms": forms[best]["p99_ms"],
                   "subint_length_ms": 512.0, "forms": forms, "autotune_report": report}
-    target = None
-    if not args.no_target:
-        from difx_amd import config as cf
-        freqs = [4800.0 + 40.0 * k for k in range(8)]
-        tg = {}
-        for xe, steps_t in (("fp16", 3), ("fp32", 2)):
-            tcfg = cf.standard_config(16, 2, freqs, 32.0, 1024, subint_ns=500_000_000, xengine=xe, seed=20261205)
-            print(f"[bench] 16-station target shape, {xe}, 1 warmup + {steps_t} steps", file=sys.stderr, flush=True)
-            r = time_path(tcfg, steps_t, 1, torch, dev, offsets)
-            tl = np.array(r["lat_ms"])
-            tg[xe] = {"value": samples_per_subint(tcfg) * steps_t / r["wall_s"] / 1e9,
-                      "p99_ms": float(np.percentile(tl, 99)),
-                      "f_ms_per_step": r["timing"]["f_ms"] / steps_t, "x_ms_per_step": r["timing"]["x_ms"] / steps_t}
-        target = {"workload": "16 stations dual-pol, 8 x 32 MHz, nchan 1024, 0.5 s subint",
-                  "mfma_fp16": tg["fp16"], "plain_fp32": tg["fp32"],
-                  "speedup_mfma_vs_plain": tg["fp16"]["value"] / tg["fp32"]["value"],
-                  "subint_length_ms": 500.0}
     serving = None
     if not args.no_serving:
         print("[bench] serving: Poisson job stream", file=sys.stderr, flush=True)
         serving = serving_bench(torch, dev)
+    kcfg = workload_config("fp16", nstations=10)
+    kbase = None
+    if not (args.no_kurtosis and args.no_pcal):
+        print("[bench] 10-station reference run for the side products, 1 warmup + 3 steps", file=sys.stderr, flush=True)
+        kbase = time_path(kcfg, 3, 1, torch, dev, offsets)
     kurt = None
     if not args.no_kurtosis:
-        print("[bench] headline path with spectral kurtosis, 1 warmup + 3 steps", file=sys.stderr, flush=True)
-        kr = time_path(cfg, 3, 1, torch, dev, offsets, kurtosis=True)
+        print("[bench] 10 stations with spectral kurtosis, 1 warmup + 3 steps", file=sys.stderr, flush=True)
+        kr = time_path(kcfg, 3, 1, torch, dev, offsets, kurtosis=True)
         k_ms = kr["wall_s"] * 1e3 / 3
-        spec_bytes = float(samples_per_subint(cfg)) / 2 * 4           # FP16 complex spectra: C = L/2 per FFT
-        kurt = {"workload": "headline subint with Mode's kurtosis sums (|X|^2, |X|^4 per band and channel)",
-                "ms_per_step": k_ms, "extra_ms_vs_headline": k_ms - head["wall_s"] * 1e3 / args.steps,
-                "other_ms_per_step": kr["timing"]["other_ms"] / 3,
+        spec_bytes = float(samples_per_subint(kcfg)) / 2 * 4           # FP16 complex spectra: C = L/2 per FFT
+        extra = (kr["timing"]["other_ms"] - kbase["timing"]["other_ms"]) / 3
+        kurt = {"workload": "10-station subint with Mode's kurtosis sums (|X|^2, |X|^4 per band and channel)",
+                "ms_per_step": k_ms, "extra_ms_vs_without": k_ms - kbase["wall_s"] * 1e3 / 3,
                 "spectra_bytes_read_per_step": spec_bytes,
-                "kurtosis_pass_GBps": spec_bytes / max((kr["timing"]["other_ms"] - tm["other_ms"] / args.steps * 3) / 3 / 1e3, 1e-9) / 1e9}
+                "kurtosis_pass_GBps": spec_bytes / max(extra / 1e3, 1e-9) / 1e9}
     pcal = None
     if not args.no_pcal:
-        print("[bench] headline path with phase-cal extraction, 1 warmup + 3 steps", file=sys.stderr, flush=True)
-        pr = time_path(cfg, 3, 1, torch, dev, offsets, pcal_hz=1_000_000)
+        print("[bench] 10 stations with phase-cal extraction, 1 warmup + 3 steps", file=sys.stderr, flush=True)
+        pr = time_path(kcfg, 3, 1, torch, dev, offsets, pcal_hz=1_000_000)
         p_ms = pr["wall_s"] * 1e3 / 3
-        extra_other = (pr["timing"]["other_ms"] - tm["other_ms"] / args.steps * 3) / 3
-        raw_bytes = float(samples_per_subint(cfg)) * cfg.datastreams[0].nbits / 8   # sample bytes read once
-        pcal = {"workload": "headline subint with phase-cal extraction, 1 MHz comb on every band",
-                "ms_per_step": p_ms, "extra_ms_vs_headline": p_ms - head["wall_s"] * 1e3 / args.steps,
+        extra_other = (pr["timing"]["other_ms"] - kbase["timing"]["other_ms"]) / 3
+        raw_bytes = float(samples_per_subint(kcfg)) * kcfg.datastreams[0].nbits / 8   # sample bytes read once
+        pcal = {"workload": "10-station subint with phase-cal extraction, 1 MHz comb on every band",
+                "ms_per_step": p_ms, "extra_ms_vs_without": p_ms - kbase["wall_s"] * 1e3 / 3,
                 "pcal_ms_per_step": extra_other,
                 "sample_bytes_per_step": raw_bytes,
                 "pcal_pass_GBps": raw_bytes / max(extra_other / 1e3, 1e-9) / 1e9}
@@ -634,10 +736,12 @@ def main():
     if not args.no_mux:
         print("[bench] VDIF thread corner turn, 1 warmup + 5 steps", file=sys.stderr, flush=True)
         mux = mux_bench(torch, dev)
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu:
-        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        print("[bench] CPU baseline, all host threads", file=sys.stderr, flush=True)
         cpu = cpu_baseline()
+        print("[bench] CPU baseline, one thread", file=sys.stderr, flush=True)
+        cpu1 = cpu_baseline(budget_s=8.0, nthreads=1)
         if mux is not None:
             mux["cpu_baseline"] = mux_cpu_baseline()
     realtime = nsamp / (cfg.subint_ns * 1e-9) / 1e9
@@ -648,31 +752,37 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["wall_s"] * 1e3 / args.steps,
+        "ms_per_step": hs["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 F-engine, fp16 X-engine inputs, f32 accumulate",
+        "dtype": "f32 F-engine (f64 delay and fringe phase), fp16 X-engine inputs, f32 accumulate",
         "data": "synthetic (device-generated 2-bit VDIF, seeded quintic delay models)",
-        "config": {"workload": "10 stations dual-pol, 8 x 32 MHz 2-bit VDIF, nchan 1024, 0.5 s subint",
-                   "stations": 10, "pols": 2, "subbands": 8, "bw_mhz": 32, "nchan": 1024,
+        "config": {"workload": "16 stations dual-pol, 8 x 32 MHz 2-bit VDIF, nchan 1024, 0.5 s subint "
+                               "(the MFMA-vs-plain target shape)",
+                   "stations": 16, "pols": 2, "subbands": 8, "bw_mhz": 32, "nchan": 1024,
                    "subint_s": 0.5, "fft_blocks_per_subint": B, "parallelism": "single GPU"},
-        "p50_subint_latency_ms": float(np.percentile(lat, 50)),
-        "p99_subint_latency_ms": float(np.percentile(lat, 99)),
+        "p50_subint_latency_ms": hs["p50_ms"],
+        "p99_subint_latency_ms": hs["p99_ms"],
         "subint_length_ms": cfg.subint_ns / 1e6,
         "realtime_factor": value / realtime,
-        "kernels": {"f_engine_ms_per_step": tm["f_ms"] / args.steps,
-                    "x_engine_ms_per_step": tm["x_ms"] / args.steps,
-                    "other_ms_per_step": tm["other_ms"] / args.steps,
+        "plain_fp32_value": plain["value"] if plain else None,
+        "plain_fp32_p99_subint_latency_ms": plain["p99_ms"] if plain else None,
+        "speedup_mfma_vs_plain": value / plain["value"] if plain else None,
+        "kernels": {"f_engine_ms_per_step": hs["f_ms_per_step"],
+                    "x_engine_ms_per_step": hs["x_ms_per_step"],
+                    "other_ms_per_step": hs["other_ms_per_step"],
                     "f_launches_per_step": nchunks_per_step,
-                    "x_engine_useful_tflops": x_engine_flops(cfg, B) / (tm["x_ms"] / args.steps / 1e3) / 1e12},
-        "roofline": {"kernel": "k_fengine<2048,false,1,256> (fused unpack+rotate+FFT+post-F)",
+                    "x_engine_useful_tflops": x_engine_flops(cfg, B) / (hs["x_ms_per_step"] / 1e3) / 1e12},
+        "roofline": {"kernel": "k_fengine<2048,false,1,256> (fused unpack+rotate+FFT+post-F, FP16 out)",
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(ROOT / "profiles" / "k_fengine_2048_traffic.txt", ffts_per_launch),
                      "avg_launch_ms": f_avg_ms, "algorithmic_bytes_per_launch": fbytes},
+        "issue_roofline": issue,
         "plain_fp32": plain,
-        "target_16_stations": target,
+        "mfma_bf16": bf16,
+        "cfg1_10_stations": cfg1,
         "uneven_bands": uneven,
         "serving": serving,
         "vdif_corner_turn": mux,
@@ -680,6 +790,7 @@ def main():
         "kurtosis": kurt,
         "phase_cal": pcal,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
     }
     print(json.dumps(line))
 

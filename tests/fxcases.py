@@ -38,6 +38,27 @@ def case_cfg3(B: int = 32, xengine: str = "fp32", nstations: int = 3, seed: int 
                              seed=seed, **kw)
 
 
+def case_cfg3_full(B: int = 2, xengine: str = "fp32", nstations: int = 16, seed: int = 20261203, **kw):
+    """The uneven-band configuration as the benchmark runs it, in one configuration: per
+    station a 4 x 64 MHz (nchan 16384, L 32768), a 4 x 32 MHz (nchan 8192, L 16384) and an
+    8 x 8 MHz (nchan 2048, L 4096) datastream, dual pol, one 256 us FFT duration, plus six zoom
+    windows of 0.5-2 MHz (128-512 channels)."""
+    groups = [(64.0, 16384, [4800.0 + 64.0 * k for k in range(4)]),
+              (32.0, 8192, [5200.0 + 32.0 * k for k in range(4)]),
+              (8.0, 2048, [5400.0 + 8.0 * k for k in range(8)])]
+    zooms = [(0, 0, 10.0, 2.0), (0, 2, 30.0, 0.5), (1, 1, 4.0, 1.0), (1, 3, 20.0, 2.0),
+             (2, 0, 1.0, 0.5), (2, 5, 3.0, 2.0)]
+    return cf.batched_config(nstations, groups, zooms, subint_ns=B * 256000, xengine=xengine,
+                             seed=seed, **kw)
+
+
+def case_target16(B: int = 32, xengine: str = "fp32", seed: int = 20261205, **kw):
+    """The MFMA-vs-plain target shape: 16 stations dual pol (32 X-engine inputs, one full
+    32-row MFMA tile per freq), 8 x 32 MHz 2-bit VDIF, nchan 1024."""
+    freqs = [4800.0 + 40.0 * k for k in range(8)]
+    return cf.standard_config(16, 2, freqs, 32.0, 1024, subint_ns=B * 32000, xengine=xengine, seed=seed, **kw)
+
+
 def case_long_ffts(B: int = 8, xengine: str = "fp32", nstations: int = 3, seed: int = 6, **kw):
     """The longest fused FFTs: 64 MHz / 4096 channels (L = 8192) beside 32 MHz / 2048
     channels (L = 4096), single pol, with a 1 MHz zoom in each."""
