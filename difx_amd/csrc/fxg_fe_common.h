@@ -71,12 +71,39 @@ __device__ __forceinline__ void dft8(float2* v) {
   v[3] = cadd(w6, w7); v[7] = csub(w6, w7);
 }
 
-// LDS exchange index: an XOR swizzle of the low five bits by bits 3-7.  It permutes each
-// aligned block of 32 entries, so the buffer needs no padding, and it makes every access of
-// the exchange conflict-free for 8-byte entries with 32 or 64 banks: the consecutive reads
-// (position t + s*L/8) and the Stockham writes at spans 1, 8 and 64 alike (a one-in-eight pad
-// costs 1.3-1.6x in bank conflicts on the reads and the span-64 writes).
-__device__ __forceinline__ int padx(int e) { return e ^ ((e >> 3) & 31); }
+// LDS exchange layout.  Each exchange of the Stockham FFT has its own layout, chosen by the
+// span Ns of the pass that writes it, so that both sides of the exchange address the buffer as
+// one per-thread base plus compile-time offsets (one address register per side instead of one
+// per entry) and neither side has bank conflicts for 8-byte entries:
+//   Ns = 1:  a thread writes 8 consecutive entries 8t..8t+7; one pad entry per 2^XPADB keeps
+//            the lanes of a bank group apart (XPADB 4 for 32 banks, 5 for 64);
+//   Ns = 8:  a thread writes 64c + k + 8r (c = t/8, k = t%8); eight pad entries per 64 place the
+//            two (or four) c of a bank group in different halves (quarters) of the banks;
+//   Ns >= 64: consecutive threads write consecutive entries; no padding.
+// The reads at position t + s*NT follow the same layout.
+#ifndef FXG_XPADB
+#define FXG_XPADB 4
+#endif
+__host__ __device__ constexpr int xlay(int Ns, int e) {
+  return Ns == 1 ? e + (e >> FXG_XPADB) : Ns == 8 ? e + 8 * (e >> 6) : e;
+}
+__host__ __device__ constexpr int sbuf_entries(int L) { return L + L / 8; }
+// Stride in entries between the reads t + s*NT and t + (s+1)*NT of the layout of span Nw, or 0
+// when the layout is not affine in s there (Nw = 8 with NT < 64).
+__host__ __device__ constexpr int xread_stride(int Nw, int NT) {
+  return Nw == 1 ? NT + (NT >> FXG_XPADB) : Nw == 8 ? (NT % 64 == 0 ? NT + NT / 8 : 0) : NT;
+}
+// Entry of position t + s*NT in the layout of span Nw.
+template <int NT>
+__device__ __forceinline__ int xread_idx(int Nw, int t, int s) {
+  const int st = xread_stride(Nw, NT);
+  return st ? xlay(Nw, t) + s * st : xlay(Nw, t + s * NT);
+}
+// Entry of the r-th output (base + r*Ns) of a thread whose first output is idxD (idxD % Ns = k,
+// idxD = (t/Ns)*8*Ns + k), in the layout of span Ns: the base's layout plus r's offset.
+__device__ __forceinline__ int xwrite_idx(int Ns, int idxD, int r) {
+  return xlay(Ns, idxD) + r * (Ns == 1 ? 1 : Ns == 8 ? 8 : Ns);
+}
 
 // Which per-freq quantities fe_range keeps in registers across the bands of a freq (bit 0:
 // the rotator of the thread's samples, bit 1: the fractional-sample phasors of its channels).
@@ -105,7 +132,7 @@ struct Plan {
 
 // Stockham FFT of L points over NT = L/8 butterfly positions, held V per thread: position
 // vt = tl + u*TPF (u < V) holds v[u][s] = x[vt + s*NT] on entry and X[vt + s*NT] on exit.
-// sbuf has L entries (swizzled by padx); tw holds Plan<L>::TWN twiddles exp(-2 pi i k r / (Ns R))
+// sbuf has sbuf_entries(L) entries (laid out by xlay); tw holds Plan<L>::TWN twiddles exp(-2 pi i k r / (Ns R))
 // per pass.  Every thread of the workgroup calls it (the barriers are workgroup-wide); with
 // several FFTs per workgroup each group passes its own sbuf.
 template <int L, int V, int TPF>
@@ -121,7 +148,7 @@ __device__ __forceinline__ void fft_stockham(float2 (*v)[8], float2* sbuf, const
       for (int u = 0; u < V; ++u) {
         const int vt = tl + u * TPF;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) v[u][s] = sbuf[padx(vt + s * NT)];
+        for (int s = 0; s < 8; ++s) v[u][s] = sbuf[xread_idx<NT>(Ns / 8, vt, s)];
         const int k = vt & (Ns - 1);
         const float2* twp = tw + P::off8(p) + k;
 #pragma unroll
@@ -140,7 +167,7 @@ __device__ __forceinline__ void fft_stockham(float2 (*v)[8], float2* sbuf, const
         const int k = vt & (Ns - 1);
         const int idxD = (vt / Ns) * Ns * 8 + k;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) sbuf[padx(idxD + r * Ns)] = v[u][r];
+        for (int r = 0; r < 8; ++r) sbuf[xwrite_idx(Ns, idxD, r)] = v[u][r];
       }
       __syncthreads();
     }
@@ -151,7 +178,7 @@ __device__ __forceinline__ void fft_stockham(float2 (*v)[8], float2* sbuf, const
     for (int u = 0; u < V; ++u) {
       const int vt = tl + u * TPF;
 #pragma unroll
-      for (int s = 0; s < 8; ++s) v[u][s] = sbuf[padx(vt + s * NT)];
+      for (int s = 0; s < 8; ++s) v[u][s] = sbuf[xread_idx<NT>(Ns / 8, vt, s)];
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         const int j = vt + b * NT;
@@ -170,7 +197,7 @@ __device__ __forceinline__ void fft_stockham(float2 (*v)[8], float2* sbuf, const
     for (int u = 0; u < V; ++u) {
       const int vt = tl + u * TPF;
 #pragma unroll
-      for (int s = 0; s < 8; ++s) v[u][s] = sbuf[padx(vt + s * NT)];
+      for (int s = 0; s < 8; ++s) v[u][s] = sbuf[xread_idx<NT>(Ns / 8, vt, s)];
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const int j = vt + m * NT;
@@ -397,7 +424,7 @@ struct Geo {
   static constexpr int TPF = NT < NTB ? NT : NTB;
   static constexpr int G = NTB / TPF;
   static constexpr int V = NT / TPF;
-  static constexpr int SBUF = L;                // exchange entries of one FFT (swizzled, unpadded)
+  static constexpr int SBUF = sbuf_entries(L); // exchange entries of one FFT
 };
 
 // LDS bytes of one (L, NTB) geometry: G exchange buffers, the twiddles, G rotator and G
@@ -795,8 +822,8 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
   using P = Plan<BM>;
   const int tid = threadIdx.x;
   const int g = tid / BNT, t = tid - g * BNT;
-  float2* sbuf = smem + g * BM;
-  float2* stw = smem + G * BM;
+  float2* sbuf = smem + g * sbuf_entries(BM);
+  float2* stw = smem + G * sbuf_entries(BM);
   float2* stl = stw + P::TWN;                     // W_L^(64 jh), jh < L/64, then W_L^jl, jl < 64
   float2* srot = stl + BTWL + g * args.phcap;
   float2* sfrac = stl + BTWL + (G + g) * args.phcap;
@@ -1038,7 +1065,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
 template <int NTB>
 size_t fb_lds_bytes(int phcap) {
   constexpr int G = NTB / BNT;
-  return sizeof(float2) * ((size_t)G * BM + Plan<BM>::TWN + BTWL + 2 * (size_t)G * phcap) +
+  return sizeof(float2) * ((size_t)G * sbuf_entries(BM) + Plan<BM>::TWN + BTWL + 2 * (size_t)G * phcap) +
          sizeof(float) * ((size_t)G * phcap + 264) + stage_lds_bytes(phcap);
 }
 
