@@ -53,6 +53,10 @@ class Datastream:
     # delay polynomials of phase centres 1..P, each [nsamples, 6] like `model` (which is then
     # the pointing centre's); only read when the config has more than one phase centre
     phasecentre_models: Optional[List[np.ndarray]] = None
+    # phase-cal tone interval and base (Configuration's phasecalintervalhz, phasecalbasehz):
+    # a nonzero interval extracts the tones every subint into the results' phase-cal block
+    phasecal_interval_hz: int = 0
+    phasecal_base_hz: int = 0
 
     @property
     def framebytes(self) -> int:
@@ -176,6 +180,8 @@ class Config:
                 assert pcm.shape == (self.nphasecentres,) + m.shape
                 keep.append(pcm)
                 x.phasecentre_delay_us = pcm.ctypes.data_as(C.POINTER(C.c_double))
+            x.phasecal_interval_hz = int(d.phasecal_interval_hz)
+            x.phasecal_base_hz = int(d.phasecal_base_hz)
         keep.append(dss)
         nb = len(self.baselines)
         bls = (abi.fxg_baseline * max(nb, 1))()

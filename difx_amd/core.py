@@ -48,6 +48,7 @@ def load_library() -> C.CDLL:
     lib.fxg_get_layout.argtypes = [C.c_void_p, P(abi.fxg_layout)]
     lib.fxg_data_bytes.argtypes = [C.c_void_p, C.c_int32, P(C.c_int64)]
     lib.fxg_result_offsets.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_int32)]
+    lib.fxg_result_pcal_offsets.argtypes = [C.c_void_p, P(C.c_int32)]
     lib.fxg_process_subint.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_void_p), P(C.c_int32),
                                        P(C.c_void_p), C.c_void_p]
     lib.fxg_process_subint_device.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_void_p), P(C.c_int32),
@@ -176,6 +177,12 @@ class FxCore:
         self._check(self.lib.fxg_result_offsets(self.h, b.ctypes.data_as(P), w.ctypes.data_as(P),
                                                 a.ctypes.data_as(P), aw.ctypes.data_as(P)))
         return b[: nf * nb].reshape(nf, nb), w[: nf * nb].reshape(nf, nb), a, aw
+
+    def result_pcal_offsets(self) -> np.ndarray:
+        """Configuration::getCoreResultPCalOffset of every datastream (cf32 units)."""
+        p = np.zeros(self.ndatastreams, np.int32)
+        self._check(self.lib.fxg_result_pcal_offsets(self.h, p.ctypes.data_as(C.POINTER(C.c_int32))))
+        return p
 
     # ---- Core::processdata ----
     def processdata(self, offsets: Sequence[int], data: Sequence[np.ndarray],

@@ -42,6 +42,11 @@ int calcstridelength(int n) {
 
 bool is_pow2(long long v) { return v > 0 && (v & (v - 1)) == 0; }
 
+long long gcd_ll(long long a, long long b) {
+  while (b) { const long long t = a % b; a = b; b = t; }
+  return a;
+}
+
 struct Slot {
   hipStream_t copy = nullptr;
   hipEvent_t copied = nullptr, done = nullptr;
@@ -194,6 +199,7 @@ struct fxg_ctx {
     long long offset_hz;           // baseband frequency of tone 0 (-1: no tone in the band)
     long long interval_hz, fs_hz;
     int ntones;
+    int tone0;                     // first entry of the band's tones in the device tone buffer
   };
   std::vector<long long> pcal_interval, pcal_base;   // [ds]; interval 0 = off
   std::vector<PcalTones> pcal_tones;
@@ -205,6 +211,12 @@ struct fxg_ctx {
   float* d_pcal_part = nullptr;    // per-workgroup partial bins
   float* d_pcal_bins = nullptr;
   std::vector<std::array<int, 3>> pcal_runs;   // {form, first group, groups}
+  std::vector<PcalToneDev> pcal_tdev;           // every tone an extractor reads
+  PcalToneDev* d_pcal_tdev = nullptr;
+  float2* d_pcal_tonebuf = nullptr;             // the last subint's tones of every band
+  int pcal_ntone_total = 0;
+  std::vector<long long> cfg_pcal_interval, cfg_pcal_base;   // [ds] of the configuration
+  std::vector<int> pcaloff;                     // [ds] cf32 offset of the results' phase-cal block
   long long pcal_part_len = 0;
   int pcal_max_gsize = 0, pcal_nsplit = 1;
   // streams, staging, ring
@@ -273,6 +285,7 @@ int validate_and_derive(fxg_ctx* ctx) {
     for (int k = 0; k < ds.nrecfreqs; ++k)
       if (ds.recfreq_index[k] < 0 || ds.recfreq_index[k] >= ctx->nfreqs)
         return fail(ctx, FXG_E_INVALID, "recorded freq index out of range");
+    if (ds.phasecal_interval_hz < 0) return fail(ctx, FXG_E_INVALID, "negative phase-cal interval");
     for (int k = 0; k < ds.nzoomfreqs; ++k)
       if (ds.zoomfreq_index[k] < 0 || ds.zoomfreq_index[k] >= ctx->nfreqs)
         return fail(ctx, FXG_E_INVALID, "zoom freq index out of range");
@@ -464,6 +477,29 @@ void band_source(const fxg_datastream& ds, int bi, int* rec, int* choff) {
 
 // Configuration::populateResultLengths for one config, one phase centre, no pulsar bins
 // (configuration.cpp:2348-2630) with the used/equivalent freq flags of :2270-2320.
+// The tones of one recorded freq (Configuration's phase-cal table, configuration.cpp:1768-1835,
+// for a whole-Hz interval): the first tone inside the band away from the LO edge, their count,
+// and the baseband offset of that tone.
+static void pcal_tones_of(double lofreq, double bw, bool lsb, long long interval, long long base, int* ntones,
+                          long long* offset) {
+  const double stepMHz = interval * 1e-6;
+  long harmonic = (long)((lofreq - base * 1e-6) / stepMHz);
+  double tonefreq = harmonic * stepMHz + base * 1e-6;
+  int n = 0;
+  if (lsb) {
+    if (tonefreq >= lofreq) { tonefreq -= stepMHz; harmonic -= 1; }
+    if (tonefreq >= lofreq - bw)
+      while (tonefreq - n * stepMHz >= lofreq - bw) n++;
+    *offset = n > 0 ? (long long)(1e6 * lofreq) - (long long)harmonic * interval : -1;
+  } else {
+    if (tonefreq <= lofreq) { tonefreq += stepMHz; harmonic += 1; }
+    if (tonefreq <= lofreq + bw)
+      while (tonefreq + n * stepMHz <= lofreq + bw) n++;
+    *offset = n > 0 ? (long long)harmonic * interval - (long long)(1e6 * lofreq) : -1;
+  }
+  *ntones = n > 0 ? n : 1;                         // one placeholder tone when none fits
+}
+
 void build_layout(fxg_ctx* ctx) {
   const int nf = ctx->nfreqs, nb = ctx->nbl;
   ctx->used.assign(nf, 0);
@@ -540,6 +576,23 @@ void build_layout(fxg_ctx* ctx) {
     }
     toadd = (toadd + 1) / 2;
     idx += std::max(toadd, 1);
+  }
+  // the phase-cal tones of datastreams with a phase-cal interval, per recorded freq its pols
+  // times its tones (configuration.cpp:2611-2619)
+  ctx->pcaloff.assign(ctx->nds, 0);
+  for (int d = 0; d < ctx->nds; ++d) {
+    const fxg_datastream& ds = ctx->dss[d];
+    ctx->pcaloff[d] = (int)idx;
+    if (ctx->cfg_pcal_interval[d] <= 0) continue;
+    for (int j = 0; j < ds.nrecfreqs; ++j) {
+      int npol = 0;
+      for (int b = 0; b < ds.nrecbands; ++b) npol += ds.recband_localfreq[b] == j;
+      const fxg_freq& f = ctx->freqs[ds.recfreq_index[j]];
+      int nt;
+      long long off;
+      pcal_tones_of(f.freq_mhz, f.bw_mhz, f.lsb != 0, ctx->cfg_pcal_interval[d], ctx->cfg_pcal_base[d], &nt, &off);
+      idx += (long long)npol * nt;
+    }
   }
   ctx->coreresultlength = idx;
 }
@@ -657,6 +710,31 @@ int build_norm(fxg_ctx* ctx) {
           widx++;
         }
       }
+    }
+  }
+  for (int d = 0; d < nds; ++d) {                  // phase-cal tones (visibility.cpp:650-675)
+    if (ctx->cfg_pcal_interval[d] <= 0) continue;
+    const fxg_datastream& s = ctx->dss[d];
+    int idx = ctx->pcaloff[d];
+    for (int j = 0; j < s.nrecbands; ++j) {
+      const fxg_freq& fr = ctx->freqs[s.recfreq_index[s.recband_localfreq[j]]];
+      int nt;
+      long long off;
+      pcal_tones_of(fr.freq_mhz, fr.bw_mhz, fr.lsb != 0, ctx->cfg_pcal_interval[d], ctx->cfg_pcal_base[d], &nt, &off);
+      NormSeg g{};
+      g.kind = 2;
+      g.decf = -1;
+      g.off = idx;
+      g.nout = nt;
+      g.wf = ctx->acw_of[(size_t)d * MB + j];     // the band's parallel autocorrelation weight, -1: none
+      g.pcz = s.nzoombands > 0;                    // numtotalbands > numrecordedbands
+      g.ds1 = g.ds2 = d;
+      g.cal1 = g.cal2 = zero;
+      g.aw1 = g.aw2 = -1;
+      g.zs1 = g.zs2 = 1.0f;
+      g.chanfactor = (float)(B * 2 * (fr.nchan / fr.chanavg) * fr.chanavg);
+      ctx->normseg.push_back(g);
+      idx += nt;
     }
   }
   if (int rc = dev_upload(ctx, &ctx->d_normseg, ctx->normseg)) return rc;
@@ -1693,6 +1771,10 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
     TimedLaunch tp(ctx, 2, st);
     if (fxg_launch_pcal_reduce(pa, npg, ctx->pcal_max_gsize, st))
       return fail(ctx, FXG_E_HIP, "phase-cal reduce launch");
+    // the tones (getFinalPCal), also added into the results' phase-cal block
+    if (fxg_launch_pcal_tones(ctx->d_pcal_tdev, (int)ctx->pcal_tdev.size(), ctx->d_pcal_bins, ctx->d_pcal_tonebuf,
+                              d_results, st))
+      return fail(ctx, FXG_E_HIP, "phase-cal tone launch");
   }
   if (!uvs)
     if (int rc = accumulate(uv)) return rc;
@@ -1793,7 +1875,7 @@ void free_ctx(fxg_ctx* ctx) {
   f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   f(ctx->d_normseg); f(ctx->d_normcal); f(ctx->d_calmean); f(ctx->d_tsys); f(ctx->d_dpct);
-  f(ctx->d_kurt_tab); f(ctx->d_kurt_acc); f(ctx->d_pcal_groups); f(ctx->d_pcal_part); f(ctx->d_pcal_bins);
+  f(ctx->d_kurt_tab); f(ctx->d_kurt_acc); f(ctx->d_pcal_groups); f(ctx->d_pcal_part); f(ctx->d_pcal_bins); f(ctx->d_pcal_tdev); f(ctx->d_pcal_tonebuf);
   f(ctx->d_uvfreq); f(ctx->d_uvdec_idx);
   for (auto& sg : ctx->stage) {
     f(sg.d_uvdd); f(sg.d_uvdec);
@@ -1861,6 +1943,179 @@ namespace {
 
 // fxg_create, or with dry set a pass that validates and sizes everything without touching
 // the device (fxg_footprint).
+// Phase-cal extraction of datastream ds with a whole-Hz interval (0: off), and the tables of
+// every datastream with it on: per recorded band the tone table (Configuration,
+// configuration.cpp:1765-1835), the rational frequency of each tone the reference's extractor
+// reads (PCal::getNew and the extractors' getFinalPCal, pcal.cpp:191-274), the fold period,
+// the groups of sample fields the device folds together and the device tone table.
+int pcal_configure(fxg_ctx* ctx, int ds, long long interval_hz, long long base_hz) {
+  const fxg_datastream& xs = ctx->dss[ds];
+  if (interval_hz > 0) {
+    if (xs.nbits != 2 && xs.nbits != 8)
+      return fail(ctx, FXG_E_INVALID, "phase-cal extraction needs 2-bit or 8-bit samples");
+    if (xs.complex_sampling && xs.nbits != 8)
+      return fail(ctx, FXG_E_INVALID, "phase-cal extraction of complex data needs 8-bit samples");
+    if (xs.framebytes % 4 || xs.headerbytes % 4)
+      return fail(ctx, FXG_E_INVALID, "phase-cal extraction needs 4-byte aligned frame payloads");
+  }
+  ctx->pcal_interval.resize(ctx->nds, 0);
+  ctx->pcal_base.resize(ctx->nds, 0);
+  const long long prev_interval = ctx->pcal_interval[ds], prev_base = ctx->pcal_base[ds];
+  ctx->pcal_interval[ds] = interval_hz;
+  ctx->pcal_base[ds] = base_hz;
+  auto refuse = [&](const char* why) {             // keep the previous setting and tables
+    ctx->pcal_interval[ds] = prev_interval;
+    ctx->pcal_base[ds] = prev_base;
+    return fail(ctx, FXG_E_INVALID, why);
+  };
+  std::vector<fxg_ctx::PcalTones> tones;
+  std::vector<fxg_ctx::PcalBins> tab;
+  std::vector<int> entry;
+  std::vector<PcalGroup> groups;
+  std::vector<PcalToneDev> tdev;
+  long long off = 0;
+  int ntone_total = 0;
+  for (int d = 0; d < ctx->nds; ++d) {
+    if (ctx->pcal_interval[d] <= 0) continue;
+    const fxg_datastream& x = ctx->dss[d];
+    const bool cplx = x.complex_sampling != 0, dsb = x.complex_sampling == 2;
+    const int nf = cplx ? 2 : 1;                  // sample fields per band
+    int res = ctx->cfg_pcal_interval[d] > 0 ? ctx->pcaloff[d] : -1;   // Core::copyPCalTones order
+    for (int b = 0; b < x.nrecbands; ++b) {
+      const fxg_freq& f = ctx->freqs[x.recfreq_index[x.recband_localfreq[b]]];
+      fxg_ctx::PcalTones t{};
+      t.ds = d;
+      t.band = b;
+      t.interval_hz = ctx->pcal_interval[d];
+      t.fs_hz = (long long)std::llround((cplx ? 1e6 : 2e6) * f.bw_mhz);
+      pcal_tones_of(f.freq_mhz, f.bw_mhz, f.lsb != 0, t.interval_hz, ctx->pcal_base[d], &t.ntones, &t.offset_hz);
+      t.tone0 = ntone_total;
+      ntone_total += t.ntones;
+      // the tones the extractor reads, as reduced fractions num/den of cycles per sample
+      std::vector<std::pair<long long, long long>> fr((size_t)t.ntones, {-1, -1});
+      auto red = [](long long num, long long den) {
+        num %= den;
+        const long long g = gcd_ll(num, den);
+        return std::make_pair(num / g, den / g);
+      };
+      const long long fs = t.fs_hz, iv = t.interval_hz, of = t.offset_hz;
+      if (of >= 0) {
+        if (!cplx) {                              // trivial, implicit-shift and shifting extractors
+          const long long first = of == 0 ? iv : of;
+          for (int k = 0; k < t.ntones; ++k) fr[k] = red(first + k * iv, fs);
+        } else {
+          const long long no = fs / gcd_ll(fs, of), np = fs / gcd_ll(fs, iv);
+          const bool implicit = no % np == 0;     // PCalExtractorImplicitShiftComplex, else Shifting
+          long long n = implicit ? no : np;       // DFT length: doubled to 1 MHz resolution (mode.cpp:402)
+          while (fs / (2 * n) > 1000000) n *= 2;
+          const long long step = (long long)std::floor((double)n * (double)iv / (double)fs);
+          const long long obin = implicit ? (long long)std::floor((double)n * (double)of / (double)fs) : 0;
+          for (int k = 0; k < t.ntones; ++k) {
+            const long long bin = obin + (long long)k * step;
+            if (bin >= n) break;                  // the extractor stops there: the tone stays zero
+            const long long bb = dsb ? n - 1 - bin : bin;   // DSB: the DFT output reversed
+            fr[k] = implicit ? red(bb, n) : red(of * n + bb * fs, fs * n);
+          }
+        }
+      }
+      long long M = 0;                            // fold period
+      if (!cplx) {
+        if (of >= 0) M = fs / std::max<long long>(gcd_ll(fs, gcd_ll(of, iv)), 1);   // offset 0: gcd = interval
+      } else {
+        for (const auto& q : fr)
+          if (q.second > 0) {
+            M = M == 0 ? q.second : M / gcd_ll(M, q.second) * q.second;
+            if (M > 16384) break;
+          }
+      }
+      int e = -1;
+      if (M > 0) {
+        if (M > 16384) return refuse("phase-cal tone period longer than 16384 samples");
+        for (int fd = 0; fd < nf; ++fd) {         // fields of the band: real (and imaginary)
+          const int field = b * nf + fd;
+          // extend the open group (same datastream, the previous field, same M, room in LDS)
+          PcalGroup* gr = groups.empty() ? nullptr : &groups.back();
+          if (!(gr && gr->ds == d && gr->band0 + gr->nb == field && gr->M == M && gr->nb < PCAL_MAXNB &&
+                (long long)(gr->nb + 1) * M <= PCAL_MAX_GSIZE)) {
+            PcalGroup ng{};
+            ng.ds = d;
+            ng.band0 = field;
+            ng.nb = 0;
+            ng.M = (int)M;
+            ng.bins_off = (int)off;
+            ng.fps = x.nrecbands * nf;
+            groups.push_back(ng);
+            gr = &groups.back();
+          }
+          gr->nb += 1;
+          if (fd == 0) {
+            e = (int)tab.size();
+            tab.push_back({(int)M, (int)off});
+          }
+          off += M;
+        }
+        for (int k = 0; k < t.ntones; ++k)
+          if (fr[k].second > 0)
+            tdev.push_back(PcalToneDev{tab[e].bins_off, (int)M, cplx ? 1 : 0, (int)fr[k].first, (int)fr[k].second,
+                                       t.tone0 + k, res >= 0 ? res + k : -1});
+      }
+      if (res >= 0) res += t.ntones;
+      entry.push_back(e);
+      tones.push_back(t);
+    }
+  }
+  // kernel form of each group, and the groups ordered by form (one launch per form)
+  for (auto& g : groups) {
+    const fxg_datastream& x = ctx->dss[g.ds];
+    g.form = (256 % g.M == 0 ? PCAL_FORM_REG : 0) | (x.nbits == 8 ? PCAL_FORM_8BIT : 0) |
+             (g.fps * x.nbits == 32 && g.band0 == 0 && g.nb == g.fps ? PCAL_FORM_WHOLE : 0);
+  }
+  std::stable_sort(groups.begin(), groups.end(),
+                   [](const PcalGroup& p, const PcalGroup& q) { return p.form < q.form; });
+  std::vector<std::array<int, 3>> runs;
+  for (int i = 0; i < (int)groups.size(); ++i) {
+    if (runs.empty() || runs.back()[0] != groups[i].form) runs.push_back({groups[i].form, i, 0});
+    runs.back()[2] += 1;
+  }
+  // workgroups per group: about eight per CU over all groups, at most one per FFT of a chunk
+  const int ng = (int)groups.size();
+  const int nsplit = ng ? std::max(1, std::min(ctx->chunkcap, (8 * ctx->ncu + ng - 1) / ng)) : 1;
+  long long part = 0;
+  int max_gsize = 0;
+  for (auto& g : groups) {
+    g.part_off = part;
+    part += (long long)nsplit * g.nb * g.M;
+    max_gsize = std::max(max_gsize, g.nb * g.M);
+  }
+  if (ctx->d_pcal_groups) { (void)hipFree(ctx->d_pcal_groups); ctx->d_pcal_groups = nullptr; }
+  if (ctx->d_pcal_part) { (void)hipFree(ctx->d_pcal_part); ctx->d_pcal_part = nullptr; }
+  if (ctx->d_pcal_bins) { (void)hipFree(ctx->d_pcal_bins); ctx->d_pcal_bins = nullptr; }
+  if (ctx->d_pcal_tdev) { (void)hipFree(ctx->d_pcal_tdev); ctx->d_pcal_tdev = nullptr; }
+  if (ctx->d_pcal_tonebuf) { (void)hipFree(ctx->d_pcal_tonebuf); ctx->d_pcal_tonebuf = nullptr; }
+  ctx->pcal_tones = tones;
+  ctx->pcal_tab = tab;
+  ctx->pcal_entry = entry;
+  ctx->pcal_groups = groups;
+  ctx->pcal_nsplit = nsplit;
+  ctx->pcal_runs = runs;
+  ctx->pcal_part_len = part;
+  ctx->pcal_max_gsize = max_gsize;
+  ctx->pcal_tdev = tdev;
+  ctx->pcal_ntone_total = ntone_total;
+  if (!ctx->dry && ntone_total > 0) {             // a host-only context keeps the tables only
+    if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_tonebuf, sizeof(float2) * ntone_total)) return rc;
+    if (int rc = dev_zero(ctx, ctx->d_pcal_tonebuf, sizeof(float2) * ntone_total)) return rc;
+  }
+  if (!groups.empty() && !ctx->dry) {
+    if (int rc = dev_upload(ctx, &ctx->d_pcal_groups, ctx->pcal_groups)) return rc;
+    if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_part, sizeof(float) * part)) return rc;
+    if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_bins, sizeof(float) * off)) return rc;
+    if (int rc = dev_zero(ctx, ctx->d_pcal_bins, sizeof(float) * off)) return rc;
+    if (int rc = dev_upload(ctx, &ctx->d_pcal_tdev, ctx->pcal_tdev)) return rc;
+  }
+  return FXG_OK;
+}
+
 int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
   if (!cfg || !out) return FXG_E_INVALID;
   *out = nullptr;
@@ -1886,6 +2141,12 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
     if (s.model_delay_us && s.model_nsamples > 0)
       ctx->models[d].assign(s.model_delay_us, s.model_delay_us + (size_t)s.model_nsamples * FXG_MAX_POLY);
     ctx->dss[d].model_delay_us = ctx->models[d].empty() ? nullptr : ctx->models[d].data();
+  }
+  ctx->cfg_pcal_interval.assign(ctx->nds, 0);
+  ctx->cfg_pcal_base.assign(ctx->nds, 0);
+  for (int d = 0; d < ctx->nds; ++d) {
+    ctx->cfg_pcal_interval[d] = ctx->dss[d].phasecal_interval_hz;
+    ctx->cfg_pcal_base[d] = ctx->dss[d].phasecal_base_hz;
   }
   ctx->xms = cfg->xmac_stride_len;
   ctx->nphase = cfg->nphasecentres > 1 ? cfg->nphasecentres : 1;
@@ -1924,6 +2185,8 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
   if (rc == FXG_OK) rc = build_engines(ctx, cfg->chunk_ffts);
   if (rc == FXG_OK) rc = build_items(ctx);
   if (rc == FXG_OK) rc = build_uvshift(ctx);
+  for (int d = 0; rc == FXG_OK && d < ctx->nds; ++d)    // the configuration's phase-cal extraction
+    if (ctx->cfg_pcal_interval[d] > 0) rc = pcal_configure(ctx, d, ctx->cfg_pcal_interval[d], ctx->cfg_pcal_base[d]);
   if (rc == FXG_OK && !dry) {
     const size_t ndd = std::max<size_t>(ctx->ivals.size() * ctx->nbl * ctx->nphase, 1);
     const size_t ndec = std::max<size_t>(ctx->uvdec_idx.size(), 1);
@@ -2021,11 +2284,6 @@ static int frames_per_second(const fxg_ctx* ctx, int ds) {
   const int payload = d.framebytes - d.headerbytes;
   if (payload < 1) return 0;
   return (int)(samplerate * d.nrecbands * d.nbits / (8.0 * payload));
-}
-
-static long long gcd_ll(long long a, long long b) {
-  while (b) { const long long t = a % b; a = b; b = t; }
-  return a;
 }
 
 // DataStream::updateConfig's per-segment constants (datastream.cpp:745-763), with the
@@ -2375,136 +2633,12 @@ int fxg_get_data_weights(const fxg_ctx* ctx, int32_t ds, float* out) {
   return FXG_OK;
 }
 
-// The tones of one recorded freq (Configuration's phase-cal table, configuration.cpp:1768-1835,
-// for a whole-Hz interval): the first tone inside the band away from the LO edge, their count,
-// and the baseband offset of that tone.
-static void pcal_tones_of(double lofreq, double bw, bool lsb, long long interval, long long base, int* ntones,
-                          long long* offset) {
-  const double stepMHz = interval * 1e-6;
-  long harmonic = (long)((lofreq - base * 1e-6) / stepMHz);
-  double tonefreq = harmonic * stepMHz + base * 1e-6;
-  int n = 0;
-  if (lsb) {
-    if (tonefreq >= lofreq) { tonefreq -= stepMHz; harmonic -= 1; }
-    if (tonefreq >= lofreq - bw)
-      while (tonefreq - n * stepMHz >= lofreq - bw) n++;
-    *offset = n > 0 ? (long long)(1e6 * lofreq) - (long long)harmonic * interval : -1;
-  } else {
-    if (tonefreq <= lofreq) { tonefreq += stepMHz; harmonic += 1; }
-    if (tonefreq <= lofreq + bw)
-      while (tonefreq + n * stepMHz <= lofreq + bw) n++;
-    *offset = n > 0 ? (long long)harmonic * interval - (long long)(1e6 * lofreq) : -1;
-  }
-  *ntones = n > 0 ? n : 1;                         // one placeholder tone when none fits
-}
-
 int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz) {
   if (!ctx || ds < 0 || ds >= ctx->nds || interval_hz < 0) return FXG_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mu);
-  const fxg_datastream& xs = ctx->dss[ds];
-  if (interval_hz > 0) {
-    if (xs.complex_sampling)
-      return fail(ctx, FXG_E_INVALID, "phase-cal extraction of complex-sampled data is not supported");
-    if (xs.nbits != 2 && xs.nbits != 8)
-      return fail(ctx, FXG_E_INVALID, "phase-cal extraction needs 2-bit or 8-bit samples");
-    if (xs.framebytes % 4 || xs.headerbytes % 4)
-      return fail(ctx, FXG_E_INVALID, "phase-cal extraction needs 4-byte aligned frame payloads");
-  }
-  ctx->pcal_interval.resize(ctx->nds, 0);
-  ctx->pcal_base.resize(ctx->nds, 0);
-  const long long prev_interval = ctx->pcal_interval[ds], prev_base = ctx->pcal_base[ds];
-  ctx->pcal_interval[ds] = interval_hz;
-  ctx->pcal_base[ds] = base_hz;
-  // rebuild the tone table, the band groups and the device bins for every datastream with
-  // phase-cal on
-  std::vector<fxg_ctx::PcalTones> tones;
-  std::vector<fxg_ctx::PcalBins> tab;
-  std::vector<int> entry;
-  std::vector<PcalGroup> groups;
-  long long off = 0;
-  for (int d = 0; d < ctx->nds; ++d) {
-    if (ctx->pcal_interval[d] <= 0) continue;
-    const fxg_datastream& x = ctx->dss[d];
-    for (int b = 0; b < x.nrecbands; ++b) {
-      const fxg_freq& f = ctx->freqs[x.recfreq_index[x.recband_localfreq[b]]];
-      fxg_ctx::PcalTones t{};
-      t.ds = d;
-      t.band = b;
-      t.interval_hz = ctx->pcal_interval[d];
-      t.fs_hz = (long long)std::llround(2e6 * f.bw_mhz);
-      pcal_tones_of(f.freq_mhz, f.bw_mhz, f.lsb != 0, t.interval_hz, ctx->pcal_base[d], &t.ntones, &t.offset_hz);
-      int e = -1;
-      if (t.offset_hz >= 0) {
-        const long long g = gcd_ll(t.fs_hz, gcd_ll(t.offset_hz, t.interval_hz));   // offset 0: gcd = interval
-        const long long M = t.fs_hz / std::max<long long>(g, 1);
-        if (M > 16384) {                         // keep the previous setting and tables
-          ctx->pcal_interval[ds] = prev_interval;
-          ctx->pcal_base[ds] = prev_base;
-          return fail(ctx, FXG_E_INVALID, "phase-cal tone period longer than 16384 samples");
-        }
-        // extend the open group (same datastream, the previous band, same M, room in LDS)
-        PcalGroup* gr = groups.empty() ? nullptr : &groups.back();
-        if (!(gr && gr->ds == d && gr->band0 + gr->nb == b && gr->M == M && gr->nb < PCAL_MAXNB &&
-              (long long)(gr->nb + 1) * M <= PCAL_MAX_GSIZE)) {
-          PcalGroup ng{};
-          ng.ds = d;
-          ng.band0 = b;
-          ng.nb = 0;
-          ng.M = (int)M;
-          ng.bins_off = (int)off;
-          groups.push_back(ng);
-          gr = &groups.back();
-        }
-        gr->nb += 1;
-        e = (int)tab.size();
-        tab.push_back({(int)M, (int)off});
-        off += M;
-      }
-      entry.push_back(e);
-      tones.push_back(t);
-    }
-  }
-  // kernel form of each group, and the groups ordered by form (one launch per form)
-  for (auto& g : groups) {
-    const fxg_datastream& x = ctx->dss[g.ds];
-    g.form = (256 % g.M == 0 ? PCAL_FORM_REG : 0) | (x.nbits == 8 ? PCAL_FORM_8BIT : 0) |
-             (x.nrecbands * x.nbits == 32 && g.band0 == 0 && g.nb == x.nrecbands ? PCAL_FORM_WHOLE : 0);
-  }
-  std::stable_sort(groups.begin(), groups.end(),
-                   [](const PcalGroup& p, const PcalGroup& q) { return p.form < q.form; });
-  std::vector<std::array<int, 3>> runs;
-  for (int i = 0; i < (int)groups.size(); ++i) {
-    if (runs.empty() || runs.back()[0] != groups[i].form) runs.push_back({groups[i].form, i, 0});
-    runs.back()[2] += 1;
-  }
-  // workgroups per group: about eight per CU over all groups, at most one per FFT of a chunk
-  const int ng = (int)groups.size();
-  const int nsplit = ng ? std::max(1, std::min(ctx->chunkcap, (8 * ctx->ncu + ng - 1) / ng)) : 1;
-  long long part = 0;
-  int max_gsize = 0;
-  for (auto& g : groups) {
-    g.part_off = part;
-    part += (long long)nsplit * g.nb * g.M;
-    max_gsize = std::max(max_gsize, g.nb * g.M);
-  }
-  if (ctx->d_pcal_groups) { (void)hipFree(ctx->d_pcal_groups); ctx->d_pcal_groups = nullptr; }
-  if (ctx->d_pcal_part) { (void)hipFree(ctx->d_pcal_part); ctx->d_pcal_part = nullptr; }
-  if (ctx->d_pcal_bins) { (void)hipFree(ctx->d_pcal_bins); ctx->d_pcal_bins = nullptr; }
-  ctx->pcal_tones = tones;
-  ctx->pcal_tab = tab;
-  ctx->pcal_entry = entry;
-  ctx->pcal_groups = groups;
-  ctx->pcal_nsplit = nsplit;
-  ctx->pcal_runs = runs;
-  ctx->pcal_part_len = part;
-  ctx->pcal_max_gsize = max_gsize;
-  if (!groups.empty() && !ctx->dry) {              // a host-only context keeps the tables only
-    if (int rc = dev_upload(ctx, &ctx->d_pcal_groups, ctx->pcal_groups)) return rc;
-    if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_part, sizeof(float) * part)) return rc;
-    if (int rc = dev_malloc(ctx, (void**)&ctx->d_pcal_bins, sizeof(float) * off)) return rc;
-    if (int rc = dev_zero(ctx, ctx->d_pcal_bins, sizeof(float) * off)) return rc;
-  }
-  return FXG_OK;
+  if (ctx->cfg_pcal_interval[ds] > 0 && (interval_hz != ctx->cfg_pcal_interval[ds] || base_hz != ctx->cfg_pcal_base[ds]))
+    return fail(ctx, FXG_E_INVALID, "the configuration's phase-cal block fixes this datastream's phase-cal setting");
+  return pcal_configure(ctx, ds, interval_hz, base_hz);
 }
 
 int fxg_pcal_info(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t* ntones, int64_t* offset_hz,
@@ -2521,9 +2655,9 @@ int fxg_pcal_info(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t* ntones,
   return FXG_E_STATE;                              // phase-cal not on for this band
 }
 
-// The extractors' getFinalPCal tones for the last subint (pcal.cpp:550-580, 746-790,
-// 1171-1215): sum over the folded bins j of bins[j] exp(-2 pi i (offset + k interval) j / fs),
-// formed in double
+// The extractors' getFinalPCal tones for the last subint, as k_pcal_tones formed them; a flagged
+// offset (-1 for no tone in the band, or negative through base_hz) leaves them zero, as the
+// reference's dummy extractor does.
 int fxg_get_pcal(const fxg_ctx* ctx, int32_t ds, int32_t band, float* out, int32_t* ntones) {
   if (ctx && ctx->dry) return FXG_E_STATE;          // host-only context
   if (!ctx || !out || !ntones || ds < 0 || ds >= ctx->nds) return FXG_E_INVALID;
@@ -2532,34 +2666,16 @@ int fxg_get_pcal(const fxg_ctx* ctx, int32_t ds, int32_t band, float* out, int32
     if (ctx->pcal_tones[i].ds == ds && ctx->pcal_tones[i].band == band) t = (int)i;
   if (t < 0) return FXG_E_STATE;                   // phase-cal not on for this band
   const fxg_ctx::PcalTones& pt = ctx->pcal_tones[t];
-  const int e = ctx->pcal_entry[t];
-  if (e < 0) {                                     // offset flagged (-1 for no tone in the band,
-    for (int k = 0; k < 2 * pt.ntones; ++k)        // or negative through base_hz): the reference's
-      out[k] = 0.0f;                               // dummy extractor leaves the tones at zero
-    *ntones = pt.ntones;
-    return FXG_OK;
-  }
-  const fxg_ctx::PcalBins& pb = ctx->pcal_tab[e];
-  std::vector<float> bins((size_t)pb.M);
   if (hipDeviceSynchronize() != hipSuccess) return FXG_E_HIP;
-  if (hipMemcpy(bins.data(), ctx->d_pcal_bins + pb.bins_off, sizeof(float) * pb.M, hipMemcpyDeviceToHost) != hipSuccess)
+  if (hipMemcpy(out, ctx->d_pcal_tonebuf + pt.tone0, sizeof(float2) * pt.ntones, hipMemcpyDeviceToHost) != hipSuccess)
     return FXG_E_HIP;
-  for (int k = 0; k < pt.ntones; ++k) {
-    // a zero offset leaves the DC bin out: the tones are then interval, 2 interval, ...
-    // (PCalExtractorTrivial::getFinalPCal, pcal.cpp:573-580)
-    const long long f = pt.offset_hz + (long long)(pt.offset_hz == 0 ? k + 1 : k) * pt.interval_hz;
-    double re = 0.0, im = 0.0;
-    for (int j = 0; j < pb.M; ++j) {
-      // phase (f j / fs) mod 1, exact in integers: f * j mod fs
-      const long long r = f * j % pt.fs_hz;
-      const double ph = -2.0 * M_PI * (double)r / (double)pt.fs_hz;
-      re += bins[j] * std::cos(ph);
-      im += bins[j] * std::sin(ph);
-    }
-    out[2 * k] = (float)re;
-    out[2 * k + 1] = (float)im;
-  }
   *ntones = pt.ntones;
+  return FXG_OK;
+}
+
+int fxg_result_pcal_offsets(const fxg_ctx* ctx, int32_t* pcal) {
+  if (!ctx || !pcal) return FXG_E_INVALID;
+  for (int d = 0; d < ctx->nds; ++d) pcal[d] = ctx->pcaloff[d];
   return FXG_OK;
 }
 

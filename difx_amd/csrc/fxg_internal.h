@@ -283,7 +283,7 @@ struct NormSeg {
   int off;            // cf32 offset of the first channel in the results
   int nout;           // channels
   int wf;             // float index of the segment's weight in the results
-  int kind;           // 0 cross, 1 autocorrelation
+  int kind;           // 0 cross, 1 autocorrelation, 2 phase-cal tones
   int ds1, ds2;       // datastreams (autos: ds2 = ds1)
   int cal1, cal2;     // Tsys: band-mean indices (cross: parallel autos of each band; autos: k and
                       // its matching band); -1 when that datastream has no Tsys
@@ -292,6 +292,8 @@ struct NormSeg {
   float chanfactor;   // (float)(blockspersend * 2 * freqchannels * chanavg)
   int decf;           // several phase centres: float index of this segment's shift decorrelation
                       // sum, the scale divided by it over the integration time; -1 otherwise
+  int pcz;            // kind 2 (phase-cal tones of a band): a zero autocorrelation weight counts
+                      // as 1 (the datastream has zoom bands, visibility.cpp:665-667)
 };
 // one band mean of a parallel autocorrelation (the Tsys calibration's autocorrcalibs)
 struct NormCal {
@@ -340,11 +342,20 @@ constexpr int PCAL_MAX_GSIZE = 32768;    // nb M floats per group (LDS)
 constexpr int PCAL_FORM_REG = 1;         // 256 is a multiple of M: register sums
 constexpr int PCAL_FORM_WHOLE = 2;       // the group is every band of a one-time-sample word
 constexpr int PCAL_FORM_8BIT = 4;        // 8-bit samples (else 2-bit)
+// A run of consecutive sample fields of one datastream folded with one period M: the fields
+// of a time sample are its recorded bands, two per band (real, imaginary) for complex data.
 struct PcalGroup {
-  int ds, band0, nb, M;
+  int ds, band0, nb, M;                  // first field, fields, period
   int bins_off;
   int form;                              // PCAL_FORM_* bits
   long long part_off;
+  int fps;                               // fields per time sample
+};
+// One extracted tone: sum over the M folded bins j (real, or real and imaginary fields) of
+// bin[j] exp(-2 pi i ((num j) mod den) / den); written to the tone buffer at `out` and added
+// to the results at cf32 index `res` (-1: no results block).
+struct PcalToneDev {
+  int bins_off, M, cplx, num, den, out, res;
 };
 struct PcalArgs {
   const DevDs* ds;
@@ -359,3 +370,5 @@ struct PcalArgs {
 size_t fxg_pcal_lds(int max_gsize);
 int fxg_launch_pcal(const PcalArgs& a, int form, int ngroups, int max_gsize, hipStream_t st);
 int fxg_launch_pcal_reduce(const PcalArgs& a, int ngroups, int max_gsize, hipStream_t st);
+int fxg_launch_pcal_tones(const PcalToneDev* t, int n, const float* bins, float2* tones, float2* results,
+                          hipStream_t st);

@@ -1,10 +1,12 @@
-// Phase-cal extraction on the device.  Mode::process hands each recorded band's unpacked real
-// samples of every valid FFT to its PCal extractor, numbered from the scan start
-// (mpifxcorr/src/mode.cpp:758-772); every extractor PCal::getNew picks (pcal.cpp:191-274)
-// computes, for tone k at baseband frequency offset + k interval, sum x[m] exp(-2 pi i f m / fs)
-// over those samples.  Here the samples are folded by their sample number modulo the tone
-// period M (M = fs / gcd(fs, offset, interval)); the tones follow from the M bins by an M-point
-// transform when they are read.
+// Phase-cal extraction on the device.  Mode::process hands each recorded band's unpacked
+// samples (real, or complex for complex sampling) of every valid FFT to its PCal extractor,
+// numbered from the scan start (mpifxcorr/src/mode.cpp:758-772); every extractor PCal::getNew
+// picks (pcal.cpp:191-274) computes, per tone, sum x[m] exp(-2 pi i f m) over those samples at
+// a rational frequency f = num / den cycles per sample that depends on the extractor (real:
+// (offset + k interval) / fs; complex: the DFT bin the implicit-shift or shifting complex
+// extractor reads, reversed for double sideband).  Here the samples are folded by their sample
+// number modulo the tone period M (a multiple of every den; complex samples fold their real
+// and imaginary fields separately), and k_pcal_tones forms each tone from the M bins.
 //
 // Work split: a workgroup takes a group of consecutive recorded bands of one datastream that
 // share M, and a contiguous share of the chunk's FFTs.  Thread t takes time samples t, t + 256,
@@ -56,7 +58,7 @@ __global__ void __launch_bounds__(PCAL_NT) k_pcal(PcalArgs a) {
     const long long s0 = (long long)(((double)run.datasec * 1e9 + (double)run.datans) / (ds.sampletime_us * 1e3) + 0.5);
     const uint8_t* __restrict__ fv = a.fvalid + ds.fvalid_off;
     const uint8_t* __restrict__ data = run.data;
-    const int L = ds.L, spf = ds.spf, nrec = ds.nrecbands, nb = g.nb;
+    const int L = ds.L, spf = ds.spf, nrec = g.fps, nb = g.nb;   // nrec: fields per time sample
     const int nsamp = run.nframes * spf;
     const int step = PCAL_NT % M;                  // bin advance per time step (0 when REG)
     float acc[PCAL_MAXNB];
@@ -168,6 +170,42 @@ __global__ void __launch_bounds__(64) k_pcal_reduce(PcalArgs a) {
   }
 }
 
+// One tone per workgroup: the M-point transform of its band's folded bins at the tone's
+// rational frequency, in FP64 with the phase reduced exactly in integers.
+__global__ void __launch_bounds__(256) k_pcal_tones(const PcalToneDev* tt, const float* bins, float2* tones,
+                                                    float2* results) {
+  const PcalToneDev t = tt[blockIdx.x];
+  __shared__ double sre[256], sim[256];
+  double re = 0.0, im = 0.0;
+  const float* br = bins + t.bins_off;
+  const float* bi = br + t.M;
+  for (int j = threadIdx.x; j < t.M; j += 256) {
+    const int r = (int)(((long long)t.num * j) % t.den);
+    double sn, cs;
+    sincospi(-2.0 * (double)r / (double)t.den, &sn, &cs);
+    const double xr = br[j], xi = t.cplx ? (double)bi[j] : 0.0;
+    re += xr * cs - xi * sn;
+    im += xr * sn + xi * cs;
+  }
+  sre[threadIdx.x] = re;
+  sim[threadIdx.x] = im;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) { sre[threadIdx.x] += sre[threadIdx.x + o]; sim[threadIdx.x] += sim[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float2 v = make_float2((float)sre[0], (float)sim[0]);
+    tones[t.out] = v;
+    if (t.res >= 0) {                              // Core::copyPCalTones adds into the slot results
+      float2 r = results[t.res];
+      r.x += v.x;
+      r.y += v.y;
+      results[t.res] = r;
+    }
+  }
+}
+
 }  // namespace
 
 size_t fxg_pcal_lds(int max_gsize) { return sizeof(float) * ((size_t)max_gsize + 288); }
@@ -201,5 +239,12 @@ int fxg_launch_pcal_reduce(const PcalArgs& a, int ngroups, int max_gsize, hipStr
   if (ngroups <= 0) return 0;
   const int gy = std::max(1, std::min(512, (max_gsize + 63) / 64));
   hipLaunchKernelGGL(k_pcal_reduce, dim3(ngroups, gy), dim3(64), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int fxg_launch_pcal_tones(const PcalToneDev* t, int n, const float* bins, float2* tones, float2* results,
+                          hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_pcal_tones, dim3(n), dim3(256), 0, st, t, bins, tones, results);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }

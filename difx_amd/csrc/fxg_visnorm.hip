@@ -24,6 +24,12 @@ __global__ void __launch_bounds__(256) k_norm_cal(NormArgs a) {
 }
 
 __device__ float seg_scale(const NormArgs& a, const NormSeg& g) {
+  if (g.kind == 2) {                                             // phase-cal tones, visibility.cpp:650-675
+    float acw = g.wf >= 0 ? (float)(a.res[g.wf] / a.fftsperint) : 0.0f;
+    if (acw == 0.0f && g.pcz) acw = 1.0f;
+    if (!(acw > 0.0f)) return 0.0f;                              // left unscaled
+    return (float)(1.0 / ((double)acw * a.msi * (double)g.chanfactor));
+  }
   const float w = (float)(a.res[g.wf] / a.fftsperint);          // baselineweights / autocorrweights
   if (g.kind == 0) {
     const double t1 = a.tsys[g.ds1], t2 = a.tsys[g.ds2];

@@ -71,6 +71,8 @@ class fxg_datastream(C.Structure):
         ("model_nsamples", C.c_int32),
         ("model_delay_us", C.POINTER(C.c_double)),
         ("phasecentre_delay_us", C.POINTER(C.c_double)),
+        ("phasecal_interval_hz", C.c_int64),
+        ("phasecal_base_hz", C.c_int64),
     ]
 
 
@@ -211,7 +213,7 @@ class fxg_difx_meta(C.Structure):
 # exported symbols of libfxg.so (every function declared in include/fxg.h)
 EXPORTED_SYMBOLS = [
     "fxg_create", "fxg_create_layout", "fxg_destroy", "fxg_strerror", "fxg_last_error", "fxg_get_layout",
-    "fxg_data_bytes", "fxg_result_offsets", "fxg_process_subint",
+    "fxg_data_bytes", "fxg_result_offsets", "fxg_result_pcal_offsets", "fxg_process_subint",
     "fxg_process_subint_device", "fxg_ring_length", "fxg_submit_subint", "fxg_wait",
     "fxg_get_spectrum", "fxg_get_data_weights", "fxg_unpack_device",
     "fxg_delay_interpolator", "fxg_set_timing", "fxg_get_timing", "fxg_autotune_report",

@@ -92,6 +92,12 @@ typedef struct fxg_datastream {
   const double* phasecentre_delay_us; /* [nphasecentres][model_nsamples][FXG_MAX_POLY]: phase
                                    centres 1..P (Model source indices 1..P); read only when the
                                    config has nphasecentres > 1, else may be NULL */
+  int64_t phasecal_interval_hz; /* phasecalintervalhz: > 0 extracts the phase-cal tones of every
+                                   recorded band each subint into the results' phase-cal block
+                                   (Mode's extractors, Core::copyPCalTones, core.cpp:1131-1163);
+                                   0 = off (fxg_set_pcal may still turn extraction on, without
+                                   a results block) */
+  int64_t phasecal_base_hz;     /* phasecalbasehz */
 } fxg_datastream;
 
 /* One baseline (Configuration::baselinedata, configuration.h): datastream pair and, per
@@ -173,6 +179,11 @@ int fxg_data_bytes(const fxg_ctx* ctx, int32_t ds, int64_t* bytes);
  * bweight[freq*nbaselines+b] (or -1), autocorr[ds], acweight[ds].  Any pointer may be null. */
 int fxg_result_offsets(const fxg_ctx* ctx, int32_t* baseline, int32_t* bweight,
                        int32_t* autocorr, int32_t* acweight);
+/* Configuration::getCoreResultPCalOffset (configuration.cpp:2611-2619) of every datastream, in
+ * cf32 units: pcal[ds]; the block holds, for datastreams with a phase-cal interval in the
+ * configuration, each recorded band's tones in band order (Core::copyPCalTones,
+ * core.cpp:1131-1163). */
+int fxg_result_pcal_offsets(const fxg_ctx* ctx, int32_t* pcal);
 
 /* Core::processdata for one slot, host buffers (core.cpp:657-1129): offsets = {scan, sec, ns}
  * of the subint; data[ds] = the slot's data buffer of nbytes[ds] bytes; control[ds] =
@@ -223,18 +234,26 @@ int fxg_get_kurtosis(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t maxch
                      int32_t* nout);
 
 /* Phase-cal extraction of datastream ds (Mode's PCal extractors, mode.cpp:401-420, 758-772,
- * 1552-1568; tone table of Configuration, configuration.cpp:1765-1835), real-sampled data with
- * a whole-Hz tone interval.  interval_hz = 0 turns it off.  The tones of each recorded band
- * follow Configuration: the first tone above (USB) or below (LSB) the band edge, counted up to
- * the far edge; tone k sits at baseband frequency offset + k interval with
- * offset = harmonic interval - long(1e6 lofreq) (USB; mirrored for LSB; base_hz moves the comb
- * but, as in the reference, not the offset).  Every subint folds the samples of each valid FFT
- * of the band by their sample number since the scan start modulo the tone period
- * M = fs / gcd(fs, offset, interval) (at most 16384).  fxg_get_pcal gives the last subint's
- * tones, sum over samples x[m] exp(-2 pi i f_k m / fs), as interleaved (re, im) floats;
+ * 1552-1568; tone table of Configuration, configuration.cpp:1765-1835), real-sampled (2- or
+ * 8-bit) or complex-sampled (8-bit) data with a whole-Hz tone interval.  interval_hz = 0 turns
+ * it off.  The tones of each recorded band follow Configuration: the first tone above (USB) or
+ * below (LSB) the band edge, counted up to the far edge; offset = harmonic interval -
+ * long(1e6 lofreq) (USB; mirrored for LSB; base_hz moves the comb but, as in the reference, not
+ * the offset).  Tone k is sum over the valid FFTs' samples x[m] exp(-2 pi i f_k m), m the
+ * sample number since the scan start, at the frequency f_k (cycles per sample) that the
+ * reference's extractor reads: real data (offset + k interval) / fs, fs = 2 bw (the trivial
+ * extractor skips DC: (k + 1) interval / fs for a zero offset); complex data (fs = bw) the DFT
+ * bin of the implicit-shift or shifting complex extractor (pcal.cpp:808-1020, 1419-1590), whose
+ * length doubles to 1 MHz resolution, reversed for double sideband, and zero for a bin past
+ * the DFT.  The device folds the samples by m modulo the tone period M (a multiple of every
+ * tone's denominator, at most 16384; real and imaginary parts separately) and forms the tones
+ * after the subint.  fxg_get_pcal gives the last subint's tones as interleaved (re, im) floats;
  * *ntones = the count.  A flagged offset (a band with no tone: one placeholder tone; or an offset
  * made negative by base_hz) gives zero tones, as the reference's dummy extractor.  FXG_E_STATE
- * when phase-cal is off for the band. */
+ * when phase-cal is off for the band.  A datastream whose configuration sets
+ * phasecal_interval_hz has extraction on from fxg_create, its tones also added every subint
+ * into the results' phase-cal block; fxg_set_pcal with another setting for it is refused
+ * (FXG_E_INVALID), as the block's size depends on it. */
 int fxg_set_pcal(fxg_ctx* ctx, int32_t ds, int64_t interval_hz, int64_t base_hz);
 int fxg_get_pcal(const fxg_ctx* ctx, int32_t ds, int32_t band, float* out, int32_t* ntones);
 /* The tone table entry of a recorded band (Configuration's getDRecordedFreqNumPCalTones and

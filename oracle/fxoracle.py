@@ -217,6 +217,7 @@ class Layout:
     bandsperautocorr: int
     shiftdecorr_off: Dict[Tuple[int, int], int] = None   # (freq, baseline) -> cf32 offset (P > 1)
     nphasecentres: int = 1
+    pcal_off: List[int] = None               # per datastream: cf32 offset of its phase-cal tones
 
 
 def freq_used(cfg) -> Tuple[List[bool], List[bool]]:
@@ -253,7 +254,7 @@ def result_layout(cfg) -> Layout:
     """populateResultLengths (configuration.cpp:2348-2630) for one config, no pulsar bins:
     per used freq and baseline a cross block of P phase centres x products x output channels,
     then the baseline weights, then (P > 1) P shift-decorrelation floats per (freq, baseline),
-    then the autocorrelations and their weights."""
+    then the autocorrelations and their weights, then the phase-cal tones."""
     nf = len(cfg.freqs)
     P = nphasecentres(cfg)
     used, equiv = freq_used(cfg)
@@ -313,7 +314,17 @@ def result_layout(cfg) -> Layout:
                 toadd += bpa
         toadd = (toadd + 1) // 2
         idx += max(toadd, 1)
-    return Layout(idx, boff, bw, ac, acw, bpa, sdo, P)
+    pco = []
+    for d in cfg.datastreams:                # phase-cal tones (:2611-2619): per recorded freq, its
+        pco.append(idx)                      # pols times its tones
+        if d.phasecal_interval_hz > 0:
+            import pcal as pc
+            for lf, fi in enumerate(d.recfreqs):
+                fr = cfg.freqs[fi]
+                npol = sum(1 for lb, _ in d.recbands if lb == lf)
+                nt = pc.tone_table(fr.freq_mhz, fr.bw_mhz, fr.lsb, d.phasecal_interval_hz, d.phasecal_base_hz)[0]
+                idx += npol * nt
+    return Layout(idx, boff, bw, ac, acw, bpa, sdo, P, pco)
 
 
 # --------------------------------------------------------------------------------------
@@ -805,6 +816,16 @@ def process_subint(cfg, offsets, data: Sequence[np.ndarray], control: Sequence[n
                     continue
                 fres[widx] += st.acweight[w, band_record_index(ds, k)]
                 widx += 1
+    # phase-cal tones of every recorded band, in band order (Core::copyPCalTones, core.cpp:1131-1163)
+    for d, ds in enumerate(cfg.datastreams):
+        if ds.phasecal_interval_hz <= 0:
+            continue
+        import pcal as pc
+        tones = pc.extract(cfg, d, offsets, data[d], control[d], ds.phasecal_interval_hz, ds.phasecal_base_hz)
+        o = lay.pcal_off[d]
+        for t in tones:
+            res[o: o + len(t)] += t
+            o += len(t)
     return res, stations
 
 

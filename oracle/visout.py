@@ -39,6 +39,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 
 import fxoracle as fo
+import pcal as pcal_restatement
 
 SYNC_WORD = 0xFF00FF00
 HEADER_BYTES = 74
@@ -171,6 +172,22 @@ def normalise(cfg, res: np.ndarray, lay, meansubints: float, tsys: Optional[Sequ
                     apply(off, fc, sc)
                     off += fc
                     widx += 1
+    for d in range(nds):                                     # phase-cal tones (visibility.cpp:650-675)
+        ds = cfg.datastreams[d]
+        if ds.phasecal_interval_hz <= 0:
+            continue
+        off = lay.pcal_off[d]
+        for k in range(len(ds.recbands)):
+            f = cfg.freqs[band_freq(ds, k)]
+            nt = pcal_restatement.tone_table(f.freq_mhz, f.bw_mhz, f.lsb, ds.phasecal_interval_hz,
+                                             ds.phasecal_base_hz)[0]
+            acw = acw0.get((d, k), F32(0))
+            if acw == 0 and len(ds.zoombands) > 0:               # zoom mode without matching autos
+                acw = F32(1.0)
+            if acw > 0:
+                fc = f.nchan // f.chanavg
+                apply(off, nt, F32(1.0 / (F64(acw) * msi * F64(F32(B * 2 * fc * f.chanavg)))))
+            off += nt
     return out
 
 

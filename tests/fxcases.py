@@ -293,3 +293,43 @@ def pcal_tone_slot(cfg, tones, offsets=(0, 1, 0), noise: float = 0.0, seed: int 
         ctrl.append(synth.control_words(cfg, d, list(offsets), start_ns))
         funcs.append(lambda mm, lv=levels: (np.clip(np.round(128.0 + 3.3 * lv(mm)), 0, 255) - 128.0) / 3.3)
     return data, ctrl, funcs
+
+
+def pcal_complex_tone_slot(cfg, tones, offsets=(0, 1, 0)):
+    """Complex 8-bit slot whose every band carries sum_j A_j exp(i (2 pi f_j m / fs + phi_j)),
+    m the sample number since the scan start and fs = bw, levels round(128 + 3.3 x) for the
+    real and imaginary parts (the 8-bit decode is (level - 128) / 3.3).  tones: [(f_hz, A, phi)]
+    with f_hz whole and possibly negative.  Returns (data, control, decoded) with decoded[d]
+    the function m -> decoded complex value of datastream d's sample number m."""
+    data, ctrl, funcs = [], [], []
+    for d, ds in enumerate(cfg.datastreams):
+        assert ds.nbits == 8 and ds.complex_sampling
+        start_ns = synth.buffer_start_ns(cfg, d, list(offsets))
+        nbytes = synth.slot_bytes(cfg, d, list(offsets), start_ns)
+        nframes = nbytes // ds.framebytes
+        nb = len(ds.recbands)
+        spf = (ds.framebytes - ds.headerbytes) // (2 * nb)
+        dt_ns = cfg.sampletime_us(d) * 1e3
+        s_start = start_ns / dt_ns
+        assert s_start == int(s_start), "buffer start on a sample boundary"
+        fs = int(round(1e9 / dt_ns))
+
+        def signal(m, fs=fs):
+            m = np.asarray(m, dtype=np.int64)
+            x = np.zeros(m.shape, complex)
+            for f, a, ph in tones:
+                x += a * np.exp(1j * (2 * np.pi * ((int(f) * (m % fs)) % fs) / fs + ph))
+            return x
+
+        def lev(x):
+            return np.clip(np.round(128.0 + 3.3 * x), 0, 255)
+
+        m = int(s_start) + np.arange(nframes * spf, dtype=np.int64)
+        x = signal(m)
+        pair = np.stack([lev(x.real), lev(x.imag)], axis=1).astype(np.uint8)      # [samples, (re, im)]
+        payload = np.repeat(pair[:, None, :], nb, axis=1).reshape(-1)           # every band the same
+        frame_ns = int(round(spf * dt_ns))
+        data.append(synth.frame_payloads(payload, ds, int(start_ns // frame_ns)))
+        ctrl.append(synth.control_words(cfg, d, list(offsets), start_ns))
+        funcs.append(lambda mm, sg=signal: ((lev(sg(mm).real) - 128.0) / 3.3) + 1j * ((lev(sg(mm).imag) - 128.0) / 3.3))
+    return data, ctrl, funcs

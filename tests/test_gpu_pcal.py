@@ -7,10 +7,12 @@ against FP64 sums)."""
 import numpy as np
 import pytest
 
+import fxoracle as fo
 import pcal as pc
+import visout as vo
 from difx_amd import config as cf
-from fxcases import (case_cfg0, case_cfg1, case_cfg3, case_mark5b, case_offsets, case_real8, damaged_slot,
-                     pcal_tone_slot, slot)
+from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_dsb_fraclo, case_mark5b, case_offsets,
+                     case_real8, damaged_slot, pcal_complex_tone_slot, pcal_tone_slot, record_errors, slot)
 
 pytestmark = pytest.mark.gpu
 
@@ -146,4 +148,123 @@ def test_uneven_bands_both_fengine_forms(gpu, mode):
         core.set_pcal(d, 1_000_000)
     core.processdata(offsets, data, ctrl)
     check_against_oracle(core, cfg, offsets, data, ctrl, 1_000_000)
+    core.close()
+
+
+def _cfg2_lsb():
+    cfg = case_cfg2(B=16, nstations=4, xengine="fp32")
+    cfg.freqs[0].lsb = True
+    return cfg
+
+
+@pytest.mark.parametrize("name,make,interval", [
+    ("complex 8-bit USB, 16 MHz (implicit-shift extractor)", lambda: case_cfg2(B=16, nstations=4, xengine="fp32"), 1_000_000),
+    ("complex 8-bit LSB, 16 MHz", _cfg2_lsb, 1_000_000),
+    ("complex 8-bit USB, 3 MHz comb, offset 2 MHz (shifting extractor)",
+     lambda: cf.standard_config(4, 1, [1399.0], 16.0, 4096, subint_ns=16 * 256000, nbits=8, complex_sampling=1,
+                                xengine="fp32", seed=3), 3_000_000),
+    ("complex double sideband, USB and LSB freqs, 5 MHz", lambda: case_dsb_fraclo(B=16), 1_000_000),
+])
+def test_complex_pcal_matches_the_oracle(gpu, name, make, interval):
+    """Complex-sampled data: the real and imaginary fields folded separately on the device,
+    the tones at the extractor's rational frequencies (implicit-shift or shifting complex
+    extractor, double-sideband reversal) against the restatement, on a damaged slot and on
+    the next subint."""
+    from difx_amd.core import FxCore
+    print("pcal", name, flush=True)
+    cfg = make()
+    core = FxCore(cfg)
+    for d in range(len(cfg.datastreams)):
+        core.set_pcal(d, interval)
+    offsets = [0, 1, 0]
+    data, ctrl = damaged_slot(cfg, offsets)
+    core.processdata(offsets, data, ctrl)
+    check_against_oracle(core, cfg, offsets, data, ctrl, interval)
+    offsets2 = [0, 1, cfg.subint_ns]
+    data2, ctrl2 = slot(cfg, offsets2, seed=33)
+    core.processdata(offsets2, data2, ctrl2)
+    check_against_oracle(core, cfg, offsets2, data2, ctrl2, interval)
+    core.close()
+
+
+@pytest.mark.parametrize("dsb", [False, True])
+def test_complex_planted_tones(gpu, dsb):
+    """Complex tones planted at +3 and -4 MHz on a 1 MHz comb (16 MHz complex band): the
+    device returns N A exp(i phi) in the tone that reads each frequency (single sideband: the
+    +3 MHz tone; double sideband: the reversed bins, -4 MHz in tone 2 and +3 MHz = -13 MHz in
+    tone 11)."""
+    from difx_amd.core import FxCore
+    print("pcal complex planted tones, dsb", dsb, flush=True)
+    cfg = cf.standard_config(2, 1, [1400.0], 16.0, 256, subint_ns=8 * 16000, nbits=8,
+                             complex_sampling=2 if dsb else 1, max_delay_us=50.0, max_rate_us_per_s=0.01,
+                             seed=22)
+    B, L = cfg.blockspersend(), cfg.fftchannels(0)
+    planted = [(3_000_000, 24.0, 0.4), (-4_000_000, 12.0, -1.0)]   # peak 36 x 3.3 < 127: no clipping
+    offsets = [0, 1, 0]
+    data, ctrl, _ = pcal_complex_tone_slot(cfg, planted, offsets)
+    core = FxCore(cfg)
+    for d in range(2):
+        core.set_pcal(d, 1_000_000)
+    core.processdata(offsets, data, ctrl)
+    where = {3_000_000: 11 if dsb else 2, -4_000_000: 2 if dsb else None}
+    for d in range(2):
+        got = core.getPcal(d, 0)
+        want = pc.extract(cfg, d, offsets, data[d], ctrl[d], 1_000_000)[0]
+        err = float(np.max(np.abs(got - want))) / (B * L * 24.0)
+        print(f"  ds {d}: max err / tone {err:.3g}", flush=True)
+        assert err <= TOL
+        for f, a, ph in planted:                    # within the 8-bit quantisation
+            k = where[f]
+            if k is None:
+                continue
+            assert abs(abs(got[k]) / (B * L * a) - 1) < 0.015
+            assert abs(np.angle(got[k] * np.exp(-1j * ph))) < 0.015
+    core.close()
+
+
+@pytest.mark.parametrize("name,make", [
+    ("2-bit VDIF dual pol, two freqs", lambda: case_cfg1(B=16, nstations=3, nfreq=2)),
+    ("complex double sideband", lambda: case_dsb_fraclo(B=16)),
+])
+def test_results_pcal_block(gpu, name, make):
+    """A phase-cal interval in the configuration: every subint's tones go into the results'
+    phase-cal block (Core::copyPCalTones), the rest of the results unchanged, against the
+    restatement's process_subint; then the block's writedata scaling on the device, bit for bit
+    against the restatement's."""
+    import torch
+    from difx_amd.core import FxCore
+    print("pcal results block:", name, flush=True)
+    cfg = make()
+    for k, ds in enumerate(cfg.datastreams):
+        ds.phasecal_interval_hz = 1_000_000 if k != 1 else 0       # datastream 1 without a comb
+    lay = fo.result_layout(cfg)
+    offsets = [0, 1, 0]
+    data, ctrl = damaged_slot(cfg, offsets)
+    core = FxCore(cfg)
+    assert core.coreresultlength == lay.coreresultlength
+    assert list(core.result_pcal_offsets()) == lay.pcal_off
+    got = core.processdata(offsets, data, ctrl)
+    ref, _ = fo.process_subint(cfg, offsets, data, ctrl)
+    e = record_errors(cfg, lay, got.astype(np.complex128), ref)
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+    worst = 0.0
+    for d, ds in enumerate(cfg.datastreams):
+        o0 = lay.pcal_off[d]
+        o1 = lay.pcal_off[d + 1] if d + 1 < len(cfg.datastreams) else lay.coreresultlength
+        if ds.phasecal_interval_hz <= 0:
+            assert o1 == o0
+            continue
+        g, w = got[o0:o1].astype(np.complex128), ref[o0:o1]
+        assert np.any(w)
+        worst = max(worst, float(np.max(np.abs(g - w)) / np.sqrt(np.mean(np.abs(w) ** 2))))
+    print("  block max |d| / rms", worst, flush=True)
+    assert worst <= TOL
+    # the device scaling of the block against the restatement's, on the device results
+    dres = torch.from_numpy(got.view(np.float32).copy()).to(gpu)
+    core.normalise_device(dres.data_ptr(), 1.0)
+    scaled = dres.cpu().numpy().view(np.complex64)
+    want = vo.normalise(cfg, got, lay, 1.0)
+    np.testing.assert_array_equal(scaled, want)
+    o = lay.pcal_off[0]
+    assert not np.array_equal(scaled[o:o + 4], got[o:o + 4])
     core.close()
