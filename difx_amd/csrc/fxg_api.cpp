@@ -929,6 +929,16 @@ int build_tables(fxg_ctx* ctx) {
 
 long long round_up(long long v, long long m) { return (v + m - 1) / m * m; }
 
+// Channel tile width of the 16-bit spectra layout the F-engine writes and the MFMA X-engine
+// reads (log2): 16 channels where the group has one 32-row tile and a multiple of 16 channels
+// (one FFT then writes 64-byte runs), else 8.
+#ifndef FXG_XTW_DEFAULT
+#define FXG_XTW_DEFAULT 4
+#endif
+int tile_width_log(const DevXGroup& g, int want) {
+  return (want == 4 && g.ntile == 1 && g.C % 16 == 0) ? 4 : 3;
+}
+
 int build_engines(fxg_ctx* ctx, int chunk_req) {
   const int nds = ctx->nds, nb = ctx->nbl;
   const bool mfma = ctx->xengine != FXG_X_FP32;
@@ -995,6 +1005,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
       if (nin > 64) return fail(ctx, FXG_E_INVALID, "MFMA X-engine supports at most 64 inputs per freq");
       ctx->groups[g].nin = nin;
       ctx->groups[g].ntile = (nin + 31) / 32;
+      ctx->groups[g].twlog = tile_width_log(ctx->groups[g], FXG_XTW_DEFAULT);
     }
   }
   // ---- chunk capacity: spectra buffer within a budget ----
@@ -1726,9 +1737,10 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
       while (g0 < ctx->groups.size()) {
         size_t g1 = g0 + 1;
         while (g1 < ctx->groups.size() && ctx->groups[g1].ntile == ctx->groups[g0].ntile &&
-               ctx->groups[g1].C == ctx->groups[g0].C)
+               ctx->groups[g1].C == ctx->groups[g0].C && ctx->groups[g1].twlog == ctx->groups[g0].twlog)
           ++g1;
-        if (fxg_launch_xmfma(xa, (int)(g1 - g0), (int)g0, ctx->groups[g0].ntile, ctx->groups[g0].C, st))
+        if (fxg_launch_xmfma(xa, (int)(g1 - g0), (int)g0, ctx->groups[g0].ntile, ctx->groups[g0].C,
+                             ctx->groups[g0].twlog, st))
           return fail(ctx, FXG_E_HIP, "MFMA X-engine launch");
         g0 = g1;
       }
@@ -1855,9 +1867,45 @@ int autotune(fxg_ctx* ctx) {
     ctx->tune_log.emplace_back(c.first, ok ? ms : -1.0);
     if (ok && ms < best) { best = ms; bestc = c.second; }
   }
-  ctx->timing = timing;
   ctx->fsel = bestc;
   if (best >= 1e30 && rc == FXG_OK) rc = fail(ctx, FXG_E_HIP, "autotune: no F-engine launch form ran");
+  // the MFMA X-engine's channel tile (8 or 16 channels per workgroup, which is also the tile
+  // width of the spectra layout the F-engine writes) for groups of one 32-row tile, timed with
+  // the chosen F-engine form
+  bool xtile = false;
+  for (const auto& g : ctx->groups) xtile |= tile_width_log(g, 4) == 4;
+  if (rc == FXG_OK && xtile && ctx->xengine != FXG_X_FP32 && !ctx->d_kurt_tab) {
+    double xbest = 1e30;
+    int xw = 3;
+    for (int want : {3, 4}) {
+      for (auto& g : ctx->groups) g.twlog = tile_width_log(g, want);
+      if (hipMemcpy(ctx->d_groups, ctx->groups.data(), sizeof(DevXGroup) * ctx->groups.size(),
+                    hipMemcpyHostToDevice) != hipSuccess) { rc = fail(ctx, FXG_E_HIP, "autotune: group upload"); break; }
+      double ms = 1e30;
+      bool ok = run_subint(ctx, offs, ptr.data(), nb.data(), cptr.data(), d_res, ctx->stream) == FXG_OK;
+      for (int rep = 0; ok && rep < 2; ++rep) {
+        ok = hipEventRecord(e0, ctx->stream) == hipSuccess &&
+             run_subint(ctx, offs, ptr.data(), nb.data(), cptr.data(), d_res, ctx->stream) == FXG_OK &&
+             hipEventRecord(e1, ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float t = 0.0f;
+        if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = std::min(ms, (double)t);
+      }
+      if (!ok) {
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->err.clear();
+      }
+      ctx->tune_log.emplace_back(want == 4 ? "MFMA X-engine, 16-channel tiles (16 channels per workgroup)"
+                                           : "MFMA X-engine, 8-channel tiles (8 channels per workgroup)",
+                                 ok ? ms : -1.0);
+      if (ok && ms < xbest) { xbest = ms; xw = want; }
+    }
+    for (auto& g : ctx->groups) g.twlog = tile_width_log(g, xw);
+    if (rc == FXG_OK && hipMemcpy(ctx->d_groups, ctx->groups.data(), sizeof(DevXGroup) * ctx->groups.size(),
+                                  hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(ctx, FXG_E_HIP, "autotune: group upload");
+  }
+  ctx->timing = timing;
   (void)hipStreamSynchronize(ctx->stream);
   for (auto b : bufs) if (b) (void)hipFree(b);
   if (d_res) (void)hipFree(d_res);
@@ -2700,6 +2748,7 @@ int fxg_set_kurtosis(fxg_ctx* ctx, int32_t enable) {
           const DevXGroup& g = ctx->groups[x.band_xfreq[b]];
           k.off = g.spec_off + (long long)x.band_xrow[b] * g.rowstride;
           k.scale_idx = g.scale_off + x.band_xrow[b];
+          k.twlog = g.twlog;
         }
         k.acc_off = (int)acc;
         acc += x.C;
@@ -2783,10 +2832,11 @@ int fxg_get_spectrum(const fxg_ctx* ctx, int32_t ds, int32_t band, int32_t fft, 
   const int row = x.band_xrow[band];
   std::vector<uint32_t> blk((size_t)C);
   float sc = 1.0f;
-  for (int ct = 0; ct < C / 8; ++ct) {
+  const int tw = 1 << g.twlog;
+  for (int ct = 0; ct < C / tw; ++ct) {
     const uint32_t* src = ctx->d_spec16 + g.spec_off + (long long)row * g.rowstride +
-                          ((long long)ct * ctx->chunkcap + lt) * 8;
-    if (hipMemcpy(blk.data() + 8 * ct, src, sizeof(uint32_t) * 8, hipMemcpyDeviceToHost) != hipSuccess)
+                          ((long long)ct * ctx->chunkcap + lt) * tw;
+    if (hipMemcpy(blk.data() + tw * ct, src, sizeof(uint32_t) * tw, hipMemcpyDeviceToHost) != hipSuccess)
       return FXG_E_HIP;
   }
   if (hipMemcpy(&sc, ctx->d_xscale + g.scale_off + row, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)

@@ -719,13 +719,17 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
         }
         auto store16 = [&](auto mode_c) {
           constexpr int OM = decltype(mode_c)::value;
+          // channel c of FFT lt sits at (c / TW) cap TW + lt TW + c % TW of the row; NT is a
+          // multiple of TW, so per thread a base and a uniform stride per m
           const CDevXGroup xgr = ((CDevXGroup)args.xgroups) + dsp->band_xfreq[band];
-          uint32_t* __restrict__ o16 = args.spec16 + xgr->spec_off + (long long)dsp->band_xrow[band] * xgr->rowstride + lt * 8;
-          const long long cs16 = (long long)(NT / 8) * cap * 8;
+          const int twl = xgr->twlog, twm = (1 << twl) - 1;
+          uint32_t* __restrict__ o16 = args.spec16 + xgr->spec_off + (long long)dsp->band_xrow[band] * xgr->rowstride +
+                                       ((long long)lt << twl);
+          const long long cs16 = ((long long)(NT >> twl) * cap) << twl;
 #pragma unroll
           for (int u = 0; u < V; ++u) {
             const int vt = tl + u * TPF;
-            uint32_t* __restrict__ o16t = o16 + (long long)(vt >> 3) * cap * 8 + (vt & 7);
+            uint32_t* __restrict__ o16t = o16 + (((long long)(vt >> twl) * cap) << twl) + (vt & twm);
 #pragma unroll
             for (int m = 0; m < NKEEP; ++m) o16t[m * cs16] = pack16m<OM>(Fo[u][m]);
           }
@@ -734,13 +738,15 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             if (dsp->zout_band[z] != band) continue;
             const int c0 = dsp->zout_c0[z], nz = dsp->zout_n[z];
             const CDevXGroup zg = ((CDevXGroup)args.xgroups) + dsp->zout_group[z];
-            uint32_t* __restrict__ oz = args.spec16 + zg->spec_off + (long long)dsp->zout_row[z] * zg->rowstride + lt * 8;
+            const int zl = zg->twlog, zm = (1 << zl) - 1;
+            uint32_t* __restrict__ oz = args.spec16 + zg->spec_off + (long long)dsp->zout_row[z] * zg->rowstride +
+                                        ((long long)lt << zl);
 #pragma unroll
             for (int u = 0; u < V; ++u) {
 #pragma unroll
               for (int m = 0; m < NKEEP; ++m) {
                 const int cz = tl + u * TPF + m * NT - c0;
-                if (cz >= 0 && cz < nz) oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16m<OM>(Fo[u][m]);
+                if (cz >= 0 && cz < nz) oz[(((long long)(cz >> zl) * cap) << zl) + (cz & zm)] = pack16m<OM>(Fo[u][m]);
               }
             }
           }
@@ -1002,11 +1008,13 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
           constexpr int OM = decltype(mode_c)::value;   // 0: FP32, 1: FP16, 2: BF16
           float2* __restrict__ o32 = nullptr;
           uint32_t* __restrict__ o16 = nullptr;
+          int twl = 3;
           if (OM == 0) {
             o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
           } else {
             const CDevXGroup xgr = ((CDevXGroup)args.xgroups) + dsp->band_xfreq[band];
-            o16 = args.spec16 + xgr->spec_off + (long long)dsp->band_xrow[band] * xgr->rowstride + lt * 8;
+            twl = xgr->twlog;
+            o16 = args.spec16 + xgr->spec_off + (long long)dsp->band_xrow[band] * xgr->rowstride + ((long long)lt << twl);
           }
           auto load_z = [&](float2* z, int s) {
 #pragma unroll
@@ -1031,14 +1039,16 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
               if (OM == 0) {
                 o32[c] = F;
               } else {
-                o16[(c >> 3) * cap * 8 + (c & 7)] = pack16m<OM>(F);
+                o16[(((long long)(c >> twl) * cap) << twl) + (c & ((1 << twl) - 1))] = pack16m<OM>(F);
                 for (int zz = 0; zz < dsp->nzout; ++zz) {          // zoom windows of this band
                   if (dsp->zout_band[zz] != band) continue;
                   const int cz = c - dsp->zout_c0[zz];
                   if (cz < 0 || cz >= dsp->zout_n[zz]) continue;
                   const CDevXGroup zg = ((CDevXGroup)args.xgroups) + dsp->zout_group[zz];
-                  uint32_t* __restrict__ oz = args.spec16 + zg->spec_off + (long long)dsp->zout_row[zz] * zg->rowstride + lt * 8;
-                  oz[(cz >> 3) * cap * 8 + (cz & 7)] = pack16m<OM>(F);
+                  const int zl = zg->twlog;
+                  uint32_t* __restrict__ oz = args.spec16 + zg->spec_off + (long long)dsp->zout_row[zz] * zg->rowstride +
+                                              ((long long)lt << zl);
+                  oz[(((long long)(cz >> zl) * cap) << zl) + (cz & ((1 << zl) - 1))] = pack16m<OM>(F);
                 }
               }
             }

@@ -103,6 +103,9 @@ struct DevXGroup {
   long long spec_off;     // element offset (32-bit units) of its spectra block
   long long rowstride;    // 32-bit units per input row: C x chunk FFTs plus one 128-byte line
   long long scale_off;    // float offset of its per-row scales [rows]
+  int twlog;              // log2 of the channel tile width TW (8 or 16): spectra rows are
+                          // [C/TW][t][TW] words, so one FFT writes TW channels contiguously
+  int pad_;
 };
 
 // Per-FFT delay chain of one datastream (mode.cpp:671-675, 743-757, 781-788).
@@ -134,7 +137,7 @@ struct FengineArgs {
   double scan_start_sec;
   int out_mode;           // 0 = FP32 [ds][band][t][C], 1 = FP16, 2 = BF16 MFMA layout
   float2* spec32;         // FP32 spectra
-  uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row (rowstride)][C/8][t][8]
+  uint32_t* spec16;       // half2/bf16x2 spectra, layout [group][row (rowstride)][C/TW][t][TW]
   const DevXGroup* xgroups;
   const float* xscale;    // per group [rows] scales applied before 16-bit conversion
   const float* chanfreq;  // per datastream: sub[S], then usb, lsb and dsb steps [NFS] (mode.cpp:303-324)
@@ -264,7 +267,7 @@ int fxg_launch_fbig(const FengineArgs& a, int L, int iscomplex, int nds, int wgx
                     const float2* twL, hipStream_t st);
 size_t fxg_fbig_lds(int ntb, int phcap);
 int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st);
-int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st);
+int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, int twlog, hipStream_t st);
 int fxg_launch_fin_plain(const FinPlain* e, int n, int maxout, const float2* acc, float2* res, const UvArgs& uv,
                          hipStream_t st);
 int fxg_launch_fin_mfma(const FinMfma* e, int n, int maxout, const DevXGroup* g, const float* acc,
@@ -319,7 +322,7 @@ struct KurtBand {
   int C;              // channels
   int scale_idx;      // FP16/BF16: xscale index of the row
   int acc_off;        // float2 offset of the band's (s1, s2) sums
-  int pad;
+  int twlog;          // FP16/BF16: log2 of the row's channel tile width
 };
 struct KurtArgs {
   const KurtBand* tab;

@@ -12,7 +12,8 @@ namespace {
 // sums take out once at the end (|X/sc|^2 = |X|^2 / sc^2)
 __device__ __forceinline__ float2 spec_raw(const KurtArgs& a, const KurtBand& b, int c, int lt) {
   if (a.mode == FXG_X_FP32) return a.spec32[b.off + (long long)lt * b.C + c];
-  const uint32_t u = a.spec16[b.off + ((long long)(c >> 3) * a.chunkcap + lt) * 8 + (c & 7)];
+  const uint32_t u = a.spec16[b.off + ((((long long)(c >> b.twlog) * a.chunkcap + lt)) << b.twlog) +
+                              (c & ((1 << b.twlog) - 1))];
   if (a.mode == FXG_X_FP16) {
     const __half2 h = *reinterpret_cast<const __half2*>(&u);
     return __half22float2(h);
@@ -21,8 +22,8 @@ __device__ __forceinline__ float2 spec_raw(const KurtArgs& a, const KurtBand& b,
 }
 
 // One workgroup per (band, tile of channels): its lanes cover FFTS_PER_PASS FFTs x CH channels
-// of the spectra layout (FP16/BF16: the 8 channels of one 32-byte group, so a pass reads
-// 32 FFTs x 32 bytes contiguous; FP32: 64 channels of 8 bytes, 4 FFTs per pass), each lane
+// of the spectra layout (FP16/BF16: 8 channels of a tile of 8 or 16, so a pass reads 32 FFTs x
+// 32 bytes of contiguous runs; FP32: 64 channels of 8 bytes, 4 FFTs per pass), each lane
 // summing every FFTS_PER_PASS-th FFT, and the partial sums are added over the FFTs in LDS.
 template <int CH>
 __global__ void __launch_bounds__(256) k_kurt(KurtArgs a) {

@@ -76,29 +76,30 @@ __device__ __forceinline__ uint4 rot_negi(uint4 u) {
   return r;
 }
 
-// One workgroup = one tile of 8 channels of one group; each wavefront owns CPW channels
-// (4 wavefronts x 2 for one 32-row tile, 8 wavefronts x 1 for two).  The F-engine writes the
-// 16-bit spectra channel-tile-blocked ([row][C/8][t][8]), so per k-step (8 FFTs) every input
-// row contributes one contiguous 256-byte run.  The workgroup streams those runs into LDS as
-// [channel][row][8 FFTs], double buffered, so each lane reads its MFMA operand (one row,
-// 4 consecutive FFTs = 8 halves) with one 16-byte LDS load.  Rows beyond the group's inputs
-// stay zero in LDS and are never fetched.
-template <int NTILE>
+// One workgroup = one tile of CT channels (8 or 16, the tile width of the group's spectra
+// layout) of one group; each wavefront owns CPW channels (2 for one 32-row tile, 1 for two).
+// The F-engine writes the 16-bit spectra channel-tile-blocked ([row][C/CT][t][CT]), so per
+// k-step (8 FFTs) every input row contributes one contiguous run of 32 CT bytes.  The workgroup
+// streams those runs into LDS as [channel][row][8 FFTs], double buffered, so each lane reads its
+// MFMA operand (one row, 4 consecutive FFTs = 8 halves) with one 16-byte LDS load.  Rows beyond
+// the group's inputs stay zero in LDS and are never fetched.
+template <int NTILE, int CT_>
 struct XTile {
   static constexpr int CPW = (NTILE == 1) ? 2 : 1;      // channels per wavefront
-  static constexpr int CT = 8;                           // channels per workgroup
+  static constexpr int CT = CT_;                         // channels per workgroup
   static constexpr int NW = CT / CPW;                    // wavefronts per workgroup
   static constexpr int NTH = 64 * NW;
   static constexpr int R = 32 * NTILE;                   // operand rows
   static constexpr int CSTR = R * 8 + 8;                 // 32-bit words per channel in LDS (padded:
                                                          // conflict-free chunk stores)
-  static constexpr int QPR = 16;                         // 16-byte chunks per row per k-step
+  static constexpr int QPR = 2 * CT;                     // 16-byte chunks per row per k-step
   static constexpr int QMAX = (R * QPR + NTH - 1) / NTH; // chunks per thread per k-step
+  static_assert(NTH <= 1024, "workgroup size");
 };
 
-template <typename V8, int NTILE>
-__global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
-  using T = XTile<NTILE>;
+template <typename V8, int NTILE, int CT_>
+__global__ void __launch_bounds__((XTile<NTILE, CT_>::NTH)) k_xmac_mfma(XMfmaArgs a) {
+  using T = XTile<NTILE, CT_>;
   constexpr int NPAIR = NTILE * (NTILE + 1) / 2;
   constexpr int CPW = T::CPW, CT = T::CT, CSTR = T::CSTR, QPR = T::QPR, QMAX = T::QMAX, NTH = T::NTH;
   __shared__ uint4 lds_raw[2][(CT * CSTR) / 4];
@@ -111,7 +112,7 @@ __global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
   const int wave = tid >> 6, lane = tid & 63;
   const int row = lane & 31, half = lane >> 5;
   const int nq = g.nin * QPR;
-  const uint32_t* gbase = a.spec16 + g.spec_off + (long long)ct * a.chunkcap * 8;
+  const uint32_t* gbase = a.spec16 + g.spec_off + (long long)ct * a.chunkcap * CT;
 
   for (int i = tid; i < 2 * (CT * CSTR) / 4; i += NTH)
     (&lds_raw[0][0])[i] = make_uint4(0u, 0u, 0u, 0u);
@@ -133,16 +134,17 @@ __global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
     }
   }
 
-  // chunk q of a k-step: row r = q / 16, FFT t = (q % 16) / 2, channel quad h = q % 2
-  // The loads are unconditional (a chunk past the group's inputs re-reads row 0 and is not
-  // stored), so the compiler can count them exactly in its waits.
+  // chunk q of a k-step: row r = q / QPR, FFT t = (q % QPR) / (CT/4), channel quad
+  // h = q % (CT/4).  The loads are unconditional (a chunk past the group's inputs re-reads
+  // row 0 and is not stored), so the compiler can count them exactly in its waits.
+  constexpr int QPC = CT / 4;                    // channel quads per FFT
   auto gload = [&](int k, uint4* b) {
 #pragma unroll
     for (int i = 0; i < QMAX; ++i) {
       const int q0 = tid + NTH * i;
-      const int q = q0 < nq ? q0 : (q0 & 15);
-      const int r = q >> 4, rem = q & 15;
-      b[i] = *reinterpret_cast<const uint4*>(gbase + (long long)r * g.rowstride + (long long)(8 * k) * 8 + rem * 4);
+      const int q = q0 < nq ? q0 : (q0 % QPR);
+      const int r = q / QPR, rem = q % QPR;
+      b[i] = *reinterpret_cast<const uint4*>(gbase + (long long)r * g.rowstride + (long long)(8 * k) * CT + rem * 4);
     }
   };
   auto lstore = [&](uint32_t* L, const uint4* b) {
@@ -150,8 +152,8 @@ __global__ void __launch_bounds__(XTile<NTILE>::NTH) k_xmac_mfma(XMfmaArgs a) {
     for (int i = 0; i < QMAX; ++i) {
       const int q = tid + NTH * i;
       if (q < nq) {
-        const int r = q >> 4, rem = q & 15;
-        const int t = rem >> 1, h = rem & 1;
+        const int r = q / QPR, rem = q % QPR;
+        const int t = rem / QPC, h = rem % QPC;
         uint32_t* d = L + (4 * h) * CSTR + r * 8 + t;
         d[0] = b[i].x; d[CSTR] = b[i].y; d[2 * CSTR] = b[i].z; d[3 * CSTR] = b[i].w;
       }
@@ -380,19 +382,26 @@ int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t s
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, hipStream_t st) {
+template <typename V8, int NTILE, int CT>
+static void launch_x(const XMfmaArgs& b, int C, int ngroups, hipStream_t st) {
+  hipLaunchKernelGGL((k_xmac_mfma<V8, NTILE, CT>), dim3(C / CT, ngroups), dim3(XTile<NTILE, CT>::NTH), 0, st, b);
+}
+
+// groups [group0, group0 + ngroup_launch) share the tile count, channel count and tile width
+int fxg_launch_xmfma(const XMfmaArgs& a, int ngroup_launch, int group0, int ntile, int C, int twlog, hipStream_t st) {
   XMfmaArgs b = a;
   b.groups = a.groups + group0;
-  if (C % 8) return -2;
-  dim3 grid(C / 8, ngroup_launch);
-  if (a.mode == 1) {
-    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 1>), grid, dim3(XTile<1>::NTH), 0, st, b);
-    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<half8_t, 2>), grid, dim3(XTile<2>::NTH), 0, st, b);
-    else return -2;
-  } else {
-    if (ntile == 1) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 1>), grid, dim3(XTile<1>::NTH), 0, st, b);
-    else if (ntile == 2) hipLaunchKernelGGL((k_xmac_mfma<bf16x8_t, 2>), grid, dim3(XTile<2>::NTH), 0, st, b);
-    else return -2;
+  const int ct = 1 << twlog;
+  if (C % ct || (ct != 8 && ct != 16) || (ntile != 1 && ntile != 2) || (ntile == 2 && ct != 8)) return -2;
+  const int form = (a.mode == 1 ? 0 : 4) + (ntile == 1 ? 0 : 2) + (ct == 8 ? 0 : 1);
+  switch (form) {
+    case 0: launch_x<half8_t, 1, 8>(b, C, ngroup_launch, st); break;
+    case 1: launch_x<half8_t, 1, 16>(b, C, ngroup_launch, st); break;
+    case 2: launch_x<half8_t, 2, 8>(b, C, ngroup_launch, st); break;
+    case 4: launch_x<bf16x8_t, 1, 8>(b, C, ngroup_launch, st); break;
+    case 5: launch_x<bf16x8_t, 1, 16>(b, C, ngroup_launch, st); break;
+    case 6: launch_x<bf16x8_t, 2, 8>(b, C, ngroup_launch, st); break;
+    default: return -2;
   }
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
