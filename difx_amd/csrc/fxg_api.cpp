@@ -140,6 +140,7 @@ struct fxg_ctx {
   double* d_model = nullptr;
   float* d_chanfreq = nullptr;
   uint8_t* d_fvalid = nullptr;
+  int2* d_fzone = nullptr;                 // per frame: decoding samples of a partial frame
   long long fvalid_total = 0;
   int maxframes = 0;
   float2* d_twiddle = nullptr;
@@ -864,6 +865,8 @@ int build_tables(fxg_ctx* ctx) {
   if ((rc = dev_upload(ctx, &ctx->d_chanfreq, chanfreq))) return rc;
   if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_fvalid, std::max<long long>(fvo, 16))) return rc_;
   if (int rc_ = dev_zero(ctx, ctx->d_fvalid, std::max<long long>(fvo, 16))) return rc_;
+  if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_fzone, sizeof(int2) * std::max<long long>(fvo, 16))) return rc_;
+  if (int rc_ = dev_zero(ctx, ctx->d_fzone, sizeof(int2) * std::max<long long>(fvo, 16))) return rc_;
   // FFT-length classes, in order of first appearance
   ctx->fclasses.clear();
   for (int d = 0; d < nds; ++d) {
@@ -1629,6 +1632,7 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   sa.run = ctx->d_run;
   sa.model = ctx->d_model;
   sa.fvalid = ctx->d_fvalid;
+  sa.fzone = ctx->d_fzone;
   sa.nds = nds;
   sa.offscan = offsets[0];
   sa.offsec = offsets[1];
@@ -1658,6 +1662,7 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   fa.xscale = ctx->d_xscale;
   fa.chanfreq = ctx->d_chanfreq;
   fa.fvalid = ctx->d_fvalid;
+  fa.fzone = ctx->d_fzone;
   fa.chunkcap = ctx->chunkcap;
   fa.rotorder = ctx->rotorder;
   if (ctx->kurt_on && fxg_launch_zero(ctx->d_kurt_acc, sizeof(float2) * ctx->kurt_len, st))
@@ -1667,6 +1672,7 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
   pa.run = ctx->d_run;
   pa.params = ctx->d_params;
   pa.fvalid = ctx->d_fvalid;
+  pa.fzone = ctx->d_fzone;
   pa.groups = ctx->d_pcal_groups;
   pa.part = ctx->d_pcal_part;
   pa.bins = ctx->d_pcal_bins;
@@ -1916,7 +1922,7 @@ int autotune(fxg_ctx* ctx) {
 
 void free_ctx(fxg_ctx* ctx) {
   auto f = [](void* p) { if (p) (void)hipFree(p); };
-  f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid);
+  f(ctx->d_ds); f(ctx->d_run); f(ctx->d_model); f(ctx->d_chanfreq); f(ctx->d_fvalid); f(ctx->d_fzone);
   f(ctx->d_twiddle); f(ctx->d_dslist); f(ctx->d_params); f(ctx->d_items[0]); f(ctx->d_items[1]);
   f(ctx->d_iclass[0]); f(ctx->d_iclass[1]);
   f(ctx->d_counter); f(ctx->d_bigscratch); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
@@ -2891,6 +2897,7 @@ int fxg_unpack_device(fxg_ctx* ctx, int32_t ds, const uint8_t* d_data, int32_t n
   sa.run = ctx->d_run;
   sa.model = ctx->d_model;
   sa.fvalid = ctx->d_fvalid;
+  sa.fzone = ctx->d_fzone;
   sa.nds = ctx->nds;
   sa.offscan = 0; sa.offsec = 0; sa.offns = 0;
   sa.model_inc_secs = ctx->model_inc;
@@ -2902,6 +2909,7 @@ int fxg_unpack_device(fxg_ctx* ctx, int32_t ds, const uint8_t* d_data, int32_t n
   ua.d = ds;
   ua.data = d_data;
   ua.fv = ctx->d_fvalid + ctx->h_ds[ds].fvalid_off;
+  ua.fz = ctx->d_fzone + ctx->h_ds[ds].fvalid_off;
   ua.nframes = nframes;
   ua.start = startsample;
   ua.nsamples = nsamples;
@@ -2928,6 +2936,7 @@ int fxg_delay_interpolator(fxg_ctx* ctx, int32_t ds, int32_t sec, int32_t ns, do
   sa.run = ctx->d_run;
   sa.model = ctx->d_model;
   sa.fvalid = ctx->d_fvalid;
+  sa.fzone = ctx->d_fzone;
   sa.nds = ctx->nds;
   sa.offscan = 0; sa.offsec = sec; sa.offns = ns;
   sa.model_inc_secs = ctx->model_inc;

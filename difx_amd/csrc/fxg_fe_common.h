@@ -210,17 +210,18 @@ __device__ __forceinline__ void fft_stockham(float2 (*v)[8], float2* sbuf, const
   }
 }
 
-// Valid samples of [g0, g0+n) given per-frame validity (mk5mode.cpp:94-174: goodsamples is the
-// count of samples from valid frames inside the FFT window).
-__device__ int count_valid(const uint8_t* fv, int nframes, int spf, long long g0, int n) {
+// Valid samples of [g0, g0+n) given the frame flags (mk5mode.cpp:94-174: goodsamples is the
+// count of decoded samples inside the FFT window).
+__device__ int count_valid(const uint8_t* fv, const int2* fz, int nframes, int spf, long long g0, int n) {
   long long g1 = g0 + n;
   int f0 = (int)(g0 / spf), f1 = (int)((g1 - 1) / spf);
   int good = 0;
   for (int f = f0; f <= f1; ++f) {
     if (f >= nframes || !fv[f]) continue;
     long long a = (long long)f * spf, b = a + spf;
+    if (fv[f] == FXG_FRAME_PARTIAL) { b = a + fz[f].y; a += fz[f].x; }
     long long lo = a > g0 ? a : g0, hi = b < g1 ? b : g1;
-    good += (int)(hi - lo);
+    if (hi > lo) good += (int)(hi - lo);
   }
   return good;
 }
@@ -229,7 +230,7 @@ __device__ int count_valid(const uint8_t* fv, int nframes, int spf, long long g0
 //   order 0: integerdelay = (int)averagedelay, b = averagedelay - integerdelay (post-F phase)
 //   order 1: a = d2 - d0, b = d0 + (d1 - (a/2 + d0))/3 - integerdelay
 //   order 2: a = q0, b = q1 + 2 i q0, c = q2 + i q1 + i^2 q0 - integerdelay
-__device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_t* fv, int i,
+__device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_t* fv, const int2* fz, int i,
                               int offsec, int offns, double scan_start_sec, int order) {
   FftDelay r;
   r.ok = 0; r.weight = 0.0f; r.n0 = 0; r.fse = 0.0f; r.a = 0; r.b = 0; r.c = 0; r.intdelay = 0;
@@ -247,7 +248,7 @@ __device__ FftDelay fft_delay(const DevDs& ds, const DevDsRun& run, const uint8_
   int n0 = (int)(starttime / dt + 0.5);
   if (n0 < -1 || (((long long)(n0 + L)) * ds.bpb_num) / ds.bpb_den > run.datalen) return r;
   if (n0 == -1) n0 = 0;
-  const int good = count_valid(fv, run.nframes, ds.spf, n0, L);
+  const int good = count_valid(fv, fz, run.nframes, ds.spf, n0, L);
   const float w = (float)good / (float)L;
   if (!(w > 0.0f)) return r;
   r.ok = 1;
@@ -476,6 +477,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
   k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us; k.dsb = dsp->dsb;
   const long long spec_off32 = dsp->spec_off;
   const GBytes fv = (GBytes)(args.fvalid + dsp->fvalid_off);
+  const int2* __restrict__ fz = args.fzone + dsp->fvalid_off;
   const float* __restrict__ cf = scf;
   const GBytes data = (GBytes)args.run[d].data;
   const int nframes = args.run[d].nframes;
@@ -511,6 +513,9 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
     // (all sixteen loads of a position are issued before any result is used: the frame index
     // is clamped to a frame of the buffer, and a sample past the data is masked afterwards)
     const bool anyload = fok_all && nframes > 0;
+    // weight 1: every sample of the window decodes (the good-sample count equals L), so the
+    // frame flags need not be read
+    const bool whole = fok_all && fd.weight == 1.0f;
 #pragma unroll
     for (int u = 0; u < V; ++u) {
       const unsigned g0 = (unsigned)fd.n0 + (unsigned)(tl + u * TPF);
@@ -533,19 +538,21 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
           const unsigned frc = min(frs[s], (unsigned)(nframes - 1));
-          fvb[s] = fv[frc];
+          if (!whole) fvb[s] = fv[frc];
           if (twobit) {
             const unsigned q = wis[s] * (unsigned)k.nrec;   // first field of the sample
             w8[s] = *(GWords)(data + frc * (unsigned)k.framebytes + (unsigned)k.headerbytes + ((q >> 4) << 2));
           }
         }
       }
-      fok[u] = 0u;
+      fok[u] = whole ? 0xFFu : 0u;
       qlo[u] = 0u;
 #pragma unroll
       for (int s = 0; s < 8; ++s) {
-        const bool ok = fvb[s] != 0u && frs[s] < (unsigned)nframes;
-        fok[u] |= (ok ? 1u : 0u) << s;
+        if (!whole) {
+          const bool ok = frs[s] < (unsigned)nframes && sample_ok(fvb[s], fz, frs[s], wis[s]);
+          fok[u] |= (ok ? 1u : 0u) << s;
+        }
         const unsigned q = wis[s] * (unsigned)k.nrec;
         qlo[u] |= (q & 15u) << (4 * s);
         wv[u][s] = twobit ? w8[s]
@@ -854,6 +861,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
   k.fractionallo = dsp->fractionallo; k.sampletime_us = dsp->sampletime_us; k.dsb = dsp->dsb;
   const long long spec_off32 = dsp->spec_off;
   const GBytes fv = (GBytes)(args.fvalid + dsp->fvalid_off);
+  const int2* __restrict__ fz = args.fzone + dsp->fvalid_off;
   const float* __restrict__ cf = scf;
   const GBytes data = (GBytes)args.run[d].data;
   const int nframes = args.run[d].nframes;
@@ -965,7 +973,7 @@ __device__ void fb_range(const FengineArgs& args, const float2* __restrict__ tw2
               unsigned fr = (unsigned)((double)gs * inv_spf);
               unsigned within = gs - fr * (unsigned)k.spf;
               if (within >= (unsigned)k.spf) { within -= k.spf; ++fr; }
-              const bool ok = fok_all && ((int)fr < nframes) && fv[fr];
+              const bool ok = fok_all && ((int)fr < nframes) && sample_ok(fv[fr], fz, fr, within);
               const unsigned fbase = fr * (unsigned)k.framebytes + (unsigned)k.headerbytes;
               const float2 rot = rot_next(s);
               if (!CPLX) {

@@ -31,8 +31,8 @@ __global__ void k_fftparams(FengineArgs args) {
   fd.intwall = 0; fd.fracwall = 0;
   if (lt < args.nffts) {
     const int i = args.chunk0 + lt;
-    fd = fft_delay(ds, run, args.fvalid + ds.fvalid_off, i, args.offsec, args.offns, args.scan_start_sec,
-                   args.rotorder);
+    fd = fft_delay(ds, run, args.fvalid + ds.fvalid_off, args.fzone + ds.fvalid_off, i, args.offsec, args.offns,
+                   args.scan_start_sec, args.rotorder);
     args.weights[(long long)d * args.B + i] = fd.weight;
   }
   args.params[(long long)d * args.chunkcap + lt] = fd;
@@ -53,33 +53,75 @@ __global__ void __launch_bounds__(NTB, FXG_F_MINWAVES) k_fengine(FengineArgs arg
   fe_range<L, CPLX, ORDER, NTB>(args, args.twiddle, d, u0, u1, fx_smem, true);
 }
 
+constexpr unsigned long long FILL64 = 0x1122334411223344ULL;   // blanker_mark5.c:36
+constexpr uint32_t FILL32 = 0x11223344u;                          // blanker_mark5.c:35
+
+// blanker_mark5's halving search over payload words [0, n) (blanker_mark5.c:43-111): probes
+// start at n/2 and move by n/4, n/8, ..., 1, down after a probe whose word is (fill == want)
+// and up otherwise; the lowest such probe, n - 1 if none, in bytes.
+__device__ int zone_edge(const unsigned long long* w, int n, bool want) {
+  int probe = n >> 1, hit = n - 1;
+  for (int step = n >> 2; step > 0; step >>= 1) {
+    const bool m = (w[probe] == FILL64) == want;
+    hit = m ? probe : hit;
+    probe += m ? -step : step;
+  }
+  return hit * 8;
+}
+
+// Zone 0 of a Mark5B payload (blanker_mark5.c:112-172) as the time samples [lo, hi) whose
+// first byte (n * bits per sample) / 8 lies in the blanker's byte range
+// (mark5_format_mark5b.c:241-243).  Returns the frame flag: 0, 1 or FXG_FRAME_PARTIAL.
+__device__ int mark5_zone(const uint8_t* pay, const DevDs& ds, int2* zone) {
+  const unsigned long long* w = reinterpret_cast<const unsigned long long*>(pay);
+  const uint32_t* w32 = reinterpret_cast<const uint32_t*>(pay);
+  const int e = min((ds.framebytes - ds.headerbytes) >> 3, 4096);   // 32768-byte zone
+  const bool sok = w[0] != FILL64, eok = w[e - 1] != FILL64;
+  if (sok && eok) return 1;
+  if (!sok && !eok) return 0;
+  long long sv = 0, ev = 1LL << 30;
+  if (sok) {
+    ev = zone_edge(w, e, true);
+    if (w32[ev / 4 - 1] == FILL32) ev -= 4;      // a fill half word just before
+  } else {
+    sv = zone_edge(w, e, false);
+    if (w32[sv / 4 - 2] == FILL32) sv += 4;      // the half word the reference tests
+  }
+  const long long bps = (long long)ds.nbits * (ds.iscomplex ? 2 : 1) * ds.nrecbands;
+  const int lo = (int)min((8 * sv + bps - 1) / bps, (long long)ds.spf);
+  const int hi = (int)min((8 * ev + bps - 1) / bps, (long long)ds.spf);
+  *zone = make_int2(lo, hi);
+  return hi <= lo ? 0 : (lo == 0 && hi == ds.spf) ? 1 : FXG_FRAME_PARTIAL;
+}
+
 // Setup for one subint: the delay interpolator of each datastream in FP64 on the device
 // (Model::calculateDelayInterpolator, model.cpp:219-289, via Mode::setOffsets,
-// mode.cpp:1502-1532) and the per-frame validity of each slot buffer.
+// mode.cpp:1502-1532) and the frame flags of each slot buffer.
 __global__ void k_setup(SetupArgs a) {
   const int d = blockIdx.y;
   const DevDs& ds = a.ds[d];
   DevDsRun& run = a.run[d];
-  // frame validity: VDIF word2 != 0 and invalid bit clear (format_vdif.c:7619-7660),
-  // Mark5B invalid flag (mark5_format_mark5b.c:1723), fill pattern at either end of the
-  // payload blanks the frame (blanker_mark5.c:272-310).
+  // frame flags: VDIF word2 != 0 and invalid bit clear (format_vdif.c:7619-7660), fill pattern
+  // at either end of the payload blanks the frame (blanker_mark5.c:272-310); Mark5B invalid
+  // flag (mark5_format_mark5b.c:1723) and the fill-pattern zone of the payload.
   for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < run.nframes; f += gridDim.x * blockDim.x) {
     const uint8_t* fr = run.data + (long long)f * ds.framebytes;
+    const uint8_t* pay = fr + ds.headerbytes;
     int ok;
+    int2 zone = make_int2(0, 0);
     if (ds.format == FXG_FMT_VDIF) {
       const uint32_t w0 = *reinterpret_cast<const uint32_t*>(fr);
       const uint32_t w2 = *reinterpret_cast<const uint32_t*>(fr + 8);
       ok = (w2 != 0u) && (((w0 >> 31) & 1u) == 0u);
+      const int pb = ds.framebytes - ds.headerbytes;
+      const unsigned long long first = *reinterpret_cast<const unsigned long long*>(pay);
+      const unsigned long long last = *reinterpret_cast<const unsigned long long*>(pay + pb - 8);
+      if (first == FILL64 || last == FILL64) ok = 0;
     } else {
-      ok = (fr[5] & 0x80) == 0;
+      ok = (fr[5] & 0x80) == 0 ? mark5_zone(pay, ds, &zone) : 0;
     }
-    const uint8_t* pay = fr + ds.headerbytes;
-    const int pb = ds.framebytes - ds.headerbytes;
-    const unsigned long long fill = 0x1122334411223344ULL;
-    const unsigned long long first = *reinterpret_cast<const unsigned long long*>(pay);
-    const unsigned long long last = *reinterpret_cast<const unsigned long long*>(pay + pb - 8);
-    if (first == fill || last == fill) ok = 0;
     a.fvalid[ds.fvalid_off + f] = (uint8_t)ok;
+    a.fzone[ds.fvalid_off + f] = zone;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // Mode::setOffsets: scan mismatch or invalid data torch the subint (mode.cpp:1509-1516)
@@ -118,7 +160,7 @@ __global__ void k_unpack(UnpackArgs a) {
     const long long g = a.start + n;
     const int fr = (int)(g / ds.spf);
     const int within = (int)(g - (long long)fr * ds.spf);
-    const int ok = (fr < a.nframes) && a.fv[fr];
+    const int ok = (fr < a.nframes) && sample_ok(a.fv[fr], a.fz, fr, within);
     const long long fbase = (long long)fr * ds.framebytes + ds.headerbytes;
     if (ok) atomicAdd(a.good, 1);
     for (int b = 0; b < nrec; ++b) {

@@ -108,6 +108,17 @@ struct DevXGroup {
   int pad_;
 };
 
+// Frame flags of a slot buffer: 0 blanked, 1 every sample decodes, 2 only the time samples
+// [zone.x, zone.y) decode (a Mark5B payload with the fill pattern at one end; the byte range
+// of blanker_mark5.c:112-172 as samples).
+#define FXG_FRAME_PARTIAL 2
+__device__ __forceinline__ bool sample_ok(unsigned flag, const int2* __restrict__ zone, unsigned fr,
+                                          unsigned within) {
+  if (flag != FXG_FRAME_PARTIAL) return flag != 0u;
+  const int2 z = zone[fr];
+  return (int)within >= z.x && (int)within < z.y;
+}
+
 // Per-FFT delay chain of one datastream (mode.cpp:671-675, 743-757, 781-788).
 struct FftDelay {
   int ok;               // 1 = processed, 0 = flagged (zero spectra, weight 0)
@@ -141,7 +152,8 @@ struct FengineArgs {
   const DevXGroup* xgroups;
   const float* xscale;    // per group [rows] scales applied before 16-bit conversion
   const float* chanfreq;  // per datastream: sub[S], then usb, lsb and dsb steps [NFS] (mode.cpp:303-324)
-  const uint8_t* fvalid;  // frame valid flags
+  const uint8_t* fvalid;  // frame flags
+  const int2* fzone;      // per frame: the decoding samples of a partial frame
   int chunkcap;           // FFT capacity of the chunk buffers (multiple of FXG_XPAD)
   int units_per_wg;       // per-length launch: units of 8 FFTs per workgroup
   const int* dslist;      // datastreams of this launch, indexed by grid.y (one FFT-length class)
@@ -172,6 +184,7 @@ struct SetupArgs {
   DevDsRun* run;
   const double* model;        // [..][FXG_MAX_POLY]
   uint8_t* fvalid;
+  int2* fzone;
   int nds;
   int offscan, offsec, offns;
   double model_inc_secs;
@@ -184,6 +197,7 @@ struct UnpackArgs {
   int d;
   const uint8_t* data;
   const uint8_t* fv;
+  const int2* fz;
   int nframes;
   long long start;
   int nsamples;
@@ -365,6 +379,7 @@ struct PcalArgs {
   const DevDsRun* run;
   const FftDelay* params;    // [nds][chunkcap] of the current chunk
   const uint8_t* fvalid;
+  const int2* fzone;
   const PcalGroup* groups;
   float* part;
   float* bins;

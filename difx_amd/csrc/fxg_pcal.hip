@@ -57,6 +57,7 @@ __global__ void __launch_bounds__(PCAL_NT) k_pcal(PcalArgs a) {
     // sample number of the buffer start since the scan start (mode.cpp:760)
     const long long s0 = (long long)(((double)run.datasec * 1e9 + (double)run.datans) / (ds.sampletime_us * 1e3) + 0.5);
     const uint8_t* __restrict__ fv = a.fvalid + ds.fvalid_off;
+    const int2* __restrict__ fz = a.fzone + ds.fvalid_off;
     const uint8_t* __restrict__ data = run.data;
     const int L = ds.L, spf = ds.spf, nrec = g.fps, nb = g.nb;   // nrec: fields per time sample
     const int nsamp = run.nframes * spf;
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(PCAL_NT) k_pcal(PcalArgs a) {
           pay[u] = data + (long long)fr * ds.framebytes + ds.headerbytes;
           q0[u] = (unsigned)(within * nrec + g.band0);
           w[u] = in ? *reinterpret_cast<const uint32_t*>(pay[u] + 4u * (q0[u] >> LGF)) : 0u;
-          ok[u] = in ? fv[fr] != 0 : false;
+          ok[u] = in ? sample_ok(fv[fr], fz, fr, within) : false;
           binu[u] = bin;
           t += PCAL_NT;
           within += PCAL_NT;

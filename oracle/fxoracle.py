@@ -8,7 +8,7 @@ arithmetic, what the reference computes for one subint and one Core slot:
                   (mark5_stream_unpacker.c:119-138), VDIF LUTs (format_vdif.c:66-145,
                   834-880, 1621-1680, 5215-5250), Mark5B LUT (mark5_format_mark5b.c:49-86,
                   1708-1760), frame validation / blanking (format_vdif.c:7619-7660,
-                  blanker_mark5.c:272-310, mark5_stream.c:144-193)
+                  blanker_mark5.c:43-172, 272-310, mark5_stream.c:144-193)
   delay model     Model::calculateDelayInterpolator (model.cpp:219-289), Mode::setOffsets
                   (mode.cpp:1502-1532)
   F-engine        Mode::process (mode.cpp:634-1374), fringe rotation order 1, fractional
@@ -79,30 +79,115 @@ def lut8bit(offset: float = 128.0) -> np.ndarray:
 # --------------------------------------------------------------------------------------
 # Frames and unpacking
 # --------------------------------------------------------------------------------------
+FILL_WORD32 = 0x11223344                      # blanker_mark5.c:35 (little endian)
+ZONE_ALL = 1 << 30                            # blanker_mark5.c:149, 153: "no bound"
+ZONE_WORDS = 1 << (15 - 3)                    # blanker_mark5.c:121, 135: 32768-byte zones
+
+
+def _first_invalid(w64: np.ndarray, w32: np.ndarray, start: int, end: int) -> int:
+    """findfirstinvalid (blanker_mark5.c:43-76): a halving search from the middle of words
+    [start, end) for the fill pattern, keeping the lowest probe that held it (end-1 if none),
+    as a byte offset; 4 bytes less when the 32-bit word just before it is fill (a straggler)."""
+    n = end - start
+    i, r, s = n // 2, n - 1, n // 4
+    while s > 0:
+        if int(w64[start + i]) == FILL_WORD64:
+            r = i
+            i -= s
+        else:
+            i += s
+        s >>= 1
+    r = 8 * (start + r)
+    if int(w32[r // 4 - 1]) == FILL_WORD32:
+        r -= 4
+    return r
+
+
+def _first_valid(w64: np.ndarray, w32: np.ndarray, start: int, end: int) -> int:
+    """findfirstvalid (blanker_mark5.c:78-111): the same search for a word that is not fill;
+    4 bytes more when the 32-bit word two before the result is fill (the reference tests
+    w32[r/4 - 2], the low half of the previous 64-bit word, and the restatement keeps that)."""
+    n = end - start
+    i, r, s = n // 2, n - 1, n // 4
+    while s > 0:
+        if int(w64[start + i]) != FILL_WORD64:
+            r = i
+            i -= s
+        else:
+            i += s
+        s >>= 1
+    r = 8 * (start + r)
+    if int(w32[r // 4 - 2]) == FILL_WORD32:
+        r += 4
+    return r
+
+
+def mark5_zone(payload: np.ndarray) -> Tuple[int, int]:
+    """blanker_mark5 (blanker_mark5.c:112-172) on one payload: the byte range
+    [startvalid, endvalid) of zone 0, the only zone the Mark5B decoders read
+    (mark5_format_mark5b.c:241-243: sample bytes i with startvalid <= i < endvalid decode).
+    Both end words valid: the whole payload, whatever lies between; both fill: nothing; one
+    end fill: the valid side up to the boundary the search finds."""
+    nword = len(payload) // 8
+    if nword == 0:
+        return 0, 0
+    w64 = payload[: nword * 8].copy().view("<u8")
+    w32 = payload[: nword * 8].copy().view("<u4")
+    e = min(ZONE_WORDS, nword)
+    start_ok = int(w64[0]) != FILL_WORD64
+    end_ok = int(w64[e - 1]) != FILL_WORD64
+    if start_ok and end_ok:
+        return 0, ZONE_ALL
+    if not start_ok and not end_ok:
+        return ZONE_ALL, 0
+    if start_ok:
+        return 0, _first_invalid(w64, w32, 0, e)
+    return _first_valid(w64, w32, 0, e), ZONE_ALL
+
+
 def frame_valid(buf: np.ndarray, fmt: str, framebytes: int, headerbytes: int) -> np.ndarray:
-    """Per-frame validity of a slot buffer: frames that fail validation or carry the fill
-    pattern are blanked.  VDIF: header word 2 == 0 or invalid bit (word 0 bit 31) fails
-    (format_vdif.c:7619-7660); blanker_vdif blanks a payload whose first or last 64-bit word
-    is the fill pattern (blanker_mark5.c:272-310).  Mark5B: the invalid flag is header byte 5
-    bit 7 (mark5_format_mark5b.c:1723, buf[-11] & 0x80); a payload starting or ending with the
-    fill pattern is blanked whole (the zone blanker's partial-frame zones are not restated).
-    Frames extending past the buffer are invalid."""
+    """Per-frame validity of a slot buffer (whole-frame part).  VDIF: header word 2 == 0 or
+    invalid bit (word 0 bit 31) fails (format_vdif.c:7619-7660); blanker_vdif blanks a payload
+    whose first or last 64-bit word is the fill pattern (blanker_mark5.c:272-310).  Mark5B:
+    the invalid flag is header byte 5 bit 7 (mark5_format_mark5b.c:1723, buf[-11] & 0x80);
+    its fill pattern blanks zones of the payload (mark5_zone, applied per sample by
+    sample_valid).  Frames extending past the buffer are invalid."""
     nframes = len(buf) // framebytes
     out = np.zeros(nframes, dtype=bool)
     if nframes == 0:
         return out
     fr = buf[: nframes * framebytes].reshape(nframes, framebytes)
-    payload = fr[:, headerbytes:]
-    first = payload[:, :8].copy().view("<u8")[:, 0]
-    last = payload[:, -8:].copy().view("<u8")[:, 0]
-    fill = (first == FILL_WORD64) | (last == FILL_WORD64)
     if fmt == "VDIF":
+        payload = fr[:, headerbytes:]
+        first = payload[:, :8].copy().view("<u8")[:, 0]
+        last = payload[:, -8:].copy().view("<u8")[:, 0]
+        fill = (first == FILL_WORD64) | (last == FILL_WORD64)
         w0 = fr[:, 0:4].copy().view("<u4")[:, 0]
         w2 = fr[:, 8:12].copy().view("<u4")[:, 0]
-        ok = (w2 != 0) & (((w0 >> 31) & 1) == 0)
-    else:
-        ok = (fr[:, 5] & 0x80) == 0
-    return ok & ~fill
+        return (w2 != 0) & (((w0 >> 31) & 1) == 0) & ~fill
+    return (fr[:, 5] & 0x80) == 0
+
+
+def sample_valid(buf: np.ndarray, ds) -> np.ndarray:
+    """Per-sample validity of a slot buffer: the frame's validity and, for Mark5B, the zone
+    test on the byte where the time sample's fields begin, i = (n * bits per time sample) / 8
+    within the payload (mark5_format_mark5b.c:1295-1343, 1868-1932: one test per sample, at
+    the readposition before its bytes are read)."""
+    fmt = ds.format.upper()
+    fb, hb = ds.framebytes, ds.headerbytes
+    nframes = len(buf) // fb
+    spf = samples_per_frame(ds)
+    valid_f = frame_valid(buf, fmt, fb, hb)
+    valid = np.repeat(valid_f, spf)
+    if fmt == "MARK5B" and nframes > 0:
+        bps = ds.nbits * (2 if ds.complex_sampling else 1) * len(ds.recbands)
+        byte = (np.arange(spf, dtype=np.int64) * bps) // 8
+        fr = buf[: nframes * fb].reshape(nframes, fb)
+        for f in range(nframes):
+            sv, ev = mark5_zone(fr[f, hb:])
+            if (sv, ev) != (0, ZONE_ALL):
+                valid[f * spf: (f + 1) * spf] &= (byte >= sv) & (byte < ev)
+    return valid
 
 
 def decode_stream(buf: np.ndarray, ds) -> Tuple[np.ndarray, np.ndarray]:
@@ -115,7 +200,6 @@ def decode_stream(buf: np.ndarray, ds) -> Tuple[np.ndarray, np.ndarray]:
     fb, hb = ds.framebytes, ds.headerbytes
     pay = fb - hb
     nframes = len(buf) // fb
-    valid_f = frame_valid(buf, fmt, fb, hb)
     fr = buf[: nframes * fb].reshape(nframes, fb)[:, hb:]
     if ds.nbits == 2 and not ds.complex_sampling:
         lut = vdif_lut2bit() if fmt == "VDIF" else mark5b_lut2bit()
@@ -134,7 +218,7 @@ def decode_stream(buf: np.ndarray, ds) -> Tuple[np.ndarray, np.ndarray]:
         vals = lut[fr.reshape(-1)].reshape(nframes * spf, nb).T
     else:
         raise NotImplementedError(f"{ds.nbits}-bit complex={ds.complex_sampling}")
-    valid = np.repeat(valid_f, spf)
+    valid = sample_valid(buf, ds)
     vals = np.where(valid[None, :], vals, np.zeros((), vals.dtype))   # blanked samples are +0.0
     return vals, valid
 

@@ -178,8 +178,48 @@ static float lut2(int fmt, unsigned field) {
 }
 static float lut8(unsigned b) { return (float)(((double)b - 128.0) / 3.3); }
 
-static int frame_ok(const port_ds* d, const uint8_t* fv, long long fr) {
-  return fr >= 0 && fr < d->nframes && fv[fr];
+/* Frame flags: 0 blanked, 1 every sample decodes, 2 samples [zone[2 fr], zone[2 fr + 1]) of
+ * the frame decode (a Mark5B payload with fill pattern at one end, blanker_mark5.c:112-172). */
+static int sample_ok(const port_ds* d, const uint8_t* fv, const int* zone, long long fr, long long within) {
+  if (fr < 0 || fr >= d->nframes || !fv[fr]) return 0;
+  return fv[fr] == 1 || (within >= zone[2 * fr] && within < zone[2 * fr + 1]);
+}
+
+static uint64_t word64(const uint8_t* p, int k) { uint64_t w; memcpy(&w, p + 8 * (long long)k, 8); return w; }
+static uint32_t word32(const uint8_t* p, int k) { uint32_t w; memcpy(&w, p + 4 * (long long)k, 4); return w; }
+
+/* blanker_mark5's halving search over payload words [0, n) (blanker_mark5.c:43-111): probes
+ * start at n/2 and move by n/4, n/8, ..., 1, down after a probe whose word is (fill == want)
+ * and up otherwise; the lowest such probe, n - 1 if none, in bytes. */
+static int zone_edge(const uint8_t* pay, int n, int want) {
+  int probe = n / 2, hit = n - 1;
+  for (int step = n / 4; step > 0; step /= 2) {
+    const int m = (word64(pay, probe) == 0x1122334411223344ULL) == want;
+    if (m) { hit = probe; probe -= step; } else { probe += step; }
+  }
+  return 8 * hit;
+}
+
+/* Zone 0 of a Mark5B payload as the time samples [lo, hi) whose first byte
+ * (n * bits per sample) / 8 lies in the blanker's byte range (mark5_format_mark5b.c:241-243). */
+static void mark5_zone(const port_ds* d, const uint8_t* pay, int* lo, int* hi) {
+  const int nword = (d->framebytes - d->headerbytes) / 8;
+  const int e = nword < 4096 ? nword : 4096;
+  const uint64_t fill = 0x1122334411223344ULL;
+  const int sok = word64(pay, 0) != fill, eok = word64(pay, e - 1) != fill;
+  long long sv = 0, ev = 1LL << 30;
+  if (!sok && !eok) { sv = 1LL << 30; ev = 0; }
+  else if (sok && !eok) {
+    ev = zone_edge(pay, e, 1);
+    if (word32(pay, (int)(ev / 4) - 1) == 0x11223344u) ev -= 4;   /* straggler half word */
+  } else if (!sok) {
+    sv = zone_edge(pay, e, 0);
+    if (word32(pay, (int)(sv / 4) - 2) == 0x11223344u) sv += 4;   /* as tested at blanker_mark5.c:104 */
+  }
+  const long long bps = (long long)d->nbits * (d->cplx ? 2 : 1) * d->nrec;
+  long long a = (8 * sv + bps - 1) / bps, b = (8 * ev + bps - 1) / bps;
+  *lo = (int)(a < d->spf ? a : d->spf);
+  *hi = (int)(b < d->spf ? b : d->spf);
 }
 
 /* ------------------------------------------------------------------ worker ---------- */
@@ -188,6 +228,7 @@ typedef struct {
   double scan_start;
   const port_ds* ds;
   const uint8_t** fvalid;
+  const int** fzone;
   int nprod;
   const port_prod* prods;
   long long acclen;
@@ -229,6 +270,7 @@ static void* worker(void* arg) {
     for (int d = 0; d < nds; ++d) {
       const port_ds* D = &J->ds[d];
       const uint8_t* fv = J->fvalid[d];
+      const int* zn = J->fzone[d];
       const int L = D->L, C = D->C, S = D->S, NS = D->NS, NFS = D->NFS, nrec = D->nrec;
       const double dt = D->sampletime_us;
       for (int sl = 0; sl < nsub; ++sl) {
@@ -253,7 +295,7 @@ static void* worker(void* arg) {
         long long within = (long long)n0 - fr * D->spf - 1;
         for (int n = 0; n < L; ++n) {
           if (++within == D->spf) { within = 0; ++fr; }
-          const int ok = frame_ok(D, fv, fr);
+          const int ok = sample_ok(D, fv, zn, fr, within);
           good += ok;
           const uint8_t* pay = D->data + fr * D->framebytes + D->headerbytes;
           for (int b = 0; b < nrec; ++b) {
@@ -454,21 +496,35 @@ int fxport_run(int nds, const port_ds* ds, int B, int offsec, int offns, double 
   if (nthreads < 1) nthreads = 1;
   if (nthreads > B) nthreads = B;
   if (nbuffered < 1) nbuffered = 1;
-  /* frame validity per datastream: VDIF word 2 / invalid bit, Mark5B invalid flag, fill pattern */
+  /* frame flags per datastream: VDIF word 2 / invalid bit and fill pattern at either end;
+   * Mark5B invalid flag and the fill-pattern zone of the payload */
   uint8_t** fvalid = (uint8_t**)calloc(nds, sizeof(uint8_t*));
+  int** fzone = (int**)calloc(nds, sizeof(int*));
   for (int d = 0; d < nds; ++d) {
     const port_ds* D = &ds[d];
     fvalid[d] = (uint8_t*)calloc(D->nframes + 1, 1);
+    fzone[d] = (int*)calloc(2 * (size_t)D->nframes + 2, sizeof(int));
     for (int f = 0; f < D->nframes; ++f) {
       const uint8_t* fr = D->data + (long long)f * D->framebytes;
+      const uint8_t* pay = fr + D->headerbytes;
       uint32_t w0, w2;
       memcpy(&w0, fr, 4);
       memcpy(&w2, fr + 8, 4);
-      int ok = D->fmt == 0 ? (w2 != 0u && ((w0 >> 31) & 1u) == 0u) : ((fr[5] & 0x80) == 0);
-      uint64_t first, last;
-      memcpy(&first, fr + D->headerbytes, 8);
-      memcpy(&last, fr + D->framebytes - 8, 8);
-      if (first == 0x1122334411223344ULL || last == 0x1122334411223344ULL) ok = 0;
+      int ok;
+      if (D->fmt == 0) {
+        ok = w2 != 0u && ((w0 >> 31) & 1u) == 0u;
+        if (word64(pay, 0) == 0x1122334411223344ULL ||
+            word64(fr + D->framebytes - 8, 0) == 0x1122334411223344ULL) ok = 0;
+      } else {
+        ok = (fr[5] & 0x80) == 0;
+        if (ok) {
+          int lo, hi;
+          mark5_zone(D, pay, &lo, &hi);
+          ok = hi <= lo ? 0 : (lo == 0 && hi == D->spf) ? 1 : 2;
+          fzone[d][2 * f] = lo;
+          fzone[d][2 * f + 1] = hi;
+        }
+      }
       fvalid[d][f] = (uint8_t)ok;
     }
   }
@@ -478,6 +534,7 @@ int fxport_run(int nds, const port_ds* ds, int B, int offsec, int offns, double 
     job* J = &jobs[t];
     J->nds = nds; J->B = B; J->offsec = offsec; J->offns = offns; J->scan_start = scan_start;
     J->nbuffered = nbuffered; J->order = order; J->ds = ds; J->fvalid = (const uint8_t**)fvalid;
+    J->fzone = (const int**)fzone;
     J->nprod = nprod; J->prods = prods; J->acclen = acclen; J->weights = weights;
     J->b0 = (int)((long long)B * t / nthreads);
     J->b1 = (int)((long long)B * (t + 1) / nthreads);
@@ -493,7 +550,7 @@ int fxport_run(int nds, const port_ds* ds, int B, int offsec, int offns, double 
     }
     free(jobs[t].acc);
   }
-  for (int d = 0; d < nds; ++d) free(fvalid[d]);
-  free(fvalid); free(th); free(jobs);
+  for (int d = 0; d < nds; ++d) { free(fvalid[d]); free(fzone[d]); }
+  free(fvalid); free(fzone); free(th); free(jobs);
   return fft_backend;
 }

@@ -204,6 +204,39 @@ def damaged_slot(cfg, offsets=(0, 1, 0), seed: int = 12):
     return data, ctrl
 
 
+FILL64 = np.frombuffer(np.uint64(0x1122334411223344).tobytes(), np.uint8)
+
+
+def mark5_fill(payload: np.ndarray, words, half: int = -1):
+    """Write the fill pattern over 64-bit words `words` of a payload (in place); `half` >= 0
+    also fills the upper 32-bit half of that word."""
+    for k in words:
+        payload[8 * k: 8 * k + 8] = FILL64
+    if half >= 0:
+        payload[8 * half + 4: 8 * half + 8] = FILL64[:4]
+
+
+def mark5b_partial_slot(cfg, offsets=(0, 1, 0), seed: int = 13):
+    """A noise slot of Mark5B frames (10000-byte payloads, 1250 words) in which station 1's
+    frames carry the fill pattern over parts of their payloads: the tail from word 625, the
+    tail from word 313, all but word 0, the tail from 625 with the upper half of word 624
+    (a straggler), the head to word 624, word 0 alone, both ends, and a middle run only."""
+    data, ctrl = synth.noise_slot(cfg, list(offsets), seed=seed)
+    ds = cfg.datastreams[1]
+    buf = data[1].copy()
+    fb, hb = ds.framebytes, ds.headerbytes
+    nfr = len(buf) // fb
+    plans = [(range(625, 1250), -1), (range(313, 1250), -1), (range(1, 1250), -1), (range(625, 1250), 624),
+             (range(0, 625), -1), (range(0, 1), -1), (list(range(0, 3)) + list(range(1240, 1250)), -1),
+             (range(100, 300), -1)]
+    for j, (words, half) in enumerate(plans):
+        f = 1 + (j * max(1, nfr - 2)) // len(plans)
+        if f < nfr:
+            mark5_fill(buf[f * fb + hb: (f + 1) * fb], words, half)
+    data[1] = buf
+    return data, ctrl
+
+
 def record_errors(cfg, lay, got: np.ndarray, ref: np.ndarray):
     """Per-record errors in the diffDiFX convention (utilities/vis2screen/diffDiFX.py:137-151:
     mean|got-ref| / mean|ref| over the channels of one record) for every baseline product

@@ -14,7 +14,7 @@ from difx_amd import synth
 from fxcases import (case_cfg0, case_cfg1, case_cfg2, case_cfg3, case_complex_fourstep,
                      case_fourstep_ffts, case_long_ffts, case_mark5b, case_offsets, case_real8,
                      case_offsets_fourstep, case_dsb_fraclo,
-                     damaged_slot, record_errors, slot, channel_errors)
+                     damaged_slot, mark5b_partial_slot, record_errors, slot, channel_errors)
 
 pytestmark = pytest.mark.gpu
 
@@ -29,10 +29,12 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("kind", ["vdif1", "vdif16", "mark5b8", "cplx8"])
+@pytest.mark.parametrize("kind", ["vdif1", "vdif16", "mark5b8", "mark5b_zones", "cplx8"])
 def test_unpack_bitexact(gpu, kind):
     import torch
-    if kind == "vdif1":
+    if kind == "mark5b_zones":
+        cfg = case_mark5b(B=32)
+    elif kind == "vdif1":
         cfg = case_cfg0(B=128)
     elif kind == "vdif16":
         cfg = case_cfg1(B=32, nstations=2)
@@ -44,20 +46,25 @@ def test_unpack_bitexact(gpu, kind):
         cfg = case_cfg2(B=16, nstations=2)
     _say("unpack case", kind)
     core = _core(cfg)
-    data, ctrl = slot(cfg, seed=5)
-    buf = data[0].copy()
     ds = cfg.datastreams[0]
-    # invalidate frame 1 (invalid bit / Mark5B flag) and put fill pattern in frame 2's payload
     fb, hb = ds.framebytes, ds.headerbytes
-    if ds.format == "VDIF":
-        buf[fb + 3] |= 0x80
+    if kind == "mark5b_zones":
+        # station 1's frames with fill-pattern zones (tail, head, straggler, both ends, middle)
+        data, ctrl = mark5b_partial_slot(cfg, seed=5)
+        buf = data[1].copy()
     else:
-        buf[fb + 5] |= 0x80
-    fill = np.frombuffer(np.uint64(0x1122334411223344).tobytes(), np.uint8)
-    buf[2 * fb + hb: 2 * fb + hb + 8] = fill
+        data, ctrl = slot(cfg, seed=5)
+        buf = data[0].copy()
+        # invalidate frame 1 (invalid bit / Mark5B flag) and put fill pattern in frame 2's payload
+        if ds.format == "VDIF":
+            buf[fb + 3] |= 0x80
+        else:
+            buf[fb + 5] |= 0x80
+        fill = np.frombuffer(np.uint64(0x1122334411223344).tobytes(), np.uint8)
+        buf[2 * fb + hb: 2 * fb + hb + 8] = fill
     vals, valid = fo.decode_stream(buf, ds)
     nsamp = vals.shape[1]
-    start, n = 3, min(nsamp - 3, 50000)
+    start, n = 3, (nsamp - 3 if kind == "mark5b_zones" else min(nsamp - 3, 50000))
     d_buf = torch.from_numpy(buf).to(gpu)
     nb = len(ds.recbands)
     out = torch.zeros(nb * n * (2 if ds.complex_sampling else 1), dtype=torch.float32, device=gpu)
@@ -95,9 +102,11 @@ def test_delay_interpolator(gpu):
     core.close()
 
 
-def _run_and_compare(cfg, seed=11, offsets=(0, 1, 0), spectra_check=True, damaged=False):
+def _run_and_compare(cfg, seed=11, offsets=(0, 1, 0), spectra_check=True, damaged=False, make_slot=None):
     core = _core(cfg)
-    data, ctrl = damaged_slot(cfg, offsets, seed=seed) if damaged else slot(cfg, offsets, seed=seed)
+    if make_slot is None:
+        make_slot = damaged_slot if damaged else slot
+    data, ctrl = make_slot(cfg, offsets, seed=seed)
     got = core.processdata(offsets, data, ctrl)
     ref, stations = fo.process_subint(cfg, list(offsets), data, ctrl)
     lay = fo.result_layout(cfg)
@@ -305,27 +314,29 @@ def test_fourstep_in_persistent_launch_bit_identical(gpu):
         np.testing.assert_array_equal(v.view(np.uint64), ref.view(np.uint64))
 
 
-@pytest.mark.parametrize("name", ["offsets", "offsets_fourstep", "dsb_fraclo", "mark5b", "real8", "damaged"])
+@pytest.mark.parametrize("name", ["offsets", "offsets_fourstep", "dsb_fraclo", "mark5b", "mark5b_zones", "real8",
+                                  "damaged"])
 @pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
 def test_more_cases(gpu, name, xe, bound):
     """Per-freq LO offsets (integer and fractional Hz), a fractional-MHz band edge, clock
     offsets, per-pol clock deltas and pol phase offsets with channel averaging 4 (also with L
     recorded before R, and on the four-step F-engine); double-sideband data whose band centre
-    is a fractional MHz; Mark5B frames; 8-bit real samples; a slot with flagged FFTs, an
-    invalid frame and a fill-pattern frame."""
+    is a fractional MHz; Mark5B frames, also with fill-pattern zones inside frames; 8-bit real
+    samples; a slot with flagged FFTs, an invalid frame and a fill-pattern frame."""
     if name == "offsets":
         cfg = case_offsets(B=32, xengine=xe)
     elif name == "offsets_fourstep":
         cfg = case_offsets_fourstep(B=4, xengine=xe)
     elif name == "dsb_fraclo":
         cfg = case_dsb_fraclo(B=16, xengine=xe)
-    elif name == "mark5b":
+    elif name in ("mark5b", "mark5b_zones"):
         cfg = case_mark5b(B=32, xengine=xe)
     elif name == "real8":
         cfg = case_real8(B=32, xengine=xe)
     else:
         cfg = case_cfg1(B=32, nstations=3, nfreq=2, xengine=xe)
     got, ref, e, lay = _run_and_compare(cfg, damaged=(name == "damaged"),
-                                        spectra_check=not name.startswith("offsets"))
+                                        spectra_check=not name.startswith("offsets"),
+                                        make_slot=mark5b_partial_slot if name == "mark5b_zones" else None)
     _say(f"{name} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
     assert max(e["cross"]) < bound and max(e["auto"]) < bound
