@@ -636,7 +636,7 @@ def main():
     f_avg_ms = tm["f_ms"] / max(tm["f_launches"], 1)
     nchunks_per_step = tm["f_launches"] / args.steps
     ffts_per_launch = B / nchunks_per_step
-    fbytes = f_engine_bytes_per_launch(cfg, ffts_per_launch, 2)      # FP16 spectra out
+    fbytes = f_engine_bytes_per_launch(cfg, ffts_per_launch, 4)      # FP16 complex spectra out (half2)
     achieved = fbytes / (f_avg_ms / 1e3) / 1e9
     f_samples_per_launch = ffts_per_launch * sum(2 * cfg.freqs[d.recfreqs[0]].nchan * len(d.recbands)
                                                  for d in cfg.datastreams)
