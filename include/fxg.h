@@ -132,7 +132,8 @@ typedef struct fxg_config {
   int32_t chunk_ffts;         /* FFTs per device spectra chunk; 0 = automatic */
   int32_t fengine_mode;       /* FXG_F_AUTO, FXG_F_PER_LENGTH or FXG_F_PERSISTENT */
   int32_t fengine_threads;    /* workgroup size 128, 256 or 512 (persistent: 256, 512); 0 = by FFT length (persistent: 512) */
-  int32_t autotune;           /* 1: time the F-engine launch candidates at create and keep the fastest */
+  int32_t autotune;           /* 1: time the F-engine launch candidates and the MFMA X-engine's
+                                 8- and 16-channel tiles at create and keep the fastest */
   int32_t nphasecentres;      /* Model::getNumPhaseCentres; 0 or 1 = one phase centre, up to
                                  FXG_MAX_PHASECENTRES: results per phase centre, uv-shifted
                                  (core.cpp:1605-1938) */
