@@ -272,6 +272,30 @@ def test_fengine_launch_forms_bit_identical(gpu, xe):
     assert max(e["cross"]) < (1e-5 if xe == "fp32" else 1e-3)
 
 
+def test_two_positions_per_thread_form(gpu):
+    """L = 4096 at 256 threads (two butterfly positions per thread) against 512 threads (one):
+    a different instruction stream, so not bit-identical, but equal to FP32 rounding and both
+    within the FP32 bar against the oracle."""
+    from difx_amd import config as cf
+    offsets = (0, 1, 0)
+    mk = lambda t: cf.standard_config(4, 2, [4800.0, 4840.0], 32.0, 2048, subint_ns=8 * 64000, seed=33,
+                                      fengine_threads=t, max_delay_us=200.0)
+    data, ctrl = slot(mk(0), offsets, seed=34)
+    out = {}
+    for t in (256, 512):
+        core = _core(mk(t))
+        out[t] = core.processdata(offsets, data, ctrl).astype(np.complex128)
+        core.close()
+    ref, _ = fo.process_subint(mk(0), list(offsets), data, ctrl)
+    lay = fo.result_layout(mk(0))
+    rel = float(np.mean(np.abs(out[256] - out[512])) / np.mean(np.abs(out[512])))
+    _say("L 4096, 256 vs 512 threads: mean |diff| / mean |V|", rel)
+    assert rel < 1e-6
+    for t in (256, 512):
+        e = record_errors(mk(t), lay, out[t], ref)
+        assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+
+
 def test_autotune_picks_a_form(gpu):
     cfg = case_cfg3(B=64, xengine="fp16", autotune=1)
     core = _core(cfg)
