@@ -25,7 +25,8 @@ Further sections of the same JSON line (each can be skipped with its --no-* flag
   cfg1_10_stations    the 10-station array-scale configuration, MFMA and plain
   mfma_bf16           BF16 X-engine inputs on the 16-station workload
   uneven_bands        mixed bandwidths and zooms, both F-engine launch forms, autotuner report
-  serving             Poisson job stream through the server (p50 / p99 subint latency)
+  serving             Poisson job stream through the server (p50 / p99 subint latency), also
+                      under an HBM budget of a third of its peak footprint
   kurtosis            the 10-station subint with the spectral-kurtosis sums enabled
   phase_cal           the 10-station subint with phase-cal extraction on every band (1 MHz comb)
   output_side         device scaling (writedata) and DIFX records of one subint's results
@@ -138,8 +139,7 @@ def serving_jobs(rng, window_s: float, rate_per_s: float, subints_per_job: int):
     draws its station count from {4, 8, 16, 32}, its bandwidth as m x 32 MHz with m from
     {2, 4, 8, 16} (2-bit bands of one sample word must be a power of two) recorded as m/2
     freqs x 2 pols, nchan from {256, 1024, 4096}; 0.512 s subints (0.5 s is not a whole
-    number of 4096-channel FFTs at 32 MHz).  Delays stay within +-36 us so that the stations
-    of a job can share one input buffer (see serving_inputs)."""
+    number of 4096-channel FFTs at 32 MHz); delay models of up to +-10 ms and 1 us/s."""
     from difx_amd import config as cf
     t, jobs = 0.0, []
     while True:
@@ -155,34 +155,42 @@ def serving_jobs(rng, window_s: float, rate_per_s: float, subints_per_job: int):
         per_fft = 4 * nchan * m * 32 * ((2 * n + 31) // 32)
         chunk = max(64, min(B, (512 << 20) // per_fft)) // 8 * 8
         cfg = cf.standard_config(n, 2, freqs, 32.0, nchan, subint_ns=512_000_000, xengine="fp16",
-                                 seed=int(rng.integers(1 << 30)), chunk_ffts=chunk, max_delay_us=40.0)
+                                 seed=int(rng.integers(1 << 30)), chunk_ffts=chunk, max_delay_us=1e4)
         jobs.append((t, cfg, subints_per_job, (n, m, nchan)))
     return jobs
 
 
 def serving_inputs(cfg, k: int):
-    """Shared-buffer inputs of subint k of a serving job: (buffer bytes, control words) valid
-    for every station.  The buffer holds the slot capacity (getDataBytes plus four frames) and
-    starts one frame before the earliest sample any station needs (its delay at the subint
-    start), which leaves room for the +-36 us spread of the stations' delays."""
-    import math
+    """Inputs of subint k of a serving job: the slot buffer bytes (getDataBytes plus four
+    frames) and each station's control words.  Every station's buffer starts one frame before
+    the earliest sample it needs (its own delay at the subint start), so the FFT windows of
+    every station lie inside its buffer whatever the delay spread."""
     from difx_amd import synth
     ds = cfg.datastreams[0]
-    frame_ns = int(round(synth.samples_per_frame(ds) * cfg.sampletime_us(0) * 1000.0))
-    t0 = (1 + k) * 1e9
-    first = min(t0 - 1000.0 * synth.eval_delay(d.model, d.model_order, t0 / 1e9, cfg.model_inc_secs)
-                for d in cfg.datastreams)
-    start = int(math.floor(first / frame_ns) - 1) * frame_ns
     nbytes = synth.data_bytes(cfg, 0) + 4 * ds.framebytes
-    return nbytes, synth.control_words(cfg, 0, [0, 1 + k, 0], start)
+    offs = [0, 1 + k, 0]
+    ctrl = [synth.control_words(cfg, d, offs, synth.buffer_start_ns(cfg, d, offs))
+            for d in range(len(cfg.datastreams))]
+    return nbytes, ctrl
 
 
-def serving_bench(torch, dev, window_s=6.0, rate_per_s=4.0, subints_per_job=8, seed=20261204):
-    """Serving run: jobs arrive as a Poisson process, each pushes one 0.512 s subint every
-    0.512 s of wall time from its arrival (real-time pacing), the library's scheduler admits
-    them by device footprint and overlaps them.  Input buffers are device-resident and shared
-    by every station and job of one datastream shape (their content does not affect the
-    timing; every FFT window lies inside them, so no work is skipped)."""
+def serving_footprint(cfg, inflight: int = 2) -> int:
+    """The server's admission footprint of a job with device-resident inputs: the context's
+    device bytes plus its result buffers (fxg_server.cpp, fxg_server_add_job)."""
+    from difx_amd.core import footprint
+    dev, crl, _ = footprint(cfg)
+    return dev + 8 * max(crl, 1) * (1 + inflight)
+
+
+def serving_bench(torch, dev, window_s=3.0, rate_per_s=4.0, subints_per_job=59, seed=20261204,
+                  hbm_budget_bytes=0):
+    """Serving run: jobs of 59 subints of 0.512 s (30.2 s) arrive as a Poisson process of 4 jobs/s
+    over `window_s`; each pushes one subint every 0.512 s of wall time from its arrival
+    (real-time pacing); the library's scheduler admits them by device footprint (within
+    `hbm_budget_bytes`, 0 = the device's free memory) and overlaps them.  The stations of one
+    datastream shape read the same device-resident bytes, each through its own control words
+    (its own buffer start time for its delay): the content does not affect the timing and every
+    FFT window lies inside the buffer, so no work is skipped."""
     from difx_amd.server import FxServer
     rng = np.random.default_rng(seed)
     jobs = serving_jobs(rng, window_s, rate_per_s, subints_per_job)
@@ -205,10 +213,11 @@ def serving_bench(torch, dev, window_s=6.0, rate_per_s=4.0, subints_per_job=8, s
         for k in range(nsub):
             events.append((t0 + k * 0.512, j, k))
     events.sort()
+    inputs = {(j, k): serving_inputs(jobs[j][1], k) for _, j, k in events}   # before the clock starts
     torch.cuda.synchronize()
     ids = {}
     total_samples = sum(samples_per_subint(c) * n for _, c, n, _ in jobs)
-    with FxServer() as srv:
+    with FxServer(hbm_budget_bytes=hbm_budget_bytes) as srv:
         t_start = time.perf_counter()
         for t_ev, j, k in events:
             delay = t_ev - (time.perf_counter() - t_start)
@@ -217,9 +226,9 @@ def serving_bench(torch, dev, window_s=6.0, rate_per_s=4.0, subints_per_job=8, s
             t0, cfg, nsub, shape = jobs[j]
             if k == 0:
                 ids[j] = srv.add_job(cfg, nsub, data_on_device=True)
-            nbytes, ctrl = serving_inputs(cfg, k)
+            nbytes, ctrl = inputs[(j, k)]
             nds = len(cfg.datastreams)
-            srv.push(ids[j], (0, 1 + k, 0), [pool[shape[1:]].data_ptr()] * nds, [nbytes] * nds, [ctrl] * nds)
+            srv.push(ids[j], (0, 1 + k, 0), [pool[shape[1:]].data_ptr()] * nds, [nbytes] * nds, ctrl)
         per_job = [srv.wait_job(ids[j]) for j in range(len(jobs))]
         t_end = time.perf_counter()
         st = srv.stats()
@@ -227,12 +236,14 @@ def serving_bench(torch, dev, window_s=6.0, rate_per_s=4.0, subints_per_job=8, s
     offered = total_samples / (events[-1][0] + 0.512) / 1e9 if events else 0.0
     return {"jobs": len(jobs), "subints": st["subints_done"],
             "arrival_rate_jobs_per_s": rate_per_s, "window_s": window_s, "subints_per_job": subints_per_job,
+            "hbm_budget_gb": hbm_budget_bytes / 1e9 if hbm_budget_bytes else None,
             "offered_gsamples_per_s": offered,
             "throughput_gsamples_per_s": st["samples_done"] / busy / 1e9 if busy > 0 else 0.0,
             "p50_subint_latency_ms": st["latency_p50_ms"], "p99_subint_latency_ms": st["latency_p99_ms"],
             "max_subint_latency_ms": st["latency_max_ms"], "subint_length_ms": 512.0,
             "max_concurrent_jobs": st["max_active"],
             "peak_admitted_gb": st["peak_admitted_bytes"] / 1e9,
+            "largest_job_footprint_gb": max(serving_footprint(c) for _, c, _, _ in jobs) / 1e9 if jobs else 0.0,
             "worst_job_p99_ms": max(p[2] for p in per_job) if per_job else 0.0,
             "wall_s": t_end - t_start,
             "job_shapes_n_m_nchan": [list(j[3]) for j in jobs]}
@@ -700,6 +711,11 @@ def main():
     if not args.no_serving:
         print("[bench] serving: Poisson job stream", file=sys.stderr, flush=True)
         serving = serving_bench(torch, dev)
+        # the same stream under an HBM budget of a third of what it admitted at its peak (at
+        # least the largest job): admission by footprint makes later jobs wait for earlier ones
+        budget = int(max(serving["peak_admitted_gb"] / 3, serving["largest_job_footprint_gb"]) * 1e9)
+        print("[bench] serving: the same stream under a third of its peak footprint", file=sys.stderr, flush=True)
+        serving["under_budget"] = serving_bench(torch, dev, hbm_budget_bytes=budget)
     kcfg = workload_config("fp16", nstations=10)
     kbase = None
     if not (args.no_kurtosis and args.no_pcal):

@@ -165,8 +165,7 @@ def control_words(cfg, d: int, offsets, start_ns: int, B: Optional[int] = None,
     flags[:B] = 1
     for i in invalid_ffts:
         flags[i] = 0
-    for k in range(nflag):
-        c[3 + k] = int(sum(int(flags[k * 30 + j]) << j for j in range(30)))
+    c[3:] = (flags.reshape(nflag, 30) << np.arange(30, dtype=np.int64)).sum(axis=1).astype(np.int32)
     return c
 
 

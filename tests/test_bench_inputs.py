@@ -1,5 +1,6 @@
-"""CPU check of the serving benchmark's shared input buffers: with every station of a job
-reading one buffer, each station's every FFT window must lie inside it, or the benchmark would
+"""CPU check of the serving benchmark's shared input buffers: every station of a job reads the
+same device bytes through its own control words (its own buffer start for its delay of up to
++-10 ms); each station's every FFT window must lie inside the buffer, or the benchmark would
 skip work and overstate its throughput."""
 import numpy as np
 
@@ -28,7 +29,7 @@ def test_serving_inputs_cover_every_fft():
                                                offsets[1] + offsets[2] / 1e9, B * f0.nchan / f0.bw_mhz / 1e6, B)
                 interp = interp.copy()
                 interp[2] -= 1e6 * fo.intclockseconds(ds)
-                ns = fo.fft_params(cfg, d, offsets, ctrl, interp, B)[0]
+                ns = fo.fft_params(cfg, d, offsets, ctrl[d], interp, B)[0]
                 assert ns.min() >= 0 and ns.max() + L <= nsamples, (shape, k, d, ns.min(), ns.max() + L, nsamples)
                 checked += 1
     assert checked > 50
