@@ -110,7 +110,7 @@ def footprint(cfg: Config):
     dev, crl, slot = C.c_int64(), C.c_int64(), C.c_int64()
     rc = lib.fxg_footprint(C.byref(fc), C.byref(dev), C.byref(crl), C.byref(slot))
     if rc != abi.FXG_OK:
-        raise FxgError(f"fxg_footprint: {lib.fxg_strerror(rc).decode()}")
+        raise FxgError(f"fxg_footprint: {lib.fxg_strerror(rc).decode()}: {lib.fxg_last_error(None).decode()}")
     return dev.value, crl.value, slot.value
 
 
@@ -131,7 +131,8 @@ class FxCore:
         h = C.c_void_p()
         rc = self.lib.fxg_create(C.byref(self._abi), C.byref(h))
         if rc != abi.FXG_OK:
-            raise FxgError(f"fxg_create failed: {self.lib.fxg_strerror(rc).decode()}")
+            raise FxgError(f"fxg_create failed: {self.lib.fxg_strerror(rc).decode()}: "
+                           f"{self.lib.fxg_last_error(None).decode()}")
         self.h = h
         lay = abi.fxg_layout()
         self._check(self.lib.fxg_get_layout(self.h, C.byref(lay)))

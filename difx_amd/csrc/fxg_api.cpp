@@ -235,6 +235,12 @@ struct fxg_ctx {
 
 namespace {
 
+// The message of this thread's last failed create (the context it belonged to is gone).
+std::string& create_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+
 int fail(fxg_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -1989,7 +1995,10 @@ const char* fxg_strerror(int code) {
   return "unknown error";
 }
 
-const char* fxg_last_error(const fxg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* fxg_last_error(const fxg_ctx* ctx) {
+  if (ctx) return ctx->err.c_str();
+  return create_error().empty() ? "null context" : create_error().c_str();
+}
 
 }  // extern "C"
 
@@ -2171,14 +2180,23 @@ int pcal_configure(fxg_ctx* ctx, int ds, long long interval_hz, long long base_h
 }
 
 int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
-  if (!cfg || !out) return FXG_E_INVALID;
+  create_error().clear();
+  if (!cfg || !out) {
+    create_error() = "null configuration or output pointer";
+    return FXG_E_INVALID;
+  }
   *out = nullptr;
   if (cfg->nfreqs < 1 || !cfg->freqs || cfg->ndatastreams < 1 || !cfg->datastreams ||
-      cfg->nbaselines < 0 || (cfg->nbaselines > 0 && !cfg->baselines))
+      cfg->nbaselines < 0 || (cfg->nbaselines > 0 && !cfg->baselines)) {
+    create_error() = "empty or missing freq, datastream or baseline table";
     return FXG_E_INVALID;
+  }
   if (!dry) {
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FXG_E_NODEVICE;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+      create_error() = "no HIP device";
+      return FXG_E_NODEVICE;
+    }
   }
   fxg_ctx* ctx = new fxg_ctx();
   ctx->dry = dry;
@@ -2257,7 +2275,7 @@ int create_impl(const fxg_config* cfg, fxg_ctx** out, bool dry) {
   }
   if (rc == FXG_OK && !dry && ctx->cfg_autotune) rc = autotune(ctx);
   if (rc != FXG_OK) {
-    std::fprintf(stderr, "fxg_create: %s\n", ctx->err.c_str());
+    create_error() = ctx->err;                   // fxg_last_error(NULL) returns it
     free_ctx(ctx);
     delete ctx;
     return rc;

@@ -171,6 +171,8 @@ int fxg_create(const fxg_config* cfg, fxg_ctx** out);
 int fxg_create_layout(const fxg_config* cfg, fxg_ctx** out);
 int fxg_destroy(fxg_ctx* ctx);
 const char* fxg_strerror(int code);
+/* The detailed message of the context's last failure; with a null context, that of this
+ * thread's last failed fxg_create / fxg_create_layout / fxg_footprint. */
 const char* fxg_last_error(const fxg_ctx* ctx);
 
 int fxg_get_layout(const fxg_ctx* ctx, fxg_layout* out);

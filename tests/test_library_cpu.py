@@ -98,3 +98,23 @@ def test_validation(mutate, ok):
     else:
         with pytest.raises(FxgError):
             footprint(cfg)
+
+
+def test_create_failure_message_without_a_context():
+    """A failed create leaves no context; fxg_last_error(NULL) gives its message."""
+    import ctypes as C
+    from difx_amd import fxg_abi as abi
+    from difx_amd.core import load_library
+    from fxcases import case_cfg0
+    lib = load_library()
+    cfg = case_cfg0(B=8)
+    fc, keep = cfg.to_abi()
+    fc.fringe_rot_order = 7
+    h = C.c_void_p()
+    assert lib.fxg_create_layout(C.byref(fc), C.byref(h)) == abi.FXG_E_INVALID
+    assert not h.value
+    assert b"fringe rotation order" in lib.fxg_last_error(None)
+    fc.fringe_rot_order = 1
+    assert lib.fxg_create_layout(C.byref(fc), C.byref(h)) == abi.FXG_OK
+    assert lib.fxg_last_error(None) == b"null context"
+    lib.fxg_destroy(h)
