@@ -1779,7 +1779,9 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
       ka.chunkcap = ctx->chunkcap;
       ka.mode = ctx->xengine;
       TimedLaunch tk(ctx, 2, st);
-      if (fxg_launch_kurt(ka, (int)ctx->kurt_tab.size(), ctx->kurt_maxC, st))
+      int mintw = 4;
+      for (const auto& kb : ctx->kurt_tab) mintw = std::min(mintw, kb.twlog);
+      if (fxg_launch_kurt(ka, (int)ctx->kurt_tab.size(), ctx->kurt_maxC, mintw, st))
         return fail(ctx, FXG_E_HIP, "kurtosis launch");
     }
     ctx->last_chunk0 = c0;

@@ -346,7 +346,7 @@ struct KurtArgs {
   float2* acc;
   int nffts, chunkcap, mode;   // mode: FXG_X_FP32, FXG_X_FP16 or FXG_X_BF16
 };
-int fxg_launch_kurt(const KurtArgs& a, int nband, int maxC, hipStream_t st);
+int fxg_launch_kurt(const KurtArgs& a, int nband, int maxC, int mintwlog, hipStream_t st);
 
 // Phase-cal folding (the PCal extractors fed by Mode::process, mode.cpp:758-772): the unpacked
 // samples of each valid FFT, indexed by their sample number since the scan start, summed into
