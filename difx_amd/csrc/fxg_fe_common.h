@@ -584,6 +584,17 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
       constexpr bool HOIST_ROT = (V == 1) && (FXG_F_HOIST & 1);
       constexpr bool HOIST = (V == 1) && (FXG_F_HOIST & 2);
       constexpr int NKEEP = CPLX ? 8 : 4;
+      // Post-F rotation of real data (order 0): the FFT input is real, so two bands of the
+      // freq go through one complex FFT as a + ib and are split after it,
+      // A[j] = (Z[j] + conj Z[-j]) / 2 and B[j] = -i (Z[j] - conj Z[-j]) / 2 (the reference
+      // takes one real-to-complex FFT per band, mode.cpp:1139-1156).  Bands pair (0, 1),
+      // (2, 3) when their count is even and each pair shares its phasor table; the 1/2 rides
+      // on the phasors.
+      constexpr bool PAIRABLE = (ORDER == 0) && !CPLX;
+      bool fpair = PAIRABLE && (rf.nbands % 2 == 0);
+      if (PAIRABLE)
+        for (int kb = 1; kb < rf.nbands; kb += 2)
+          if (dsp->rf[f].newfrac[kb]) fpair = false;
       auto make_rot = [&](int u, float2* rot8) {
         const int vt = tl + u * TPF;
         const float2 sub = ORDER != 0 ? srot[vt & (S - 1)] : make_float2(1.0f, 0.0f);
@@ -620,7 +631,8 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
       auto make_frq = [&](int u, float2* frqk) {
         const int vt = tl + u * TPF;
         const float2 fsub0 = sfrac[vt & (S - 1)];         // the same sub-phasor for every m
-        const float2 fsub = make_float2(fsub0.x * fs, fsub0.y * fs);
+        const float fsx = (PAIRABLE && fpair) ? 0.5f * fs : fs;
+        const float2 fsub = make_float2(fsub0.x * fsx, fsub0.y * fsx);
         const int f0 = S + vt / S;
 #pragma unroll
         for (int m = 0; m < NKEEP; ++m) frqk[m] = cmul(fsub, sfrac[f0 + m * (NT / S)]);
@@ -634,8 +646,9 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           if (!((fok[0] >> s) & 1u)) rot_h[s] = make_float2(0.0f, 0.0f);
       }
       if (HOIST) make_frq(0, frq_h);
-      for (int kb = 0; kb < rf.nbands; ++kb) {
+      for (int kb = 0; kb < rf.nbands; kb += (PAIRABLE && fpair) ? 2 : 1) {
         const int band = dsp->rf[f].band[kb];
+        const int band2 = (PAIRABLE && fpair) ? dsp->rf[f].band[kb + 1] : band;
         if (kb > 0 && dsp->rf[f].newfrac[kb]) {  // another pol's phasor table (uniform branch)
           __syncthreads();                        // the previous band's phasor reads are done
           build_frac<ORDER>(sfrac, k, rf, CPLX, cf, fd, dsp->rf[f].bhd[kb], make_float2(dsp->rf[f].pcre[kb], dsp->rf[f].pcim[kb]), tl, TPF);
@@ -650,26 +663,30 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
           if (!CPLX) {
             // the sample width is tested once per band, not per sample, so the eight decodes
             // are straight-line code
-            float x[8];
+            float x[8], x2[8];
             if (twobit) {                            // levels from the LDS table of the format
 #pragma unroll
               for (int s = 0; s < 8; ++s) {
-                const int sh = 2 * (int)(((qlo[u] >> (4 * s)) & 15u) + band);
-                const float lv = slev2[(wv[u][s] >> sh) & 3u];
+                const int q = (int)((qlo[u] >> (4 * s)) & 15u);
+                const float lv = slev2[(wv[u][s] >> (2 * (q + band))) & 3u];
                 // invalid samples: zeroed through the rotator when it is kept per freq
                 x[s] = (HOIST_ROT && ORDER != 0) || ((fok[u] >> s) & 1u) ? lv : 0.0f;
+                x2[s] = 0.0f;
+                if (PAIRABLE && fpair && ((fok[u] >> s) & 1u)) x2[s] = slev2[(wv[u][s] >> (2 * (q + band2))) & 3u];
               }
             } else {
 #pragma unroll
               for (int s = 0; s < 8; ++s) {
                 x[s] = 0.0f;
+                x2[s] = 0.0f;
                 if ((fok[u] >> s) & 1u) x[s] = slut8[data[wv[u][s] + band]];
+                if (PAIRABLE && fpair && ((fok[u] >> s) & 1u)) x2[s] = slut8[data[wv[u][s] + band2]];
               }
             }
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
               const float2 rot = HOIST_ROT ? rot_h[s] : rot_l[HOIST_ROT ? 0 : s];
-              v[u][s] = ORDER == 0 ? make_float2(x[s], 0.0f)              // real FFT input
+              v[u][s] = ORDER == 0 ? make_float2(x[s], x2[s])             // real input(s): a + ib
                                    : make_float2(x[s] * rot.x, x[s] * rot.y);  // (x + 0i) * rotator
             }
             continue;
@@ -692,6 +709,36 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
         fft_stockham<L, V, TPF>(v, sbuf, stw, tl);
         // channel selection and fractional-sample correction
         float2 Fo[V][NKEEP];                      // output channels vt + m*NT
+        float2 Fo2[PAIRABLE ? V : 1][PAIRABLE ? NKEEP : 1];   // the pair's second band
+        if (PAIRABLE && fpair) {
+          // Z[-j] of every kept channel j sits with another thread: through sbuf in natural
+          // order (L <= sbuf entries)
+          __syncthreads();                        // the FFT's last reads of sbuf are done
+#pragma unroll
+          for (int u = 0; u < V; ++u) {
+            const int vt = tl + u * TPF;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) sbuf[vt + s * NT] = v[u][s];
+          }
+          __syncthreads();
+#pragma unroll
+          for (int u = 0; u < V; ++u) {
+            const int vt = tl + u * TPF;
+            float2 frq_l[HOIST ? 1 : NKEEP];
+            if (!HOIST) make_frq(u, frq_l);
+#pragma unroll
+            for (int m = 0; m < NKEEP; ++m) {
+              // USB keeps X[vt + m*NT], real LSB X[C + vt + m*NT] (mode.cpp:1205)
+              const bool upper = rf.lsb != 0;
+              const int j = vt + m * NT + (upper ? 4 * NT : 0);
+              const float2 frq = HOIST ? frq_h[m] : frq_l[HOIST ? 0 : m];
+              const float2 zj = upper ? v[u][(m + 4) & 7] : v[u][m];
+              const float2 zm = sbuf[(L - j) & (L - 1)];
+              Fo[u][m] = cmul(make_float2(zj.x + zm.x, zj.y - zm.y), frq);
+              Fo2[u][m] = cmul(make_float2(zj.y + zm.y, zm.x - zj.x), frq);   // -i (Z[j] - conj Z[-j])
+            }
+          }
+        } else {
 #pragma unroll
         for (int u = 0; u < V; ++u) {
           const int vt = tl + u * TPF;
@@ -710,10 +757,12 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
             Fo[u][m] = cmul(X, frq);
           }
         }
+        }
         if (!active) continue;
         // the stores: the output form is tested once per band, so each form's stores are
         // straight-line code.  Channel c = vt + m*NT; NT is a multiple of 8, so c >> 3 =
         // (vt >> 3) + m*NT/8 and c & 7 = vt & 7: per-thread bases with uniform strides.
+        auto store_band = [&](const int band, const float2 (&Fo)[V][NKEEP]) {
         if (out_mode == 0) {
           float2* __restrict__ o32 = args.spec32 + spec_off32 + ((long long)band * cap + lt) * C;
 #pragma unroll
@@ -722,7 +771,7 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
 #pragma unroll
             for (int m = 0; m < NKEEP; ++m) o32t[m * NT] = Fo[u][m];
           }
-          continue;
+          return;
         }
         auto store16 = [&](auto mode_c) {
           constexpr int OM = decltype(mode_c)::value;
@@ -760,6 +809,11 @@ __device__ void fe_range(const FengineArgs& args, const float2* __restrict__ twi
         };
         if (out_mode == 1) store16(IntC<1>());
         else store16(IntC<2>());
+        };
+        store_band(band, Fo);
+        if constexpr (PAIRABLE) {
+          if (fpair) store_band(band2, Fo2);
+        }
       }
     }
   }

@@ -238,6 +238,60 @@ def test_rotation_orders(gpu, order, xe, bound):
     assert max(e["cross"]) < bound and max(e["auto"]) < bound
 
 
+@pytest.mark.parametrize("name", ["usb", "lsb", "offsets", "zero_delta", "mark5b_zones", "real8", "damaged",
+                                  "long_ffts"])
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
+def test_order0_band_pairs(gpu, name, xe, bound):
+    """Post-F rotation (order 0) of real data: the two pols of a freq go through one complex
+    FFT as a + ib and are split after it.  Dual-pol USB and LSB freqs, pols with their own
+    phasor tables (clock deltas and phase offsets: one FFT per band) and without (deltas and
+    offsets zero: pairs), Mark5B fill-pattern zones, 8-bit samples, flagged FFTs and invalid
+    frames, and 4096/8192-point FFTs with zooms (several butterfly positions per thread).
+    Every band's FP32 spectra are checked, the pair's second band included."""
+    from difx_amd import config as cf
+    make_slot = None
+    if name in ("usb", "lsb", "damaged"):
+        cfg = case_cfg1(B=16, nstations=4, nfreq=2, xengine=xe, lsb=(name == "lsb"), fringe_rot_order=0)
+    elif name in ("offsets", "zero_delta"):
+        cfg = case_offsets(B=32, xengine=xe, fringe_rot_order=0)
+        if name == "zero_delta":
+            for d in cfg.datastreams:
+                d.clock_offset_delta_us = [0.0 for _ in d.recfreqs]
+                d.phase_offset_deg = [0.0 for _ in d.recfreqs]
+    elif name == "mark5b_zones":
+        cfg = case_mark5b(B=32, xengine=xe, fringe_rot_order=0)
+        make_slot = mark5b_partial_slot
+    elif name == "real8":
+        cfg = case_real8(B=32, xengine=xe, fringe_rot_order=0)
+    else:
+        groups = [(64.0, 4096, [6000.0]), (32.0, 2048, [6100.0])]
+        zooms = [(0, 0, 5.0, 1.0), (1, 0, 3.0, 1.0)]
+        cfg = cf.batched_config(3, groups, zooms, subint_ns=8 * 64000, npol=2, xengine=xe, seed=6,
+                                fringe_rot_order=0)
+    got, ref, e, lay = _run_and_compare(cfg, damaged=(name == "damaged"), make_slot=make_slot,
+                                        spectra_check=name not in ("offsets", "zero_delta"))
+    _say(f"order 0 pairs {name} {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound
+    if xe != "fp32" or name in ("offsets", "zero_delta"):
+        return
+    core = _core(cfg)
+    offsets = (0, 1, 0)
+    data, ctrl = (make_slot or (damaged_slot if name == "damaged" else slot))(cfg, offsets, seed=11)
+    core.processdata(offsets, data, ctrl)
+    _, stations = fo.process_subint(cfg, list(offsets), data, ctrl)
+    B = cfg.blockspersend()
+    worst = 0.0
+    for d in range(len(cfg.datastreams)):
+        for band in range(len(cfg.datastreams[d].recbands)):
+            for i in (0, B // 2, B - 1):
+                F = core.getFreqs(d, band, i)
+                R = stations[d].spectra[band, i]
+                worst = max(worst, float(np.max(np.abs(F - R)) / max(np.max(np.abs(R)), 1e-30)))
+    core.close()
+    _say(f"order 0 pairs {name}: worst spectrum error", worst)
+    assert worst < 2e-5
+
+
 @pytest.mark.parametrize("cs,lsb", [(1, True), (2, False), (2, True)])
 @pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
 def test_complex_sidebands(gpu, cs, lsb, xe, bound):
