@@ -32,6 +32,14 @@ constexpr int FLAGS_PER_INT = 30;                             // mpifxcorr.h:29
 constexpr int INVALID_SUBINT = -99999999;                     // mode.h:222
 constexpr int RING = 4;                                       // RECEIVE_RING_LENGTH, core.cpp:156
 
+// Two freqs covering the same sky range (lower band edge = edge - bandwidth for LSB) with the
+// same bandwidth, channels and averaging (configuration.cpp:3108-3135)
+bool same_sky_band(const fxg_freq& a, const fxg_freq& b) {
+  const double la = a.freq_mhz - (a.lsb ? a.bw_mhz : 0.0), lb = b.freq_mhz - (b.lsb ? b.bw_mhz : 0.0);
+  return std::fabs(la - lb) < TINY && std::fabs(a.bw_mhz - b.bw_mhz) < TINY && a.nchan == b.nchan &&
+         a.chanavg == b.chanavg;
+}
+
 // configuration.cpp:38-51
 int calcstridelength(int n) {
   if (n <= 16) return n;
@@ -133,6 +141,7 @@ struct fxg_ctx {
   std::vector<int> uvdec_idx;      // float index in results of each shift-decorrelation sum
   int* d_uvdec_idx = nullptr;
   std::vector<int> used, equiv;
+  std::vector<int> aclabel;        // [freq] freq index of its autocorrelation records
   // device tables
   std::vector<DevDs> h_ds;
   DevDs* d_ds = nullptr;
@@ -418,8 +427,15 @@ int validate_and_derive(fxg_ctx* ctx) {
             return fail(ctx, FXG_E_INVALID, "baseline band index out of range");
           int fi = bi < ds.nrecbands ? ds.recfreq_index[ds.recband_localfreq[bi]]
                                      : ds.zoomfreq_index[ds.zoomband_localfreq[bi - ds.nrecbands]];
-          if (fi != b.freq_index[k])
-            return fail(ctx, FXG_E_INVALID, "baseline products must correlate bands of the baseline freq");
+          // USB x LSB (configuration.cpp:3125-3135): the second datastream's recorded band may
+          // be the opposite sideband of the same sky band, real or single-sideband complex data,
+          // against a recorded band of the first
+          const bool mixed = s == 1 && bi < ds.nrecbands && bands[0] < ctx->dss[b.ds1].nrecbands &&
+                             ds.complex_sampling != 2 && ctx->dss[b.ds1].complex_sampling != 2 &&
+                             same_sky_band(ctx->freqs[fi], f) && ctx->freqs[fi].lsb != f.lsb;
+          if (fi != b.freq_index[k] && !mixed)
+            return fail(ctx, FXG_E_INVALID, "baseline products must correlate bands of the baseline freq "
+                                            "(or, on the second datastream, its opposite sideband)");
         }
       }
       if (f.nchan % f.chanavg) return fail(ctx, FXG_E_INVALID, "channel averaging must divide nchan");
@@ -507,8 +523,51 @@ static void pcal_tones_of(double lofreq, double bw, bool lsb, long long interval
   *ntones = n > 0 ? n : 1;                         // one placeholder tone when none fits
 }
 
+// Configuration::getOppositeSidebandFreqIndex (configuration.cpp:774-791): the last freq of the
+// other sideband with the same bandwidth and channels whose band covers the same sky range
+int opposite_sideband_freq(const fxg_ctx* ctx, int fi) {
+  const fxg_freq& f1 = ctx->freqs[fi];
+  int r = -1;
+  for (int i = 0; i < ctx->nfreqs; ++i) {
+    const fxg_freq& f = ctx->freqs[i];
+    if (f.bw_mhz == f1.bw_mhz && f.lsb != f1.lsb && f.nchan == f1.nchan && f.chanavg == f1.chanavg) {
+      if (f.lsb && f1.freq_mhz + f1.bw_mhz == f.freq_mhz) r = i;
+      if (!f.lsb && f.freq_mhz + f.bw_mhz == f1.freq_mhz) r = i;
+    }
+  }
+  return r;
+}
+
+// The freq index under which each freq's autocorrelations are written: with USB x LSB
+// products anywhere, an LSB freq correlated against a USB one is written as its opposite
+// sideband freq (visibility.cpp:912-925; correlatedagainstupper from the first product of
+// each baseline freq, configuration.cpp:1131-1158)
+void autocorr_labels(fxg_ctx* ctx) {
+  const int nf = ctx->nfreqs;
+  ctx->aclabel.resize(nf);
+  for (int i = 0; i < nf; ++i) ctx->aclabel[i] = i;
+  std::vector<char> against_upper(nf, 0);
+  bool any = false;
+  for (const auto& b : ctx->bls)
+    for (int k = 0; k < b.nfreqs; ++k) {
+      const int f1 = freq_of_band(ctx->dss[b.ds1], b.ds1band[k][0]);
+      const int f2 = freq_of_band(ctx->dss[b.ds2], b.ds2band[k][0]);
+      if (ctx->freqs[f1].lsb != ctx->freqs[f2].lsb) {
+        any = true;
+        against_upper[ctx->freqs[f1].lsb ? f1 : f2] = 1;
+      }
+    }
+  if (!any) return;
+  for (int i = 0; i < nf; ++i)
+    if (ctx->freqs[i].lsb && against_upper[i]) {
+      const int o = opposite_sideband_freq(ctx, i);
+      if (o >= 0) ctx->aclabel[i] = o;
+    }
+}
+
 void build_layout(fxg_ctx* ctx) {
   const int nf = ctx->nfreqs, nb = ctx->nbl;
+  autocorr_labels(ctx);
   ctx->used.assign(nf, 0);
   ctx->equiv.assign(nf, 0);
   int maxprod = 0;
@@ -953,11 +1012,23 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
   const bool mfma = ctx->xengine != FXG_X_FP32;
   // ---- MFMA groups: one per recorded freq table entry, rows = station bands ----
   if (mfma) {
+    // freqs that meet in USB x LSB products share a group (one contraction over the rows of
+    // both sidebands), keyed by the smallest freq index among them
+    std::vector<int> gkey(ctx->nfreqs);
+    for (int i = 0; i < ctx->nfreqs; ++i) gkey[i] = i;
+    auto root = [&](int i) { while (gkey[i] != i) i = gkey[i]; return i; };
+    for (const auto& bl : ctx->bls)
+      for (int k = 0; k < bl.nfreqs; ++k)
+        for (int p = 0; p < bl.npolprod[k]; ++p) {
+          const int a = root(freq_of_band(ctx->dss[bl.ds1], bl.ds1band[k][p]));
+          const int c = root(freq_of_band(ctx->dss[bl.ds2], bl.ds2band[k][p]));
+          if (a != c) gkey[std::max(a, c)] = std::min(a, c);
+        }
     std::map<int, int> gof;
     for (int d = 0; d < nds; ++d) {
       const fxg_datastream& s = ctx->dss[d];
       for (int b = 0; b < s.nrecbands; ++b) {
-        int fi = s.recfreq_index[s.recband_localfreq[b]];
+        int fi = root(s.recfreq_index[s.recband_localfreq[b]]);
         if (!gof.count(fi)) {
           gof[fi] = (int)ctx->groups.size();
           DevXGroup g;
@@ -984,7 +1055,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
             const int bi = side ? bl.ds2band[k][p] : bl.ds1band[k][p];
             const fxg_datastream& s = ctx->dss[d];
             if (bi < s.nrecbands || ctx->rowof.count({d, bi})) continue;
-            const int fi = freq_of_band(s, bi);
+            const int fi = root(freq_of_band(s, bi));
             if (!gof.count(fi)) {
               gof[fi] = (int)ctx->groups.size();
               DevXGroup g;
@@ -1164,6 +1235,7 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
         for (int p = 0; p < b.npolprod[k]; ++p) {
           auto g1 = ctx->rowof[{b.ds1, b.ds1band[k][p]}];
           auto g2 = ctx->rowof[{b.ds2, b.ds2band[k][p]}];
+          if (g1.first != g2.first) return fail(ctx, FXG_E_INVALID, "product rows in different MFMA groups");
           FinMfma e;
           e.group = g1.first;
           e.row1 = g1.second;
@@ -2559,7 +2631,7 @@ int fxg_difx_records(const fxg_ctx* ctx, const float* results, const fxg_difx_me
       int widx = ctx->acwoff[d] * 2;
       for (int j = 0; j < ctx->bpa; ++j) {
         for (int k = 0; k < s.nrecbands + s.nzoombands; ++k) {
-          const int fi = freq_of_band(s, k);
+          const int fi = ctx->aclabel[freq_of_band(s, k)];
           if (!(ctx->used[fi] || ctx->equiv[fi])) continue;
           const int fc = ctx->freqs[fi].nchan / ctx->freqs[fi].chanavg;
           const float w = (float)(results[widx++] / m->fftsperintegration);

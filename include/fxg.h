@@ -102,7 +102,9 @@ typedef struct fxg_datastream {
 
 /* One baseline (Configuration::baselinedata, configuration.h): datastream pair and, per
  * baseline freq, the band index pairs of each polarisation product.  Band indices count
- * recorded bands first, then zoom bands (mode.cpp:164-196). */
+ * recorded bands first, then zoom bands (mode.cpp:164-196).  Both bands of a product belong
+ * to the baseline freq, except for USB x LSB: the second datastream's band may be a recorded
+ * band of the opposite sideband covering the same sky band (configuration.cpp:3125-3135). */
 typedef struct fxg_baseline {
   int32_t ds1, ds2;                 /* ordered datastream indices */
   int32_t nfreqs;

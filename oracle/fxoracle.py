@@ -304,6 +304,52 @@ class Layout:
     pcal_off: List[int] = None               # per datastream: cf32 offset of its phase-cal tones
 
 
+def band_freq_index(ds, k: int) -> int:
+    """Freq table index of band k of a datastream (recorded bands, then zoom bands)."""
+    nrec = len(ds.recbands)
+    return ds.recfreqs[ds.recbands[k][0]] if k < nrec else ds.zoomfreqs[ds.zoombands[k - nrec][0]][0]
+
+
+def opposite_sideband_freq(cfg, fi: int) -> int:
+    """Configuration::getOppositeSidebandFreqIndex (configuration.cpp:774-791): the last freq
+    of the other sideband with the same bandwidth, channels and averaging whose band covers
+    the same sky range (exact comparisons, as there), or -1."""
+    f1 = cfg.freqs[fi]
+    r = -1
+    for i, f in enumerate(cfg.freqs):
+        if f.bw_mhz == f1.bw_mhz and f.lsb != f1.lsb and f.nchan == f1.nchan and f.chanavg == f1.chanavg:
+            if f.lsb and f1.freq_mhz + f1.bw_mhz == f.freq_mhz:
+                r = i
+            if not f.lsb and f.freq_mhz + f.bw_mhz == f1.freq_mhz:
+                r = i
+    return r
+
+
+def autocorr_freq_labels(cfg) -> List[int]:
+    """The freq index each freq's autocorrelation records carry (visibility.cpp:912-925): with
+    a USB x LSB product on any baseline (anyusbxlsb, configuration.cpp:3192-3203), an LSB freq
+    correlated against a USB freq (correlatedagainstupper, set from the first product of each
+    baseline freq, configuration.cpp:1131-1158) is written as its opposite sideband freq."""
+    labels = list(range(len(cfg.freqs)))
+    against_upper = [False] * len(cfg.freqs)
+    anymixed = False
+    for b in cfg.baselines:
+        for _, prods in b.freqs:
+            b1, b2 = prods[0]
+            f1 = band_freq_index(cfg.datastreams[b.ds1], b1)
+            f2 = band_freq_index(cfg.datastreams[b.ds2], b2)
+            if cfg.freqs[f1].lsb != cfg.freqs[f2].lsb:
+                anymixed = True
+                against_upper[f1 if cfg.freqs[f1].lsb else f2] = True
+    if anymixed:
+        for i, f in enumerate(cfg.freqs):
+            if f.lsb and against_upper[i]:
+                o = opposite_sideband_freq(cfg, i)
+                if o >= 0:
+                    labels[i] = o
+    return labels
+
+
 def freq_used(cfg) -> Tuple[List[bool], List[bool]]:
     """configuration.cpp:2270-2320: a freq is used if some baseline correlates it; an
     'equivalent' freq (same edge, bandwidth, channels, averaging) of a used freq is also

@@ -232,11 +232,14 @@ def records(cfg, res: np.ndarray, lay, mjd: int, seconds: float, configindex: in
                 o = core + (pc * len(prods) + p) * fc
                 out.append(src[o: o + fc].tobytes())
     if cfg.write_cross_autocorrs and pc == 0:
+        label = fo.autocorr_freq_labels(cfg)            # USB x LSB (visibility.cpp:912-925)
         for d, ds in enumerate(cfg.datastreams):
             off, widx = lay.autocorr_off[d], lay.acweight_off[d] * 2
             for j in range(lay.bandsperautocorr):
-                for k in _autocorr_bands(cfg, d, used, equiv):
-                    fi = band_freq(ds, k)
+                for k in range(len(ds.recbands) + len(ds.zoombands)):
+                    fi = label[band_freq(ds, k)]
+                    if not (used[fi] or equiv[fi]):
+                        continue
                     fc = cfg.freqs[fi].nchan // cfg.freqs[fi].chanavg
                     w = F32(F64(f32[widx]) / fftsperintegration)
                     if w > 0:

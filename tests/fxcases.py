@@ -131,6 +131,22 @@ def case_dsb_fraclo(B: int = 16, xengine: str = "fp32", seed: int = 10, **kw):
     return cfg
 
 
+def case_usb_lsb(B: int = 32, xengine: str = "fp32", seed: int = 21, **kw):
+    """USB x LSB: 4 stations dual pol; stations 0 and 1 record 4800-4832 MHz as USB (freq 0,
+    edge 4800), stations 2 and 3 record the same sky band as LSB (freq 1, edge 4832),
+    nchan 256.  Baselines within a sideband correlate their own freq; baselines across
+    correlate freq 0 (the first datastream's) with the LSB bands on the second datastream
+    (configuration.cpp:3125-3135)."""
+    cfg = cf.standard_config(4, 2, [4800.0], 32.0, 256, subint_ns=B * 8000, xengine=xengine, seed=seed, **kw)
+    cfg.freqs.append(cf.Freq(4832.0, 32.0, 256, lsb=True))
+    for d in (2, 3):
+        cfg.datastreams[d].recfreqs = [1]
+    for b in cfg.baselines:
+        if b.ds1 >= 2:
+            b.freqs = [(1, prods) for _, prods in b.freqs]
+    return cfg
+
+
 def case_mark5b(B: int = 32, xengine: str = "fp32", **kw):
     """cfg1 shape (4 stations, 4 freqs dual pol) recorded as Mark5B frames."""
     cfg = case_cfg1(B=B, xengine=xengine, nstations=4, nfreq=4, **kw)

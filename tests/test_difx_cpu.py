@@ -298,3 +298,79 @@ def test_phase_centre_layout_records_and_scale():
         sc = np.float32(1.0 / (np.float64(wf) * msi * np.float64(np.float32(cfg.blockspersend() * 2 * fc * fr.chanavg))))
         dec = np.float32(f32[2 * so + pc] / (1e9 * (msi * cfg.subint_ns / 1e9)))
         np.testing.assert_array_equal(out[o:o + fc], (res[o:o + fc].view(np.float32) * np.float32(sc / dec)).view(np.complex64))
+
+
+def test_usb_lsb_layout_and_autocorr_labels():
+    """USB x LSB products (configuration.cpp:3125-3135) on a host-only context: the layout
+    equals the restatement's, and the records equal it byte for byte.  Worked by hand: the
+    LSB freq 1 (edge 4832 MHz, 32 MHz) covers the sky band of the USB freq 0 (edge 4800 MHz)
+    and is correlated against it, so stations 2 and 3's autocorrelations are written under
+    freq 0 (visibility.cpp:912-925); the LSB x LSB baseline (2, 3) keeps freq 1."""
+    from fxcases import case_usb_lsb
+    lib = _lib()
+    cfg = case_usb_lsb(B=32)
+    assert fo.autocorr_freq_labels(cfg) == [0, 0]
+    assert fo.opposite_sideband_freq(cfg, 1) == 0 and fo.opposite_sideband_freq(cfg, 0) == 1
+    lay = fo.result_layout(cfg)
+    rng = np.random.default_rng(9)
+    res = _results(cfg, lay, rng, zero_some=False)
+    ffts = 2.0 * cfg.blockspersend()
+    h, keep = _layout_ctx(lib, cfg)
+    try:
+        from difx_amd import fxg_abi as abi
+        lt = abi.fxg_layout()
+        assert lib.fxg_get_layout(h, C.byref(lt)) == 0
+        assert lt.coreresultlength == lay.coreresultlength
+        m, arrs = _meta(cfg, rng, ffts)
+        nbytes, nrec = C.c_int64(), C.c_int32()
+        fres = np.ascontiguousarray(res.view(np.float32))
+        assert lib.fxg_difx_records(h, fres.ctypes.data, C.byref(m), None, 0, C.byref(nbytes), C.byref(nrec)) == 0
+        out = np.zeros(nbytes.value, dtype=np.uint8)
+        assert lib.fxg_difx_records(h, fres.ctypes.data, C.byref(m), out.ctypes.data, out.size, C.byref(nbytes),
+                                    C.byref(nrec)) == 0
+    finally:
+        lib.fxg_destroy(h)
+    bl, uvw, tel = arrs
+    assert out.tobytes() == vo.records(cfg, res, lay, 59000, 43200.25, 0, 2, ffts, bl, uvw, tel)
+    recs = vo.parse(out.tobytes(), {0: 256, 1: 256})
+    autos = [r for r in recs if r["baseline"] % 257 == 0]
+    assert autos and all(r["freq"] == 0 for r in autos)
+    cross = {(r["baseline"], r["freq"]) for r in recs if r["baseline"] % 257 != 0}
+    assert (256 * 3 + 4, 1) in cross and (256 * 1 + 3, 0) in cross
+
+
+@pytest.mark.parametrize("change", ["bandwidth", "edge", "nchan", "first_side", "dsb"])
+def test_usb_lsb_mismatches_rejected(change):
+    """A product across freqs is accepted only for the opposite sideband of the same sky band,
+    recorded on the second datastream, from single-sideband data."""
+    from fxcases import case_usb_lsb
+    lib = _lib()
+    cfg = case_usb_lsb(B=32)
+    f1 = cfg.freqs[1]
+    if change == "bandwidth":
+        f1.bw_mhz, f1.nchan = 16.0, 128
+    elif change == "edge":
+        f1.freq_mhz = 4840.0
+    elif change == "nchan":
+        f1.nchan = 512
+    elif change == "first_side":
+        b = cfg.baselines[1]                       # (0, 2) on freq 0: make it (2, 0) on freq 1
+        b.ds1, b.ds2 = b.ds2, b.ds1
+        b.freqs = [(1, prods) for _, prods in b.freqs]
+        cfg.baselines[1] = b
+        for d in (0, 1):
+            cfg.datastreams[d].recfreqs = [0]
+        # the first datastream's band is of the baseline freq (1, LSB), the second's is USB: accepted
+        fa, keep = cfg.to_abi()
+        h = C.c_void_p()
+        assert lib.fxg_create_layout(C.byref(fa), C.byref(h)) == 0
+        lib.fxg_destroy(h)
+        b.freqs = [(0, prods) for _, prods in b.freqs]   # baseline freq of the second datastream: refused
+    else:
+        for d in range(4):
+            cfg.datastreams[d].complex_sampling = 2
+            cfg.datastreams[d].nbits = 8
+    fa, keep = cfg.to_abi()
+    h = C.c_void_p()
+    assert lib.fxg_create_layout(C.byref(fa), C.byref(h)) != 0
+    assert b"opposite sideband" in lib.fxg_last_error(None)

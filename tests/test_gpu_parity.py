@@ -292,6 +292,19 @@ def test_order0_band_pairs(gpu, name, xe, bound):
     assert worst < 2e-5
 
 
+@pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3), ("bf16", 1e-2)])
+def test_usb_lsb_products(gpu, xe, bound):
+    """USB x LSB: two stations record a sky band as USB, two as LSB; the cross-sideband
+    baselines multiply channel c of the USB band by channel c of the LSB band (both ascending
+    in sky frequency, core.cpp:869-871).  The MFMA X-engine contracts both sidebands' rows in
+    one group."""
+    from fxcases import case_usb_lsb
+    cfg = case_usb_lsb(B=32, xengine=xe)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"usb x lsb {xe} cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < bound and max(e["auto"]) < bound
+
+
 @pytest.mark.parametrize("cs,lsb", [(1, True), (2, False), (2, True)])
 @pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3)])
 def test_complex_sidebands(gpu, cs, lsb, xe, bound):
