@@ -24,6 +24,8 @@ on a bounded sample of the same workload on the host, with all host threads and 
 Further sections of the same JSON line (each can be skipped with its --no-* flag):
   cfg1_10_stations    the 10-station array-scale configuration, MFMA and plain
   mfma_bf16           BF16 X-engine inputs on the 16-station workload
+  mfma_post_f         the 16-station workload with post-F fringe rotation (order 0), where
+                      the F-engine puts the two pols of a freq through one complex FFT
   uneven_bands        mixed bandwidths and zooms, both F-engine launch forms, autotuner report
   serving             Poisson job stream through the server (p50 / p99 subint latency), also
                       under an HBM budget of a third of its peak footprint
@@ -34,8 +36,8 @@ Further sections of the same JSON line (each can be skipped with its --no-* flag
                       a device-copy yardstick and the numpy restatement's rate
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-plain] [--no-uneven]
-                    [--no-serving] [--no-cfg1] [--no-bf16] [--no-kurtosis] [--no-pcal]
-                    [--no-output] [--no-mux]
+                    [--no-serving] [--no-cfg1] [--no-bf16] [--no-postf] [--no-kurtosis]
+                    [--no-pcal] [--no-output] [--no-mux]
     python bench.py --cpu-table      # the C restatement per configuration, 1 and all threads
 """
 from __future__ import annotations
@@ -56,12 +58,12 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
-def workload_config(xengine: str, nstations: int = 16):
+def workload_config(xengine: str, nstations: int = 16, **kw):
     """The array workload: nstations dual-pol, 8 x 32 MHz 2-bit VDIF, nchan 1024, 0.5 s."""
     from difx_amd import config as cf
     freqs = [4800.0 + 40.0 * k for k in range(8)]
     return cf.standard_config(nstations, 2, freqs, 32.0, 1024, subint_ns=500_000_000,
-                              xengine=xengine, seed=20261202 if nstations == 10 else 20261205)
+                              xengine=xengine, seed=20261202 if nstations == 10 else 20261205, **kw)
 
 
 def device_slot(cfg, offsets, torch, dev, seed=7):
@@ -609,6 +611,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cfg1", action="store_true")
     ap.add_argument("--no-bf16", action="store_true")
+    ap.add_argument("--no-postf", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mux", action="store_true")
@@ -674,6 +677,13 @@ def main():
         print(f"[bench] 16 stations, BF16 MFMA path, 1 warmup + {bsteps} steps", file=sys.stderr, flush=True)
         r = time_path(workload_config("bf16"), bsteps, 1, torch, dev, offsets)
         bf16 = path_summary(r, nsamp, bsteps)
+    postf = None
+    if not args.no_postf:
+        psteps = max(1, min(args.steps, 10))
+        print(f"[bench] 16 stations, post-F fringe rotation, MFMA path, 1 warmup + {psteps} steps",
+              file=sys.stderr, flush=True)
+        r = time_path(workload_config("fp16", fringe_rot_order=0), psteps, 1, torch, dev, offsets)
+        postf = path_summary(r, nsamp, psteps)
     cfg1 = None
     if not args.no_cfg1:
         c1 = workload_config("fp16", nstations=10)
@@ -798,6 +808,7 @@ def main():
         "issue_roofline": issue,
         "plain_fp32": plain,
         "mfma_bf16": bf16,
+        "mfma_post_f": postf,
         "cfg1_10_stations": cfg1,
         "uneven_bands": uneven,
         "serving": serving,
