@@ -164,6 +164,18 @@ struct fxg_ctx {
   // plain X
   std::vector<DevProd> prods;
   DevProd* d_prods = nullptr;
+  // blocked plain X-engine: groups of rows, their pair tables and work items; the products it
+  // does not take (rows of groups above 64, repeated row pairs) stay with the per-product kernel
+  std::vector<XPGroup> xp_groups;
+  std::vector<XPRow> xp_rows;
+  std::vector<int> xp_pairs;
+  std::vector<XPItem> xp_items;
+  std::vector<DevProd> prods_rest;
+  XPGroup* d_xp_groups = nullptr;
+  XPRow* d_xp_rows = nullptr;
+  int* d_xp_pairs = nullptr;
+  XPItem* d_xp_items = nullptr;
+  DevProd* d_prods_rest = nullptr;
   float2* d_xacc = nullptr;
   long long xacc_len = 0;
   int maxchan = 0;
@@ -1007,6 +1019,115 @@ int tile_width_log(const DevXGroup& g, int want) {
   return (want == 4 && g.ntile == 1 && g.C % 16 == 0) ? 4 : 3;
 }
 
+// The blocked plain X-engine's tables.  A row is a spectra band or zoom window with its channel
+// count, (datastream, band, channel offset, channels); products join rows into groups.  Each
+// group of at most 64 rows gets a pair table (the accumulator of the product of rows (i, j), or
+// -1) and work items of 8 x 8 row blocks over 64-channel tiles.  Products of groups above 64
+// rows or whose rows' datastreams differ in FFT stride, a second product of the same row pair,
+// and every product when the FP32 spectra reach 4 GiB (the kernel's offsets are 32-bit) stay
+// with the per-product kernel.
+void build_xplain_blocks(fxg_ctx* ctx) {
+  const auto& P = ctx->prods;
+  ctx->xp_groups.clear(); ctx->xp_rows.clear(); ctx->xp_pairs.clear(); ctx->xp_items.clear();
+  ctx->prods_rest.clear();
+  std::map<std::array<int, 4>, int> rid;
+  std::vector<std::array<int, 4>> rkey;
+  auto row = [&](int d, int b, int co, int nc) {
+    const std::array<int, 4> k = {d, b, co, nc};
+    auto f = rid.find(k);
+    if (f != rid.end()) return f->second;
+    const int r = (int)rkey.size();
+    rid[k] = r;
+    rkey.push_back(k);
+    return r;
+  };
+  std::vector<int> r1(P.size()), r2(P.size());
+  for (size_t i = 0; i < P.size(); ++i) {
+    r1[i] = row(P[i].ds1, P[i].band1, P[i].choff1, P[i].nchan);
+    r2[i] = row(P[i].ds2, P[i].band2, P[i].choff2, P[i].nchan);
+  }
+  std::vector<int> par(rkey.size());
+  for (size_t r = 0; r < par.size(); ++r) par[r] = (int)r;
+  auto root = [&](int x) { while (par[x] != x) x = par[x] = par[par[x]]; return x; };
+  for (size_t i = 0; i < P.size(); ++i) {
+    const int a = root(r1[i]), b = root(r2[i]);
+    if (a != b) par[std::max(a, b)] = std::min(a, b);
+  }
+  std::map<int, int> gid;
+  std::vector<std::vector<int>> grows;
+  std::vector<int> local(rkey.size());
+  for (size_t r = 0; r < rkey.size(); ++r) {
+    const int g = root((int)r);
+    if (!gid.count(g)) { gid[g] = (int)grows.size(); grows.emplace_back(); }
+    local[r] = (int)grows[gid[g]].size();
+    grows[gid[g]].push_back((int)r);
+  }
+  if (8LL * ctx->spec_elems >= (1LL << 32)) {
+    ctx->prods_rest = P;
+    return;
+  }
+  std::vector<int> gslot(grows.size(), -1);
+  for (size_t g = 0; g < grows.size(); ++g) {
+    const int n = (int)grows[g].size();
+    if (n > 64) continue;
+    bool onestride = true;
+    for (int r : grows[g]) onestride = onestride && ctx->h_ds[rkey[r][0]].C == ctx->h_ds[rkey[grows[g][0]][0]].C;
+    if (!onestride) continue;
+    XPGroup G;
+    G.nrows = n;
+    G.C = rkey[grows[g][0]][3];
+    G.stride = ctx->h_ds[rkey[grows[g][0]][0]].C;
+    G.pad = 0;
+    G.row0 = (int)ctx->xp_rows.size();
+    G.pair0 = (int)ctx->xp_pairs.size();
+    for (int r : grows[g]) {
+      const DevDs& x = ctx->h_ds[rkey[r][0]];
+      XPRow R;
+      R.off = x.spec_off + (long long)rkey[r][1] * ctx->chunkcap * x.C + rkey[r][2];
+      R.stride = x.C;
+      R.pad = 0;
+      ctx->xp_rows.push_back(R);
+    }
+    ctx->xp_pairs.resize(ctx->xp_pairs.size() + (size_t)n * n, -1);
+    gslot[g] = (int)ctx->xp_groups.size();
+    ctx->xp_groups.push_back(G);
+  }
+  for (size_t i = 0; i < P.size(); ++i) {
+    const int gs = gslot[gid[root(r1[i])]];
+    if (gs < 0) { ctx->prods_rest.push_back(P[i]); continue; }
+    const XPGroup& G = ctx->xp_groups[gs];
+    int& slot = ctx->xp_pairs[G.pair0 + (size_t)local[r1[i]] * G.nrows + local[r2[i]]];
+    if (slot >= 0) { ctx->prods_rest.push_back(P[i]); continue; }
+    slot = P[i].acc_off;
+  }
+  // Workgroups are dispatched to the eight XCDs in turn by workgroup index, and each XCD has
+  // its own L2: the blocks of one (group, channel tile), which read the same rows, go to one
+  // XCD (unit u to XCD u mod 8, its blocks at indices x, x + 8, ...); the shorter XCD lists
+  // are padded with empty items (group -1).
+  constexpr int NXCD = 8;
+  std::vector<std::vector<XPItem>> per(NXCD);
+  int unit = 0;
+  for (size_t gi = 0; gi < ctx->xp_groups.size(); ++gi) {
+    const XPGroup& G = ctx->xp_groups[gi];
+    const int nbk = (G.nrows + 7) / 8;
+    for (int c0 = 0; c0 < G.C; c0 += 64, ++unit)
+      for (int rb = 0; rb < nbk; ++rb)
+        for (int cb = rb; cb < nbk; ++cb) {
+          XPItem it;
+          it.group = (int)gi; it.rb = rb; it.cb = cb; it.c0 = c0;
+          per[unit % NXCD].push_back(it);
+        }
+  }
+  size_t len = 0;
+  for (const auto& v : per) len = std::max(len, v.size());
+  for (size_t k = 0; k < len; ++k)
+    for (int x = 0; x < NXCD; ++x) {
+      XPItem it;
+      it.group = -1; it.rb = 0; it.cb = 0; it.c0 = 0;
+      ctx->xp_items.push_back(k < per[x].size() ? per[x][k] : it);
+    }
+}
+
 int build_engines(fxg_ctx* ctx, int chunk_req) {
   const int nds = ctx->nds, nb = ctx->nbl;
   const bool mfma = ctx->xengine != FXG_X_FP32;
@@ -1095,7 +1216,8 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
   } else {
     for (int d = 0; d < nds; ++d) per_fft += (long long)ctx->h_ds[d].nrecbands * ctx->h_ds[d].C * 8;
   }
-  const long long budget = 4LL << 30;
+  // (FP32 spectra stay below 4 GiB: the blocked plain X-engine addresses them with 32-bit offsets)
+  const long long budget = mfma ? (4LL << 30) : (4LL << 30) - (256LL << 20);
   long long cap = chunk_req > 0 ? chunk_req : std::max<long long>(64, budget / std::max<long long>(per_fft, 1));
   cap = std::min<long long>(cap, round_up(ctx->B, FXG_XPAD));
   cap = round_up(cap, FXG_XPAD);
@@ -1225,6 +1347,15 @@ int build_engines(fxg_ctx* ctx, int chunk_req) {
     if (int rc_ = dev_malloc(ctx, (void**)&ctx->d_xacc, sizeof(float2) * std::max<long long>(acc, 2))) return rc_;
     if (int rc = dev_upload(ctx, &ctx->d_prods, ctx->prods)) return rc;
     if (int rc = dev_upload(ctx, &ctx->d_finp, ctx->finp)) return rc;
+    build_xplain_blocks(ctx);
+    if (!ctx->xp_items.empty()) {
+      if (int rc = dev_upload(ctx, &ctx->d_xp_groups, ctx->xp_groups)) return rc;
+      if (int rc = dev_upload(ctx, &ctx->d_xp_rows, ctx->xp_rows)) return rc;
+      if (int rc = dev_upload(ctx, &ctx->d_xp_pairs, ctx->xp_pairs)) return rc;
+      if (int rc = dev_upload(ctx, &ctx->d_xp_items, ctx->xp_items)) return rc;
+    }
+    if (!ctx->prods_rest.empty())
+      if (int rc = dev_upload(ctx, &ctx->d_prods_rest, ctx->prods_rest)) return rc;
   } else {
     for (int j = 0; j < nb; ++j) {
       const fxg_baseline& b = ctx->bls[j];
@@ -1829,16 +1960,29 @@ int run_subint(fxg_ctx* ctx, const int32_t offsets[3], const uint8_t* const* d_d
         g0 = g1;
       }
     } else {
-      XPlainArgs xa;
-      xa.prods = ctx->d_prods;
-      xa.ds = ctx->d_ds;
-      xa.spec32 = ctx->d_spec32;
-      xa.acc = ctx->d_xacc;
-      xa.nffts = n;
-      xa.chunkcap = ctx->chunkcap;
       TimedLaunch tl(ctx, 1, st);
-      if (fxg_launch_xplain(xa, (int)ctx->prods.size(), ctx->maxchan, st))
-        return fail(ctx, FXG_E_HIP, "X-engine launch");
+      if (!ctx->xp_items.empty()) {
+        XPBlkArgs ba;
+        ba.items = ctx->d_xp_items;
+        ba.groups = ctx->d_xp_groups;
+        ba.rows = ctx->d_xp_rows;
+        ba.pairs = ctx->d_xp_pairs;
+        ba.spec32 = ctx->d_spec32;
+        ba.acc = ctx->d_xacc;
+        ba.nffts = n;
+        if (fxg_launch_xplain_blk(ba, (int)ctx->xp_items.size(), st)) return fail(ctx, FXG_E_HIP, "X-engine launch");
+      }
+      if (!ctx->prods_rest.empty()) {
+        XPlainArgs xa;
+        xa.prods = ctx->d_prods_rest;
+        xa.ds = ctx->d_ds;
+        xa.spec32 = ctx->d_spec32;
+        xa.acc = ctx->d_xacc;
+        xa.nffts = n;
+        xa.chunkcap = ctx->chunkcap;
+        if (fxg_launch_xplain(xa, (int)ctx->prods_rest.size(), ctx->maxchan, st))
+          return fail(ctx, FXG_E_HIP, "X-engine launch");
+      }
     }
     if (ctx->kurt_on) {                            // before the next chunk overwrites the spectra
       KurtArgs ka;
@@ -2006,7 +2150,8 @@ void free_ctx(fxg_ctx* ctx) {
   f(ctx->d_twiddle); f(ctx->d_dslist); f(ctx->d_params); f(ctx->d_items[0]); f(ctx->d_items[1]);
   f(ctx->d_iclass[0]); f(ctx->d_iclass[1]);
   f(ctx->d_counter); f(ctx->d_bigscratch); f(ctx->d_weights); f(ctx->d_flags); f(ctx->d_spec32); f(ctx->d_spec16);
-  f(ctx->d_prods); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
+  f(ctx->d_prods); f(ctx->d_xp_groups); f(ctx->d_xp_rows); f(ctx->d_xp_pairs); f(ctx->d_xp_items);
+  f(ctx->d_prods_rest); f(ctx->d_xacc); f(ctx->d_groups); f(ctx->d_vacc); f(ctx->d_xscale);
   f(ctx->d_finp); f(ctx->d_finm); f(ctx->d_finw);
   f(ctx->d_normseg); f(ctx->d_normcal); f(ctx->d_calmean); f(ctx->d_tsys); f(ctx->d_dpct);
   f(ctx->d_kurt_tab); f(ctx->d_kurt_acc); f(ctx->d_pcal_groups); f(ctx->d_pcal_part); f(ctx->d_pcal_bins); f(ctx->d_pcal_tdev); f(ctx->d_pcal_tonebuf);

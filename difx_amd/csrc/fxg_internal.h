@@ -214,6 +214,32 @@ struct XPlainArgs {
   int chunkcap;
 };
 
+// Blocked plain FP32 X-engine.  A group is a set of rows (spectra bands, or zoom windows of
+// them, with one channel count) that products connect; a work item is one 8 x 8 block (row
+// block rb <= column block cb) of the group's row x row matrix over 64 consecutive channels.
+struct XPRow {
+  long long off;          // cf32 offset in the FP32 spectra of (band, FFT 0, window start)
+  int stride;             // cf32 entries between consecutive FFTs
+  int pad;
+};
+struct XPGroup {
+  int nrows, C, row0, pair0;   // pairs[pair0 + i*nrows + j]: accumulator of product (i, j) or -1
+  int stride;                  // cf32 entries between consecutive FFTs, the same for every row
+  int pad;
+};
+struct XPItem {
+  int group, rb, cb, c0;
+};
+struct XPBlkArgs {
+  const XPItem* items;
+  const XPGroup* groups;
+  const XPRow* rows;
+  const int* pairs;
+  const float2* spec32;
+  float2* acc;
+  int nffts;
+};
+
 struct XMfmaArgs {
   const DevXGroup* groups;
   const uint32_t* spec16;
@@ -274,6 +300,7 @@ int fxg_launch_setup(const SetupArgs& a, int maxframes, hipStream_t st);
 int fxg_launch_unpack(const UnpackArgs& a, hipStream_t st);
 int fxg_launch_fengine(const FengineArgs& a, int L, int iscomplex, int nds, int ntb, hipStream_t st);
 int fxg_launch_fftparams(const FengineArgs& a, int nds, hipStream_t st);
+int fxg_launch_xplain_blk(const XPBlkArgs& a, int nitems, hipStream_t st);
 int fxg_launch_fpersist(const FengineArgs& a, const FItem* items, const IClass* cls, int ncls, int* counters,
                         int nwg, int ntb, int iscomplex, size_t lds, hipStream_t st);
 size_t fxg_fengine_lds(int L, int ntb, int phcap, int order);

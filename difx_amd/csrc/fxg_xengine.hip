@@ -54,6 +54,115 @@ __global__ void __launch_bounds__(256) k_xmac_f32(XPlainArgs a) {
 }
 
 
+// Blocked form of the plain FP32 X-engine: one workgroup per work item (an 8 x 8 block of a
+// group's row x row matrix over 64 channels, one channel per lane).  Each of the four
+// wavefronts sums X_i conj(X_j) over a quarter of the chunk's FFTs in FFT order, loading 16
+// spectra values per FFT for 64 products; wave 0 then adds the four partial sums in wave order
+// and adds the result to each product's accumulator: product (i, j) takes the block entry,
+// product (j, i) its conjugate (off the diagonal blocks).  Autocorrelations (i = j) keep a zero
+// imaginary part.  Rows past the group and channels past C are clamped for the loads and never
+// written.  Items with group -1 are padding and do nothing.
+__global__ void __launch_bounds__(256) k_xmac_f32_blk(XPBlkArgs a) {
+  __shared__ float2 red[3][16][64];
+  const XPItem it = a.items[blockIdx.x];
+  if (it.group < 0) return;                      // padding item: the whole workgroup leaves
+  const XPGroup g = a.groups[it.group];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform: scalar loop and pointers
+  const int c = it.c0 + lane;
+  const int cl = min(c, g.C - 1);
+  // the row pointers are the same for every lane (scalar registers); each lane adds its channel
+  const int t0 = (int)((long long)w * a.nffts / 4), t1 = (int)((long long)(w + 1) * a.nffts / 4);
+  // 32-bit byte offsets from the spectra base (the host keeps the FP32 spectra below 4 GiB
+  // for this kernel), so each load is the scalar base plus one offset register
+  typedef float v2f __attribute__((ext_vector_type(2)));
+  typedef const __attribute__((address_space(1))) v2f* GF2;
+  typedef const __attribute__((address_space(1))) char* GB;
+  // The rows of a group share one FFT stride, so the row offsets are the same for every lane
+  // (scalar registers) and each load adds the lane's channel offset.
+  const GB base = (GB)a.spec32;
+  unsigned ox[8], oy[8];
+  const unsigned st = 8u * (unsigned)g.stride;
+  const unsigned co = (unsigned)cl * 8u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    ox[i] = (unsigned)(8LL * (a.rows[g.row0 + min(it.rb * 8 + i, g.nrows - 1)].off + (long long)t0 * g.stride));
+    oy[i] = (unsigned)(8LL * (a.rows[g.row0 + min(it.cb * 8 + i, g.nrows - 1)].off + (long long)t0 * g.stride));
+  }
+  float2 acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = make_float2(0.0f, 0.0f);
+  for (int t = t0; t < t1; ++t) {
+    float2 x[8], y[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const v2f vx = *(GF2)(base + (ox[i] + co));
+      const v2f vy = *(GF2)(base + (oy[i] + co));
+      x[i] = make_float2(vx.x, vx.y);
+      y[i] = make_float2(vy.x, vy.y);
+      ox[i] += st;
+      oy[i] += st;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        acc[i][j].x = fmaf(x[i].x, y[j].x, acc[i][j].x);
+        acc[i][j].x = fmaf(x[i].y, y[j].y, acc[i][j].x);
+        acc[i][j].y = fmaf(x[i].y, y[j].x, acc[i][j].y);
+        acc[i][j].y = fmaf(-x[i].x, y[j].y, acc[i][j].y);
+      }
+  }
+  // the partial sums of waves 1-3 through LDS, 16 entries per round
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    __syncthreads();
+    if (w > 0) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) red[w - 1][k][lane] = acc[(r * 16 + k) / 8][(r * 16 + k) % 8];
+    }
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        float2 v = acc[(r * 16 + k) / 8][(r * 16 + k) % 8];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { v.x += red[q][k][lane].x; v.y += red[q][k][lane].y; }
+        acc[(r * 16 + k) / 8][(r * 16 + k) % 8] = v;
+      }
+    }
+  }
+  if (w != 0 || c >= g.C) return;
+  const int n = g.nrows;
+  const int* __restrict__ pairs = a.pairs + g.pair0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int ri = it.rb * 8 + i;
+    if (ri >= n) break;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int cj = it.cb * 8 + j;
+      if (cj >= n) break;
+      const int p = pairs[ri * n + cj];
+      if (p >= 0) {
+        float2 v = acc[i][j];
+        if (ri == cj) v.y = 0.0f;
+        float2* d = a.acc + p + c;
+        *d = make_float2(d->x + v.x, d->y + v.y);
+      }
+      if (it.rb != it.cb) {
+        const int q = pairs[cj * n + ri];
+        if (q >= 0) {
+          float2* d = a.acc + q + c;
+          *d = make_float2(d->x + acc[i][j].x, d->y - acc[i][j].y);
+        }
+      }
+    }
+  }
+}
+
 template <typename V8>
 __device__ __forceinline__ floatx16 mfma32(V8 a, V8 b, floatx16 c);
 template <>
@@ -376,6 +485,12 @@ __global__ void k_fill(float* p, long long n, float v) {
 }
 
 // ---- host-side launchers -------------------------------------------------------------
+int fxg_launch_xplain_blk(const XPBlkArgs& a, int nitems, hipStream_t st) {
+  if (nitems == 0) return 0;
+  hipLaunchKernelGGL(k_xmac_f32_blk, dim3(nitems), dim3(256), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int fxg_launch_xplain(const XPlainArgs& a, int nprod, int maxchan, hipStream_t st) {
   dim3 grid((maxchan + 255) / 256, nprod);
   hipLaunchKernelGGL(k_xmac_f32, grid, dim3(256), 0, st, a);
