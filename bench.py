@@ -11,7 +11,8 @@ in HBM), so `value` excludes PCIe transfers.
 
 Paths timed on the same inputs, with the same number of steps:
   MFMA      fused F-engine + FP16 MFMA Hermitian X-engine   -> `value`
-  plain     fused F-engine (FP32 spectra) + FP32 CMAC X-engine per baseline product
+  plain     fused F-engine (FP32 spectra) + FP32 CMAC X-engine (8 x 8 blocks of each freq's
+            input x input matrix per channel, FP32 FMA)
             -> `plain_fp32_value`, and `speedup_mfma_vs_plain` = value / plain
   BF16      the MFMA path with BF16 inputs (section mfma_bf16)
 
