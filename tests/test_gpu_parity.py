@@ -292,6 +292,18 @@ def test_order0_band_pairs(gpu, name, xe, bound):
     assert worst < 2e-5
 
 
+@pytest.mark.parametrize("nst,npol", [(5, 1), (32, 2), (36, 2)])
+def test_plain_x_engine_group_sizes(gpu, nst, npol):
+    """The blocked plain FP32 X-engine over groups of 5 rows (one partial 8 x 8 block), 64 rows
+    (eight row blocks, 36 blocks per channel tile) and 72 rows (above its 64-row limit: every
+    product of the group goes to the per-product kernel), nchan 128 (two 64-channel tiles)."""
+    from difx_amd import config as cf
+    cfg = cf.standard_config(nst, npol, [1600.0], 16.0, 128, subint_ns=8 * 8000, xengine="fp32", seed=30 + nst)
+    got, ref, e, lay = _run_and_compare(cfg)
+    _say(f"plain X {nst} stations x {npol} pol cross max", max(e["cross"]), "auto max", max(e["auto"]))
+    assert max(e["cross"]) < 1e-5 and max(e["auto"]) < 1e-5
+
+
 @pytest.mark.parametrize("xe,bound", [("fp32", 1e-5), ("fp16", 1e-3), ("bf16", 1e-2)])
 def test_usb_lsb_products(gpu, xe, bound):
     """USB x LSB: two stations record a sky band as USB, two as LSB; the cross-sideband
