@@ -119,3 +119,36 @@ def test_footprint_is_what_create_holds(gpu):
         core.lib.fxg_device_bytes(core.h, C.byref(held))
         core.close()
         assert held.value == dev
+
+
+def test_serving_mix_against_oracle(gpu):
+    """Jobs of the serving stream's shapes (4, 8, 16 and 32 stations dual-pol, 2-bit VDIF,
+    1-2 x 32 MHz, nchan 256-4096, FP16 MFMA X-engine, one FP32 job) admitted together and
+    interleaved: each job's device integration over its subints against the sum of the FP64
+    oracle's subint results, within the bar of its X-engine (1e-3 FP16, 1e-5 FP32)."""
+    import fxoracle as fo
+    from fxcases import record_errors
+    from difx_amd import config as cf
+    from difx_amd.server import FxServer
+    jobs = {
+        "n4_fp32": cf.standard_config(4, 2, [4800.0, 4840.0], 32.0, 256, subint_ns=16 * 8000, xengine="fp32", seed=41),
+        "n8": cf.standard_config(8, 2, [4800.0], 32.0, 1024, subint_ns=8 * 32000, xengine="fp16", seed=42),
+        "n16": cf.standard_config(16, 2, [4800.0, 4840.0], 32.0, 256, subint_ns=16 * 8000, xengine="fp16", seed=43),
+        "n32": cf.standard_config(32, 2, [4800.0], 32.0, 4096, subint_ns=4 * 128000, xengine="fp16", seed=44),
+    }
+    subs = {name: _slots(cfg, gpu, 300 + 10 * i) for i, (name, cfg) in enumerate(jobs.items())}
+    with FxServer() as srv:
+        mix = _run(srv, jobs, subs, list(jobs))
+        st = srv.stats()
+    _say("mix stats", st)
+    assert st["jobs_finished"] == len(jobs) and st["max_active"] >= 2
+    for name, cfg in jobs.items():
+        lay = fo.result_layout(cfg)
+        ref = np.zeros(lay.coreresultlength, np.complex128)
+        for offsets, dev, nb, ctrl, host in subs[name]:
+            r, _ = fo.process_subint(cfg, list(offsets), host, ctrl)
+            ref += r
+        e = record_errors(cfg, lay, mix[name][0].astype(np.complex128), ref)
+        bound = 1e-5 if cfg.xengine == "fp32" else 1e-3
+        _say(f"serving {name} vs oracle: cross max {max(e['cross']):.3e}, auto max {max(e['auto']):.3e}")
+        assert max(e["cross"]) < bound and max(e["auto"]) < bound
