@@ -1,10 +1,13 @@
 // X-engines and accumulation of the FX hot path on CDNA4.
 //
-// Plain FP32 path: one thread per (product, channel) accumulates
+// Plain FP32 path: every product accumulates
 //   acc += F1[t][c] * conj(F2[t][c])
-// sequentially over the FFTs of the subint, exactly the vectorAddProduct_cf32 of
-// Core::processdata (mpifxcorr/src/core.cpp:884-979, :963) for the cross products and of
-// Mode::process for the autocorrelations (mode.cpp:1272, 1333-1339).
+// over the FFTs of the subint in FP32, the vectorAddProduct_cf32 of Core::processdata
+// (mpifxcorr/src/core.cpp:884-979, :963) for the cross products and of Mode::process for the
+// autocorrelations (mode.cpp:1272, 1333-1339).  The blocked kernel takes 8 x 8 blocks of the
+// input x input matrix of the products' inputs (FFT order within each quarter of a chunk, the
+// quarters in order); the per-product kernel (one thread per product and channel, FFT order)
+// takes the products it leaves.
 //
 // MFMA path: per channel the visibilities of all inputs (station bands of one frequency)
 // form the Hermitian matrix V = sum_t x_t x_t^H.  With x stored as interleaved (re, im)
