@@ -12,7 +12,8 @@ in HBM), so `value` excludes PCIe transfers.
 Paths timed on the same inputs, with the same number of steps:
   MFMA      fused F-engine + FP16 MFMA Hermitian X-engine   -> `value`
   plain     fused F-engine (FP32 spectra) + FP32 CMAC X-engine (8 x 8 blocks of each freq's
-            input x input matrix per channel, FP32 FMA)
+            input x input matrix per channel, FP32 FMA), with the X-engine's VALU issue
+            fraction (plain_fp32.x_engine_valu_issue)
             -> `plain_fp32_value`, and `speedup_mfma_vs_plain` = value / plain
   BF16      the MFMA path with BF16 inputs (section mfma_bf16)
 
@@ -672,6 +673,19 @@ def main():
         # agreement of the two paths on identical inputs (diffDiFX metric over the whole array)
         a, b = head["results"], p["results"]
         plain["mfma_vs_plain_rel_diff"] = float(np.mean(np.abs(a - b)) / max(np.mean(np.abs(b)), 1e-30))
+        # the blocked FP32 X-engine against the VALU issue rate: per freq of n inputs it
+        # computes nb (nb + 1) / 2 blocks of 8 x 8 products (nb = ceil(n / 8)) per channel and
+        # FFT, 4 FMA lane-instructions per product
+        lane_ops = 0.0
+        for fi, fr in enumerate(pcfg.freqs):
+            n = sum(1 for ds in pcfg.datastreams for lf, _ in ds.recbands if ds.recfreqs[lf] == fi)
+            nb = (n + 7) // 8
+            lane_ops += nb * (nb + 1) / 2 * 64 * 4 * fr.nchan * B
+        xs = plain["x_ms_per_step"] / 1e3
+        plain["x_engine_valu_issue"] = {
+            "kernel": "k_xmac_f32_blk", "achieved": lane_ops / xs, "peak": VALU_PEAK_LANE_OPS,
+            "unit": "lane-instructions/s", "frac": lane_ops / xs / VALU_PEAK_LANE_OPS,
+            "fma_lane_instructions_per_subint": lane_ops}
     bf16 = None
     if not args.no_bf16:
         bsteps = max(1, min(args.steps, 10))
